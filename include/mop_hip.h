@@ -1,0 +1,301 @@
+/*
+ * mop_hip.h -- C ABI of the MI355X (gfx950) HIP backend for Master of
+ * Puppets' software-rasterizer frame path.
+ *
+ * The backend plugs in behind MOP's RHI function table.  Every mop_hip_*
+ * entry point below replaces one slot of `MopRhiBackend` (src/rhi/rhi.h:132-339)
+ * or one of the software calls the viewport core makes directly on the CPU
+ * backend's framebuffer (src/core/viewport.c:2433,2699,2729,2953-2959).
+ * mop_rhi_backend_hip() returns the filled table; a maintainer adds the
+ * MOP_BACKEND_HIP enum value and a case in mop_rhi_get_backend (rhi.c:50-71).
+ *
+ * Plain C types only: pointers, sizes and POD structs whose layout is
+ * identical to the reference's (see the Mirror types section; a test compiles
+ * both and compares sizeof/offsetof field by field).
+ *
+ * Error behaviour follows the RHI: creates return NULL on failure, draw and
+ * frame calls are void and log on failure (rhi.h:115-129, cpu_backend.c:584).
+ * The library never falls back to a CPU path: without a usable gfx950 device,
+ * mop_hip_device_create returns NULL and logs why.
+ */
+#ifndef MOP_HIP_H
+#define MOP_HIP_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Mirror types: same field order, types and therefore layout as the
+ * reference value types, so a pointer to either may be passed.
+ * ------------------------------------------------------------------------ */
+
+typedef struct MopHipVec3 { float x, y, z; } MopHipVec3;        /* types.h:23-25 */
+typedef struct MopHipVec4 { float x, y, z, w; } MopHipVec4;     /* types.h:27-29 */
+typedef struct MopHipMat4 { float d[16]; } MopHipMat4;          /* types.h:39-41, column-major */
+typedef struct MopHipColor { float r, g, b, a; } MopHipColor;   /* types.h:47-49 */
+
+/* 48-byte vertex: types.h:58-63 */
+typedef struct MopHipVertex {
+  MopHipVec3 position;
+  MopHipVec3 normal;
+  MopHipColor color;
+  float u, v;
+} MopHipVertex;
+
+/* light.h:24-28 / 37-62 (72 bytes) */
+enum { MOP_HIP_LIGHT_DIRECTIONAL = 0, MOP_HIP_LIGHT_POINT = 1, MOP_HIP_LIGHT_SPOT = 2 };
+typedef struct MopHipLight {
+  int32_t type;
+  MopHipVec3 position;
+  MopHipVec3 direction;
+  MopHipColor color;
+  float intensity;
+  float range;
+  float spot_inner_cos;
+  float spot_outer_cos;
+  bool active;
+  bool cast_shadows;
+  void *viewport;
+} MopHipLight;
+
+/* vertex_format.h:25-70 */
+enum {
+  MOP_HIP_ATTRIB_POSITION = 0, MOP_HIP_ATTRIB_NORMAL = 1, MOP_HIP_ATTRIB_COLOR = 2,
+  MOP_HIP_ATTRIB_TEXCOORD0 = 3
+};
+enum {
+  MOP_HIP_FORMAT_FLOAT = 0, MOP_HIP_FORMAT_FLOAT2 = 1, MOP_HIP_FORMAT_FLOAT3 = 2,
+  MOP_HIP_FORMAT_FLOAT4 = 3, MOP_HIP_FORMAT_UBYTE4 = 4
+};
+typedef struct MopHipVertexAttrib {
+  int32_t semantic;
+  int32_t format;
+  uint32_t offset;
+} MopHipVertexAttrib;
+typedef struct MopHipVertexFormat {
+  MopHipVertexAttrib attribs[12];
+  uint32_t attrib_count;
+  uint32_t stride;
+} MopHipVertexFormat;
+
+/* postprocess.h:20-39 */
+enum {
+  MOP_HIP_POST_GAMMA = 1 << 0, MOP_HIP_POST_TONEMAP = 1 << 1,
+  MOP_HIP_POST_VIGNETTE = 1 << 2, MOP_HIP_POST_FOG = 1 << 3,
+  MOP_HIP_POST_FXAA = 1 << 4
+};
+typedef struct MopHipFogParams {
+  MopHipColor color;
+  float near_dist;
+  float far_dist;
+} MopHipFogParams;
+
+/* types.h:77-96 */
+enum { MOP_HIP_BLEND_OPAQUE = 0, MOP_HIP_BLEND_ALPHA = 1, MOP_HIP_BLEND_ADDITIVE = 2,
+       MOP_HIP_BLEND_MULTIPLY = 3 };
+enum { MOP_HIP_SHADING_FLAT = 0, MOP_HIP_SHADING_SMOOTH = 1 };
+
+typedef struct MopHipDevice MopHipDevice;
+typedef struct MopHipBuffer MopHipBuffer;
+typedef struct MopHipFramebuffer MopHipFramebuffer;
+typedef struct MopHipTexture MopHipTexture;
+typedef struct MopHipShader MopHipShader;
+
+typedef struct MopHipBufferDesc { const void *data; size_t size; } MopHipBufferDesc; /* rhi.h:33-36 */
+typedef struct MopHipFramebufferDesc { int width, height; } MopHipFramebufferDesc;  /* rhi.h:42-45 */
+
+/* rhi.h:51-110 -- everything one draw needs */
+typedef struct MopHipDrawCall {
+  MopHipBuffer *vertex_buffer;
+  MopHipBuffer *index_buffer;
+  uint32_t vertex_count;
+  uint32_t index_count;
+  uint32_t object_id;
+  MopHipMat4 model;
+  MopHipMat4 view;
+  MopHipMat4 projection;
+  MopHipMat4 mvp;
+  MopHipColor base_color;
+  float opacity;
+  MopHipVec3 light_dir;
+  float ambient;
+  int32_t shading_mode;
+  bool wireframe;
+  bool depth_test;
+  bool depth_write;
+  bool backface_cull;
+  MopHipTexture *texture;
+  MopHipTexture *normal_map;
+  MopHipTexture *metallic_roughness_map;
+  MopHipTexture *ao_map;
+  int32_t blend_mode;
+  float metallic;
+  float roughness;
+  MopHipVec3 emissive;
+  const MopHipLight *lights;
+  uint32_t light_count;
+  MopHipVec3 cam_eye;
+  const MopHipVertexFormat *vertex_format;
+  float line_width;
+  float depth_bias;
+  MopHipVec3 aabb_min;
+  MopHipVec3 aabb_max;
+  bool cast_shadows;
+} MopHipDrawCall;
+
+/* ------------------------------------------------------------------------
+ * RHI slots (rhi.h:132-339).  Each comment names the slot it fills.
+ * ------------------------------------------------------------------------ */
+
+MopHipDevice *mop_hip_device_create(void);                                  /* device_create  rhi.h:135 */
+void mop_hip_device_destroy(MopHipDevice *dev);                             /* device_destroy rhi.h:136 */
+
+MopHipBuffer *mop_hip_buffer_create(MopHipDevice *dev, const MopHipBufferDesc *desc); /* rhi.h:139-140; copies desc->data */
+void mop_hip_buffer_destroy(MopHipDevice *dev, MopHipBuffer *buf);          /* rhi.h:141 */
+void mop_hip_buffer_update(MopHipDevice *dev, MopHipBuffer *buf, const void *data,
+                           size_t offset, size_t size);                     /* buffer_update rhi.h:224-225 */
+const void *mop_hip_buffer_read(MopHipBuffer *buf);                         /* buffer_read rhi.h:229; host copy */
+
+MopHipFramebuffer *mop_hip_framebuffer_create(MopHipDevice *dev,
+                                              const MopHipFramebufferDesc *desc); /* rhi.h:144-145 */
+void mop_hip_framebuffer_destroy(MopHipDevice *dev, MopHipFramebuffer *fb); /* rhi.h:146 */
+void mop_hip_framebuffer_resize(MopHipDevice *dev, MopHipFramebuffer *fb, int width,
+                                int height);                                /* rhi.h:147-148 */
+
+void mop_hip_frame_begin(MopHipDevice *dev, MopHipFramebuffer *fb, MopHipColor clear); /* rhi.h:160-161 */
+void mop_hip_frame_end(MopHipDevice *dev, MopHipFramebuffer *fb);          /* rhi.h:162 */
+void mop_hip_frame_submit(MopHipDevice *dev, MopHipFramebuffer *fb);       /* rhi.h:163 */
+void mop_hip_frame_wait_readback(MopHipDevice *dev, MopHipFramebuffer *fb); /* rhi.h:171 */
+void mop_hip_draw(MopHipDevice *dev, MopHipFramebuffer *fb, const MopHipDrawCall *call); /* rhi.h:173-174 */
+void mop_hip_draw_instanced(MopHipDevice *dev, MopHipFramebuffer *fb,
+                            const MopHipDrawCall *call, const MopHipMat4 *instance_transforms,
+                            uint32_t instance_count);                      /* rhi.h:218-221 */
+
+uint32_t mop_hip_pick_read_id(MopHipDevice *dev, MopHipFramebuffer *fb, int x, int y);  /* rhi.h:177-178 */
+float mop_hip_pick_read_depth(MopHipDevice *dev, MopHipFramebuffer *fb, int x, int y);  /* rhi.h:179-180 */
+const uint8_t *mop_hip_framebuffer_read_color(MopHipDevice *dev, MopHipFramebuffer *fb,
+                                              int *out_w, int *out_h);     /* rhi.h:183-185 */
+const uint32_t *mop_hip_framebuffer_read_object_id(MopHipDevice *dev, MopHipFramebuffer *fb,
+                                                   int *out_w, int *out_h); /* rhi.h:188-191 */
+const float *mop_hip_framebuffer_read_depth(MopHipDevice *dev, MopHipFramebuffer *fb,
+                                            int *out_w, int *out_h);       /* rhi.h:194-196 */
+
+MopHipTexture *mop_hip_texture_create(MopHipDevice *dev, int w, int h,
+                                      const uint8_t *rgba);                /* rhi.h:199-200 */
+MopHipTexture *mop_hip_texture_create_hdr(MopHipDevice *dev, int w, int h,
+                                          const float *rgba);              /* rhi.h:201-203 */
+void mop_hip_texture_destroy(MopHipDevice *dev, MopHipTexture *tex);       /* rhi.h:215 */
+bool mop_hip_texture_read_rgba8(MopHipDevice *dev, MopHipTexture *tex, uint8_t *out,
+                                size_t size);                              /* rhi.h:319-320 */
+float mop_hip_frame_gpu_time_ms(MopHipDevice *dev);                        /* rhi.h:233 */
+void mop_hip_set_exposure(MopHipDevice *dev, float exposure);              /* rhi.h:238 */
+
+/* The filled MopRhiBackend-layout table (name "hip"). */
+const void *mop_rhi_backend_hip(void);
+
+/* ------------------------------------------------------------------------
+ * Hooks for the viewport core's CPU-gated software calls.  Same arguments as
+ * the software function each replaces, plus the device.
+ * ------------------------------------------------------------------------ */
+
+/* mop_sw_shadow_render_mesh (rasterizer.c:357-493; called viewport.c:2433).
+ * `vertices`/`indices` are the host pointers buffer_read returned (looked up
+ * to the resident device buffers) or any host arrays (uploaded).  `size` is
+ * the square shadow-map size; `first` != 0 starts a new shadow map cleared to
+ * 1.0 (pass_shadow clears before its first mesh, viewport.c:2365-2368). */
+void mop_hip_shadow_render_mesh(MopHipDevice *dev, const void *vertices, uint32_t vertex_count,
+                                const uint32_t *indices, uint32_t index_count,
+                                const MopHipMat4 *model, const MopHipMat4 *light_vp, int size,
+                                int first);
+/* mop_sw_shadow_set (rasterizer.c:141-146): shading now tests this map. */
+void mop_hip_shadow_set(MopHipDevice *dev, int w, int h, const MopHipMat4 *light_vp);
+/* mop_sw_shadow_clear (rasterizer.c:148-152). */
+void mop_hip_shadow_clear(MopHipDevice *dev);
+/* mop_sw_ibl_set (rasterizer.c:167-181; rasterizer.h:212-215): tables are
+ * copied to HBM (re-uploaded only when their contents change). */
+void mop_hip_ibl_set(MopHipDevice *dev, const float *irradiance, int irr_w, int irr_h,
+                     const float *prefiltered, int pf_w, int pf_h, int pf_levels,
+                     const float *brdf_lut, int brdf_size, float rotation, float intensity);
+/* mop_sw_ibl_clear (rasterizer.c:183). */
+void mop_hip_ibl_clear(MopHipDevice *dev);
+/* mop_sw_hdr_resolve (rasterizer.c:2776-2803). */
+void mop_hip_hdr_resolve(MopHipDevice *dev, MopHipFramebuffer *fb, float exposure);
+/* mop_postprocess_apply (postprocess.c:24-116): FXAA then fog/tonemap/gamma/vignette. */
+void mop_hip_postprocess_apply(MopHipDevice *dev, MopHipFramebuffer *fb, uint32_t effects,
+                               const MopHipFogParams *fog);
+/* The RGBA32F HDR attachment (MopSwFramebuffer.color_hdr, rasterizer.h:33),
+ * read back to host memory owned by the framebuffer. */
+const float *mop_hip_framebuffer_read_hdr(MopHipDevice *dev, MopHipFramebuffer *fb, int *out_w,
+                                          int *out_h);
+
+/* ------------------------------------------------------------------------
+ * Device-resident shadow maps and IBL tables (no counterpart in the
+ * reference, whose shadow/IBL state is process-global: rasterizer.c:141-183).
+ * They let many scenes keep their lighting state in HBM and let one device
+ * render frames of different scenes back to back (batching, serving).
+ * ------------------------------------------------------------------------ */
+
+typedef struct MopHipShadowMap MopHipShadowMap;
+typedef struct MopHipIbl MopHipIbl;
+
+MopHipShadowMap *mop_hip_shadow_map_create(MopHipDevice *dev, int size);
+void mop_hip_shadow_map_destroy(MopHipDevice *dev, MopHipShadowMap *map);
+/* Start a caster list for `map`; mop_hip_shadow_draw adds resident meshes
+ * (model matrix, light view-projection as pass_shadow computes them,
+ * viewport.c:2312-2415); mop_hip_shadow_end renders the map (cleared to 1.0)
+ * and makes later draws test it, like mop_sw_shadow_set. */
+void mop_hip_shadow_begin(MopHipDevice *dev, MopHipShadowMap *map);
+void mop_hip_shadow_draw(MopHipDevice *dev, MopHipBuffer *vb, uint32_t vertex_count,
+                         MopHipBuffer *ib, uint32_t index_count, const MopHipMat4 *model,
+                         const MopHipMat4 *light_vp);
+void mop_hip_shadow_end(MopHipDevice *dev, const MopHipMat4 *light_vp);
+/* Make later draws test an already rendered map. */
+void mop_hip_shadow_bind(MopHipDevice *dev, const MopHipShadowMap *map,
+                         const MopHipMat4 *light_vp);
+
+/* IBL tables computed on the device from an equirect RGBA32F environment
+ * (environment.c:115-484: irradiance 64x32, prefiltered 128x64 x 5 levels,
+ * BRDF LUT 256^2). */
+MopHipIbl *mop_hip_ibl_create(MopHipDevice *dev, const float *env_rgba, int w, int h);
+void mop_hip_ibl_destroy(MopHipDevice *dev, MopHipIbl *ibl);
+/* Like mop_hip_ibl_set with resident tables. */
+void mop_hip_ibl_bind(MopHipDevice *dev, const MopHipIbl *ibl, float rotation, float intensity);
+/* Copy the tables to host arrays (64*32*4, 5-level pyramid, 256*256*4 floats). */
+int mop_hip_ibl_download(MopHipDevice *dev, const MopHipIbl *ibl, float *irr, float *pf,
+                         float *lut);
+
+/* ------------------------------------------------------------------------
+ * Diagnostics (no counterpart in the reference).
+ * ------------------------------------------------------------------------ */
+
+/* Last error message of this thread ("" when none). */
+const char *mop_hip_last_error(void);
+
+/* Per-frame statistics of the last executed frame. */
+typedef struct MopHipFrameStats {
+  uint32_t triangles_in;     /* triangles submitted (viewport.c:2032 counting) */
+  uint32_t subtris;          /* screen triangles after cull/clip/fan */
+  uint32_t clipped;          /* triangles that went through the clipper */
+  uint32_t big_subtris;      /* screen triangles walked row-parallel */
+  uint32_t shadow_tris;      /* shadow-caster triangles */
+  float gpu_ms;              /* device time of the frame (hipEvents) */
+} MopHipFrameStats;
+void mop_hip_get_frame_stats(MopHipDevice *dev, MopHipFrameStats *out);
+
+/* Device-side stage timers (hipEvents on the backend's stream around each
+ * stage: vertex, setup, raster, shade, resolve, post, shadow).  Enabling
+ * resets the sums; reading synchronises the stream. */
+void mop_hip_timers_enable(MopHipDevice *dev, int on);
+int mop_hip_timers_read(MopHipDevice *dev, const char *stage, double *total_ms,
+                        uint64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOP_HIP_H */

@@ -1,0 +1,1568 @@
+/*
+ * kernels.hip -- gfx950 kernels of the software-rasterizer frame path.
+ *
+ * Every arithmetic step that feeds coverage, depth or the object-ID buffer
+ * follows the reference expression by expression (see devmath.h), because
+ * those buffers must come out bit-identical to the CPU backend.  The layout
+ * around that arithmetic is MI355X-first: one lane per vertex / triangle /
+ * pixel, a visibility buffer updated with 64-bit atomic min in L2 instead of
+ * ordered per-tile loops, and row-per-lane walks for large triangles.
+ */
+#include "devmath.h"
+#include "mop_hip_internal.h"
+
+#define WG 256
+
+/* --------------------------------------------------------------------- */
+/* draw lookup: draws are sorted by vtx_base / tri_base                   */
+/* --------------------------------------------------------------------- */
+
+DEV uint32_t find_draw_by_vtx(const DrawGPU *d, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (d[mid].vtx_base <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+DEV uint32_t find_draw_by_tri(const DrawGPU *d, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (d[mid].tri_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* --------------------------------------------------------------------- */
+/* k_vertex: cpu_prepare_triangle (cpu_backend.c:224-324) and the flexible */
+/* format variant (cpu_backend.c:330-526), once per vertex.               */
+/* --------------------------------------------------------------------- */
+
+DEV void read_f3(const uint8_t *base, int off, int fmt, float o[3]) {
+  const float *f = (const float *)(base + off);
+  switch (fmt) {
+  case MOP_HIP_FORMAT_FLOAT3:
+  case MOP_HIP_FORMAT_FLOAT4:
+    o[0] = f[0]; o[1] = f[1]; o[2] = f[2];
+    break;
+  case MOP_HIP_FORMAT_FLOAT2:
+    o[0] = f[0]; o[1] = f[1]; o[2] = 0.0f;
+    break;
+  case MOP_HIP_FORMAT_FLOAT:
+    o[0] = f[0]; o[1] = 0.0f; o[2] = 0.0f;
+    break;
+  default:
+    o[0] = o[1] = o[2] = 0.0f;
+  }
+}
+
+DEV void read_f4(const uint8_t *base, int off, int fmt, float o[4]) {
+  const float *f = (const float *)(base + off);
+  switch (fmt) {
+  case MOP_HIP_FORMAT_FLOAT4:
+    o[0] = f[0]; o[1] = f[1]; o[2] = f[2]; o[3] = f[3];
+    break;
+  case MOP_HIP_FORMAT_FLOAT3:
+    o[0] = f[0]; o[1] = f[1]; o[2] = f[2]; o[3] = 1.0f;
+    break;
+  case MOP_HIP_FORMAT_FLOAT2:
+    o[0] = f[0]; o[1] = f[1]; o[2] = 0.0f; o[3] = 1.0f;
+    break;
+  case MOP_HIP_FORMAT_FLOAT:
+    o[0] = f[0]; o[1] = 0.0f; o[2] = 0.0f; o[3] = 1.0f;
+    break;
+  default:
+    o[0] = o[1] = o[2] = 0.0f; o[3] = 1.0f;
+  }
+}
+
+DEV void read_f2(const uint8_t *base, int off, int fmt, float o[2]) {
+  const float *f = (const float *)(base + off);
+  switch (fmt) {
+  case MOP_HIP_FORMAT_FLOAT2:
+  case MOP_HIP_FORMAT_FLOAT3:
+  case MOP_HIP_FORMAT_FLOAT4:
+    o[0] = f[0]; o[1] = f[1];
+    break;
+  case MOP_HIP_FORMAT_FLOAT:
+    o[0] = f[0]; o[1] = 0.0f;
+    break;
+  default:
+    o[0] = o[1] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
+  uint32_t v = blockIdx.x * WG + threadIdx.x;
+  if (v >= fa.n_verts)
+    return;
+  uint32_t di = find_draw_by_vtx(fa.draws, fa.ndraws, v);
+  const DrawGPU &d = fa.draws[di];
+  uint32_t i = v - d.vtx_base;
+  const uint8_t *vb = d.vbuf + (size_t)i * d.stride;
+  float p[3], n[3], c[4], uv[2];
+  bool has_n = true;
+  if (!(d.flags & MOPH_DRAW_FLEX)) {
+    const float *f = (const float *)vb;
+    p[0] = f[0]; p[1] = f[1]; p[2] = f[2];
+    n[0] = f[3]; n[1] = f[4]; n[2] = f[5];
+    c[0] = f[6]; c[1] = f[7]; c[2] = f[8]; c[3] = f[9];
+    uv[0] = f[10]; uv[1] = f[11];
+  } else {
+    read_f3(vb, d.pos_off, d.pos_fmt, p);
+    if (d.nrm_off >= 0)
+      read_f3(vb, d.nrm_off, d.nrm_fmt, n);
+    else
+      has_n = false;
+    if (d.col_off >= 0)
+      read_f4(vb, d.col_off, d.col_fmt, c);
+    else
+      c[0] = c[1] = c[2] = c[3] = 1.0f;
+    if (d.uv_off >= 0)
+      read_f2(vb, d.uv_off, d.uv_fmt, uv);
+    else
+      uv[0] = uv[1] = 0.0f;
+  }
+  float4 clip = mat4_mul_vec4(d.mvp, p[0], p[1], p[2], 1.0f);
+  if (d.depth_bias != 0.0f)
+    clip.z += d.depth_bias * clip.w;
+  float4 wp = mat4_mul_vec4(d.model, p[0], p[1], p[2], 1.0f);
+  float nx, ny, nz;
+  if (has_n) {
+    float4 tn = mat4_mul_vec4(d.model, n[0], n[1], n[2], 0.0f);
+    nx = tn.x; ny = tn.y; nz = tn.z;
+  } else {
+    nx = 0.0f; ny = 1.0f; nz = 0.0f;
+  }
+  if (d.tex) {
+    /* nearest texel at the vertex UV modulates the colour (cpu_backend.c:277-303) */
+    float tu = uv[0] - floorf(uv[0]);
+    float tv = uv[1] - floorf(uv[1]);
+    int tx = cvt_i32(tu * (float)(d.tex_w - 1) + 0.5f);
+    int ty = cvt_i32(tv * (float)(d.tex_h - 1) + 0.5f);
+    if (tx < 0) tx = 0;
+    if (tx >= d.tex_w) tx = d.tex_w - 1;
+    if (ty < 0) ty = 0;
+    if (ty >= d.tex_h) ty = d.tex_h - 1;
+    const uint8_t *t = d.tex + ((size_t)ty * (size_t)d.tex_w + (size_t)tx) * 4;
+    c[0] *= (float)t[0] / 255.0f;
+    c[1] *= (float)t[1] / 255.0f;
+    c[2] *= (float)t[2] / 255.0f;
+    c[3] *= (float)t[3] / 255.0f;
+  }
+  fa.vpos[v] = clip;
+  float4 *a = fa.vattr + (size_t)v * 3;
+  a[0] = make_float4(nx, ny, nz, wp.x);
+  a[1] = make_float4(wp.y, wp.z, c[0], c[1]);
+  a[2] = make_float4(c[2], c[3], 0.0f, 0.0f);
+}
+
+void launch_vertex(const FrameArgs &fa, hipStream_t s) {
+  if (!fa.n_verts)
+    return;
+  hipLaunchKernelGGL(k_vertex, dim3((fa.n_verts + WG - 1) / WG), dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* classification and fan checks (rasterizer.c:2458-2553)                */
+/* --------------------------------------------------------------------- */
+
+DEV bool trivially_rejected(float4 a, float4 b, float4 c) {
+  return (a.x < -a.w && b.x < -b.w && c.x < -c.w) ||
+         (a.x > a.w && b.x > b.w && c.x > c.w) ||
+         (a.y < -a.w && b.y < -b.w && c.y < -c.w) ||
+         (a.y > a.w && b.y > b.w && c.y > c.w) ||
+         (a.z < -a.w && b.z < -b.w && c.z < -c.w) ||
+         (a.z > a.w && b.z > b.w && c.z > c.w);
+}
+
+DEV bool inside_all(float4 a) {
+  return a.x >= -a.w && a.x <= a.w && a.y >= -a.w && a.y <= a.w &&
+         a.z >= -a.w && a.z <= a.w;
+}
+
+/* One fan triangle survives: |w| guard, screen-space cull, bbox, |area|. */
+DEV bool fan_ok(float4 p0, float4 p1, float4 p2, bool cull, int W, int H) {
+  if (fabsf(p0.w) < 1e-7f || fabsf(p1.w) < 1e-7f || fabsf(p2.w) < 1e-7f)
+    return false;
+  float half_w = (float)W * 0.5f, half_h = (float)H * 0.5f;
+  float sx[3], sy[3], sz[3], iw[3];
+  screen_xform(p0, half_w, half_h, sx[0], sy[0], sz[0], iw[0]);
+  screen_xform(p1, half_w, half_h, sx[1], sy[1], sz[1], iw[1]);
+  screen_xform(p2, half_w, half_h, sx[2], sy[2], sz[2], iw[2]);
+  float signed_area =
+      (sx[1] - sx[0]) * (sy[2] - sy[0]) - (sx[2] - sx[0]) * (sy[1] - sy[0]);
+  if (cull && signed_area >= 0.0f)
+    return false;
+  EdgeSetup es;
+  return edge_setup(sx, sy, W, H, es);
+}
+
+__global__ __launch_bounds__(WG) void k_classify(FrameArgs fa) {
+  uint32_t t = blockIdx.x * WG + threadIdx.x;
+  if (t >= fa.n_tris)
+    return;
+  uint32_t di = find_draw_by_tri(fa.draws, fa.ndraws, t);
+  const DrawGPU &d = fa.draws[di];
+  uint32_t k = t - d.tri_base;
+  uint32_t i0 = d.ibuf[3 * k], i1 = d.ibuf[3 * k + 1], i2 = d.ibuf[3 * k + 2];
+  uint32_t cnt = 0, slot = 0xffffffffu;
+  if (i0 < d.vertex_count && i1 < d.vertex_count && i2 < d.vertex_count) {
+    float4 a = fa.vpos[d.vtx_base + i0];
+    float4 b = fa.vpos[d.vtx_base + i1];
+    float4 c = fa.vpos[d.vtx_base + i2];
+    bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
+    bool alive = !trivially_rejected(a, b, c);
+    if (alive && cull && a.w > 0.0f && b.w > 0.0f && c.w > 0.0f) {
+      float ex = b.x * a.w - a.x * b.w;
+      float ey = b.y * a.w - a.y * b.w;
+      float fx = c.x * a.w - a.x * c.w;
+      float fy = c.y * a.w - a.y * c.w;
+      if (ex * fy - ey * fx <= 0.0f)
+        alive = false;
+    }
+    if (alive) {
+      if (a.w > 0.0f && b.w > 0.0f && c.w > 0.0f && inside_all(a) &&
+          inside_all(b) && inside_all(c)) {
+        cnt = fan_ok(a, b, c, cull, fa.width, fa.height) ? 1u : 0u;
+      } else {
+        uint32_t q = atomicAdd(&fa.counters[CNT_CLIP], 1u);
+        if (q < fa.clip_cap) {
+          fa.clip_queue[q] = t;
+          slot = q;
+        } else {
+          atomicOr(&fa.counters[CNT_OVERFLOW], 1u);
+        }
+      }
+    }
+  }
+  fa.tri_counts[t] = cnt;
+  fa.clip_slot[t] = slot;
+}
+
+void launch_classify(const FrameArgs &fa, hipStream_t s) {
+  if (!fa.n_tris)
+    return;
+  hipLaunchKernelGGL(k_classify, dim3((fa.n_tris + WG - 1) / WG), dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* k_clip: mop_sw_clip_polygon (rasterizer.c:510-626) for queued triangles */
+/* Attributes carried: position, normal, world position, colour -- the    */
+/* ones the smooth multi-light path reads.                               */
+/* --------------------------------------------------------------------- */
+
+struct CV {
+  float p[4];
+  float a[10]; /* n.xyz, wp.xyz, c.rgba */
+};
+
+DEV float clip_dist(int plane, const float *p) {
+  /* FRUSTUM_PLANES (rasterizer.c:510-517), all four products kept */
+  const float N[6][4] = {{1.0f, 0.0f, 0.0f, 1.0f},  {-1.0f, 0.0f, 0.0f, 1.0f},
+                         {0.0f, 1.0f, 0.0f, 1.0f},  {0.0f, -1.0f, 0.0f, 1.0f},
+                         {0.0f, 0.0f, 1.0f, 1.0f},  {0.0f, 0.0f, -1.0f, 1.0f}};
+  return N[plane][0] * p[0] + N[plane][1] * p[1] + N[plane][2] * p[2] +
+         N[plane][3] * p[3];
+}
+
+DEV void lerp_cv(const CV &a, const CV &b, float t, CV &r) {
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    r.p[k] = a.p[k] + t * (b.p[k] - a.p[k]);
+#pragma unroll
+  for (int k = 0; k < 10; k++)
+    r.a[k] = a.a[k] + t * (b.a[k] - a.a[k]);
+}
+
+DEV int clip_plane(const CV *in, int n, CV *out, int plane) {
+  if (n < 1)
+    return 0;
+  int oc = 0;
+  CV prev = in[n - 1];
+  float prev_d = clip_dist(plane, prev.p);
+  for (int i = 0; i < n; i++) {
+    CV curr = in[i];
+    float curr_d = clip_dist(plane, curr.p);
+    if (curr_d >= 0.0f) {
+      if (prev_d < 0.0f) {
+        float t = prev_d / (prev_d - curr_d);
+        if (oc < 24)
+          lerp_cv(prev, curr, t, out[oc++]);
+      }
+      if (oc < 24)
+        out[oc++] = curr;
+    } else if (prev_d >= 0.0f) {
+      float t = prev_d / (prev_d - curr_d);
+      if (oc < 24)
+        lerp_cv(prev, curr, t, out[oc++]);
+    }
+    prev = curr;
+    prev_d = curr_d;
+  }
+  return oc;
+}
+
+DEV void load_cv(const FrameArgs &fa, uint32_t v, CV &o) {
+  float4 p = fa.vpos[v];
+  const float4 *a = fa.vattr + (size_t)v * 3;
+  float4 a0 = a[0], a1 = a[1], a2 = a[2];
+  o.p[0] = p.x; o.p[1] = p.y; o.p[2] = p.z; o.p[3] = p.w;
+  o.a[0] = a0.x; o.a[1] = a0.y; o.a[2] = a0.z;
+  o.a[3] = a0.w; o.a[4] = a1.x; o.a[5] = a1.y;
+  o.a[6] = a1.z; o.a[7] = a1.w; o.a[8] = a2.x; o.a[9] = a2.y;
+}
+
+__global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
+  uint32_t nq = fa.counters[CNT_CLIP];
+  if (nq > fa.clip_cap)
+    nq = fa.clip_cap;
+  for (uint32_t q = blockIdx.x * 64 + threadIdx.x; q < nq; q += gridDim.x * 64) {
+    uint32_t t = fa.clip_queue[q];
+    uint32_t di = find_draw_by_tri(fa.draws, fa.ndraws, t);
+    const DrawGPU &d = fa.draws[di];
+    uint32_t k = t - d.tri_base;
+    CV bufA[24], bufB[24]; /* MAX_CLIP_VERTICES, rasterizer.c:596 */
+    load_cv(fa, d.vtx_base + d.ibuf[3 * k], bufA[0]);
+    load_cv(fa, d.vtx_base + d.ibuf[3 * k + 1], bufA[1]);
+    load_cv(fa, d.vtx_base + d.ibuf[3 * k + 2], bufA[2]);
+    int n = 3;
+    CV *src = bufA, *dst = bufB;
+    for (int pl = 0; pl < 6 && n > 0; pl++) {
+      n = clip_plane(src, n, dst, pl);
+      CV *tmp = src;
+      src = dst;
+      dst = tmp;
+    }
+    uint32_t cnt = 0;
+    if (n > MOPH_MAX_POLY) {
+      atomicOr(&fa.counters[CNT_OVERFLOW], 8u);
+      n = 0;
+    }
+    if (n >= 3) {
+      bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
+      for (int i = 1; i < n - 1; i++) {
+        float4 p0 = make_float4(src[0].p[0], src[0].p[1], src[0].p[2], src[0].p[3]);
+        float4 p1 = make_float4(src[i].p[0], src[i].p[1], src[i].p[2], src[i].p[3]);
+        float4 p2 = make_float4(src[i + 1].p[0], src[i + 1].p[1], src[i + 1].p[2],
+                                src[i + 1].p[3]);
+        if (fan_ok(p0, p1, p2, cull, fa.width, fa.height))
+          cnt++;
+      }
+      size_t base = (size_t)q * MOPH_MAX_POLY;
+      for (int i = 0; i < n; i++) {
+        fa.cpos[base + i] = make_float4(src[i].p[0], src[i].p[1], src[i].p[2], src[i].p[3]);
+        float4 *ca = fa.cattr + (base + i) * 3;
+        ca[0] = make_float4(src[i].a[0], src[i].a[1], src[i].a[2], src[i].a[3]);
+        ca[1] = make_float4(src[i].a[4], src[i].a[5], src[i].a[6], src[i].a[7]);
+        ca[2] = make_float4(src[i].a[8], src[i].a[9], 0.0f, 0.0f);
+      }
+    } else {
+      n = 0;
+    }
+    fa.cnt_poly[q] = (uint32_t)n;
+    fa.tri_counts[t] = cnt;
+  }
+}
+
+void launch_clip(const FrameArgs &fa, hipStream_t s) {
+  if (!fa.n_tris || !fa.clip_cap)
+    return;
+  uint32_t grid = (fa.clip_cap + 63) / 64;
+  if (grid > 4096)
+    grid = 4096;
+  hipLaunchKernelGGL(k_clip, dim3(grid), dim3(64), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* exclusive scan of per-triangle screen-triangle counts                  */
+/* --------------------------------------------------------------------- */
+
+#define SCAN_ITEMS 16
+#define SCAN_TILE (WG * SCAN_ITEMS)
+
+DEV uint32_t block_excl_scan(uint32_t v, uint32_t *lds, uint32_t &total) {
+  /* wave64 inclusive scan, then across the 4 waves */
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o)
+      x += y;
+  }
+  if (lane == 63)
+    lds[wid] = x;
+  __syncthreads();
+  uint32_t wpre = 0, tot = 0;
+  for (int w = 0; w < WG / 64; w++) {
+    if (w < wid)
+      wpre += lds[w];
+    tot += lds[w];
+  }
+  __syncthreads();
+  total = tot;
+  return wpre + x - v;
+}
+
+__global__ __launch_bounds__(WG) void k_scan_reduce(const uint32_t *in, uint32_t n,
+                                                    uint32_t *sums) {
+  __shared__ uint32_t lds[WG / 64];
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++)
+    if (base + k < n)
+      acc += in[base + k];
+  uint32_t tot;
+  block_excl_scan(acc, lds, tot);
+  if (threadIdx.x == 0)
+    sums[blockIdx.x] = tot;
+}
+
+/* single workgroup: exclusive scan of the block sums, in place */
+__global__ __launch_bounds__(WG) void k_scan_sums(uint32_t *sums, uint32_t nb) {
+  __shared__ uint32_t lds[WG / 64];
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nb; b0 += WG) {
+    uint32_t i = b0 + threadIdx.x;
+    uint32_t v = i < nb ? sums[i] : 0;
+    uint32_t tot;
+    uint32_t ex = block_excl_scan(v, lds, tot);
+    if (i < nb)
+      sums[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_scan_apply(uint32_t *io, uint32_t n,
+                                                   const uint32_t *sums,
+                                                   uint32_t *counters,
+                                                   uint32_t cap) {
+  __shared__ uint32_t lds[WG / 64];
+  size_t base = (size_t)blockIdx.x * SCAN_TILE + (size_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    v[k] = (base + k < n) ? io[base + k] : 0;
+    acc += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = sums[blockIdx.x] + block_excl_scan(acc, lds, tot);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    if (base + k < n)
+      io[base + k] = run;
+    run += v[k];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == WG - 1) {
+    io[n] = run; /* total */
+    counters[CNT_SUBTRI] = run;
+    if (run > cap)
+      atomicOr(&counters[CNT_OVERFLOW], 2u);
+  }
+}
+
+void launch_scan_counts(const FrameArgs &fa, uint32_t *block_sums, hipStream_t s) {
+  uint32_t n = fa.n_tris;
+  uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 0)
+    nb = 1;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(WG), 0, s, fa.tri_counts, n, block_sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(WG), 0, s, block_sums, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(WG), 0, s, fa.tri_counts, n, block_sums,
+                     fa.counters, fa.subtri_cap);
+}
+
+/* --------------------------------------------------------------------- */
+/* k_emit: write screen triangles at their scanned index                  */
+/* --------------------------------------------------------------------- */
+
+DEV void write_subtri(const FrameArgs &fa, uint32_t idx, uint32_t draw, float4 p0,
+                      float4 p1, float4 p2, uint32_t a0, uint32_t a1, uint32_t a2) {
+  if (idx >= fa.subtri_cap)
+    return;
+  float half_w = (float)fa.width * 0.5f, half_h = (float)fa.height * 0.5f;
+  SubTri st;
+  screen_xform(p0, half_w, half_h, st.sx[0], st.sy[0], st.sz[0], st.iw[0]);
+  screen_xform(p1, half_w, half_h, st.sx[1], st.sy[1], st.sz[1], st.iw[1]);
+  screen_xform(p2, half_w, half_h, st.sx[2], st.sy[2], st.sz[2], st.iw[2]);
+  st.draw = draw;
+  st.a[0] = a0;
+  st.a[1] = a1;
+  st.a[2] = a2;
+  fa.subtris[idx] = st;
+}
+
+__global__ __launch_bounds__(WG) void k_emit(FrameArgs fa) {
+  uint32_t t = blockIdx.x * WG + threadIdx.x;
+  if (t >= fa.n_tris)
+    return;
+  uint32_t off = fa.tri_counts[t];
+  uint32_t cnt = fa.tri_counts[t + 1] - off;
+  if (!cnt)
+    return;
+  uint32_t di = find_draw_by_tri(fa.draws, fa.ndraws, t);
+  const DrawGPU &d = fa.draws[di];
+  uint32_t slot = fa.clip_slot[t];
+  if (slot == 0xffffffffu) {
+    uint32_t k = t - d.tri_base;
+    uint32_t v0 = d.vtx_base + d.ibuf[3 * k], v1 = d.vtx_base + d.ibuf[3 * k + 1],
+             v2 = d.vtx_base + d.ibuf[3 * k + 2];
+    write_subtri(fa, off, di, fa.vpos[v0], fa.vpos[v1], fa.vpos[v2], v0, v1, v2);
+    return;
+  }
+  int n = (int)fa.cnt_poly[slot];
+  bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
+  size_t base = (size_t)slot * MOPH_MAX_POLY;
+  float4 p0 = fa.cpos[base];
+  uint32_t o = off;
+  for (int i = 1; i < n - 1; i++) {
+    float4 p1 = fa.cpos[base + i], p2 = fa.cpos[base + i + 1];
+    if (fan_ok(p0, p1, p2, cull, fa.width, fa.height)) {
+      write_subtri(fa, o, di, p0, p1, p2, fa.n_verts + (uint32_t)base,
+                   fa.n_verts + (uint32_t)(base + i), fa.n_verts + (uint32_t)(base + i + 1));
+      o++;
+    }
+  }
+}
+
+void launch_emit(const FrameArgs &fa, hipStream_t s) {
+  if (!fa.n_tris)
+    return;
+  hipLaunchKernelGGL(k_emit, dim3((fa.n_tris + WG - 1) / WG), dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* raster: smooth multi-light coverage + depth (rasterizer.c:1926-1957)   */
+/* --------------------------------------------------------------------- */
+
+struct RasterTri {
+  EdgeSetup es;
+  float inv_e[3];
+  float sz[3];
+};
+
+DEV bool load_raster_tri(const FrameArgs &fa, const SubTri &st, RasterTri &r) {
+  if (!edge_setup(st.sx, st.sy, fa.width, fa.height, r.es))
+    return false;
+  r.inv_e[0] = inv_edge_len(r.es.e_dx[0], r.es.e_dy[0]);
+  r.inv_e[1] = inv_edge_len(r.es.e_dx[1], r.es.e_dy[1]);
+  r.inv_e[2] = inv_edge_len(r.es.e_dx[2], r.es.e_dy[2]);
+  r.sz[0] = st.sz[0];
+  r.sz[1] = st.sz[1];
+  r.sz[2] = st.sz[2];
+  return true;
+}
+
+/* Coverage + min_d test + barycentric depth of one pixel; false if the
+ * pixel is not drawn (edge test or AA distance test fails or z is NaN). */
+DEV bool frag_depth(const RasterTri &r, float w0, float w1, float w2, float &z) {
+  if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f))
+    return false;
+  float d0 = w0 * r.inv_e[0];
+  float d1 = w1 * r.inv_e[1];
+  float d2 = w2 * r.inv_e[2];
+  float min_d = d0;
+  if (d1 < min_d)
+    min_d = d1;
+  if (d2 < min_d)
+    min_d = d2;
+  if (!(min_d >= 0.0f))
+    return false;
+  float b0 = w0 * r.es.inv_area;
+  float b1 = w1 * r.es.inv_area;
+  float b2 = w2 * r.es.inv_area;
+  z = b0 * r.sz[0] + b1 * r.sz[1] + b2 * r.sz[2];
+  return z == z;
+}
+
+DEV void vis_update(unsigned long long *vis, size_t idx, unsigned long long key) {
+  unsigned long long cur = __builtin_nontemporal_load(vis + idx);
+  if (key < cur)
+    atomicMin(vis + idx, key);
+}
+
+__global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
+  uint32_t total = fa.counters[CNT_SUBTRI];
+  if (total > fa.subtri_cap)
+    total = fa.subtri_cap;
+  for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
+    SubTri st = fa.subtris[s];
+    RasterTri r;
+    if (!load_raster_tri(fa, st, r))
+      continue;
+    if (bbox_area(r.es) > MOPH_SMALL_AREA) {
+      uint32_t q = atomicAdd(&fa.counters[CNT_BIG], 1u);
+      fa.big_queue[q] = s;
+      continue;
+    }
+    unsigned long long seq = (unsigned long long)(s + 1);
+    float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+    for (int y = r.es.min_y; y <= r.es.max_y; y++) {
+      float w0 = w0r, w1 = w1r, w2 = w2r;
+      for (int x = r.es.min_x; x <= r.es.max_x; x++) {
+        float z;
+        if (frag_depth(r, w0, w1, w2, z)) {
+          size_t idx = (size_t)y * (size_t)fa.width + (size_t)x;
+          vis_update(fa.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+        }
+        w0 += r.es.e_dx[0];
+        w1 += r.es.e_dx[1];
+        w2 += r.es.e_dx[2];
+      }
+      w0r += r.es.e_dy[0];
+      w1r += r.es.e_dy[1];
+      w2r += r.es.e_dy[2];
+    }
+  }
+}
+
+/* Big screen triangles: one workgroup each, one lane per bbox row.  The row
+ * starts are the serial chain w_row += e_dy (rasterizer.c:2148-2150); lane 0
+ * extends it 256 rows at a time into LDS.  mode 0 updates the visibility
+ * buffer; mode 1 (after all updates) stores the winner's edge values. */
+template <int MODE>
+__global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
+  __shared__ float rows[3][WG];
+  __shared__ float carry[3];
+  uint32_t nbig = fa.counters[CNT_BIG];
+  for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+    uint32_t s = fa.big_queue[b];
+    SubTri st = fa.subtris[s];
+    RasterTri r;
+    load_raster_tri(fa, st, r); /* succeeded when queued */
+    unsigned long long seq = (unsigned long long)(s + 1);
+    int nrows = r.es.max_y - r.es.min_y + 1;
+    if (threadIdx.x == 0) {
+      carry[0] = r.es.w_row[0];
+      carry[1] = r.es.w_row[1];
+      carry[2] = r.es.w_row[2];
+    }
+    for (int c0 = 0; c0 < nrows; c0 += WG) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float a0 = carry[0], a1 = carry[1], a2 = carry[2];
+        int lim = nrows - c0 < WG ? nrows - c0 : WG;
+        for (int k = 0; k < lim; k++) {
+          rows[0][k] = a0;
+          rows[1][k] = a1;
+          rows[2][k] = a2;
+          a0 += r.es.e_dy[0];
+          a1 += r.es.e_dy[1];
+          a2 += r.es.e_dy[2];
+        }
+        carry[0] = a0;
+        carry[1] = a1;
+        carry[2] = a2;
+      }
+      __syncthreads();
+      int row = c0 + (int)threadIdx.x;
+      if (row < nrows) {
+        int y = r.es.min_y + row;
+        float w0 = rows[0][threadIdx.x], w1 = rows[1][threadIdx.x], w2 = rows[2][threadIdx.x];
+        size_t rowbase = (size_t)y * (size_t)fa.width;
+        for (int x = r.es.min_x; x <= r.es.max_x; x++) {
+          float z;
+          if (frag_depth(r, w0, w1, w2, z)) {
+            unsigned long long key = ((unsigned long long)depth_key(z) << 32) | seq;
+            if (MODE == 0) {
+              vis_update(fa.vis, rowbase + x, key);
+            } else if (fa.vis[rowbase + x] == key) {
+              fa.wbuf[rowbase + x] = make_float4(w0, w1, w2, 0.0f);
+            }
+          }
+          w0 += r.es.e_dx[0];
+          w1 += r.es.e_dx[1];
+          w2 += r.es.e_dx[2];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_raster(const FrameArgs &fa, hipStream_t s) {
+  uint32_t cap = fa.subtri_cap;
+  uint32_t grid = (cap + WG - 1) / WG;
+  if (grid > 8192)
+    grid = 8192;
+  if (grid == 0)
+    grid = 1;
+  hipLaunchKernelGGL(k_raster_small, dim3(grid), dim3(WG), 0, s, fa);
+  hipLaunchKernelGGL(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
+  hipLaunchKernelGGL(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* shading (rasterizer.c:194-348, 1596-1814, 1950-2108)                   */
+/* --------------------------------------------------------------------- */
+
+/* shadow_test_pcf, rasterizer.c:305-348 */
+DEV float shadow_pcf(const FrameArgs &fa, V3 wp) {
+  float4 lp = mat4_mul_vec4(fa.shadow_vp, wp.x, wp.y, wp.z, 1.0f);
+  if (lp.w <= 0.0f)
+    return 1.0f;
+  float inv_w = 1.0f / lp.w;
+  float ndcx = lp.x * inv_w;
+  float ndcy = lp.y * inv_w;
+  float ndcz = (lp.z * inv_w + 1.0f) * 0.5f;
+  float u = (ndcx + 1.0f) * 0.5f * (float)fa.shadow_w;
+  float v = (1.0f - ndcy) * 0.5f * (float)fa.shadow_h;
+  int cx = cvt_i32(u);
+  int cy = cvt_i32(v);
+  const float bias = 0.005f;
+  int lit = 0, total = 0;
+  for (int dy = -1; dy <= 1; dy++) {
+    for (int dx = -1; dx <= 1; dx++) {
+      int sx = (int)((uint32_t)cx + (uint32_t)dx);
+      int sy = (int)((uint32_t)cy + (uint32_t)dy);
+      if (sx < 0 || sx >= fa.shadow_w || sy < 0 || sy >= fa.shadow_h) {
+        lit++;
+        total++;
+        continue;
+      }
+      float shadow_z = fa.shadow_map[(size_t)sy * (size_t)fa.shadow_w + (size_t)sx];
+      if (ndcz <= shadow_z + bias)
+        lit++;
+      total++;
+    }
+  }
+  return (float)lit / (float)total;
+}
+
+/* ibl_sample, rasterizer.c:187-207 (indices clamped: a NaN direction would
+ * read outside the table in the reference) */
+DEV void ibl_sample(const float *data, int w, int h, float u, float v, float out[3]) {
+  u = u - floorf(u);
+  v = v - floorf(v);
+  float fx = u * (float)(w - 1);
+  float fy = v * (float)(h - 1);
+  int x0 = cvt_i32(fx), y0 = cvt_i32(fy);
+  if (x0 < 0 || x0 >= w)
+    x0 = 0;
+  if (y0 < 0 || y0 >= h)
+    y0 = 0;
+  int x1 = (x0 + 1) % w;
+  int y1 = y0 + 1 < h ? y0 + 1 : h - 1;
+  float sx = fx - (float)x0, sy = fy - (float)y0;
+  const float *p00 = &data[((size_t)y0 * w + x0) * 4];
+  const float *p10 = &data[((size_t)y0 * w + x1) * 4];
+  const float *p01 = &data[((size_t)y1 * w + x0) * 4];
+  const float *p11 = &data[((size_t)y1 * w + x1) * 4];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float top = p00[c] * (1.0f - sx) + p10[c] * sx;
+    float bot = p01[c] * (1.0f - sx) + p11[c] * sx;
+    out[c] = top * (1.0f - sy) + bot * sy;
+  }
+}
+
+/* dir_to_uv, rasterizer.c:210-216 */
+DEV void dir_to_uv(float dx, float dy, float dz, float rot, float &u, float &v) {
+  float phi = atan2f(dz, dx) + rot;
+  float cdy = dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy);
+  u = phi / (2.0f * 3.14159265f) + 0.5f;
+  v = asinf(cdy) / 3.14159265f + 0.5f;
+}
+
+DEV void ibl_irradiance(const FrameArgs &fa, V3 n, float out[3]) {
+  out[0] = out[1] = out[2] = 0.0f;
+  float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+  if (len < 1e-6f)
+    return;
+  float u, v;
+  dir_to_uv(n.x / len, n.y / len, n.z / len, fa.ibl_rot, u, v);
+  ibl_sample(fa.irr, fa.irr_w, fa.irr_h, u, v, out);
+  out[0] *= fa.ibl_int;
+  out[1] *= fa.ibl_int;
+  out[2] *= fa.ibl_int;
+}
+
+DEV void ibl_prefiltered(const FrameArgs &fa, V3 refl, float roughness, float out[3]) {
+  out[0] = out[1] = out[2] = 0.0f;
+  float len = sqrtf(refl.x * refl.x + refl.y * refl.y + refl.z * refl.z);
+  if (len < 1e-6f)
+    return;
+  int level = cvt_i32(roughness * (float)(fa.pf_levels - 1));
+  if (level >= fa.pf_levels)
+    level = fa.pf_levels - 1;
+  if (level < 0)
+    level = 0;
+  size_t offset = 0;
+  for (int l = 0; l < level; l++) {
+    int lw = fa.pf_w >> l;
+    if (lw < 1)
+      lw = 1;
+    int lh = fa.pf_h >> l;
+    if (lh < 1)
+      lh = 1;
+    offset += (size_t)lw * lh;
+  }
+  int lw = fa.pf_w >> level;
+  if (lw < 1)
+    lw = 1;
+  int lh = fa.pf_h >> level;
+  if (lh < 1)
+    lh = 1;
+  float u, v;
+  dir_to_uv(refl.x / len, refl.y / len, refl.z / len, fa.ibl_rot, u, v);
+  ibl_sample(fa.pf + offset * 4, lw, lh, u, v, out);
+  out[0] *= fa.ibl_int;
+  out[1] *= fa.ibl_int;
+  out[2] *= fa.ibl_int;
+}
+
+DEV void ibl_brdf(const FrameArgs &fa, float ndotv, float roughness, float &scale,
+                  float &bias) {
+  if (ndotv < 0.0f)
+    ndotv = 0.0f;
+  if (ndotv > 1.0f)
+    ndotv = 1.0f;
+  if (roughness < 0.0f)
+    roughness = 0.0f;
+  if (roughness > 1.0f)
+    roughness = 1.0f;
+  int n = fa.lut_size;
+  int x = cvt_i32(ndotv * (float)(n - 1) + 0.5f);
+  int y = cvt_i32(roughness * (float)(n - 1) + 0.5f);
+  if (x >= n)
+    x = n - 1;
+  if (y >= n)
+    y = n - 1;
+  if (x < 0)
+    x = 0;
+  if (y < 0)
+    y = 0;
+  size_t idx = ((size_t)y * n + x) * 4;
+  scale = fa.lut[idx + 0];
+  bias = fa.lut[idx + 1];
+}
+
+DEV V3 light_pos(const MopHipLight &l) { return V3{l.position.x, l.position.y, l.position.z}; }
+DEV V3 light_dir(const MopHipLight &l) { return V3{l.direction.x, l.direction.y, l.direction.z}; }
+
+/* distance attenuation shared by point/spot (rasterizer.c:1625-1635) */
+DEV float atten(const MopHipLight &l, float dist) {
+  float a;
+  if (l.range > 0.0f) {
+    float r = dist / l.range;
+    a = 1.0f - r;
+    if (a < 0.0f)
+      a = 0.0f;
+    a *= a;
+  } else {
+    float d2 = dist * dist;
+    if (d2 < 0.01f)
+      d2 = 0.01f;
+    a = 1.0f / d2;
+  }
+  return a;
+}
+
+struct ShadeOut {
+  float r, g, b;
+};
+
+/* compute_multi_light_rgb (rasterizer.c:1596-1688) fused with
+ * compute_multi_specular_ggx (rasterizer.c:1701-1814): the two loops touch the
+ * same lights in the same order and accumulate into separate sums, so one
+ * loop with the PCF lookup shared gives the same values. */
+DEV void lights_eval(const FrameArgs &fa, const DrawGPU &d, V3 normal, V3 wp, V3 view_dir,
+                     float base_r, float base_g, float base_b, float lit[3], float spec[3]) {
+  lit[0] = lit[1] = lit[2] = d.ambient;
+  spec[0] = spec[1] = spec[2] = 0.0f;
+  float roughness = d.roughness, metallic = d.metallic;
+  float alpha = roughness * roughness;
+  float alpha2 = alpha * alpha;
+  if (alpha2 < 1e-7f)
+    alpha2 = 1e-7f;
+  float ndotv = v3dot(normal, view_dir);
+  if (ndotv < 1e-4f)
+    ndotv = 1e-4f;
+  float f0_r = 0.04f * (1.0f - metallic) + base_r * metallic;
+  float f0_g = 0.04f * (1.0f - metallic) + base_g * metallic;
+  float f0_b = 0.04f * (1.0f - metallic) + base_b * metallic;
+  float k = alpha * 0.5f;
+  float g1v = ndotv / (ndotv * (1.0f - k) + k);
+  const MopHipLight *L = fa.lights + d.light_off;
+  for (uint32_t i = 0; i < d.light_count; i++) {
+    const MopHipLight &l = L[i];
+    if (!l.active)
+      continue;
+    /* diffuse part */
+    float ndotl_d = 0.0f, att_d = 1.0f, spot = 1.0f;
+    /* specular part */
+    V3 l_dir = V3{0.0f, 0.0f, 0.0f};
+    float att_s = 1.0f;
+    if (l.type == MOP_HIP_LIGHT_DIRECTIONAL) {
+      V3 dir = v3norm(light_dir(l));
+      ndotl_d = -v3dot(normal, dir);
+      l_dir = V3{-dir.x, -dir.y, -dir.z};
+    } else if (l.type == MOP_HIP_LIGHT_POINT || l.type == MOP_HIP_LIGHT_SPOT) {
+      V3 to_light = v3sub(light_pos(l), wp);
+      float dist = v3len(to_light);
+      if (dist < 1e-6f)
+        dist = 1e-6f;
+      V3 dir = v3scale(to_light, 1.0f / dist);
+      ndotl_d = v3dot(normal, dir);
+      if (l.type == MOP_HIP_LIGHT_SPOT) {
+        V3 spot_dir = v3norm(light_dir(l));
+        float cos_angle = -v3dot(dir, spot_dir);
+        if (cos_angle < l.spot_outer_cos) {
+          spot = 0.0f;
+        } else if (cos_angle < l.spot_inner_cos) {
+          float range_s = l.spot_inner_cos - l.spot_outer_cos;
+          if (range_s > 1e-6f) {
+            float t = (cos_angle - l.spot_outer_cos) / range_s;
+            spot = t * t * (3.0f - 2.0f * t);
+          }
+        }
+      }
+      att_d = atten(l, dist);
+      att_s = att_d;
+      l_dir = dir;
+    }
+    if (ndotl_d < 0.0f)
+      ndotl_d = 0.0f;
+    bool dir_light = (l.type == MOP_HIP_LIGHT_DIRECTIONAL);
+    float shadow = 1.0f;
+    if (dir_light && fa.shadow_on)
+      shadow = shadow_pcf(fa, wp);
+    float contrib = ndotl_d * l.intensity * att_d * spot * shadow;
+    lit[0] += contrib * l.color.r;
+    lit[1] += contrib * l.color.g;
+    lit[2] += contrib * l.color.b;
+
+    float ndotl = v3dot(normal, l_dir);
+    if (ndotl <= 0.0f)
+      continue;
+    V3 h = v3norm(v3add(l_dir, view_dir));
+    float ndoth = v3dot(normal, h);
+    if (ndoth < 0.0f)
+      ndoth = 0.0f;
+    float vdoth = v3dot(view_dir, h);
+    if (vdoth < 0.0f)
+      vdoth = 0.0f;
+    float ndoth2 = ndoth * ndoth;
+    float denom_d = ndoth2 * (alpha2 - 1.0f) + 1.0f;
+    float D = alpha2 / (3.14159265358979323846f * denom_d * denom_d);
+    float g1l = ndotl / (ndotl * (1.0f - k) + k);
+    float G = g1v * g1l;
+    float omv = 1.0f - vdoth;
+    float omv2 = omv * omv;
+    float omv5 = omv2 * omv2 * omv;
+    float fr_r = f0_r + (1.0f - f0_r) * omv5;
+    float fr_g = f0_g + (1.0f - f0_g) * omv5;
+    float fr_b = f0_b + (1.0f - f0_b) * omv5;
+    float spec_denom = 4.0f * ndotl * ndotv;
+    if (spec_denom < 1e-6f)
+      spec_denom = 1e-6f;
+    float spec_term = D * G / spec_denom;
+    float weight = spec_term * l.intensity * att_s * ndotl * shadow *
+                   3.14159265358979323846f;
+    spec[0] += weight * fr_r * l.color.r;
+    spec[1] += weight * fr_g * l.color.g;
+    spec[2] += weight * fr_b * l.color.b;
+  }
+}
+
+DEV void load_attr(const FrameArgs &fa, uint32_t slot, float4 &a0, float4 &a1, float4 &a2) {
+  const float4 *p = (slot < fa.n_verts) ? fa.vattr + (size_t)slot * 3
+                                        : fa.cattr + (size_t)(slot - fa.n_verts) * 3;
+  a0 = p[0];
+  a1 = p[1];
+  a2 = p[2];
+}
+
+__global__ __launch_bounds__(WG) void k_shade(FrameArgs fa) {
+  int px = blockIdx.x * 16 + (threadIdx.x & 15);
+  int py = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (px >= fa.width || py >= fa.height)
+    return;
+  size_t idx = (size_t)py * (size_t)fa.width + (size_t)px;
+  unsigned long long key = fa.vis[idx];
+  uint32_t s1 = (uint32_t)key;
+  if (s1 == 0)
+    return;
+  uint32_t s = s1 - 1;
+  SubTri st = fa.subtris[s];
+  RasterTri r;
+  load_raster_tri(fa, st, r);
+  float w0, w1, w2;
+  if (bbox_area(r.es) > MOPH_SMALL_AREA) {
+    float4 w = fa.wbuf[idx];
+    w0 = w.x;
+    w1 = w.y;
+    w2 = w.z;
+  } else {
+    /* replay the reference's incremental walk up to this pixel */
+    float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+    for (int y = r.es.min_y; y < py; y++) {
+      w0r += r.es.e_dy[0];
+      w1r += r.es.e_dy[1];
+      w2r += r.es.e_dy[2];
+    }
+    w0 = w0r;
+    w1 = w1r;
+    w2 = w2r;
+    for (int x = r.es.min_x; x < px; x++) {
+      w0 += r.es.e_dx[0];
+      w1 += r.es.e_dx[1];
+      w2 += r.es.e_dx[2];
+    }
+  }
+  float b0 = w0 * r.es.inv_area;
+  float b1 = w1 * r.es.inv_area;
+  float b2 = w2 * r.es.inv_area;
+  float z = b0 * st.sz[0] + b1 * st.sz[1] + b2 * st.sz[2];
+
+  const DrawGPU &d = fa.draws[st.draw];
+  float pc0 = b0 * st.iw[0];
+  float pc1 = b1 * st.iw[1];
+  float pc2 = b2 * st.iw[2];
+  float inv_pc = 1.0f / (pc0 + pc1 + pc2);
+  pc0 *= inv_pc;
+  pc1 *= inv_pc;
+  pc2 *= inv_pc;
+  float4 A0[3], A1[3], A2[3];
+  load_attr(fa, st.a[0], A0[0], A1[0], A2[0]);
+  load_attr(fa, st.a[1], A0[1], A1[1], A2[1]);
+  load_attr(fa, st.a[2], A0[2], A1[2], A2[2]);
+  /* attribute k of vertex v: n=(A0.x,A0.y,A0.z) wp=(A0.w,A1.x,A1.y) c=(A1.z,A1.w,A2.x,A2.y) */
+  V3 n = V3{pc0 * A0[0].x + pc1 * A0[1].x + pc2 * A0[2].x,
+            pc0 * A0[0].y + pc1 * A0[1].y + pc2 * A0[2].y,
+            pc0 * A0[0].z + pc1 * A0[1].z + pc2 * A0[2].z};
+  n = v3norm(n);
+  float fr = pc0 * A1[0].z + pc1 * A1[1].z + pc2 * A1[2].z;
+  float fg = pc0 * A1[0].w + pc1 * A1[1].w + pc2 * A1[2].w;
+  float fb_ = pc0 * A2[0].x + pc1 * A2[1].x + pc2 * A2[2].x;
+  V3 wp = V3{pc0 * A0[0].w + pc1 * A0[1].w + pc2 * A0[2].w,
+             pc0 * A1[0].x + pc1 * A1[1].x + pc2 * A1[2].x,
+             pc0 * A1[0].y + pc1 * A1[1].y + pc2 * A1[2].y};
+  V3 eye = V3{d.cam_eye[0], d.cam_eye[1], d.cam_eye[2]};
+  V3 view_dir = v3norm(v3sub(eye, wp));
+
+  float lit[3], spec[3];
+  lights_eval(fa, d, n, wp, view_dir, fr, fg, fb_, lit, spec);
+  float metallic = d.metallic, roughness = d.roughness;
+  float diffuse_scale = 1.0f - metallic;
+  float env_r = 0.0f, env_g = 0.0f, env_b = 0.0f;
+  float irr[3] = {0.0f, 0.0f, 0.0f};
+  bool has_irr = fa.ibl_on && fa.irr;
+  if (has_irr)
+    ibl_irradiance(fa, n, irr);
+  {
+    float vdn = v3dot(view_dir, n);
+    V3 refl = V3{2.0f * vdn * n.x - view_dir.x, 2.0f * vdn * n.y - view_dir.y,
+                 2.0f * vdn * n.z - view_dir.z};
+    float ndv = v3dot(n, view_dir);
+    if (ndv < 0.0f)
+      ndv = 0.0f;
+    float f0_r = 0.04f * (1.0f - metallic) + fr * metallic;
+    float f0_g = 0.04f * (1.0f - metallic) + fg * metallic;
+    float f0_b = 0.04f * (1.0f - metallic) + fb_ * metallic;
+    if (fa.ibl_on && fa.pf && fa.lut) {
+      float pf[3];
+      ibl_prefiltered(fa, refl, roughness, pf);
+      float bs = 1.0f, bb = 0.0f;
+      ibl_brdf(fa, ndv, roughness, bs, bb);
+      env_r = pf[0] * (f0_r * bs + bb);
+      env_g = pf[1] * (f0_g * bs + bb);
+      env_b = pf[2] * (f0_b * bs + bb);
+    } else if (metallic > 0.01f) {
+      float sky_t = refl.y * 0.5f + 0.5f;
+      if (sky_t < 0.0f)
+        sky_t = 0.0f;
+      if (sky_t > 1.0f)
+        sky_t = 1.0f;
+      float sky_rv = 0.35f, sky_gv = 0.38f, sky_bv = 0.45f;
+      float gnd_rv = 0.08f, gnd_gv = 0.07f, gnd_bv = 0.06f;
+      float he_r = gnd_rv + sky_t * (sky_rv - gnd_rv);
+      float he_g = gnd_gv + sky_t * (sky_gv - gnd_gv);
+      float he_b = gnd_bv + sky_t * (sky_bv - gnd_bv);
+      float rough_mix = roughness * roughness;
+      float avg_env = (he_r + he_g + he_b) * (1.0f / 3.0f);
+      he_r += rough_mix * (avg_env - he_r);
+      he_g += rough_mix * (avg_env - he_g);
+      he_b += rough_mix * (avg_env - he_b);
+      float om = 1.0f - ndv;
+      float om2 = om * om;
+      float om5 = om2 * om2 * om;
+      float fe_r = f0_r + (1.0f - f0_r) * om5;
+      float fe_g = f0_g + (1.0f - f0_g) * om5;
+      float fe_b = f0_b + (1.0f - f0_b) * om5;
+      env_r = he_r * fe_r * d.ambient * 3.0f;
+      env_g = he_g * fe_g * d.ambient * 3.0f;
+      env_b = he_b * fe_b * d.ambient * 3.0f;
+    }
+  }
+  float pr, pg, pb;
+  if (has_irr) {
+    pr = fr * (lit[0] + irr[0]) * diffuse_scale + env_r + spec[0];
+    pg = fg * (lit[1] + irr[1]) * diffuse_scale + env_g + spec[1];
+    pb = fb_ * (lit[2] + irr[2]) * diffuse_scale + env_b + spec[2];
+  } else {
+    pr = fr * lit[0] * diffuse_scale + env_r + spec[0];
+    pg = fg * lit[1] * diffuse_scale + env_g + spec[1];
+    pb = fb_ * lit[2] * diffuse_scale + env_b + spec[2];
+  }
+  if (pr < 0.0f)
+    pr = 0.0f;
+  if (pg < 0.0f)
+    pg = 0.0f;
+  if (pb < 0.0f)
+    pb = 0.0f;
+  reinterpret_cast<float4 *>(fa.hdr)[idx] = make_float4(pr, pg, pb, 1.0f);
+  fa.depth[idx] = z;
+  fa.ids[idx] = d.object_id;
+}
+
+void launch_shade(const FrameArgs &fa, hipStream_t s) {
+  dim3 grid((fa.width + 15) / 16, (fa.height + 15) / 16);
+  hipLaunchKernelGGL(k_shade, grid, dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* clear (rasterizer.c:114-134), resolve (2770-2803), post (postprocess.c) */
+/* --------------------------------------------------------------------- */
+
+__global__ __launch_bounds__(WG) void k_clear(int npx, float r, float g, float b, float a,
+                                              uint8_t *color, float *hdr, float *depth,
+                                              uint32_t *ids, unsigned long long *vis) {
+  int i = blockIdx.x * WG + threadIdx.x;
+  if (i >= npx)
+    return;
+  uchar4 c;
+  c.x = cvt_u8(r * 255.0f);
+  c.y = cvt_u8(g * 255.0f);
+  c.z = cvt_u8(b * 255.0f);
+  c.w = cvt_u8(a * 255.0f);
+  reinterpret_cast<uchar4 *>(color)[i] = c;
+  reinterpret_cast<float4 *>(hdr)[i] = make_float4(r, g, b, a);
+  depth[i] = 1.0f;
+  ids[i] = 0u;
+  vis[i] = (unsigned long long)depth_key(1.0f) << 32;
+}
+
+void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr, float *depth,
+                  uint32_t *ids, unsigned long long *vis, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, clear[0],
+                     clear[1], clear[2], clear[3], color, hdr, depth, ids, vis);
+}
+
+__global__ __launch_bounds__(WG) void k_vis_from_depth(int npx, const float *depth,
+                                                       unsigned long long *vis) {
+  int i = blockIdx.x * WG + threadIdx.x;
+  if (i >= npx)
+    return;
+  float dz = depth[i];
+  /* a NaN depth can never be beaten by `z < depth` */
+  uint32_t k = (dz == dz) ? depth_key(dz) : 0u;
+  vis[i] = (unsigned long long)k << 32;
+}
+
+void launch_vis_from_depth(int npx, const float *depth, unsigned long long *vis,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_vis_from_depth, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, depth,
+                     vis);
+}
+
+DEV float aces(float x) {
+  float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f;
+  float m = (x * (a * x + b)) / (x * (c * x + d) + e);
+  return m < 0.0f ? 0.0f : (m > 1.0f ? 1.0f : m);
+}
+
+__global__ __launch_bounds__(WG) void k_resolve(int npx, const float *hdr, const uint32_t *ids,
+                                                uint8_t *color, float exposure) {
+  int i = blockIdx.x * WG + threadIdx.x;
+  if (i >= npx)
+    return;
+  float4 h = reinterpret_cast<const float4 *>(hdr)[i];
+  uchar4 c;
+  if (ids[i] == 0) {
+    c.x = cvt_u8(fminf(h.x, 1.0f) * 255.0f + 0.5f);
+    c.y = cvt_u8(fminf(h.y, 1.0f) * 255.0f + 0.5f);
+    c.z = cvt_u8(fminf(h.z, 1.0f) * 255.0f + 0.5f);
+    c.w = 255;
+  } else {
+    float r = aces(h.x * exposure);
+    float g = aces(h.y * exposure);
+    float b = aces(h.z * exposure);
+    c.x = cvt_u8(r * 255.0f + 0.5f);
+    c.y = cvt_u8(g * 255.0f + 0.5f);
+    c.z = cvt_u8(b * 255.0f + 0.5f);
+    c.w = cvt_u8(h.w * 255.0f + 0.5f);
+  }
+  reinterpret_cast<uchar4 *>(color)[i] = c;
+}
+
+void launch_resolve(int npx, const float *hdr, const uint32_t *ids, uint8_t *color,
+                    float exposure, hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, hdr, ids,
+                     color, exposure);
+}
+
+DEV float luma_at(const uint8_t *src, int w, int x, int y) {
+  const uint8_t *p = src + ((size_t)y * (size_t)w + (size_t)x) * 4;
+  return 0.299f * (float)p[0] + 0.587f * (float)p[1] + 0.114f * (float)p[2];
+}
+
+DEV float srgb_enc(float c) {
+  return c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+}
+
+/* mop_sw_fxaa (rasterizer.c:2625-2761) followed by the per-pixel effects of
+ * mop_postprocess_apply (postprocess.c:36-116); FXAA reads only the copy of
+ * the resolved image, so both fuse into one pass from src into dst. */
+__global__ __launch_bounds__(WG) void k_post(int w, int h, const uint8_t *src, uint8_t *dst,
+                                             const float *depth, uint32_t effects, float fog_r,
+                                             float fog_g, float fog_b, float fog_near,
+                                             float fog_far) {
+  int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= w || y >= h)
+    return;
+  size_t ci = ((size_t)y * (size_t)w + (size_t)x) * 4;
+  uint8_t o0 = src[ci], o1 = src[ci + 1], o2 = src[ci + 2], o3 = src[ci + 3];
+  if ((effects & MOP_HIP_POST_FXAA) && w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 &&
+      y < h - 1) {
+    const float EDGE_THRESHOLD = 0.0625f, EDGE_THRESHOLD_MIN = 0.004f, SUBPIX_QUALITY = 0.75f;
+    float lM = luma_at(src, w, x, y);
+    float lN = luma_at(src, w, x, y - 1);
+    float lS = luma_at(src, w, x, y + 1);
+    float lW = luma_at(src, w, x - 1, y);
+    float lE = luma_at(src, w, x + 1, y);
+    float range_min = lM;
+    if (lN < range_min) range_min = lN;
+    if (lS < range_min) range_min = lS;
+    if (lW < range_min) range_min = lW;
+    if (lE < range_min) range_min = lE;
+    float range_max = lM;
+    if (lN > range_max) range_max = lN;
+    if (lS > range_max) range_max = lS;
+    if (lW > range_max) range_max = lW;
+    if (lE > range_max) range_max = lE;
+    float range = range_max - range_min;
+    if (!(range < EDGE_THRESHOLD * range_max + EDGE_THRESHOLD_MIN)) {
+      float lNW = luma_at(src, w, x - 1, y - 1);
+      float lNE = luma_at(src, w, x + 1, y - 1);
+      float lSW = luma_at(src, w, x - 1, y + 1);
+      float lSE = luma_at(src, w, x + 1, y + 1);
+      float luma_avg = (lN + lS + lW + lE) * (2.0f / 12.0f) + (lNW + lNE + lSW + lSE) * (1.0f / 12.0f);
+      float subpix = fabsf(luma_avg - lM);
+      subpix = subpix / range;
+      if (subpix > 1.0f)
+        subpix = 1.0f;
+      subpix = subpix * subpix * (3.0f - 2.0f * subpix);
+      subpix *= SUBPIX_QUALITY;
+      float edge_h = fabsf(-2.0f * lW + lNW + lSW) + fabsf(-2.0f * lM + lN + lS) * 2.0f +
+                     fabsf(-2.0f * lE + lNE + lSE);
+      float edge_v = fabsf(-2.0f * lN + lNW + lNE) + fabsf(-2.0f * lM + lW + lE) * 2.0f +
+                     fabsf(-2.0f * lS + lSW + lSE);
+      bool is_horiz = (edge_h >= edge_v);
+      int step_x = is_horiz ? 0 : 1;
+      int step_y = is_horiz ? 1 : 0;
+      float luma_pos = is_horiz ? lS : lE;
+      float luma_neg = is_horiz ? lN : lW;
+      float grad_pos = fabsf(luma_pos - lM);
+      float grad_neg = fabsf(luma_neg - lM);
+      int off_x, off_y;
+      if (grad_pos >= grad_neg) {
+        off_x = step_x;
+        off_y = step_y;
+      } else {
+        off_x = -step_x;
+        off_y = -step_y;
+      }
+      float blend = subpix;
+      int nx = x + off_x, ny = y + off_y;
+      if (nx < 0) nx = 0;
+      if (nx >= w) nx = w - 1;
+      if (ny < 0) ny = 0;
+      if (ny >= h) ny = h - 1;
+      size_t ni = ((size_t)ny * (size_t)w + (size_t)nx) * 4;
+      float inv_blend = 1.0f - blend;
+      o0 = cvt_u8((float)src[ci + 0] * inv_blend + (float)src[ni + 0] * blend);
+      o1 = cvt_u8((float)src[ci + 1] * inv_blend + (float)src[ni + 1] * blend);
+      o2 = cvt_u8((float)src[ci + 2] * inv_blend + (float)src[ni + 2] * blend);
+    }
+  }
+  uint32_t rest = effects & (MOP_HIP_POST_GAMMA | MOP_HIP_POST_TONEMAP |
+                             MOP_HIP_POST_VIGNETTE | MOP_HIP_POST_FOG);
+  if (effects != 0) {
+    /* postprocess.c runs its per-pixel loop whenever effects != 0 */
+    float r = (float)o0 / 255.0f;
+    float g = (float)o1 / 255.0f;
+    float b = (float)o2 / 255.0f;
+    if (rest & MOP_HIP_POST_FOG) {
+      float dz = depth[(size_t)y * (size_t)w + (size_t)x];
+      float range = fog_far - fog_near;
+      float factor = 0.0f;
+      if (range > 0.0f) {
+        factor = (dz - fog_near) / range;
+        if (factor < 0.0f) factor = 0.0f;
+        if (factor > 1.0f) factor = 1.0f;
+      }
+      r = r + (fog_r - r) * factor;
+      g = g + (fog_g - g) * factor;
+      b = b + (fog_b - b) * factor;
+    }
+    if (rest & MOP_HIP_POST_TONEMAP) {
+      r = r / (r + 1.0f);
+      g = g / (g + 1.0f);
+      b = b / (b + 1.0f);
+    }
+    if (rest & MOP_HIP_POST_GAMMA) {
+      r = srgb_enc(r > 0.0f ? r : 0.0f);
+      g = srgb_enc(g > 0.0f ? g : 0.0f);
+      b = srgb_enc(b > 0.0f ? b : 0.0f);
+    }
+    if (rest & MOP_HIP_POST_VIGNETTE) {
+      float cx = (float)w * 0.5f, cy = (float)h * 0.5f;
+      float max_dist = sqrtf(cx * cx + cy * cy);
+      float dx = (float)x - cx, dy = (float)y - cy;
+      float dist = sqrtf(dx * dx + dy * dy) / max_dist;
+      float vig = 1.0f - dist * dist * 0.5f;
+      if (vig < 0.0f) vig = 0.0f;
+      r *= vig;
+      g *= vig;
+      b *= vig;
+    }
+    if (r > 1.0f) r = 1.0f;
+    if (g > 1.0f) g = 1.0f;
+    if (b > 1.0f) b = 1.0f;
+    if (r < 0.0f) r = 0.0f;
+    if (g < 0.0f) g = 0.0f;
+    if (b < 0.0f) b = 0.0f;
+    o0 = cvt_u8(r * 255.0f + 0.5f);
+    o1 = cvt_u8(g * 255.0f + 0.5f);
+    o2 = cvt_u8(b * 255.0f + 0.5f);
+  }
+  dst[ci] = o0;
+  dst[ci + 1] = o1;
+  dst[ci + 2] = o2;
+  dst[ci + 3] = o3;
+}
+
+void launch_post(int w, int h, const uint8_t *src, uint8_t *dst, const float *depth,
+                 uint32_t effects, const MopHipFogParams *fog, hipStream_t s) {
+  dim3 grid((w + 15) / 16, (h + 15) / 16);
+  float fr = 0, fg = 0, fb = 0, fn = 0, ff = 0;
+  if (fog) {
+    fr = fog->color.r;
+    fg = fog->color.g;
+    fb = fog->color.b;
+    fn = fog->near_dist;
+    ff = fog->far_dist;
+  } else {
+    effects &= ~(uint32_t)MOP_HIP_POST_FOG;
+  }
+  hipLaunchKernelGGL(k_post, grid, dim3(WG), 0, s, w, h, src, dst, depth, effects, fr, fg, fb,
+                     fn, ff);
+}
+
+__global__ void k_fill_u32(size_t n, uint32_t *p, uint32_t v) {
+  size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+void launch_fill_u32(size_t n, uint32_t *p, uint32_t v, hipStream_t s) {
+  if (!n)
+    return;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, s, n, p, v);
+}
+
+/* --------------------------------------------------------------------- */
+/* shadow depth pass (mop_sw_shadow_render_mesh, rasterizer.c:357-493)    */
+/* --------------------------------------------------------------------- */
+
+__global__ __launch_bounds__(WG) void k_shadow_clear(int n, uint32_t *keys) {
+  int i = blockIdx.x * WG + threadIdx.x;
+  if (i < n)
+    keys[i] = depth_key(1.0f);
+}
+
+void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow_clear, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, keys);
+}
+
+DEV uint32_t find_shadow_draw(const ShadowDrawGPU *d, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (d[mid].tri_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+struct ShadowTri {
+  EdgeSetup es;
+  float sz[3];
+};
+
+DEV bool shadow_setup(const ShadowArgsGPU &sa, uint32_t t, ShadowTri &st) {
+  uint32_t di = find_shadow_draw(sa.draws, sa.ndraws, t);
+  const ShadowDrawGPU &d = sa.draws[di];
+  uint32_t k = t - d.tri_base;
+  uint32_t i0 = d.ibuf[3 * k], i1 = d.ibuf[3 * k + 1], i2 = d.ibuf[3 * k + 2];
+  if (i0 >= d.vertex_count || i1 >= d.vertex_count || i2 >= d.vertex_count)
+    return false;
+  const float *v0 = (const float *)(d.vbuf + (size_t)i0 * sizeof(MopHipVertex));
+  const float *v1 = (const float *)(d.vbuf + (size_t)i1 * sizeof(MopHipVertex));
+  const float *v2 = (const float *)(d.vbuf + (size_t)i2 * sizeof(MopHipVertex));
+  float4 c0 = mat4_mul_vec4(d.mvp, v0[0], v0[1], v0[2], 1.0f);
+  float4 c1 = mat4_mul_vec4(d.mvp, v1[0], v1[1], v1[2], 1.0f);
+  float4 c2 = mat4_mul_vec4(d.mvp, v2[0], v2[1], v2[2], 1.0f);
+  if (trivially_rejected(c0, c1, c2))
+    return false;
+  if (c0.w <= 1e-6f || c1.w <= 1e-6f || c2.w <= 1e-6f)
+    return false;
+  float half = (float)sa.size * 0.5f;
+  float sx[3], sy[3], iw[3];
+  screen_xform(c0, half, half, sx[0], sy[0], st.sz[0], iw[0]);
+  screen_xform(c1, half, half, sx[1], sy[1], st.sz[1], iw[1]);
+  screen_xform(c2, half, half, sx[2], sy[2], st.sz[2], iw[2]);
+  return edge_setup(sx, sy, sa.size, sa.size, st.es);
+}
+
+DEV void shadow_frag(const ShadowArgsGPU &sa, const ShadowTri &st, int x, int y, float w0,
+                     float w1, float w2) {
+  float b0 = w0 * st.es.inv_area, b1 = w1 * st.es.inv_area, b2 = w2 * st.es.inv_area;
+  float z = b0 * st.sz[0] + b1 * st.sz[1] + b2 * st.sz[2];
+  if (z != z)
+    return;
+  uint32_t k = depth_key(z);
+  uint32_t *p = sa.keys + (size_t)y * (size_t)sa.size + (size_t)x;
+  if (k < __builtin_nontemporal_load(p))
+    atomicMin(p, k);
+}
+
+__global__ __launch_bounds__(WG) void k_shadow_small(ShadowArgsGPU sa) {
+  uint32_t t = blockIdx.x * WG + threadIdx.x;
+  if (t >= sa.n_tris)
+    return;
+  ShadowTri st;
+  if (!shadow_setup(sa, t, st))
+    return;
+  if (bbox_area(st.es) > MOPH_SMALL_AREA) {
+    uint32_t q = atomicAdd(&sa.counters[CNT_SHADOW_BIG], 1u);
+    if (q < sa.big_cap)
+      sa.big_queue[q] = t;
+    else
+      atomicOr(&sa.counters[CNT_OVERFLOW], 4u);
+    return;
+  }
+  float w0r = st.es.w_row[0], w1r = st.es.w_row[1], w2r = st.es.w_row[2];
+  for (int y = st.es.min_y; y <= st.es.max_y; y++) {
+    float w0 = w0r, w1 = w1r, w2 = w2r;
+    for (int x = st.es.min_x; x <= st.es.max_x; x++) {
+      if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)
+        shadow_frag(sa, st, x, y, w0, w1, w2);
+      w0 += st.es.e_dx[0];
+      w1 += st.es.e_dx[1];
+      w2 += st.es.e_dx[2];
+    }
+    w0r += st.es.e_dy[0];
+    w1r += st.es.e_dy[1];
+    w2r += st.es.e_dy[2];
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_shadow_big(ShadowArgsGPU sa) {
+  __shared__ float rows[3][WG];
+  __shared__ float carry[3];
+  uint32_t nbig = sa.counters[CNT_SHADOW_BIG];
+  if (nbig > sa.big_cap)
+    nbig = sa.big_cap;
+  for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+    ShadowTri st;
+    shadow_setup(sa, sa.big_queue[b], st);
+    int nrows = st.es.max_y - st.es.min_y + 1;
+    if (threadIdx.x == 0) {
+      carry[0] = st.es.w_row[0];
+      carry[1] = st.es.w_row[1];
+      carry[2] = st.es.w_row[2];
+    }
+    for (int c0 = 0; c0 < nrows; c0 += WG) {
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float a0 = carry[0], a1 = carry[1], a2 = carry[2];
+        int lim = nrows - c0 < WG ? nrows - c0 : WG;
+        for (int k = 0; k < lim; k++) {
+          rows[0][k] = a0;
+          rows[1][k] = a1;
+          rows[2][k] = a2;
+          a0 += st.es.e_dy[0];
+          a1 += st.es.e_dy[1];
+          a2 += st.es.e_dy[2];
+        }
+        carry[0] = a0;
+        carry[1] = a1;
+        carry[2] = a2;
+      }
+      __syncthreads();
+      int row = c0 + (int)threadIdx.x;
+      if (row < nrows) {
+        int y = st.es.min_y + row;
+        float w0 = rows[0][threadIdx.x], w1 = rows[1][threadIdx.x], w2 = rows[2][threadIdx.x];
+        for (int x = st.es.min_x; x <= st.es.max_x; x++) {
+          if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)
+            shadow_frag(sa, st, x, y, w0, w1, w2);
+          w0 += st.es.e_dx[0];
+          w1 += st.es.e_dx[1];
+          w2 += st.es.e_dx[2];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s) {
+  if (!sa.n_tris)
+    return;
+  hipLaunchKernelGGL(k_shadow_small, dim3((sa.n_tris + WG - 1) / WG), dim3(WG), 0, s, sa);
+  hipLaunchKernelGGL(k_shadow_big, dim3(2048), dim3(WG), 0, s, sa);
+}
+
+__global__ __launch_bounds__(WG) void k_shadow_finalize(int n, const uint32_t *keys,
+                                                        float *depth) {
+  int i = blockIdx.x * WG + threadIdx.x;
+  if (i < n)
+    depth[i] = key_depth(keys[i]);
+}
+
+void launch_shadow_finalize(int n, const uint32_t *keys, float *depth, hipStream_t s) {
+  hipLaunchKernelGGL(k_shadow_finalize, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, keys,
+                     depth);
+}
+
+__global__ void k_copy_u8(size_t n, const uint8_t *src, uint8_t *dst) {
+  size_t i = ((size_t)blockIdx.x * WG + threadIdx.x) * 16;
+  if (i + 16 <= n) {
+    *reinterpret_cast<uint4 *>(dst + i) = *reinterpret_cast<const uint4 *>(src + i);
+  } else {
+    for (; i < n; i++)
+      dst[i] = src[i];
+  }
+}
+
+void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s) {
+  size_t blocks = (n + WG * 16 - 1) / (WG * 16);
+  hipLaunchKernelGGL(k_copy_u8, dim3((unsigned)blocks), dim3(WG), 0, s, n, src, dst);
+}

@@ -1,0 +1,179 @@
+/*
+ * mop_hip_internal.h -- device-side data layout shared by the gfx950 kernels
+ * (kernels.hip) and the host runtime (runtime.hip).
+ *
+ * Frame pipeline (one stream, deferred to frame_end):
+ *   k_vertex      per draw vertex: clip position (MVP), world position and
+ *                 model-space-transformed normal, vertex colour (texture
+ *                 modulated) -- cpu_prepare_triangle, cpu_backend.c:224-324
+ *   k_classify    per triangle: trivial reject, clip-space backface cull,
+ *                 trivial accept -> number of screen triangles, or queue for
+ *                 the clipper -- rasterizer.c:2467-2504
+ *   k_clip        Sutherland-Hodgman over the 6 planes, fan split, per-fan
+ *                 rejects -- rasterizer.c:510-626, 2510-2553
+ *   scan          exclusive prefix sum of per-triangle counts: screen
+ *                 triangles get indices in submission order, used as the
+ *                 depth tie-break sequence number
+ *   k_emit        write screen-triangle records
+ *   k_raster_*    half-space walk with the reference's incremental edge
+ *                 recurrences; 64-bit atomic min of (depth key, sequence)
+ *                 into the visibility buffer
+ *   k_shade       per pixel: re-derive the winner's edge values, interpolate,
+ *                 multi-light Lambert + GGX + PCF + IBL -- rasterizer.c:1926-2140
+ *   k_resolve     ACES + exposure -- rasterizer.c:2776-2803
+ *   k_post        FXAA then fog / tonemap / gamma / vignette -- postprocess.c
+ */
+#ifndef MOP_HIP_INTERNAL_H
+#define MOP_HIP_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mop_hip.h"
+
+/* Screen triangles whose clamped bbox has more pixels than this are walked
+ * one row per lane instead of one triangle per lane. */
+#define MOPH_SMALL_AREA 64
+/* Stored vertices per clipped triangle (a triangle cut by 6 planes has at
+ * most 9; more is flagged as overflow). */
+#define MOPH_MAX_POLY 12
+
+enum {
+  MOPH_DRAW_CULL = 1u << 0,
+  MOPH_DRAW_FLEX = 1u << 1,
+};
+
+/* One draw (or one instance of an instanced draw) as the kernels see it. */
+struct DrawGPU {
+  float mvp[16];
+  float model[16];
+  const uint8_t *vbuf;
+  const uint32_t *ibuf;
+  const uint8_t *tex;
+  int32_t tex_w, tex_h;
+  uint32_t vertex_count;
+  uint32_t tri_count; /* index_count / 3 triangles considered */
+  uint32_t vtx_base;  /* first slot in the transformed-vertex cache */
+  uint32_t tri_base;  /* first slot in the flattened triangle space */
+  uint32_t stride;
+  int32_t pos_off, pos_fmt, nrm_off, nrm_fmt, col_off, col_fmt, uv_off, uv_fmt;
+  uint32_t object_id;
+  uint32_t flags;
+  float depth_bias;
+  float ambient;
+  float metallic;
+  float roughness;
+  float cam_eye[3];
+  uint32_t light_off;
+  uint32_t light_count;
+  uint32_t pad_;
+};
+
+/* Screen-space triangle: exactly the MopSwScreenVertex positions the CPU
+ * path hands to mop_sw_rasterize_triangle_smooth_ml. */
+struct SubTri {
+  float sx[3], sy[3], sz[3], iw[3];
+  uint32_t draw;
+  uint32_t a[3]; /* attribute slots: < n_verts -> vertex cache, else clip store */
+};
+
+/* Counters in device memory. */
+enum {
+  CNT_CLIP = 0,     /* triangles queued for clipping */
+  CNT_BIG = 1,      /* big screen triangles queued */
+  CNT_SUBTRI = 2,   /* total screen triangles (written by the scan) */
+  CNT_OVERFLOW = 3, /* bit0 clip queue, bit1 subtri store, bit2 shadow big queue,
+                       bit3 clipped polygon longer than MOPH_MAX_POLY */
+  CNT_SHADOW_BIG = 4,
+  CNT_N = 8
+};
+
+/* Everything the frame kernels read, passed by value. */
+struct FrameArgs {
+  const DrawGPU *draws;
+  uint32_t ndraws;
+  uint32_t n_verts;
+  uint32_t n_tris;
+  uint32_t clip_cap;
+  uint32_t subtri_cap;
+  int32_t width, height;
+  int32_t pad0_;
+  float4 *vpos;   /* n_verts clip positions */
+  float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */
+  float4 *cpos;   /* clip_cap * MOPH_MAX_POLY */
+  float4 *cattr;  /* clip_cap * MOPH_MAX_POLY * 3 */
+  uint32_t *cnt_poly; /* clip_cap: polygon vertex count per queued triangle */
+  uint32_t *tri_counts; /* n_tris (+1): screen-triangle count, scanned in place */
+  uint32_t *clip_queue;
+  uint32_t *clip_slot; /* n_tris: clip-queue slot of a triangle or ~0 */
+  uint32_t *counters;
+  SubTri *subtris;
+  uint32_t *big_queue;
+  unsigned long long *vis;
+  float4 *wbuf;
+  float *hdr;
+  float *depth;
+  uint32_t *ids;
+  const MopHipLight *lights;
+  /* shadow lookup state (mop_sw_shadow_set) */
+  const float *shadow_map;
+  int32_t shadow_w, shadow_h;
+  int32_t shadow_on;
+  float shadow_vp[16];
+  /* IBL state (mop_sw_ibl_set) */
+  const float *irr;
+  const float *pf;
+  const float *lut;
+  int32_t irr_w, irr_h, pf_w, pf_h, pf_levels, lut_size;
+  float ibl_rot, ibl_int;
+  int32_t ibl_on;
+};
+
+/* One shadow caster mesh (mop_sw_shadow_render_mesh arguments). */
+struct ShadowDrawGPU {
+  float mvp[16]; /* light_vp * model, mop_mat4_multiply order */
+  const uint8_t *vbuf;
+  const uint32_t *ibuf;
+  uint32_t vertex_count;
+  uint32_t tri_count;
+  uint32_t tri_base;
+  uint32_t pad_;
+};
+
+struct ShadowArgsGPU {
+  const ShadowDrawGPU *draws;
+  uint32_t ndraws;
+  uint32_t n_tris;
+  int32_t size;
+  uint32_t big_cap;
+  uint32_t *keys; /* size*size ordered depth keys */
+  uint32_t *big_queue;
+  uint32_t *counters;
+};
+
+/* Kernel launchers (kernels.hip). */
+void launch_vertex(const FrameArgs &fa, hipStream_t s);
+void launch_classify(const FrameArgs &fa, hipStream_t s);
+void launch_clip(const FrameArgs &fa, hipStream_t s);
+void launch_scan_counts(const FrameArgs &fa, uint32_t *block_sums, hipStream_t s);
+void launch_emit(const FrameArgs &fa, hipStream_t s);
+void launch_raster(const FrameArgs &fa, hipStream_t s);
+void launch_shade(const FrameArgs &fa, hipStream_t s);
+void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr,
+                  float *depth, uint32_t *ids, unsigned long long *vis,
+                  hipStream_t s);
+void launch_vis_from_depth(int npx, const float *depth,
+                           unsigned long long *vis, hipStream_t s);
+void launch_resolve(int npx, const float *hdr, const uint32_t *ids,
+                    uint8_t *color, float exposure, hipStream_t s);
+void launch_post(int w, int h, const uint8_t *src, uint8_t *dst,
+                 const float *depth, uint32_t effects,
+                 const MopHipFogParams *fog, hipStream_t s);
+void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
+void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);
+void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s);
+void launch_shadow_finalize(int n, const uint32_t *keys, float *depth,
+                            hipStream_t s);
+void launch_fill_u32(size_t n, uint32_t *p, uint32_t v, hipStream_t s);
+
+#endif

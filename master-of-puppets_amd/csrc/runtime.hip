@@ -1,0 +1,1461 @@
+/*
+ * runtime.hip -- host side of the HIP backend: resources, deferred frame
+ * recording and execution, readbacks, and the hooks for the viewport core's
+ * CPU-gated calls.  All entry points are extern "C" (include/mop_hip.h).
+ *
+ * Frame model.  frame_begin records the clear, draw/draw_instanced record
+ * draws (host copies of the call data; vertex/index data are already
+ * resident), frame_end executes the recorded frame on the device stream and
+ * resolves HDR -> RGBA8 with the exposure given by set_exposure.  Readbacks
+ * synchronise the stream once and copy into pinned host memory owned by the
+ * framebuffer (valid until the next frame, resize or destroy, as the RHI
+ * requires: viewport.h:157-166).
+ *
+ * Draws are grouped into segments of consecutive draws of one class.  The
+ * class every parity scene uses -- opaque, opacity >= 1, depth test and write
+ * on, smooth shading with at least one light, not wireframe -- is order-free:
+ * the CPU's strict `z < depth` with first-submitted-wins gives the same image
+ * as a lexicographic min over (depth, submission index), which the kernels
+ * compute with 64-bit atomics.  Draws of other classes are not rendered by
+ * this backend yet; they are reported through mop_hip_last_error and counted
+ * in the frame statistics.
+ */
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "mop_hip_internal.h"
+
+/* ------------------------------------------------------------------------ */
+/* errors                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static thread_local char t_err[512];
+
+static void set_err(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_err, sizeof(t_err), fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "[mop-hip] %s\n", t_err);
+}
+
+#define HIPCHK(expr)                                                           \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+              __LINE__);                                                       \
+    }                                                                          \
+  } while (0)
+
+extern "C" const char *mop_hip_last_error(void) { return t_err; }
+
+/* ------------------------------------------------------------------------ */
+/* host matrix product in mop_mat4_multiply order (math.c:182-194)           */
+/* ------------------------------------------------------------------------ */
+
+static MopHipMat4 mat4_mul(const MopHipMat4 &a, const MopHipMat4 &b) {
+  MopHipMat4 r;
+  for (int c = 0; c < 4; c++)
+    for (int row = 0; row < 4; row++) {
+      float sum = 0.0f;
+      for (int k = 0; k < 4; k++)
+        sum += a.d[k * 4 + row] * b.d[c * 4 + k];
+      r.d[c * 4 + row] = sum;
+    }
+  return r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* resource types                                                            */
+/* ------------------------------------------------------------------------ */
+
+struct MopHipBuffer {
+  void *host = nullptr;
+  void *dev = nullptr;
+  size_t size = 0;
+};
+
+struct MopHipTexture {
+  int w = 0, h = 0;
+  bool hdr = false;
+  void *host = nullptr;
+  void *dev = nullptr; /* RGBA8 only (draw textures) */
+};
+
+struct PendingDraw {
+  DrawGPU g;              /* filled except vtx/tri bases and light_off */
+  std::vector<MopHipLight> lights;
+  int cls;                /* 0 = order-free opaque, else unsupported */
+  int state;              /* index into the frame's shading-state list */
+};
+
+/* Shading state a draw sees (mop_sw_shadow_set / mop_sw_ibl_set state at
+ * the time of the draw): device pointers of the bound tables. */
+struct ShadeState {
+  bool shadow_on = false;
+  const float *shadow_map = nullptr;
+  int shadow_w = 0, shadow_h = 0;
+  float shadow_vp[16] = {0};
+  bool ibl_on = false;
+  const float *irr = nullptr, *pf = nullptr, *lut = nullptr;
+  int irr_w = 0, irr_h = 0, pf_w = 0, pf_h = 0, pf_levels = 0, lut_size = 0;
+  float ibl_rot = 0.0f, ibl_int = 1.0f;
+};
+
+struct MopHipFramebuffer {
+  int w = 0, h = 0;
+  uint8_t *color[2] = {nullptr, nullptr};
+  int cur = 0;
+  float *hdr = nullptr, *depth = nullptr;
+  uint32_t *ids = nullptr;
+  unsigned long long *vis = nullptr;
+  float4 *wbuf = nullptr;
+  uint8_t *h_color = nullptr;
+  float *h_depth = nullptr, *h_hdr = nullptr;
+  uint32_t *h_ids = nullptr;
+  bool v_color = false, v_depth = false, v_ids = false, v_hdr = false;
+  /* recorded frame */
+  bool has_clear = false;
+  float clear[4] = {0, 0, 0, 0};
+  std::vector<PendingDraw> draws;
+  std::vector<ShadeState> states;
+  bool executed = false;
+  bool resolved = false;
+};
+
+template <class T> struct DevArr {
+  T *p = nullptr;
+  size_t cap = 0;
+  bool ensure(size_t n) {
+    if (n <= cap)
+      return true;
+    if (p)
+      (void)hipFree(p);
+    size_t nc = n + n / 4 + 256;
+    if (hipMalloc((void **)&p, nc * sizeof(T)) != hipSuccess) {
+      set_err("hipMalloc of %zu bytes failed", nc * sizeof(T));
+      p = nullptr;
+      cap = 0;
+      return false;
+    }
+    cap = nc;
+    return true;
+  }
+  void release() {
+    if (p)
+      (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct MopHipShadowMap {
+  int size = 0;
+  DevArr<uint32_t> keys;
+  DevArr<float> depth;
+};
+
+struct MopHipIbl {
+  DevArr<float> irr, pf, lut;
+  int irr_w = 0, irr_h = 0, pf_w = 0, pf_h = 0, pf_levels = 0, lut_size = 0;
+  bool has_irr = false, has_pf = false, has_lut = false;
+};
+
+struct ShadowDrawHost {
+  ShadowDrawGPU g;
+};
+
+/* Optional per-stage device timers: hipEvents recorded on the device
+ * stream around each stage while enabled; summed on read. */
+enum { TM_VERTEX, TM_SETUP, TM_RASTER, TM_SHADE, TM_RESOLVE, TM_POST, TM_SHADOW, TM_N };
+static const char *k_timer_names[TM_N] = {"vertex", "setup", "raster", "shade",
+                                          "resolve", "post", "shadow"};
+
+struct StageTimers {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double total_ms[TM_N] = {0};
+  uint64_t count[TM_N] = {0};
+  size_t next = 0;
+};
+
+struct MopHipDevice {
+  hipStream_t stream = nullptr;
+  StageTimers tm;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::mutex mu;
+  std::unordered_map<const void *, MopHipBuffer *> host2buf;
+  /* frame scratch */
+  DevArr<DrawGPU> draws;
+  DevArr<MopHipLight> lights;
+  DevArr<float4> vpos, vattr, cpos, cattr;
+  DevArr<uint32_t> cnt_poly, tri_counts, clip_queue, clip_slot, counters, block_sums,
+      big_queue;
+  DevArr<SubTri> subtris;
+  uint32_t clip_cap_hint = 0;
+  /* pinned staging */
+  void *h_stage = nullptr;
+  size_t h_stage_cap = 0;
+  uint32_t *h_counters = nullptr;
+  /* shadow: the map the hook API renders into, and the recorded casters */
+  MopHipShadowMap *def_shadow = nullptr;
+  MopHipShadowMap *rec_shadow = nullptr; /* map the recorded casters target */
+  DevArr<uint32_t> shadow_big;
+  DevArr<ShadowDrawGPU> shadow_draws_dev;
+  std::vector<ShadowDrawHost> shadow_draws;
+  std::vector<MopHipBuffer *> shadow_temp;
+  ShadeState cur_state;
+  bool state_dirty = true;
+  /* IBL tables uploaded through mop_hip_ibl_set */
+  MopHipIbl *def_ibl = nullptr;
+  uint64_t ibl_hash = 0;
+  float exposure = 1.0f;
+  MopHipFrameStats stats = {};
+  bool timing_pending = false;
+};
+
+/* ------------------------------------------------------------------------ */
+/* device / buffers / textures                                               */
+/* ------------------------------------------------------------------------ */
+
+extern "C" MopHipDevice *mop_hip_device_create(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+    set_err("no HIP device visible");
+    return nullptr;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+    set_err("hipGetDeviceProperties failed");
+    return nullptr;
+  }
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    set_err("device 0 is %s; this library is built for gfx950 only", prop.gcnArchName);
+    return nullptr;
+  }
+  MopHipDevice *d = new MopHipDevice();
+  HIPCHK(hipSetDevice(0));
+  if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_err("hipStreamCreate failed");
+    delete d;
+    return nullptr;
+  }
+  HIPCHK(hipEventCreate(&d->ev0));
+  HIPCHK(hipEventCreate(&d->ev1));
+  HIPCHK(hipHostMalloc((void **)&d->h_counters, CNT_N * sizeof(uint32_t), 0));
+  return d;
+}
+
+extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
+  if (!d)
+    return;
+  (void)hipStreamSynchronize(d->stream);
+  d->draws.release();
+  d->lights.release();
+  d->vpos.release();
+  d->vattr.release();
+  d->cpos.release();
+  d->cattr.release();
+  d->cnt_poly.release();
+  d->tri_counts.release();
+  d->clip_queue.release();
+  d->clip_slot.release();
+  d->counters.release();
+  d->block_sums.release();
+  d->big_queue.release();
+  d->subtris.release();
+  d->shadow_big.release();
+  d->shadow_draws_dev.release();
+  mop_hip_shadow_map_destroy(d, d->def_shadow);
+  mop_hip_ibl_destroy(d, d->def_ibl);
+  for (MopHipBuffer *b : d->shadow_temp) {
+    (void)hipFree(b->dev);
+    delete b;
+  }
+  if (d->h_stage)
+    (void)hipHostFree(d->h_stage);
+  if (d->h_counters)
+    (void)hipHostFree(d->h_counters);
+  (void)hipEventDestroy(d->ev0);
+  (void)hipEventDestroy(d->ev1);
+  (void)hipStreamDestroy(d->stream);
+  delete d;
+}
+
+static void *stage(MopHipDevice *d, size_t bytes) {
+  if (bytes > d->h_stage_cap) {
+    /* the previous staging buffer may still feed an in-flight copy */
+    (void)hipStreamSynchronize(d->stream);
+    if (d->h_stage)
+      (void)hipHostFree(d->h_stage);
+    size_t nc = bytes + bytes / 2 + 4096;
+    if (hipHostMalloc(&d->h_stage, nc, 0) != hipSuccess) {
+      d->h_stage = nullptr;
+      d->h_stage_cap = 0;
+      set_err("hipHostMalloc(%zu) failed", nc);
+      return nullptr;
+    }
+    d->h_stage_cap = nc;
+  } else {
+    (void)hipStreamSynchronize(d->stream);
+  }
+  return d->h_stage;
+}
+
+static void tm_collect(MopHipDevice *d);
+
+static hipEvent_t tm_event(MopHipDevice *d) {
+  StageTimers &t = d->tm;
+  if (t.next == t.pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess)
+      return nullptr;
+    t.pool.push_back(e);
+  }
+  return t.pool[t.next++];
+}
+
+struct TmScope {
+  MopHipDevice *d;
+  int id;
+  hipEvent_t a = nullptr;
+  TmScope(MopHipDevice *dev, int which) : d(dev), id(which) {
+    if (d->tm.on && (a = tm_event(d)))
+      (void)hipEventRecord(a, d->stream);
+  }
+  ~TmScope() {
+    if (!a)
+      return;
+    hipEvent_t b = tm_event(d);
+    if (!b)
+      return;
+    (void)hipEventRecord(b, d->stream);
+    d->tm.pending.push_back({id, {a, b}});
+    if (d->tm.pending.size() >= 1024)
+      tm_collect(d);
+  }
+};
+
+static void tm_collect(MopHipDevice *d) {
+  StageTimers &t = d->tm;
+  if (t.pending.empty())
+    return;
+  (void)hipStreamSynchronize(d->stream);
+  for (auto &p : t.pending) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      t.total_ms[p.first] += ms;
+      t.count[p.first]++;
+    }
+  }
+  t.pending.clear();
+  t.next = 0;
+}
+
+/* Enable/disable and reset the stage timers. */
+extern "C" void mop_hip_timers_enable(MopHipDevice *d, int on) {
+  if (!d)
+    return;
+  tm_collect(d);
+  d->tm.on = on != 0;
+  for (int i = 0; i < TM_N; i++) {
+    d->tm.total_ms[i] = 0.0;
+    d->tm.count[i] = 0;
+  }
+}
+
+/* Total milliseconds and number of timed instances of stage `name`
+ * (vertex, setup, raster, shade, resolve, post, shadow); -1 if unknown. */
+extern "C" int mop_hip_timers_read(MopHipDevice *d, const char *name, double *total_ms,
+                                   uint64_t *count) {
+  if (!d || !name)
+    return -1;
+  tm_collect(d);
+  for (int i = 0; i < TM_N; i++)
+    if (strcmp(name, k_timer_names[i]) == 0) {
+      if (total_ms)
+        *total_ms = d->tm.total_ms[i];
+      if (count)
+        *count = d->tm.count[i];
+      return 0;
+    }
+  return -1;
+}
+
+extern "C" MopHipBuffer *mop_hip_buffer_create(MopHipDevice *d, const MopHipBufferDesc *desc) {
+  if (!d || !desc)
+    return nullptr;
+  MopHipBuffer *b = new MopHipBuffer();
+  b->size = desc->size;
+  b->host = malloc(desc->size ? desc->size : 1);
+  if (!b->host) {
+    delete b;
+    return nullptr;
+  }
+  if (desc->data && desc->size)
+    memcpy(b->host, desc->data, desc->size);
+  if (hipMalloc(&b->dev, desc->size ? desc->size : 4) != hipSuccess) {
+    set_err("hipMalloc(%zu) for buffer failed", desc->size);
+    free(b->host);
+    delete b;
+    return nullptr;
+  }
+  if (desc->size)
+    HIPCHK(hipMemcpy(b->dev, b->host, desc->size, hipMemcpyHostToDevice));
+  std::lock_guard<std::mutex> lk(d->mu);
+  d->host2buf[b->host] = b;
+  return b;
+}
+
+extern "C" void mop_hip_buffer_destroy(MopHipDevice *d, MopHipBuffer *b) {
+  if (!b)
+    return;
+  if (d) {
+    (void)hipStreamSynchronize(d->stream);
+    std::lock_guard<std::mutex> lk(d->mu);
+    d->host2buf.erase(b->host);
+  }
+  (void)hipFree(b->dev);
+  free(b->host);
+  delete b;
+}
+
+extern "C" void mop_hip_buffer_update(MopHipDevice *d, MopHipBuffer *b, const void *data,
+                                      size_t offset, size_t size) {
+  if (!b || !data || offset + size > b->size)
+    return;
+  if (d)
+    (void)hipStreamSynchronize(d->stream);
+  memcpy((uint8_t *)b->host + offset, data, size);
+  HIPCHK(hipMemcpy((uint8_t *)b->dev + offset, (uint8_t *)b->host + offset, size,
+                   hipMemcpyHostToDevice));
+}
+
+extern "C" const void *mop_hip_buffer_read(MopHipBuffer *b) { return b ? b->host : nullptr; }
+
+extern "C" MopHipTexture *mop_hip_texture_create(MopHipDevice *d, int w, int h,
+                                                 const uint8_t *rgba) {
+  if (!d || w <= 0 || h <= 0 || !rgba)
+    return nullptr;
+  MopHipTexture *t = new MopHipTexture();
+  t->w = w;
+  t->h = h;
+  size_t n = (size_t)w * h * 4;
+  t->host = malloc(n);
+  memcpy(t->host, rgba, n);
+  if (hipMalloc(&t->dev, n) != hipSuccess) {
+    free(t->host);
+    delete t;
+    return nullptr;
+  }
+  HIPCHK(hipMemcpy(t->dev, rgba, n, hipMemcpyHostToDevice));
+  return t;
+}
+
+extern "C" MopHipTexture *mop_hip_texture_create_hdr(MopHipDevice *d, int w, int h,
+                                                     const float *rgba) {
+  if (!d || w <= 0 || h <= 0 || !rgba)
+    return nullptr;
+  MopHipTexture *t = new MopHipTexture();
+  t->w = w;
+  t->h = h;
+  t->hdr = true;
+  size_t n = (size_t)w * h * 4 * sizeof(float);
+  t->host = malloc(n);
+  memcpy(t->host, rgba, n);
+  return t;
+}
+
+extern "C" void mop_hip_texture_destroy(MopHipDevice *d, MopHipTexture *t) {
+  if (!t)
+    return;
+  if (d)
+    (void)hipStreamSynchronize(d->stream);
+  if (t->dev)
+    (void)hipFree(t->dev);
+  free(t->host);
+  delete t;
+}
+
+extern "C" bool mop_hip_texture_read_rgba8(MopHipDevice *d, MopHipTexture *t, uint8_t *out,
+                                           size_t size) {
+  (void)d;
+  if (!t || t->hdr || !out || size < (size_t)t->w * t->h * 4)
+    return false;
+  memcpy(out, t->host, (size_t)t->w * t->h * 4);
+  return true;
+}
+
+/* ------------------------------------------------------------------------ */
+/* framebuffers                                                              */
+/* ------------------------------------------------------------------------ */
+
+static void fb_free(MopHipFramebuffer *fb) {
+  for (int i = 0; i < 2; i++)
+    if (fb->color[i])
+      (void)hipFree(fb->color[i]);
+  if (fb->hdr) (void)hipFree(fb->hdr);
+  if (fb->depth) (void)hipFree(fb->depth);
+  if (fb->ids) (void)hipFree(fb->ids);
+  if (fb->vis) (void)hipFree(fb->vis);
+  if (fb->wbuf) (void)hipFree(fb->wbuf);
+  if (fb->h_color) (void)hipHostFree(fb->h_color);
+  if (fb->h_depth) (void)hipHostFree(fb->h_depth);
+  if (fb->h_ids) (void)hipHostFree(fb->h_ids);
+  if (fb->h_hdr) (void)hipHostFree(fb->h_hdr);
+  fb->color[0] = fb->color[1] = nullptr;
+  fb->hdr = fb->depth = nullptr;
+  fb->ids = nullptr;
+  fb->vis = nullptr;
+  fb->wbuf = nullptr;
+  fb->h_color = nullptr;
+  fb->h_depth = fb->h_hdr = nullptr;
+  fb->h_ids = nullptr;
+}
+
+static bool fb_alloc(MopHipDevice *d, MopHipFramebuffer *fb, int w, int h) {
+  if (w <= 0 || h <= 0 || w > 16384 || h > 16384) {
+    set_err("framebuffer dimensions out of range: %dx%d", w, h);
+    return false;
+  }
+  size_t n = (size_t)w * h;
+  fb->w = w;
+  fb->h = h;
+  bool ok = hipMalloc(&fb->color[0], n * 4) == hipSuccess &&
+            hipMalloc(&fb->color[1], n * 4) == hipSuccess &&
+            hipMalloc(&fb->hdr, n * 16) == hipSuccess &&
+            hipMalloc(&fb->depth, n * 4) == hipSuccess &&
+            hipMalloc(&fb->ids, n * 4) == hipSuccess &&
+            hipMalloc(&fb->vis, n * 8) == hipSuccess &&
+            hipMalloc(&fb->wbuf, n * 16) == hipSuccess &&
+            hipHostMalloc((void **)&fb->h_color, n * 4, 0) == hipSuccess &&
+            hipHostMalloc((void **)&fb->h_depth, n * 4, 0) == hipSuccess &&
+            hipHostMalloc((void **)&fb->h_ids, n * 4, 0) == hipSuccess &&
+            hipHostMalloc((void **)&fb->h_hdr, n * 16, 0) == hipSuccess;
+  if (!ok) {
+    set_err("framebuffer allocation %dx%d failed", w, h);
+    fb_free(fb);
+    return false;
+  }
+  /* calloc'd like mop_sw_framebuffer_alloc: colour/HDR/ids zero */
+  float zero[4] = {0, 0, 0, 0};
+  launch_clear((int)n, zero, fb->color[0], fb->hdr, fb->depth, fb->ids, fb->vis, d->stream);
+  HIPCHK(hipStreamSynchronize(d->stream));
+  fb->cur = 0;
+  fb->v_color = fb->v_depth = fb->v_ids = fb->v_hdr = false;
+  return true;
+}
+
+extern "C" MopHipFramebuffer *mop_hip_framebuffer_create(MopHipDevice *d,
+                                                         const MopHipFramebufferDesc *desc) {
+  if (!d || !desc)
+    return nullptr;
+  MopHipFramebuffer *fb = new MopHipFramebuffer();
+  if (!fb_alloc(d, fb, desc->width, desc->height)) {
+    delete fb;
+    return nullptr;
+  }
+  return fb;
+}
+
+extern "C" void mop_hip_framebuffer_destroy(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (!fb)
+    return;
+  if (d)
+    (void)hipStreamSynchronize(d->stream);
+  fb_free(fb);
+  delete fb;
+}
+
+extern "C" void mop_hip_framebuffer_resize(MopHipDevice *d, MopHipFramebuffer *fb, int w,
+                                           int h) {
+  if (!d || !fb)
+    return;
+  (void)hipStreamSynchronize(d->stream);
+  fb_free(fb);
+  fb->draws.clear();
+  fb->states.clear();
+  fb->executed = false;
+  fb_alloc(d, fb, w, h);
+}
+
+/* ------------------------------------------------------------------------ */
+/* frame recording                                                           */
+/* ------------------------------------------------------------------------ */
+
+static void invalidate_host(MopHipFramebuffer *fb) {
+  fb->v_color = fb->v_depth = fb->v_ids = fb->v_hdr = false;
+}
+
+extern "C" void mop_hip_frame_begin(MopHipDevice *d, MopHipFramebuffer *fb, MopHipColor c) {
+  if (!d || !fb)
+    return;
+  fb->has_clear = true;
+  fb->clear[0] = c.r;
+  fb->clear[1] = c.g;
+  fb->clear[2] = c.b;
+  fb->clear[3] = c.a;
+  fb->draws.clear();
+  fb->states.clear();
+  fb->executed = false;
+  fb->resolved = false;
+  d->state_dirty = true;
+  invalidate_host(fb);
+}
+
+static int current_state(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (d->state_dirty || fb->states.empty()) {
+    fb->states.push_back(d->cur_state);
+    d->state_dirty = false;
+  }
+  return (int)fb->states.size() - 1;
+}
+
+static const MopHipVertexAttrib *find_attr(const MopHipVertexFormat *f, int sem) {
+  for (uint32_t i = 0; i < f->attrib_count && i < 12; i++)
+    if (f->attribs[i].semantic == sem)
+      return &f->attribs[i];
+  return nullptr;
+}
+
+static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
+                        const MopHipMat4 &model, const MopHipMat4 &mvp) {
+  if (!c->vertex_buffer || !c->index_buffer)
+    return;
+  PendingDraw pd;
+  memset(&pd.g, 0, sizeof(pd.g));
+  DrawGPU &g = pd.g;
+  memcpy(g.mvp, mvp.d, sizeof(g.mvp));
+  memcpy(g.model, model.d, sizeof(g.model));
+  g.vbuf = (const uint8_t *)c->vertex_buffer->dev;
+  g.ibuf = (const uint32_t *)c->index_buffer->dev;
+  if (c->texture && !c->texture->hdr && c->texture->w >= 1 && c->texture->h >= 1) {
+    g.tex = (const uint8_t *)c->texture->dev;
+    g.tex_w = c->texture->w;
+    g.tex_h = c->texture->h;
+  }
+  g.vertex_count = c->vertex_count;
+  /* triangles i with i+2 < index_count, i += 3 (cpu_backend.c:589) */
+  g.tri_count = c->index_count / 3;
+  size_t ib_elems = c->index_buffer->size / 4;
+  if ((size_t)g.tri_count * 3 > ib_elems)
+    g.tri_count = (uint32_t)(ib_elems / 3);
+  g.stride = sizeof(MopHipVertex);
+  g.nrm_off = g.col_off = g.uv_off = -1;
+  if (c->vertex_format) {
+    const MopHipVertexFormat *f = c->vertex_format;
+    const MopHipVertexAttrib *pa = find_attr(f, MOP_HIP_ATTRIB_POSITION);
+    if (!pa) {
+      set_err("draw with a vertex format lacking POSITION skipped");
+      return;
+    }
+    g.flags |= MOPH_DRAW_FLEX;
+    g.stride = f->stride;
+    g.pos_off = (int32_t)pa->offset;
+    g.pos_fmt = pa->format;
+    const MopHipVertexAttrib *na = find_attr(f, MOP_HIP_ATTRIB_NORMAL);
+    const MopHipVertexAttrib *ca = find_attr(f, MOP_HIP_ATTRIB_COLOR);
+    const MopHipVertexAttrib *ua = find_attr(f, MOP_HIP_ATTRIB_TEXCOORD0);
+    if (na) { g.nrm_off = (int32_t)na->offset; g.nrm_fmt = na->format; }
+    if (ca) { g.col_off = (int32_t)ca->offset; g.col_fmt = ca->format; }
+    if (ua) { g.uv_off = (int32_t)ua->offset; g.uv_fmt = ua->format; }
+  }
+  /* vertices the device may touch: the draw's vertex_count, clamped to the buffer */
+  size_t vmax = c->vertex_buffer->size / (g.stride ? g.stride : 1);
+  if (g.vertex_count > vmax)
+    g.vertex_count = (uint32_t)vmax;
+  g.object_id = c->object_id;
+  if (c->backface_cull)
+    g.flags |= MOPH_DRAW_CULL;
+  g.depth_bias = c->depth_bias;
+  g.ambient = c->ambient;
+  g.metallic = c->metallic;
+  g.roughness = c->roughness;
+  g.cam_eye[0] = c->cam_eye.x;
+  g.cam_eye[1] = c->cam_eye.y;
+  g.cam_eye[2] = c->cam_eye.z;
+  if (c->lights && c->light_count)
+    pd.lights.assign(c->lights, c->lights + c->light_count);
+  g.light_count = c->light_count;
+  bool order_free = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
+                    c->light_count > 0 && c->depth_test && c->depth_write &&
+                    c->blend_mode == MOP_HIP_BLEND_OPAQUE && c->opacity >= 1.0f;
+  /* An alpha/additive draw at opacity <= 0 changes nothing: every fragment
+   * blends as dst*1 + src*0 and writes depth/id only above alpha 0.5
+   * (rasterizer.c:2109-2140).  The core draws its hidden grid that way. */
+  bool invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
+                    c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
+  pd.cls = order_free ? 0 : (invisible ? 2 : 1);
+  pd.state = current_state(d, fb);
+  fb->draws.push_back(std::move(pd));
+  fb->executed = false;
+}
+
+extern "C" void mop_hip_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c) {
+  if (!d || !fb || !c)
+    return;
+  record_draw(d, fb, c, c->model, c->mvp);
+}
+
+extern "C" void mop_hip_draw_instanced(MopHipDevice *d, MopHipFramebuffer *fb,
+                                       const MopHipDrawCall *c, const MopHipMat4 *xf,
+                                       uint32_t count) {
+  if (!d || !fb || !c || !xf || count == 0)
+    return;
+  /* per instance: model = transform, mvp = P * (V * M) (cpu_backend.c:638-657) */
+  for (uint32_t i = 0; i < count; i++) {
+    MopHipMat4 vm = mat4_mul(c->view, xf[i]);
+    MopHipMat4 mvp = mat4_mul(c->projection, vm);
+    record_draw(d, fb, c, xf[i], mvp);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* execution                                                                 */
+/* ------------------------------------------------------------------------ */
+
+static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_tris,
+                                 uint32_t ndraws, uint32_t nlights, uint32_t &clip_cap,
+                                 uint32_t &subtri_cap) {
+  clip_cap = n_tris / 16;
+  if (clip_cap < 4096)
+    clip_cap = 4096;
+  if (d->clip_cap_hint > clip_cap)
+    clip_cap = d->clip_cap_hint;
+  uint64_t sc = (uint64_t)n_tris + 10ull * clip_cap + 1;
+  if (sc > 0xffffff00ull)
+    sc = 0xffffff00ull;
+  subtri_cap = (uint32_t)sc;
+  return d->draws.ensure(ndraws ? ndraws : 1) && d->lights.ensure(nlights ? nlights : 1) &&
+         d->vpos.ensure(n_verts ? n_verts : 1) && d->vattr.ensure((size_t)(n_verts ? n_verts : 1) * 3) &&
+         d->cpos.ensure((size_t)clip_cap * MOPH_MAX_POLY) &&
+         d->cattr.ensure((size_t)clip_cap * MOPH_MAX_POLY * 3) && d->cnt_poly.ensure(clip_cap) &&
+         d->tri_counts.ensure((size_t)n_tris + 1) && d->clip_queue.ensure(clip_cap) &&
+         d->clip_slot.ensure(n_tris ? n_tris : 1) && d->counters.ensure(CNT_N) &&
+         d->block_sums.ensure(n_tris / (256 * 16) + 2) && d->subtris.ensure(subtri_cap) &&
+         d->big_queue.ensure(subtri_cap);
+}
+
+static void fill_state(const ShadeState &st, FrameArgs &fa) {
+  fa.shadow_on = (st.shadow_on && st.shadow_map) ? 1 : 0;
+  fa.shadow_map = st.shadow_map;
+  fa.shadow_w = st.shadow_w;
+  fa.shadow_h = st.shadow_h;
+  memcpy(fa.shadow_vp, st.shadow_vp, sizeof(fa.shadow_vp));
+  fa.ibl_on = st.ibl_on ? 1 : 0;
+  fa.irr = st.irr;
+  fa.pf = st.pf;
+  fa.lut = st.lut;
+  fa.irr_w = st.irr_w;
+  fa.irr_h = st.irr_h;
+  fa.pf_w = st.pf_w;
+  fa.pf_h = st.pf_h;
+  fa.pf_levels = st.pf_levels;
+  fa.lut_size = st.lut_size;
+  fa.ibl_rot = st.ibl_rot;
+  fa.ibl_int = st.ibl_int;
+}
+
+/* Runs draws [b, e) (one order-free class, one shading state). */
+static bool run_segment(MopHipDevice *d, MopHipFramebuffer *fb, size_t b, size_t e,
+                        bool first_segment) {
+  uint32_t n_verts = 0, n_tris = 0, nlights = 0;
+  std::vector<DrawGPU> gd;
+  std::vector<MopHipLight> lights;
+  gd.reserve(e - b);
+  for (size_t i = b; i < e; i++) {
+    PendingDraw &pd = fb->draws[i];
+    DrawGPU g = pd.g;
+    g.vtx_base = n_verts;
+    g.tri_base = n_tris;
+    /* share the light array with the previous draw when identical */
+    if (!lights.empty() && !gd.empty() && gd.back().light_count == pd.lights.size() &&
+        memcmp(&lights[gd.back().light_off], pd.lights.data(),
+               pd.lights.size() * sizeof(MopHipLight)) == 0) {
+      g.light_off = gd.back().light_off;
+    } else {
+      g.light_off = (uint32_t)lights.size();
+      lights.insert(lights.end(), pd.lights.begin(), pd.lights.end());
+    }
+    if ((uint64_t)n_verts + g.vertex_count > 0xffffffffull ||
+        (uint64_t)n_tris + g.tri_count > 0x7fffffffull) {
+      set_err("frame exceeds 2^31 triangles; remaining draws skipped");
+      break;
+    }
+    n_verts += g.vertex_count;
+    n_tris += g.tri_count;
+    gd.push_back(g);
+  }
+  nlights = (uint32_t)lights.size();
+  if (gd.empty())
+    return true;
+  uint32_t clip_cap, subtri_cap;
+  if (!ensure_frame_scratch(d, n_verts, n_tris, (uint32_t)gd.size(), nlights, clip_cap,
+                            subtri_cap))
+    return false;
+  size_t bytes_d = gd.size() * sizeof(DrawGPU), bytes_l = lights.size() * sizeof(MopHipLight);
+  uint8_t *stg = (uint8_t *)stage(d, bytes_d + bytes_l + 64);
+  if (!stg)
+    return false;
+  memcpy(stg, gd.data(), bytes_d);
+  if (bytes_l)
+    memcpy(stg + bytes_d, lights.data(), bytes_l);
+  HIPCHK(hipMemcpyAsync(d->draws.p, stg, bytes_d, hipMemcpyHostToDevice, d->stream));
+  if (bytes_l)
+    HIPCHK(hipMemcpyAsync(d->lights.p, stg + bytes_d, bytes_l, hipMemcpyHostToDevice, d->stream));
+
+  FrameArgs fa;
+  memset(&fa, 0, sizeof(fa));
+  fa.draws = d->draws.p;
+  fa.ndraws = (uint32_t)gd.size();
+  fa.n_verts = n_verts;
+  fa.n_tris = n_tris;
+  fa.clip_cap = clip_cap;
+  fa.subtri_cap = subtri_cap;
+  fa.width = fb->w;
+  fa.height = fb->h;
+  fa.vpos = d->vpos.p;
+  fa.vattr = d->vattr.p;
+  fa.cpos = d->cpos.p;
+  fa.cattr = d->cattr.p;
+  fa.cnt_poly = d->cnt_poly.p;
+  fa.tri_counts = d->tri_counts.p;
+  fa.clip_queue = d->clip_queue.p;
+  fa.clip_slot = d->clip_slot.p;
+  fa.counters = d->counters.p;
+  fa.subtris = d->subtris.p;
+  fa.big_queue = d->big_queue.p;
+  fa.vis = fb->vis;
+  fa.wbuf = fb->wbuf;
+  fa.hdr = fb->hdr;
+  fa.depth = fb->depth;
+  fa.ids = fb->ids;
+  fa.lights = d->lights.p;
+  fill_state(fb->states[fb->draws[b].state], fa);
+
+  int npx = fb->w * fb->h;
+  if (!first_segment)
+    launch_vis_from_depth(npx, fb->depth, fb->vis, d->stream);
+  launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
+  {
+    TmScope t(d, TM_VERTEX);
+    launch_vertex(fa, d->stream);
+  }
+  {
+    TmScope t(d, TM_SETUP);
+    launch_classify(fa, d->stream);
+    launch_clip(fa, d->stream);
+    launch_scan_counts(fa, d->block_sums.p, d->stream);
+    launch_emit(fa, d->stream);
+  }
+  {
+    TmScope t(d, TM_RASTER);
+    launch_raster(fa, d->stream);
+  }
+  {
+    TmScope t(d, TM_SHADE);
+    launch_shade(fa, d->stream);
+  }
+  HIPCHK(hipMemcpyAsync(d->h_counters, d->counters.p, CNT_N * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, d->stream));
+  d->stats.triangles_in += n_tris;
+  return true;
+}
+
+static void execute_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (fb->executed)
+    return;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    d->stats = MopHipFrameStats();
+    HIPCHK(hipEventRecord(d->ev0, d->stream));
+    int npx = fb->w * fb->h;
+    if (fb->has_clear)
+      launch_clear(npx, fb->clear, fb->color[0], fb->hdr, fb->depth, fb->ids, fb->vis,
+                   d->stream);
+    bool first = true;
+    bool overflow = false;
+    uint32_t agg[4] = {0, 0, 0, 0};
+    size_t i = 0, n = fb->draws.size();
+    size_t skipped = 0;
+    while (i < n) {
+      size_t j = i + 1;
+      while (j < n && fb->draws[j].cls == fb->draws[i].cls &&
+             fb->draws[j].state == fb->draws[i].state)
+        j++;
+      if (fb->draws[i].cls == 0) {
+        run_segment(d, fb, i, j, first);
+        first = false;
+        /* counters of this segment */
+        HIPCHK(hipStreamSynchronize(d->stream));
+        uint32_t *c = d->h_counters;
+        if (c[CNT_OVERFLOW] & 3u) {
+          overflow = true;
+          uint32_t need = c[CNT_CLIP];
+          if (need > d->clip_cap_hint)
+            d->clip_cap_hint = need + need / 4;
+        }
+        agg[0] += c[CNT_SUBTRI];
+        agg[1] += c[CNT_CLIP];
+        agg[2] += c[CNT_BIG];
+      } else if (fb->draws[i].cls == 1) {
+        skipped += j - i;
+      }
+      i = j;
+      if (overflow)
+        break;
+    }
+    if (overflow)
+      continue;
+    if (skipped)
+      set_err("%zu draw(s) of a class this backend does not render yet were skipped", skipped);
+    {
+      TmScope t(d, TM_RESOLVE);
+      launch_resolve(npx, fb->hdr, fb->ids, fb->color[0], d->exposure, d->stream);
+    }
+    fb->cur = 0;
+    HIPCHK(hipEventRecord(d->ev1, d->stream));
+    d->stats.subtris = agg[0];
+    d->stats.clipped = agg[1];
+    d->stats.big_subtris = agg[2];
+    d->timing_pending = true;
+    fb->executed = true;
+    fb->resolved = true;
+    invalidate_host(fb);
+    return;
+  }
+  set_err("frame scratch overflow persisted; frame not rendered");
+}
+
+extern "C" void mop_hip_frame_end(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (!d || !fb)
+    return;
+  execute_frame(d, fb);
+}
+
+extern "C" void mop_hip_frame_submit(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (!d || !fb)
+    return;
+  execute_frame(d, fb);
+}
+
+extern "C" void mop_hip_frame_wait_readback(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (!d || !fb)
+    return;
+  execute_frame(d, fb);
+  HIPCHK(hipStreamSynchronize(d->stream));
+}
+
+extern "C" void mop_hip_set_exposure(MopHipDevice *d, float e) {
+  if (d)
+    d->exposure = e;
+}
+
+extern "C" float mop_hip_frame_gpu_time_ms(MopHipDevice *d) {
+  if (!d)
+    return 0.0f;
+  if (d->timing_pending) {
+    HIPCHK(hipEventSynchronize(d->ev1));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, d->ev0, d->ev1) == hipSuccess)
+      d->stats.gpu_ms = ms;
+    d->timing_pending = false;
+  }
+  return d->stats.gpu_ms;
+}
+
+extern "C" void mop_hip_get_frame_stats(MopHipDevice *d, MopHipFrameStats *out) {
+  if (!d || !out)
+    return;
+  mop_hip_frame_gpu_time_ms(d);
+  *out = d->stats;
+}
+
+/* ------------------------------------------------------------------------ */
+/* readbacks                                                                 */
+/* ------------------------------------------------------------------------ */
+
+static void ensure_ready(MopHipDevice *d, MopHipFramebuffer *fb) {
+  execute_frame(d, fb);
+}
+
+extern "C" const uint8_t *mop_hip_framebuffer_read_color(MopHipDevice *d, MopHipFramebuffer *fb,
+                                                         int *ow, int *oh) {
+  if (!d || !fb)
+    return nullptr;
+  ensure_ready(d, fb);
+  if (ow) *ow = fb->w;
+  if (oh) *oh = fb->h;
+  if (!fb->v_color) {
+    HIPCHK(hipMemcpyAsync(fb->h_color, fb->color[fb->cur], (size_t)fb->w * fb->h * 4,
+                          hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    fb->v_color = true;
+  }
+  return fb->h_color;
+}
+
+extern "C" const uint32_t *mop_hip_framebuffer_read_object_id(MopHipDevice *d,
+                                                              MopHipFramebuffer *fb, int *ow,
+                                                              int *oh) {
+  if (!d || !fb)
+    return nullptr;
+  ensure_ready(d, fb);
+  if (ow) *ow = fb->w;
+  if (oh) *oh = fb->h;
+  if (!fb->v_ids) {
+    HIPCHK(hipMemcpyAsync(fb->h_ids, fb->ids, (size_t)fb->w * fb->h * 4, hipMemcpyDeviceToHost,
+                          d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    fb->v_ids = true;
+  }
+  return fb->h_ids;
+}
+
+extern "C" const float *mop_hip_framebuffer_read_depth(MopHipDevice *d, MopHipFramebuffer *fb,
+                                                       int *ow, int *oh) {
+  if (!d || !fb)
+    return nullptr;
+  ensure_ready(d, fb);
+  if (ow) *ow = fb->w;
+  if (oh) *oh = fb->h;
+  if (!fb->v_depth) {
+    HIPCHK(hipMemcpyAsync(fb->h_depth, fb->depth, (size_t)fb->w * fb->h * 4,
+                          hipMemcpyDeviceToHost, d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    fb->v_depth = true;
+  }
+  return fb->h_depth;
+}
+
+extern "C" const float *mop_hip_framebuffer_read_hdr(MopHipDevice *d, MopHipFramebuffer *fb,
+                                                     int *ow, int *oh) {
+  if (!d || !fb)
+    return nullptr;
+  ensure_ready(d, fb);
+  if (ow) *ow = fb->w;
+  if (oh) *oh = fb->h;
+  if (!fb->v_hdr) {
+    HIPCHK(hipMemcpyAsync(fb->h_hdr, fb->hdr, (size_t)fb->w * fb->h * 16, hipMemcpyDeviceToHost,
+                          d->stream));
+    HIPCHK(hipStreamSynchronize(d->stream));
+    fb->v_hdr = true;
+  }
+  return fb->h_hdr;
+}
+
+extern "C" uint32_t mop_hip_pick_read_id(MopHipDevice *d, MopHipFramebuffer *fb, int x, int y) {
+  if (!d || !fb || x < 0 || x >= fb->w || y < 0 || y >= fb->h)
+    return 0;
+  const uint32_t *ids = mop_hip_framebuffer_read_object_id(d, fb, nullptr, nullptr);
+  return ids ? ids[(size_t)y * fb->w + x] : 0;
+}
+
+extern "C" float mop_hip_pick_read_depth(MopHipDevice *d, MopHipFramebuffer *fb, int x, int y) {
+  if (!d || !fb || x < 0 || x >= fb->w || y < 0 || y >= fb->h)
+    return 1.0f;
+  const float *dp = mop_hip_framebuffer_read_depth(d, fb, nullptr, nullptr);
+  return dp ? dp[(size_t)y * fb->w + x] : 1.0f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* hooks: shadow, IBL, resolve, post                                         */
+/* ------------------------------------------------------------------------ */
+
+static MopHipBuffer *lookup_or_upload(MopHipDevice *d, const void *host, size_t bytes) {
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    auto it = d->host2buf.find(host);
+    if (it != d->host2buf.end() && it->second->size >= bytes)
+      return it->second;
+  }
+  MopHipBuffer *b = new MopHipBuffer();
+  b->size = bytes;
+  if (hipMalloc(&b->dev, bytes ? bytes : 4) != hipSuccess) {
+    delete b;
+    return nullptr;
+  }
+  HIPCHK(hipMemcpy(b->dev, host, bytes, hipMemcpyHostToDevice));
+  d->shadow_temp.push_back(b);
+  return b;
+}
+
+extern "C" MopHipShadowMap *mop_hip_shadow_map_create(MopHipDevice *d, int size) {
+  if (!d || size <= 0 || size > 16384) {
+    set_err("shadow map size %d out of range", size);
+    return nullptr;
+  }
+  MopHipShadowMap *m = new MopHipShadowMap();
+  m->size = size;
+  size_t n = (size_t)size * size;
+  if (!m->keys.ensure(n) || !m->depth.ensure(n)) {
+    delete m;
+    return nullptr;
+  }
+  return m;
+}
+
+extern "C" void mop_hip_shadow_map_destroy(MopHipDevice *d, MopHipShadowMap *m) {
+  if (!m)
+    return;
+  if (d)
+    (void)hipStreamSynchronize(d->stream);
+  m->keys.release();
+  m->depth.release();
+  delete m;
+}
+
+/* Begin a caster list for `map` (cleared to depth 1.0 when rendered). */
+static void shadow_begin(MopHipDevice *d, MopHipShadowMap *map) {
+  d->shadow_draws.clear();
+  if (!d->shadow_temp.empty()) {
+    (void)hipStreamSynchronize(d->stream);
+    for (MopHipBuffer *b : d->shadow_temp) {
+      (void)hipFree(b->dev);
+      delete b;
+    }
+    d->shadow_temp.clear();
+  }
+  d->rec_shadow = map;
+}
+
+static void shadow_add(MopHipDevice *d, const void *vdev, uint32_t vertex_count,
+                       const void *idev, uint32_t index_count, const MopHipMat4 &model,
+                       const MopHipMat4 &light_vp) {
+  ShadowDrawHost sh;
+  memset(&sh, 0, sizeof(sh));
+  /* mvp = light_vp * model (rasterizer.c:361) */
+  MopHipMat4 mvp = mat4_mul(light_vp, model);
+  memcpy(sh.g.mvp, mvp.d, sizeof(sh.g.mvp));
+  sh.g.vbuf = (const uint8_t *)vdev;
+  sh.g.ibuf = (const uint32_t *)idev;
+  sh.g.vertex_count = vertex_count;
+  sh.g.tri_count = index_count / 3;
+  d->shadow_draws.push_back(sh);
+}
+
+/* Rasterize the recorded casters into d->rec_shadow (keys -> float depth). */
+static void run_shadow(MopHipDevice *d) {
+  MopHipShadowMap *map = d->rec_shadow;
+  if (!map)
+    return;
+  TmScope t(d, TM_SHADOW);
+  int size = map->size;
+  size_t npx = (size_t)size * size;
+  launch_shadow_clear((int)npx, map->keys.p, d->stream);
+  uint32_t n_tris = 0;
+  std::vector<ShadowDrawGPU> gd;
+  for (ShadowDrawHost &sh : d->shadow_draws) {
+    ShadowDrawGPU g = sh.g;
+    g.tri_base = n_tris;
+    n_tris += g.tri_count;
+    gd.push_back(g);
+  }
+  if (!gd.empty() && n_tris) {
+    for (int attempt = 0; attempt < 3; attempt++) {
+      uint32_t big_cap = d->shadow_big.cap ? (uint32_t)d->shadow_big.cap : 4096;
+      if (!d->shadow_big.ensure(big_cap) || !d->shadow_draws_dev.ensure(gd.size()) ||
+          !d->counters.ensure(CNT_N))
+        return;
+      big_cap = (uint32_t)d->shadow_big.cap;
+      size_t bytes = gd.size() * sizeof(ShadowDrawGPU);
+      void *stg = stage(d, bytes);
+      if (!stg)
+        return;
+      memcpy(stg, gd.data(), bytes);
+      HIPCHK(hipMemcpyAsync(d->shadow_draws_dev.p, stg, bytes, hipMemcpyHostToDevice, d->stream));
+      launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
+      ShadowArgsGPU sa;
+      sa.draws = d->shadow_draws_dev.p;
+      sa.ndraws = (uint32_t)gd.size();
+      sa.n_tris = n_tris;
+      sa.size = size;
+      sa.big_cap = big_cap;
+      sa.keys = map->keys.p;
+      sa.big_queue = d->shadow_big.p;
+      sa.counters = d->counters.p;
+      launch_shadow_raster(sa, d->stream);
+      HIPCHK(hipMemcpyAsync(d->h_counters, d->counters.p, CNT_N * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, d->stream));
+      HIPCHK(hipStreamSynchronize(d->stream));
+      if (d->h_counters[CNT_OVERFLOW] & 4u) {
+        /* more big triangles than queue slots: grow and redo from a clear map */
+        d->shadow_big.ensure((size_t)d->h_counters[CNT_SHADOW_BIG] + 1);
+        launch_shadow_clear((int)npx, map->keys.p, d->stream);
+        continue;
+      }
+      d->stats.shadow_tris = n_tris;
+      break;
+    }
+  }
+  launch_shadow_finalize((int)npx, map->keys.p, map->depth.p, d->stream);
+}
+
+static void shadow_enable(MopHipDevice *d, const MopHipShadowMap *map, const MopHipMat4 &lvp) {
+  d->cur_state.shadow_on = true;
+  d->cur_state.shadow_map = map->depth.p;
+  d->cur_state.shadow_w = map->size;
+  d->cur_state.shadow_h = map->size;
+  memcpy(d->cur_state.shadow_vp, lvp.d, sizeof(d->cur_state.shadow_vp));
+  d->state_dirty = true;
+}
+
+extern "C" void mop_hip_shadow_render_mesh(MopHipDevice *d, const void *vertices,
+                                           uint32_t vertex_count, const uint32_t *indices,
+                                           uint32_t index_count, const MopHipMat4 *model,
+                                           const MopHipMat4 *light_vp, int size, int first) {
+  if (!d || !vertices || !indices || !model || !light_vp || size <= 0)
+    return;
+  if (first) {
+    if (!d->def_shadow || d->def_shadow->size != size) {
+      mop_hip_shadow_map_destroy(d, d->def_shadow);
+      d->def_shadow = mop_hip_shadow_map_create(d, size);
+      if (!d->def_shadow)
+        return;
+    }
+    shadow_begin(d, d->def_shadow);
+  }
+  MopHipBuffer *vb = lookup_or_upload(d, vertices, (size_t)vertex_count * sizeof(MopHipVertex));
+  MopHipBuffer *ib = lookup_or_upload(d, indices, (size_t)index_count * sizeof(uint32_t));
+  if (!vb || !ib)
+    return;
+  shadow_add(d, vb->dev, vertex_count, ib->dev, index_count, *model, *light_vp);
+}
+
+extern "C" void mop_hip_shadow_set(MopHipDevice *d, int w, int h, const MopHipMat4 *light_vp) {
+  if (!d || !light_vp || !d->rec_shadow)
+    return;
+  if (w != d->rec_shadow->size || h != d->rec_shadow->size) {
+    set_err("shadow map %dx%d does not match the rendered %d^2 map", w, h,
+            d->rec_shadow->size);
+    return;
+  }
+  run_shadow(d);
+  shadow_enable(d, d->rec_shadow, *light_vp);
+}
+
+extern "C" void mop_hip_shadow_clear(MopHipDevice *d) {
+  if (!d)
+    return;
+  d->cur_state.shadow_on = false;
+  d->cur_state.shadow_map = nullptr;
+  d->state_dirty = true;
+}
+
+/* Resident-scene variant: casters are device buffers already. */
+extern "C" void mop_hip_shadow_begin(MopHipDevice *d, MopHipShadowMap *map) {
+  if (d && map)
+    shadow_begin(d, map);
+}
+
+extern "C" void mop_hip_shadow_draw(MopHipDevice *d, MopHipBuffer *vb, uint32_t vertex_count,
+                                    MopHipBuffer *ib, uint32_t index_count,
+                                    const MopHipMat4 *model, const MopHipMat4 *light_vp) {
+  if (!d || !vb || !ib || !model || !light_vp || !d->rec_shadow)
+    return;
+  shadow_add(d, vb->dev, vertex_count, ib->dev, index_count, *model, *light_vp);
+}
+
+extern "C" void mop_hip_shadow_end(MopHipDevice *d, const MopHipMat4 *light_vp) {
+  if (!d || !light_vp || !d->rec_shadow)
+    return;
+  run_shadow(d);
+  shadow_enable(d, d->rec_shadow, *light_vp);
+}
+
+extern "C" void mop_hip_shadow_bind(MopHipDevice *d, const MopHipShadowMap *map,
+                                    const MopHipMat4 *light_vp) {
+  if (!d || !map || !light_vp)
+    return;
+  shadow_enable(d, map, *light_vp);
+}
+
+static uint64_t hash_bytes(const void *p, size_t n, uint64_t h) {
+  const uint8_t *b = (const uint8_t *)p;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t v;
+    memcpy(&v, b + i, 8);
+    h ^= v;
+    h *= 0x100000001b3ull;
+    h ^= h >> 29;
+  }
+  for (; i < n; i++) {
+    h ^= b[i];
+    h *= 0x100000001b3ull;
+  }
+  return h;
+}
+
+static size_t pf_floats(int pf_w, int pf_h, int pf_levels) {
+  size_t n = 0;
+  for (int l = 0; l < pf_levels; l++) {
+    int lw = pf_w >> l, lh = pf_h >> l;
+    if (lw < 1) lw = 1;
+    if (lh < 1) lh = 1;
+    n += (size_t)lw * lh * 4;
+  }
+  return n;
+}
+
+static void ibl_enable(MopHipDevice *d, const MopHipIbl *ibl, float rotation, float intensity) {
+  ShadeState &st = d->cur_state;
+  st.irr = ibl->has_irr ? ibl->irr.p : nullptr;
+  st.pf = ibl->has_pf ? ibl->pf.p : nullptr;
+  st.lut = ibl->has_lut ? ibl->lut.p : nullptr;
+  st.irr_w = ibl->irr_w;
+  st.irr_h = ibl->irr_h;
+  st.pf_w = ibl->pf_w;
+  st.pf_h = ibl->pf_h;
+  st.pf_levels = ibl->pf_levels;
+  st.lut_size = ibl->lut_size;
+  st.ibl_rot = rotation;
+  st.ibl_int = intensity;
+  st.ibl_on = st.irr || (st.pf && st.lut);
+  d->state_dirty = true;
+}
+
+extern "C" void mop_hip_ibl_set(MopHipDevice *d, const float *irradiance, int irr_w, int irr_h,
+                                const float *prefiltered, int pf_w, int pf_h, int pf_levels,
+                                const float *brdf_lut, int brdf_size, float rotation,
+                                float intensity) {
+  if (!d)
+    return;
+  size_t n_irr = irradiance ? (size_t)irr_w * irr_h * 4 : 0;
+  size_t n_pf = prefiltered ? pf_floats(pf_w, pf_h, pf_levels) : 0;
+  size_t n_lut = brdf_lut ? (size_t)brdf_size * brdf_size * 4 : 0;
+  uint64_t h = 1469598103934665603ull;
+  int dims[6] = {irr_w, irr_h, pf_w, pf_h, pf_levels, brdf_size};
+  h = hash_bytes(dims, sizeof(dims), h);
+  if (n_irr) h = hash_bytes(irradiance, n_irr * 4, h);
+  if (n_pf) h = hash_bytes(prefiltered, n_pf * 4, h);
+  if (n_lut) h = hash_bytes(brdf_lut, n_lut * 4, h);
+  if (!d->def_ibl)
+    d->def_ibl = new MopHipIbl();
+  MopHipIbl *ibl = d->def_ibl;
+  if (h != d->ibl_hash) {
+    (void)hipStreamSynchronize(d->stream);
+    ibl->has_irr = n_irr && ibl->irr.ensure(n_irr);
+    ibl->has_pf = n_pf && ibl->pf.ensure(n_pf);
+    ibl->has_lut = n_lut && ibl->lut.ensure(n_lut);
+    if (ibl->has_irr)
+      HIPCHK(hipMemcpy(ibl->irr.p, irradiance, n_irr * 4, hipMemcpyHostToDevice));
+    if (ibl->has_pf)
+      HIPCHK(hipMemcpy(ibl->pf.p, prefiltered, n_pf * 4, hipMemcpyHostToDevice));
+    if (ibl->has_lut)
+      HIPCHK(hipMemcpy(ibl->lut.p, brdf_lut, n_lut * 4, hipMemcpyHostToDevice));
+    d->ibl_hash = h;
+  }
+  ibl->irr_w = irr_w;
+  ibl->irr_h = irr_h;
+  ibl->pf_w = pf_w;
+  ibl->pf_h = pf_h;
+  ibl->pf_levels = pf_levels;
+  ibl->lut_size = brdf_size;
+  ibl_enable(d, ibl, rotation, intensity);
+}
+
+extern "C" void mop_hip_ibl_clear(MopHipDevice *d) {
+  if (!d)
+    return;
+  d->cur_state.ibl_on = false;
+  d->cur_state.irr = d->cur_state.pf = d->cur_state.lut = nullptr;
+  d->state_dirty = true;
+}
+
+void launch_ibl_precompute(const float *env, int ew, int eh, float *irr, float *pf, float *lut,
+                           hipStream_t s);
+size_t ibl_pf_texels(void);
+
+extern "C" MopHipIbl *mop_hip_ibl_create(MopHipDevice *d, const float *env_rgba, int w, int h) {
+  if (!d || !env_rgba || w < 2 || h < 2)
+    return nullptr;
+  MopHipIbl *ibl = new MopHipIbl();
+  ibl->irr_w = 64;
+  ibl->irr_h = 32;
+  ibl->pf_w = 128;
+  ibl->pf_h = 64;
+  ibl->pf_levels = 5;
+  ibl->lut_size = 256;
+  size_t n_env = (size_t)w * h * 4;
+  DevArr<float> env;
+  bool ok = env.ensure(n_env) && ibl->irr.ensure(64 * 32 * 4) &&
+            ibl->pf.ensure(ibl_pf_texels() * 4) && ibl->lut.ensure(256 * 256 * 4);
+  if (!ok) {
+    env.release();
+    mop_hip_ibl_destroy(d, ibl);
+    return nullptr;
+  }
+  HIPCHK(hipMemcpy(env.p, env_rgba, n_env * sizeof(float), hipMemcpyHostToDevice));
+  launch_ibl_precompute(env.p, w, h, ibl->irr.p, ibl->pf.p, ibl->lut.p, d->stream);
+  HIPCHK(hipStreamSynchronize(d->stream));
+  env.release();
+  ibl->has_irr = ibl->has_pf = ibl->has_lut = true;
+  return ibl;
+}
+
+extern "C" void mop_hip_ibl_destroy(MopHipDevice *d, MopHipIbl *ibl) {
+  if (!ibl)
+    return;
+  if (d)
+    (void)hipStreamSynchronize(d->stream);
+  ibl->irr.release();
+  ibl->pf.release();
+  ibl->lut.release();
+  delete ibl;
+}
+
+extern "C" void mop_hip_ibl_bind(MopHipDevice *d, const MopHipIbl *ibl, float rotation,
+                                 float intensity) {
+  if (!d || !ibl)
+    return;
+  ibl_enable(d, ibl, rotation, intensity);
+}
+
+extern "C" int mop_hip_ibl_download(MopHipDevice *d, const MopHipIbl *ibl, float *irr,
+                                    float *pf, float *lut) {
+  if (!d || !ibl)
+    return -1;
+  (void)hipStreamSynchronize(d->stream);
+  if (irr)
+    HIPCHK(hipMemcpy(irr, ibl->irr.p, (size_t)ibl->irr_w * ibl->irr_h * 16, hipMemcpyDeviceToHost));
+  if (pf)
+    HIPCHK(hipMemcpy(pf, ibl->pf.p, pf_floats(ibl->pf_w, ibl->pf_h, ibl->pf_levels) * 4,
+                     hipMemcpyDeviceToHost));
+  if (lut)
+    HIPCHK(hipMemcpy(lut, ibl->lut.p, (size_t)ibl->lut_size * ibl->lut_size * 16,
+                     hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" void mop_hip_hdr_resolve(MopHipDevice *d, MopHipFramebuffer *fb, float exposure) {
+  if (!d || !fb)
+    return;
+  execute_frame(d, fb);
+  TmScope t(d, TM_RESOLVE);
+  launch_resolve(fb->w * fb->h, fb->hdr, fb->ids, fb->color[0], exposure, d->stream);
+  fb->cur = 0;
+  fb->v_color = false;
+}
+
+extern "C" void mop_hip_postprocess_apply(MopHipDevice *d, MopHipFramebuffer *fb,
+                                          uint32_t effects, const MopHipFogParams *fog) {
+  if (!d || !fb || effects == 0)
+    return;
+  execute_frame(d, fb);
+  TmScope t(d, TM_POST);
+  launch_post(fb->w, fb->h, fb->color[fb->cur], fb->color[1 - fb->cur], fb->depth, effects, fog,
+              d->stream);
+  fb->cur = 1 - fb->cur;
+  fb->v_color = false;
+}

@@ -1,0 +1,429 @@
+/*
+ * ref_harness.c -- TEST INFRASTRUCTURE, never part of the product.
+ *
+ * Drives the *reference* MOP viewport core (compiled from the reference's own
+ * sources into oracle/_ref/libmopref.a) through its public API, so the parity
+ * tests and the CPU baseline use exactly the reference behaviour:
+ *
+ *   mode 0  reference CPU backend, multithreaded as shipped (timing baseline;
+ *           cpu_backend.c:55-75 sizes its pool to nproc-1 workers)
+ *   mode 1  reference CPU backend forced single-threaded: the wrapped
+ *           mop_sw_threadpool_create returns NULL, so cpu_draw always takes
+ *           the submission-order loop (cpu_backend.c:547,588-620).  Goldens.
+ *   mode 2  the same unmodified viewport core driving the HIP backend.  The
+ *           harness hands the HIP RHI table out for MOP_BACKEND_CPU and routes
+ *           the core's CPU-gated software calls (shadow pass, IBL state, HDR
+ *           resolve, post-process: viewport.c:2433,2699,2729,2953-2959) to the
+ *           HIP backend's hooks.  This is the drop-in exercised end to end
+ *           with no edit of the reference sources.
+ *
+ * Interposition uses the linker's --wrap (see oracle/Makefile).
+ */
+#include "core/viewport_internal.h"
+#include "rasterizer/rasterizer.h"
+#include "rasterizer/rasterizer_mt.h"
+#include "rhi/rhi.h"
+
+#include <dlfcn.h>
+#include <mop/mop.h>
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+/* tinyexr (linked from the reference build) */
+extern int SaveEXR(const float *data, const int width, const int height,
+                   const int components, const int save_as_fp16,
+                   const char *filename, const char **err);
+
+typedef const MopRhiBackend *(*fn_backend)(void);
+
+/* ---- mode + backend entry points (resolved with dlsym) ------------------ */
+
+static int g_mode = 1;
+
+/* One routed backend: the RHI table and the hooks standing in for the
+ * core's CPU-gated software calls.  Slot 0 serves mode 2 (HIP backend,
+ * prefix "mop_hip_"), slot 1 serves mode 3 (CPU restatement in
+ * oracle/mop_oracle.c, prefix "mop_oracle_"). */
+typedef struct Routed {
+  const MopRhiBackend *rhi;
+  void (*shadow_mesh)(MopRhiDevice *, const void *, uint32_t, const uint32_t *, uint32_t,
+                      const MopMat4 *, const MopMat4 *, int, int);
+  void (*shadow_set)(MopRhiDevice *, int, int, const MopMat4 *);
+  void (*shadow_clear)(MopRhiDevice *);
+  void (*ibl_set)(MopRhiDevice *, const float *, int, int, const float *, int, int, int,
+                  const float *, int, float, float);
+  void (*ibl_clear)(MopRhiDevice *);
+  void (*hdr_resolve)(MopRhiDevice *, MopRhiFramebuffer *, float);
+  void (*postprocess)(MopRhiDevice *, MopRhiFramebuffer *, uint32_t, const MopFogParams *);
+  const float *(*read_hdr)(MopRhiDevice *, MopRhiFramebuffer *, int *, int *);
+} Routed;
+
+static Routed g_routed[2];
+
+static Routed *routed(void) { return (g_mode >= 2 && g_mode <= 3) ? &g_routed[g_mode - 2] : NULL; }
+
+/* viewport whose frame is being rendered (for hooks without a vp argument) */
+static MopViewport *g_cur_vp;
+static int g_shadow_started;
+
+int mh_set_mode(int mode) {
+  if ((mode == 2 || mode == 3) && !g_routed[mode - 2].rhi)
+    return -1;
+  g_mode = mode;
+  return 0;
+}
+
+static void *sym(void *h, const char *prefix, const char *name) {
+  char buf[128];
+  snprintf(buf, sizeof(buf), "%s%s", prefix, name);
+  return dlsym(h, buf);
+}
+
+/* Loads a backend library for mode 2 (prefix "mop_hip_") or mode 3
+ * (prefix "mop_oracle_"); its table comes from mop_rhi_backend_<table>(). */
+int mh_load_backend(int mode, const char *so_path, const char *prefix, const char *table) {
+  if (mode != 2 && mode != 3)
+    return -4;
+  void *h = dlopen(so_path, RTLD_NOW | RTLD_GLOBAL);
+  if (!h) {
+    fprintf(stderr, "mh_load_backend: %s\n", dlerror());
+    return -1;
+  }
+  Routed *r = &g_routed[mode - 2];
+  char tb[64];
+  snprintf(tb, sizeof(tb), "mop_rhi_backend_%s", table);
+  fn_backend fb = (fn_backend)dlsym(h, tb);
+  r->shadow_mesh = sym(h, prefix, "shadow_render_mesh");
+  r->shadow_set = sym(h, prefix, "shadow_set");
+  r->shadow_clear = sym(h, prefix, "shadow_clear");
+  r->ibl_set = sym(h, prefix, "ibl_set");
+  r->ibl_clear = sym(h, prefix, "ibl_clear");
+  r->hdr_resolve = sym(h, prefix, "hdr_resolve");
+  r->postprocess = sym(h, prefix, "postprocess_apply");
+  r->read_hdr = sym(h, prefix, "framebuffer_read_hdr");
+  if (!fb || !r->shadow_mesh || !r->shadow_set || !r->shadow_clear || !r->ibl_set ||
+      !r->ibl_clear || !r->hdr_resolve || !r->postprocess || !r->read_hdr) {
+    fprintf(stderr, "mh_load_backend: missing symbol in %s\n", so_path);
+    return -2;
+  }
+  r->rhi = fb();
+  return r->rhi ? 0 : -3;
+}
+
+int mh_load_hip(const char *so_path) { return mh_load_backend(2, so_path, "mop_hip_", "hip"); }
+
+/* ---- wrapped reference symbols ----------------------------------------- */
+
+MopSwThreadPool *__real_mop_sw_threadpool_create(int num_threads);
+MopSwThreadPool *__wrap_mop_sw_threadpool_create(int num_threads) {
+  if (g_mode == 1)
+    return NULL;
+  return __real_mop_sw_threadpool_create(num_threads);
+}
+
+const MopRhiBackend *__real_mop_rhi_get_backend(MopBackendType type);
+const MopRhiBackend *__wrap_mop_rhi_get_backend(MopBackendType type) {
+  Routed *r = routed();
+  if (r && (type == MOP_BACKEND_CPU || type == MOP_BACKEND_AUTO))
+    return r->rhi;
+  return __real_mop_rhi_get_backend(type);
+}
+
+void __real_mop_sw_shadow_render_mesh(const MopVertex *, uint32_t,
+                                      const uint32_t *, uint32_t, MopMat4,
+                                      MopMat4, MopSwFramebuffer *);
+void __wrap_mop_sw_shadow_render_mesh(const MopVertex *v, uint32_t vc,
+                                      const uint32_t *idx, uint32_t ic,
+                                      MopMat4 model, MopMat4 light_vp,
+                                      MopSwFramebuffer *sfb) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_sw_shadow_render_mesh(v, vc, idx, ic, model, light_vp, sfb);
+    return;
+  }
+  /* &vp->shadow_fb -> vp (pass_shadow passes the viewport's own member) */
+  MopViewport *vp = (MopViewport *)((char *)sfb - offsetof(MopViewport, shadow_fb));
+  r->shadow_mesh(vp->device, v, vc, idx, ic, &model, &light_vp, sfb->width,
+                g_shadow_started ? 0 : 1);
+  g_shadow_started = 1;
+}
+
+void __real_mop_sw_shadow_set(const float *, int, int, MopMat4);
+void __wrap_mop_sw_shadow_set(const float *depth, int w, int h, MopMat4 lvp) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_sw_shadow_set(depth, w, h, lvp);
+    return;
+  }
+  g_shadow_started = 0;
+  if (g_cur_vp)
+    r->shadow_set(g_cur_vp->device, w, h, &lvp);
+}
+
+void __real_mop_sw_shadow_clear(void);
+void __wrap_mop_sw_shadow_clear(void) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_sw_shadow_clear();
+    return;
+  }
+  g_shadow_started = 0;
+  if (g_cur_vp)
+    r->shadow_clear(g_cur_vp->device);
+}
+
+void __real_mop_sw_ibl_set(const float *, int, int, const float *, int, int,
+                           int, const float *, int, float, float);
+void __wrap_mop_sw_ibl_set(const float *irr, int iw, int ih, const float *pf,
+                           int pw, int ph, int pl, const float *lut, int ls,
+                           float rot, float inten) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_sw_ibl_set(irr, iw, ih, pf, pw, ph, pl, lut, ls, rot, inten);
+    return;
+  }
+  if (g_cur_vp)
+    r->ibl_set(g_cur_vp->device, irr, iw, ih, pf, pw, ph, pl, lut, ls, rot,
+              inten);
+}
+
+void __real_mop_sw_ibl_clear(void);
+void __wrap_mop_sw_ibl_clear(void) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_sw_ibl_clear();
+    return;
+  }
+  if (g_cur_vp)
+    r->ibl_clear(g_cur_vp->device);
+}
+
+void __real_mop_sw_hdr_resolve(MopSwFramebuffer *fb, float exposure);
+void __wrap_mop_sw_hdr_resolve(MopSwFramebuffer *fb, float exposure) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_sw_hdr_resolve(fb, exposure);
+    return;
+  }
+  /* the core passes vp->framebuffer cast to MopSwFramebuffer* */
+  r->hdr_resolve(g_cur_vp->device, (MopRhiFramebuffer *)fb, exposure);
+}
+
+void __real_mop_postprocess_apply(MopSwFramebuffer *, uint32_t,
+                                  const MopFogParams *);
+void __wrap_mop_postprocess_apply(MopSwFramebuffer *fb, uint32_t effects,
+                                  const MopFogParams *fog) {
+  Routed *r = routed();
+  if (!r) {
+    __real_mop_postprocess_apply(fb, effects, fog);
+    return;
+  }
+  r->postprocess(g_cur_vp->device, (MopRhiFramebuffer *)fb, effects, fog);
+}
+
+/* ---- scene construction through the public API ------------------------- */
+
+MopViewport *mh_viewport(int w, int h) {
+  MopViewportDesc d = {.width = w,
+                       .height = h,
+                       .backend = MOP_BACKEND_CPU,
+                       .ssaa_factor = 1};
+  MopViewport *vp = mop_viewport_create(&d);
+  if (!vp)
+    return NULL;
+  mop_viewport_set_chrome(vp, false);
+  mop_viewport_set_overlay_enabled(vp, MOP_OVERLAY_OUTLINE, false);
+  mop_viewport_clear_lights(vp);
+  return vp;
+}
+
+void mh_destroy(MopViewport *vp) {
+  if (g_cur_vp == vp)
+    g_cur_vp = NULL;
+  mop_viewport_destroy(vp);
+}
+
+/* model: 16 floats column-major (NULL = identity); material sets metallic /
+ * roughness through mop_mesh_set_material like the configs need. */
+MopMesh *mh_add_mesh(MopViewport *vp, const MopVertex *verts, uint32_t nv,
+                     const uint32_t *idx, uint32_t ni, uint32_t object_id,
+                     const float *model, int has_material, float metallic,
+                     float roughness, int blend_mode, float opacity) {
+  MopMeshDesc md = {.vertices = verts,
+                    .vertex_count = nv,
+                    .indices = idx,
+                    .index_count = ni,
+                    .object_id = object_id};
+  MopMesh *m = mop_viewport_add_mesh(vp, &md);
+  if (!m)
+    return NULL;
+  if (model) {
+    MopMat4 t;
+    memcpy(t.d, model, sizeof(t.d));
+    mop_mesh_set_transform(m, &t);
+  }
+  if (has_material) {
+    MopMaterial mat = mop_material_default();
+    mat.metallic = metallic;
+    mat.roughness = roughness;
+    mop_mesh_set_material(m, &mat);
+  }
+  if (blend_mode != 0)
+    mop_mesh_set_blend_mode(m, (MopBlendMode)blend_mode);
+  if (opacity < 1.0f) {
+    mop_mesh_set_opacity(m, opacity);
+    if (blend_mode == 0)
+      mop_mesh_set_blend_mode(m, MOP_BLEND_OPAQUE);
+  }
+  return m;
+}
+
+int mh_add_light(MopViewport *vp, int type, const float *pos, const float *dir,
+                 const float *color, float intensity, float range,
+                 float inner_cos, float outer_cos, int cast_shadows) {
+  MopLight l;
+  memset(&l, 0, sizeof(l));
+  l.type = (MopLightType)type;
+  l.position = (MopVec3){pos[0], pos[1], pos[2]};
+  l.direction = (MopVec3){dir[0], dir[1], dir[2]};
+  l.color = (MopColor){color[0], color[1], color[2], color[3]};
+  l.intensity = intensity;
+  l.range = range;
+  l.spot_inner_cos = inner_cos;
+  l.spot_outer_cos = outer_cos;
+  l.active = true;
+  l.cast_shadows = cast_shadows != 0;
+  return mop_viewport_add_light(vp, &l) ? 0 : -1;
+}
+
+void mh_set_camera(MopViewport *vp, const float *eye, const float *target,
+                   const float *up, float fov_deg, float znear, float zfar) {
+  mop_viewport_set_camera(vp, (MopVec3){eye[0], eye[1], eye[2]},
+                          (MopVec3){target[0], target[1], target[2]},
+                          (MopVec3){up[0], up[1], up[2]}, fov_deg, znear, zfar);
+}
+
+void mh_set_clear_color(MopViewport *vp, const float *c) {
+  mop_viewport_set_clear_color(vp, (MopColor){c[0], c[1], c[2], c[3]});
+}
+
+void mh_set_post_effects(MopViewport *vp, uint32_t effects) {
+  mop_viewport_set_post_effects(vp, effects);
+}
+
+void mh_set_ambient(MopViewport *vp, float a) { mop_viewport_set_ambient(vp, a); }
+
+void mh_set_shading(MopViewport *vp, int mode) {
+  mop_viewport_set_shading(vp, (MopShadingMode)mode);
+}
+
+void mh_set_exposure(MopViewport *vp, float e) { mop_viewport_set_exposure(vp, e); }
+
+/* Writes the RGBA32F equirect to `path` as float EXR (lossless) and loads it
+ * as the viewport environment (MOP_ENV_HDRI, environment.c:730-846). */
+int mh_set_environment(MopViewport *vp, const float *rgba, int w, int h,
+                       const char *path, float rotation, float intensity) {
+  const char *err = NULL;
+  if (SaveEXR(rgba, w, h, 4, 0, path, &err) != 0) {
+    fprintf(stderr, "SaveEXR failed: %s\n", err ? err : "?");
+    return -1;
+  }
+  MopEnvironmentDesc d = {.type = MOP_ENV_HDRI,
+                          .hdr_path = path,
+                          .rotation = rotation,
+                          .intensity = intensity};
+  return mop_viewport_set_environment(vp, &d) ? 0 : -2;
+}
+
+float mh_get_exposure(MopViewport *vp) { return vp->exposure; }
+
+/* IBL tables as the core holds them (inputs to mop_sw_ibl_set). */
+int mh_get_ibl(MopViewport *vp, const float **irr, int *iw, int *ih,
+               const float **pf, int *pw, int *ph, int *pl, const float **lut) {
+  if (!vp->env_irradiance_data)
+    return -1;
+  *irr = vp->env_irradiance_data;
+  *iw = vp->env_irradiance_w;
+  *ih = vp->env_irradiance_h;
+  *pf = vp->env_prefiltered_data;
+  *pw = vp->env_prefiltered_w;
+  *ph = vp->env_prefiltered_h;
+  *pl = vp->env_prefiltered_levels;
+  *lut = vp->env_brdf_lut_data;
+  return 0;
+}
+
+static double now_ms(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+/* One frame: mop_viewport_render + mop_viewport_read_color (the frame path the
+ * CPU baseline times).  Returns wall milliseconds. */
+double mh_render(MopViewport *vp) {
+  g_cur_vp = vp;
+  double t0 = now_ms();
+  mop_viewport_render(vp);
+  int w, h;
+  (void)mop_viewport_read_color(vp, &w, &h);
+  return now_ms() - t0;
+}
+
+uint32_t mh_triangle_count(MopViewport *vp) {
+  return mop_viewport_get_stats(vp).triangle_count;
+}
+
+/* Full-buffer readbacks through the RHI table the core holds. */
+int mh_read(MopViewport *vp, uint8_t *color, float *depth, uint32_t *ids,
+            float *hdr) {
+  int w = 0, h = 0;
+  size_t n;
+  if (color) {
+    const uint8_t *c = mop_viewport_read_color(vp, &w, &h);
+    if (!c)
+      return -1;
+    memcpy(color, c, (size_t)w * h * 4);
+  }
+  if (depth) {
+    const float *d = vp->rhi->framebuffer_read_depth(vp->device, vp->framebuffer,
+                                                      &w, &h);
+    if (!d)
+      return -2;
+    memcpy(depth, d, (size_t)w * h * sizeof(float));
+  }
+  if (ids) {
+    const uint32_t *o = vp->rhi->framebuffer_read_object_id(
+        vp->device, vp->framebuffer, &w, &h);
+    if (!o)
+      return -3;
+    memcpy(ids, o, (size_t)w * h * sizeof(uint32_t));
+  }
+  if (hdr) {
+    const float *src;
+    Routed *r = routed();
+    if (r) {
+      src = r->read_hdr(vp->device, vp->framebuffer, &w, &h);
+    } else {
+      MopSwFramebuffer *sw = (MopSwFramebuffer *)vp->framebuffer;
+      src = sw->color_hdr;
+      w = sw->width;
+      h = sw->height;
+    }
+    if (!src)
+      return -4;
+    n = (size_t)w * h * 4;
+    memcpy(hdr, src, n * sizeof(float));
+  }
+  return 0;
+}
+
+int mh_pick(MopViewport *vp, int x, int y, uint32_t *id, float *depth) {
+  MopPickResult r = mop_viewport_pick(vp, x, y);
+  *id = r.object_id;
+  *depth = r.depth;
+  return r.hit ? 1 : 0;
+}

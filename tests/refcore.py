@@ -1,0 +1,231 @@
+"""Test helper: render a Scene through the *reference* viewport core.
+
+Loads oracle/_ref/ref_harness.so (the reference library compiled from its own
+sources plus oracle/ref_harness.c).  Modes: 0 = reference CPU backend
+multithreaded, 1 = reference CPU backend single-threaded (goldens), 2 = the
+unmodified reference core driving this repository's HIP backend, 3 = the same
+core driving the CPU restatement in oracle/mop_oracle.c.
+"""
+import ctypes as C
+import os
+import tempfile
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HARNESS = os.path.join(REPO, "oracle", "_ref", "ref_harness.so")
+
+_h = None
+_hip_loaded = False
+
+
+def available():
+    return os.path.exists(HARNESS)
+
+
+def harness():
+    global _h
+    if _h is None:
+        if not available():
+            raise RuntimeError(f"{HARNESS} missing: run `make -C oracle`")
+        h = C.CDLL(HARNESS, mode=C.RTLD_GLOBAL)
+        VP, F = C.c_void_p, C.POINTER(C.c_float)
+        h.mh_set_mode.argtypes = [C.c_int]
+        h.mh_load_hip.argtypes = [C.c_char_p]
+        h.mh_load_backend.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]
+        h.mh_viewport.restype = VP
+        h.mh_viewport.argtypes = [C.c_int, C.c_int]
+        h.mh_destroy.argtypes = [VP]
+        h.mh_add_mesh.restype = VP
+        h.mh_add_mesh.argtypes = [VP, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                  C.c_uint32, C.c_void_p, C.c_int, C.c_float, C.c_float,
+                                  C.c_int, C.c_float]
+        h.mh_add_light.argtypes = [VP, C.c_int, F, F, F, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_int]
+        h.mh_set_camera.argtypes = [VP, F, F, F, C.c_float, C.c_float, C.c_float]
+        h.mh_set_clear_color.argtypes = [VP, F]
+        h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
+        h.mh_set_ambient.argtypes = [VP, C.c_float]
+        h.mh_set_exposure.argtypes = [VP, C.c_float]
+        h.mh_set_environment.argtypes = [VP, C.c_void_p, C.c_int, C.c_int, C.c_char_p,
+                                         C.c_float, C.c_float]
+        h.mh_get_exposure.restype = C.c_float
+        h.mh_get_exposure.argtypes = [VP]
+        h.mh_render.restype = C.c_double
+        h.mh_render.argtypes = [VP]
+        h.mh_triangle_count.restype = C.c_uint32
+        h.mh_triangle_count.argtypes = [VP]
+        h.mh_read.argtypes = [VP, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        h.mh_pick.argtypes = [VP, C.c_int, C.c_int, C.POINTER(C.c_uint32),
+                              C.POINTER(C.c_float)]
+        FP = C.POINTER(F)
+        IP = C.POINTER(C.c_int)
+        h.mh_get_ibl.argtypes = [VP, FP, IP, IP, FP, IP, IP, IP, FP]
+        _h = h
+    return _h
+
+
+ORACLE_LIB = os.path.join(REPO, "oracle", "_build", "liboracle.so")
+_oracle_loaded = False
+
+
+def use_oracle():
+    """Make mode 3 available (loads the CPU restatement oracle/mop_oracle.c)."""
+    global _oracle_loaded
+    h = harness()
+    if not _oracle_loaded:
+        rc = h.mh_load_backend(3, ORACLE_LIB.encode(), b"mop_oracle_", b"oracle")
+        if rc != 0:
+            raise RuntimeError(f"mh_load_backend(oracle) failed ({rc})")
+        _oracle_loaded = True
+
+
+def use_hip():
+    """Make mode 2 available (loads libmop_hip.so into the harness)."""
+    global _hip_loaded
+    h = harness()
+    if not _hip_loaded:
+        lib = os.path.join(REPO, "master-of-puppets_amd", "libmop_hip.so")
+        rc = h.mh_load_hip(lib.encode())
+        if rc != 0:
+            raise RuntimeError(f"mh_load_hip failed ({rc})")
+        _hip_loaded = True
+
+
+def _f3(t):
+    return (C.c_float * 4)(*(list(t) + [0.0] * (4 - len(t))))
+
+
+class RefViewport:
+    """A reference-core viewport holding one Scene."""
+
+    def __init__(self, scene, mode, env_path=None):
+        h = harness()
+        if mode == 2:
+            use_hip()
+        elif mode == 3:
+            use_oracle()
+        if h.mh_set_mode(mode) != 0:
+            raise RuntimeError(f"mode {mode} unavailable")
+        self.h, self.mode, self.scene = h, mode, scene
+        self.vp = h.mh_viewport(scene.width, scene.height)
+        if not self.vp:
+            raise RuntimeError("mop_viewport_create failed")
+        self._keep = []
+        for m in scene.meshes:
+            v = np.ascontiguousarray(m.vertices, np.float32)
+            i = np.ascontiguousarray(m.indices, np.uint32)
+            model = None
+            if m.model is not None:
+                mm = np.ascontiguousarray(m.model, np.float32)
+                self._keep.append(mm)
+                model = mm.ctypes.data
+            ok = h.mh_add_mesh(self.vp, v.ctypes.data, len(v), i.ctypes.data, len(i),
+                               m.object_id, model, int(m.has_material), m.metallic,
+                               m.roughness, 0, 1.0)
+            if not ok:
+                raise RuntimeError("mop_viewport_add_mesh failed")
+        for L in scene.lights:
+            h.mh_add_light(self.vp, L.type, _f3(L.position), _f3(L.direction), _f3(L.color),
+                           L.intensity, L.range, L.inner_cos, L.outer_cos, int(L.cast_shadows))
+        h.mh_set_camera(self.vp, _f3(scene.eye), _f3(scene.target), _f3(scene.up), scene.fov,
+                        scene.znear, scene.zfar)
+        h.mh_set_post_effects(self.vp, scene.post_effects)
+        if scene.env is not None:
+            env = np.ascontiguousarray(scene.env, np.float32)
+            path = env_path or os.path.join(tempfile.gettempdir(),
+                                            f"mop_env_{os.getpid()}_{id(self)}.exr")
+            rc = h.mh_set_environment(self.vp, env.ctypes.data, env.shape[1], env.shape[0],
+                                      path.encode(), scene.env_rotation, scene.env_intensity)
+            try:
+                os.remove(path)
+            except OSError:
+                pass
+            if rc != 0:
+                raise RuntimeError(f"set_environment failed ({rc})")
+
+    def render(self):
+        if self.h.mh_set_mode(self.mode) != 0:
+            raise RuntimeError("mode switch failed")
+        return self.h.mh_render(self.vp)
+
+    def read(self, hdr=True):
+        w, h = self.scene.width, self.scene.height
+        color = np.zeros((h, w, 4), np.uint8)
+        depth = np.zeros((h, w), np.float32)
+        ids = np.zeros((h, w), np.uint32)
+        hdr_a = np.zeros((h, w, 4), np.float32) if hdr else None
+        rc = self.h.mh_read(self.vp, color.ctypes.data, depth.ctypes.data, ids.ctypes.data,
+                            hdr_a.ctypes.data if hdr else None)
+        if rc != 0:
+            raise RuntimeError(f"mh_read failed ({rc})")
+        return {"color": color, "depth": depth, "ids": ids, "hdr": hdr_a}
+
+    def exposure(self):
+        return self.h.mh_get_exposure(self.vp)
+
+    def triangle_count(self):
+        return self.h.mh_triangle_count(self.vp)
+
+    def pick(self, x, y):
+        i, d = C.c_uint32(), C.c_float()
+        hit = self.h.mh_pick(self.vp, x, y, C.byref(i), C.byref(d))
+        return bool(hit), i.value, d.value
+
+    def ibl_tables(self):
+        F = C.POINTER(C.c_float)
+        irr, pf, lut = F(), F(), F()
+        iw, ih, pw, ph, pl = (C.c_int() for _ in range(5))
+        if self.h.mh_get_ibl(self.vp, C.byref(irr), C.byref(iw), C.byref(ih), C.byref(pf),
+                             C.byref(pw), C.byref(ph), C.byref(pl), C.byref(lut)) != 0:
+            return None
+        npf = sum(max(pw.value >> l, 1) * max(ph.value >> l, 1) for l in range(pl.value))
+        return {
+            "irr": np.ctypeslib.as_array(irr, (ih.value, iw.value, 4)).copy(),
+            "pf": np.ctypeslib.as_array(pf, (npf * 4,)).copy(),
+            "pf_w": pw.value, "pf_h": ph.value, "pf_levels": pl.value,
+            "lut": np.ctypeslib.as_array(lut, (256, 256, 4)).copy(),
+        }
+
+    def close(self):
+        if self.vp:
+            self.h.mh_destroy(self.vp)
+            self.vp = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(scene, mode, frames=1):
+    vp = RefViewport(scene, mode)
+    try:
+        for _ in range(frames):
+            vp.render()
+        out = vp.read()
+        out["exposure"] = vp.exposure()
+        out["triangles"] = vp.triangle_count()
+        return out
+    finally:
+        vp.close()
+
+
+def compare(a, b):
+    """Parity summary between two renders (dicts from read())."""
+    res = {}
+    res["depth_diff_px"] = int((a["depth"].view(np.uint32) != b["depth"].view(np.uint32)).sum())
+    res["id_diff_px"] = int((a["ids"] != b["ids"]).sum())
+    dc = np.abs(a["color"].astype(np.int16) - b["color"].astype(np.int16))
+    res["color_max_diff"] = int(dc.max())
+    res["color_diff_px"] = int((dc.max(-1) > 0).sum())
+    res["color_over_1_px"] = int((dc.max(-1) > 1).sum())
+    if a.get("hdr") is not None and b.get("hdr") is not None:
+        ha, hb = a["hdr"], b["hdr"]
+        rel = np.abs(ha - hb) / np.maximum(np.abs(hb), 1e-6)
+        tol_bad = np.abs(ha - hb) > 1e-4 * np.maximum(np.abs(hb), 1.0)
+        res["hdr_max_rel"] = float(rel.max())
+        res["hdr_over_tol_px"] = int(tol_bad.any(-1).sum())
+        res["hdr_bitdiff_px"] = int((ha.view(np.uint32) != hb.view(np.uint32)).any(-1).sum())
+    return res
