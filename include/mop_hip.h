@@ -234,6 +234,21 @@ const float *mop_hip_framebuffer_read_hdr(MopHipDevice *dev, MopHipFramebuffer *
                                           int *out_h);
 
 /* ------------------------------------------------------------------------
+ * Batched execution (no counterpart in the reference, whose frames are
+ * synchronous per viewport).  Record frames on many framebuffers with
+ * frame_begin/draw (+ mop_hip_frame_request_post), then execute them all in
+ * one set of launches; readbacks and mop_hip_frame_wait_readback work per
+ * framebuffer as usual.
+ * ------------------------------------------------------------------------ */
+
+/* Record the HDR resolve (exposure) and post-process (effects, fog: may be
+ * NULL) of the current frame of fb without executing it. */
+void mop_hip_frame_request_post(MopHipDevice *dev, MopHipFramebuffer *fb, float exposure,
+                                uint32_t effects, const MopHipFogParams *fog);
+/* Execute every recorded, not yet executed frame of fbs[0..n) as one batch. */
+void mop_hip_frames_execute(MopHipDevice *dev, MopHipFramebuffer *const *fbs, size_t n);
+
+/* ------------------------------------------------------------------------
  * Device-resident shadow maps and IBL tables (no counterpart in the
  * reference, whose shadow/IBL state is process-global: rasterizer.c:141-183).
  * They let many scenes keep their lighting state in HBM and let one device

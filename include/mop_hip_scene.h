@@ -43,6 +43,10 @@ void mop_hip_scene_set_exposure(MopHipScene *s, float exposure);
 float mop_hip_scene_exposure(const MopHipScene *s);
 void mop_hip_scene_set_clear_color(MopHipScene *s, MopHipColor c);
 void mop_hip_scene_set_post_effects(MopHipScene *s, uint32_t effects);
+/* Record one frame of s into fb without executing it (execute a set of
+ * recorded frames with mop_hip_frames_execute).  Returns triangles submitted. */
+uint32_t mop_hip_scene_record(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,
+                              int width, int height);
 /* Record and execute one frame into fb (width x height); readbacks through
  * the mop_hip_framebuffer_read_* calls.  Returns triangles submitted. */
 uint32_t mop_hip_scene_render(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,

@@ -137,6 +137,9 @@ SIGS = {
     "mop_hip_scene_set_post_effects": (None, [VP, C.c_uint32]),
     "mop_hip_scene_render": (C.c_uint32, [VP, VP, C.POINTER(Camera), C.c_int, C.c_int]),
     "mop_hip_scene_triangles": (C.c_uint32, [VP]),
+    "mop_hip_scene_record": (C.c_uint32, [VP, VP, C.POINTER(Camera), C.c_int, C.c_int]),
+    "mop_hip_frame_request_post": (None, [VP, VP, C.c_float, C.c_uint32, C.POINTER(FogParams)]),
+    "mop_hip_frames_execute": (None, [VP, C.POINTER(VP), C.c_size_t]),
 }
 
 

@@ -232,7 +232,8 @@ __global__ __launch_bounds__(WG) void k_classify(FrameArgs fa) {
     if (alive) {
       if (a.w > 0.0f && b.w > 0.0f && c.w > 0.0f && inside_all(a) &&
           inside_all(b) && inside_all(c)) {
-        cnt = fan_ok(a, b, c, cull, fa.width, fa.height) ? 1u : 0u;
+        const ViewGPU &vw = fa.views[d.view];
+        cnt = fan_ok(a, b, c, cull, vw.w, vw.h) ? 1u : 0u;
       } else {
         uint32_t q = atomicAdd(&fa.counters[CNT_CLIP], 1u);
         if (q < fa.clip_cap) {
@@ -349,12 +350,13 @@ __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
     }
     if (n >= 3) {
       bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
+      const ViewGPU &vw = fa.views[d.view];
       for (int i = 1; i < n - 1; i++) {
         float4 p0 = make_float4(src[0].p[0], src[0].p[1], src[0].p[2], src[0].p[3]);
         float4 p1 = make_float4(src[i].p[0], src[i].p[1], src[i].p[2], src[i].p[3]);
         float4 p2 = make_float4(src[i + 1].p[0], src[i + 1].p[1], src[i + 1].p[2],
                                 src[i + 1].p[3]);
-        if (fan_ok(p0, p1, p2, cull, fa.width, fa.height))
+        if (fan_ok(p0, p1, p2, cull, vw.w, vw.h))
           cnt++;
       }
       size_t base = (size_t)q * MOPH_MAX_POLY;
@@ -491,7 +493,8 @@ DEV void write_subtri(const FrameArgs &fa, uint32_t idx, uint32_t draw, float4 p
                       float4 p1, float4 p2, uint32_t a0, uint32_t a1, uint32_t a2) {
   if (idx >= fa.subtri_cap)
     return;
-  float half_w = (float)fa.width * 0.5f, half_h = (float)fa.height * 0.5f;
+  const ViewGPU &vw = fa.views[fa.draws[draw].view];
+  float half_w = (float)vw.w * 0.5f, half_h = (float)vw.h * 0.5f;
   SubTri st;
   screen_xform(p0, half_w, half_h, st.sx[0], st.sy[0], st.sz[0], st.iw[0]);
   screen_xform(p1, half_w, half_h, st.sx[1], st.sy[1], st.sz[1], st.iw[1]);
@@ -523,12 +526,13 @@ __global__ __launch_bounds__(WG) void k_emit(FrameArgs fa) {
   }
   int n = (int)fa.cnt_poly[slot];
   bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
+  const ViewGPU &vw = fa.views[d.view];
   size_t base = (size_t)slot * MOPH_MAX_POLY;
   float4 p0 = fa.cpos[base];
   uint32_t o = off;
   for (int i = 1; i < n - 1; i++) {
     float4 p1 = fa.cpos[base + i], p2 = fa.cpos[base + i + 1];
-    if (fan_ok(p0, p1, p2, cull, fa.width, fa.height)) {
+    if (fan_ok(p0, p1, p2, cull, vw.w, vw.h)) {
       write_subtri(fa, o, di, p0, p1, p2, fa.n_verts + (uint32_t)base,
                    fa.n_verts + (uint32_t)(base + i), fa.n_verts + (uint32_t)(base + i + 1));
       o++;
@@ -552,8 +556,8 @@ struct RasterTri {
   float sz[3];
 };
 
-DEV bool load_raster_tri(const FrameArgs &fa, const SubTri &st, RasterTri &r) {
-  if (!edge_setup(st.sx, st.sy, fa.width, fa.height, r.es))
+DEV bool load_raster_tri(int W, int H, const SubTri &st, RasterTri &r) {
+  if (!edge_setup(st.sx, st.sy, W, H, r.es))
     return false;
   r.inv_e[0] = inv_edge_len(r.es.e_dx[0], r.es.e_dy[0]);
   r.inv_e[1] = inv_edge_len(r.es.e_dx[1], r.es.e_dy[1]);
@@ -598,8 +602,9 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
     total = fa.subtri_cap;
   for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
     SubTri st = fa.subtris[s];
+    const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
     RasterTri r;
-    if (!load_raster_tri(fa, st, r))
+    if (!load_raster_tri(vw.w, vw.h, st, r))
       continue;
     if (bbox_area(r.es) > MOPH_SMALL_AREA) {
       uint32_t q = atomicAdd(&fa.counters[CNT_BIG], 1u);
@@ -613,8 +618,8 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
       for (int x = r.es.min_x; x <= r.es.max_x; x++) {
         float z;
         if (frag_depth(r, w0, w1, w2, z)) {
-          size_t idx = (size_t)y * (size_t)fa.width + (size_t)x;
-          vis_update(fa.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+          size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
+          vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
         }
         w0 += r.es.e_dx[0];
         w1 += r.es.e_dx[1];
@@ -639,8 +644,9 @@ __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
   for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
     uint32_t s = fa.big_queue[b];
     SubTri st = fa.subtris[s];
+    const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
     RasterTri r;
-    load_raster_tri(fa, st, r); /* succeeded when queued */
+    load_raster_tri(vw.w, vw.h, st, r); /* succeeded when queued */
     unsigned long long seq = (unsigned long long)(s + 1);
     int nrows = r.es.max_y - r.es.min_y + 1;
     if (threadIdx.x == 0) {
@@ -670,15 +676,15 @@ __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
       if (row < nrows) {
         int y = r.es.min_y + row;
         float w0 = rows[0][threadIdx.x], w1 = rows[1][threadIdx.x], w2 = rows[2][threadIdx.x];
-        size_t rowbase = (size_t)y * (size_t)fa.width;
+        size_t rowbase = (size_t)y * (size_t)vw.w;
         for (int x = r.es.min_x; x <= r.es.max_x; x++) {
           float z;
           if (frag_depth(r, w0, w1, w2, z)) {
             unsigned long long key = ((unsigned long long)depth_key(z) << 32) | seq;
             if (MODE == 0) {
-              vis_update(fa.vis, rowbase + x, key);
-            } else if (fa.vis[rowbase + x] == key) {
-              fa.wbuf[rowbase + x] = make_float4(w0, w1, w2, 0.0f);
+              vis_update(vw.vis, rowbase + x, key);
+            } else if (vw.vis[rowbase + x] == key) {
+              vw.wbuf[rowbase + x] = make_float4(w0, w1, w2, 0.0f);
             }
           }
           w0 += r.es.e_dx[0];
@@ -708,7 +714,7 @@ void launch_raster(const FrameArgs &fa, hipStream_t s) {
 /* --------------------------------------------------------------------- */
 
 /* shadow_test_pcf, rasterizer.c:305-348 */
-DEV float shadow_pcf(const FrameArgs &fa, V3 wp) {
+DEV float shadow_pcf(const ViewGPU &fa, V3 wp) {
   float4 lp = mat4_mul_vec4(fa.shadow_vp, wp.x, wp.y, wp.z, 1.0f);
   if (lp.w <= 0.0f)
     return 1.0f;
@@ -775,7 +781,7 @@ DEV void dir_to_uv(float dx, float dy, float dz, float rot, float &u, float &v) 
   v = asinf(cdy) / 3.14159265f + 0.5f;
 }
 
-DEV void ibl_irradiance(const FrameArgs &fa, V3 n, float out[3]) {
+DEV void ibl_irradiance(const ViewGPU &fa, V3 n, float out[3]) {
   out[0] = out[1] = out[2] = 0.0f;
   float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
   if (len < 1e-6f)
@@ -788,7 +794,7 @@ DEV void ibl_irradiance(const FrameArgs &fa, V3 n, float out[3]) {
   out[2] *= fa.ibl_int;
 }
 
-DEV void ibl_prefiltered(const FrameArgs &fa, V3 refl, float roughness, float out[3]) {
+DEV void ibl_prefiltered(const ViewGPU &fa, V3 refl, float roughness, float out[3]) {
   out[0] = out[1] = out[2] = 0.0f;
   float len = sqrtf(refl.x * refl.x + refl.y * refl.y + refl.z * refl.z);
   if (len < 1e-6f)
@@ -822,7 +828,7 @@ DEV void ibl_prefiltered(const FrameArgs &fa, V3 refl, float roughness, float ou
   out[2] *= fa.ibl_int;
 }
 
-DEV void ibl_brdf(const FrameArgs &fa, float ndotv, float roughness, float &scale,
+DEV void ibl_brdf(const ViewGPU &fa, float ndotv, float roughness, float &scale,
                   float &bias) {
   if (ndotv < 0.0f)
     ndotv = 0.0f;
@@ -877,7 +883,8 @@ struct ShadeOut {
  * compute_multi_specular_ggx (rasterizer.c:1701-1814): the two loops touch the
  * same lights in the same order and accumulate into separate sums, so one
  * loop with the PCF lookup shared gives the same values. */
-DEV void lights_eval(const FrameArgs &fa, const DrawGPU &d, V3 normal, V3 wp, V3 view_dir,
+DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d, V3 normal,
+                     V3 wp, V3 view_dir,
                      float base_r, float base_g, float base_b, float lit[3], float spec[3]) {
   lit[0] = lit[1] = lit[2] = d.ambient;
   spec[0] = spec[1] = spec[2] = 0.0f;
@@ -894,7 +901,7 @@ DEV void lights_eval(const FrameArgs &fa, const DrawGPU &d, V3 normal, V3 wp, V3
   float f0_b = 0.04f * (1.0f - metallic) + base_b * metallic;
   float k = alpha * 0.5f;
   float g1v = ndotv / (ndotv * (1.0f - k) + k);
-  const MopHipLight *L = fa.lights + d.light_off;
+  const MopHipLight *L = frame.lights + d.light_off;
   for (uint32_t i = 0; i < d.light_count; i++) {
     const MopHipLight &l = L[i];
     if (!l.active)
@@ -984,20 +991,38 @@ DEV void load_attr(const FrameArgs &fa, uint32_t slot, float4 &a0, float4 &a1, f
   a2 = p[2];
 }
 
-__global__ __launch_bounds__(WG) void k_shade(FrameArgs fa) {
-  int px = blockIdx.x * 16 + (threadIdx.x & 15);
-  int py = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (px >= fa.width || py >= fa.height)
+DEV uint32_t find_view_by_tile(const ViewGPU *v, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (v[mid].tile_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* One workgroup per 16x16 tile of one view; tiles of all views of the batch
+ * are numbered consecutively. */
+__global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
+  uint32_t vi = find_view_by_tile(frame.views, frame.nviews, blockIdx.x);
+  const ViewGPU &fa = frame.views[vi];
+  uint32_t t = blockIdx.x - fa.tile_base;
+  uint32_t tiles_x = ((uint32_t)fa.w + 15) / 16;
+  int px = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
+  int py = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  if (px >= fa.w || py >= fa.h)
     return;
-  size_t idx = (size_t)py * (size_t)fa.width + (size_t)px;
+  size_t idx = (size_t)py * (size_t)fa.w + (size_t)px;
   unsigned long long key = fa.vis[idx];
   uint32_t s1 = (uint32_t)key;
   if (s1 == 0)
     return;
   uint32_t s = s1 - 1;
-  SubTri st = fa.subtris[s];
+  SubTri st = frame.subtris[s];
   RasterTri r;
-  load_raster_tri(fa, st, r);
+  load_raster_tri(fa.w, fa.h, st, r);
   float w0, w1, w2;
   if (bbox_area(r.es) > MOPH_SMALL_AREA) {
     float4 w = fa.wbuf[idx];
@@ -1026,7 +1051,7 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs fa) {
   float b2 = w2 * r.es.inv_area;
   float z = b0 * st.sz[0] + b1 * st.sz[1] + b2 * st.sz[2];
 
-  const DrawGPU &d = fa.draws[st.draw];
+  const DrawGPU &d = frame.draws[st.draw];
   float pc0 = b0 * st.iw[0];
   float pc1 = b1 * st.iw[1];
   float pc2 = b2 * st.iw[2];
@@ -1035,9 +1060,9 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs fa) {
   pc1 *= inv_pc;
   pc2 *= inv_pc;
   float4 A0[3], A1[3], A2[3];
-  load_attr(fa, st.a[0], A0[0], A1[0], A2[0]);
-  load_attr(fa, st.a[1], A0[1], A1[1], A2[1]);
-  load_attr(fa, st.a[2], A0[2], A1[2], A2[2]);
+  load_attr(frame, st.a[0], A0[0], A1[0], A2[0]);
+  load_attr(frame, st.a[1], A0[1], A1[1], A2[1]);
+  load_attr(frame, st.a[2], A0[2], A1[2], A2[2]);
   /* attribute k of vertex v: n=(A0.x,A0.y,A0.z) wp=(A0.w,A1.x,A1.y) c=(A1.z,A1.w,A2.x,A2.y) */
   V3 n = V3{pc0 * A0[0].x + pc1 * A0[1].x + pc2 * A0[2].x,
             pc0 * A0[0].y + pc1 * A0[1].y + pc2 * A0[2].y,
@@ -1053,7 +1078,7 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs fa) {
   V3 view_dir = v3norm(v3sub(eye, wp));
 
   float lit[3], spec[3];
-  lights_eval(fa, d, n, wp, view_dir, fr, fg, fb_, lit, spec);
+  lights_eval(frame, fa, d, n, wp, view_dir, fr, fg, fb_, lit, spec);
   float metallic = d.metallic, roughness = d.roughness;
   float diffuse_scale = 1.0f - metallic;
   float env_r = 0.0f, env_g = 0.0f, env_b = 0.0f;
@@ -1127,9 +1152,26 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs fa) {
   fa.ids[idx] = d.object_id;
 }
 
+/* Copy the batch counters to every view's host-visible counter slot. */
+__global__ void k_export_counters(FrameArgs fa) {
+  uint32_t v = blockIdx.x * 64 + threadIdx.x;
+  if (v >= fa.nviews)
+    return;
+  uint32_t *o = fa.views[v].cnt_out;
+  for (int k = 0; k < CNT_N; k++)
+    o[k] = fa.counters[k];
+}
+
+void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
+  if (!fa.nviews)
+    return;
+  hipLaunchKernelGGL(k_export_counters, dim3((fa.nviews + 63) / 64), dim3(64), 0, s, fa);
+}
+
 void launch_shade(const FrameArgs &fa, hipStream_t s) {
-  dim3 grid((fa.width + 15) / 16, (fa.height + 15) / 16);
-  hipLaunchKernelGGL(k_shade, grid, dim3(WG), 0, s, fa);
+  if (!fa.n_tiles)
+    return;
+  hipLaunchKernelGGL(k_shade, dim3(fa.n_tiles), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1183,34 +1225,51 @@ DEV float aces(float x) {
   return m < 0.0f ? 0.0f : (m > 1.0f ? 1.0f : m);
 }
 
-__global__ __launch_bounds__(WG) void k_resolve(int npx, const float *hdr, const uint32_t *ids,
-                                                uint8_t *color, float exposure) {
-  int i = blockIdx.x * WG + threadIdx.x;
-  if (i >= npx)
+DEV uint32_t find_post_view(const PostViewGPU *v, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (v[mid].tile_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(WG) void k_resolve(const PostViewGPU *views, uint32_t nviews) {
+  const PostViewGPU &v = views[find_post_view(views, nviews, blockIdx.x)];
+  uint32_t t = blockIdx.x - v.tile_base;
+  uint32_t tiles_x = ((uint32_t)v.w + 15) / 16;
+  int x = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
+  int y = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  if (x >= v.w || y >= v.h)
     return;
-  float4 h = reinterpret_cast<const float4 *>(hdr)[i];
+  size_t i = (size_t)y * (size_t)v.w + (size_t)x;
+  float4 h = reinterpret_cast<const float4 *>(v.hdr)[i];
   uchar4 c;
-  if (ids[i] == 0) {
+  if (v.ids[i] == 0) {
     c.x = cvt_u8(fminf(h.x, 1.0f) * 255.0f + 0.5f);
     c.y = cvt_u8(fminf(h.y, 1.0f) * 255.0f + 0.5f);
     c.z = cvt_u8(fminf(h.z, 1.0f) * 255.0f + 0.5f);
     c.w = 255;
   } else {
-    float r = aces(h.x * exposure);
-    float g = aces(h.y * exposure);
-    float b = aces(h.z * exposure);
+    float r = aces(h.x * v.exposure);
+    float g = aces(h.y * v.exposure);
+    float b = aces(h.z * v.exposure);
     c.x = cvt_u8(r * 255.0f + 0.5f);
     c.y = cvt_u8(g * 255.0f + 0.5f);
     c.z = cvt_u8(b * 255.0f + 0.5f);
     c.w = cvt_u8(h.w * 255.0f + 0.5f);
   }
-  reinterpret_cast<uchar4 *>(color)[i] = c;
+  reinterpret_cast<uchar4 *>(v.dst)[i] = c;
 }
 
-void launch_resolve(int npx, const float *hdr, const uint32_t *ids, uint8_t *color,
-                    float exposure, hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, hdr, ids,
-                     color, exposure);
+void launch_resolve_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
+                          hipStream_t s) {
+  if (!n_tiles)
+    return;
+  hipLaunchKernelGGL(k_resolve, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
 }
 
 DEV float luma_at(const uint8_t *src, int w, int x, int y) {
@@ -1225,14 +1284,21 @@ DEV float srgb_enc(float c) {
 /* mop_sw_fxaa (rasterizer.c:2625-2761) followed by the per-pixel effects of
  * mop_postprocess_apply (postprocess.c:36-116); FXAA reads only the copy of
  * the resolved image, so both fuse into one pass from src into dst. */
-__global__ __launch_bounds__(WG) void k_post(int w, int h, const uint8_t *src, uint8_t *dst,
-                                             const float *depth, uint32_t effects, float fog_r,
-                                             float fog_g, float fog_b, float fog_near,
-                                             float fog_far) {
-  int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+__global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t nviews) {
+  const PostViewGPU &v = views[find_post_view(views, nviews, blockIdx.x)];
+  uint32_t t = blockIdx.x - v.tile_base;
+  uint32_t tiles_x = ((uint32_t)v.w + 15) / 16;
+  int x = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
+  int y = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  const int w = v.w, h = v.h;
   if (x >= w || y >= h)
     return;
+  const uint8_t *src = v.src;
+  uint8_t *dst = v.dst;
+  const float *depth = v.depth;
+  uint32_t effects = v.effects;
+  float fog_r = v.fog_r, fog_g = v.fog_g, fog_b = v.fog_b, fog_near = v.fog_near,
+        fog_far = v.fog_far;
   size_t ci = ((size_t)y * (size_t)w + (size_t)x) * 4;
   uint8_t o0 = src[ci], o1 = src[ci + 1], o2 = src[ci + 2], o3 = src[ci + 3];
   if ((effects & MOP_HIP_POST_FXAA) && w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 &&
@@ -1305,7 +1371,7 @@ __global__ __launch_bounds__(WG) void k_post(int w, int h, const uint8_t *src, u
     float r = (float)o0 / 255.0f;
     float g = (float)o1 / 255.0f;
     float b = (float)o2 / 255.0f;
-    if (rest & MOP_HIP_POST_FOG) {
+    if ((rest & MOP_HIP_POST_FOG) && v.has_fog) {
       float dz = depth[(size_t)y * (size_t)w + (size_t)x];
       float range = fog_far - fog_near;
       float factor = 0.0f;
@@ -1355,21 +1421,11 @@ __global__ __launch_bounds__(WG) void k_post(int w, int h, const uint8_t *src, u
   dst[ci + 3] = o3;
 }
 
-void launch_post(int w, int h, const uint8_t *src, uint8_t *dst, const float *depth,
-                 uint32_t effects, const MopHipFogParams *fog, hipStream_t s) {
-  dim3 grid((w + 15) / 16, (h + 15) / 16);
-  float fr = 0, fg = 0, fb = 0, fn = 0, ff = 0;
-  if (fog) {
-    fr = fog->color.r;
-    fg = fog->color.g;
-    fb = fog->color.b;
-    fn = fog->near_dist;
-    ff = fog->far_dist;
-  } else {
-    effects &= ~(uint32_t)MOP_HIP_POST_FOG;
-  }
-  hipLaunchKernelGGL(k_post, grid, dim3(WG), 0, s, w, h, src, dst, depth, effects, fr, fg, fb,
-                     fn, ff);
+void launch_post_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
+                       hipStream_t s) {
+  if (!n_tiles)
+    return;
+  hipLaunchKernelGGL(k_post, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
 }
 
 __global__ void k_fill_u32(size_t n, uint32_t *p, uint32_t v) {

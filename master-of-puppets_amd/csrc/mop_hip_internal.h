@@ -66,7 +66,7 @@ struct DrawGPU {
   float cam_eye[3];
   uint32_t light_off;
   uint32_t light_count;
-  uint32_t pad_;
+  uint32_t view; /* index into the batch's views */
 };
 
 /* Screen-space triangle: exactly the MopSwScreenVertex positions the CPU
@@ -88,16 +88,45 @@ enum {
   CNT_N = 8
 };
 
+/* One view (framebuffer + the shading state its draws see) of a batch.
+ * A batch renders one order-free segment of each of its views in a single
+ * set of launches; a single frame is a batch of one view. */
+struct ViewGPU {
+  int32_t w, h;
+  uint32_t px_base;   /* first pixel of this view in the batch's pixel space */
+  uint32_t tile_base; /* first 16x16 tile of this view in the batch's tiles */
+  unsigned long long *vis;
+  float4 *wbuf;
+  float *hdr;
+  float *depth;
+  uint32_t *ids;
+  uint32_t *cnt_out; /* device alias of host-pinned counters of this segment */
+  /* shadow lookup state (mop_sw_shadow_set) */
+  const float *shadow_map;
+  int32_t shadow_w, shadow_h;
+  int32_t shadow_on;
+  int32_t ibl_on;
+  float shadow_vp[16];
+  /* IBL state (mop_sw_ibl_set) */
+  const float *irr;
+  const float *pf;
+  const float *lut;
+  int32_t irr_w, irr_h, pf_w, pf_h, pf_levels, lut_size;
+  float ibl_rot, ibl_int;
+};
+
 /* Everything the frame kernels read, passed by value. */
 struct FrameArgs {
   const DrawGPU *draws;
+  const ViewGPU *views;
   uint32_t ndraws;
+  uint32_t nviews;
   uint32_t n_verts;
   uint32_t n_tris;
   uint32_t clip_cap;
   uint32_t subtri_cap;
-  int32_t width, height;
-  int32_t pad0_;
+  uint32_t n_tiles;   /* 16x16 shading tiles over all views */
+  uint32_t pad0_;
   float4 *vpos;   /* n_verts clip positions */
   float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */
   float4 *cpos;   /* clip_cap * MOPH_MAX_POLY */
@@ -109,24 +138,23 @@ struct FrameArgs {
   uint32_t *counters;
   SubTri *subtris;
   uint32_t *big_queue;
-  unsigned long long *vis;
-  float4 *wbuf;
-  float *hdr;
-  float *depth;
-  uint32_t *ids;
   const MopHipLight *lights;
-  /* shadow lookup state (mop_sw_shadow_set) */
-  const float *shadow_map;
-  int32_t shadow_w, shadow_h;
-  int32_t shadow_on;
-  float shadow_vp[16];
-  /* IBL state (mop_sw_ibl_set) */
-  const float *irr;
-  const float *pf;
-  const float *lut;
-  int32_t irr_w, irr_h, pf_w, pf_h, pf_levels, lut_size;
-  float ibl_rot, ibl_int;
-  int32_t ibl_on;
+};
+
+/* Per-view arguments of the resolve and post passes. */
+struct PostViewGPU {
+  int32_t w, h;
+  uint32_t tile_base;
+  uint32_t effects;
+  const float *hdr;
+  const float *depth;
+  const uint32_t *ids;
+  const uint8_t *src;
+  uint8_t *dst;
+  float exposure;
+  float fog_r, fog_g, fog_b, fog_near, fog_far;
+  int32_t has_fog;
+  int32_t pad_;
 };
 
 /* One shadow caster mesh (mop_sw_shadow_render_mesh arguments). */
@@ -159,16 +187,17 @@ void launch_scan_counts(const FrameArgs &fa, uint32_t *block_sums, hipStream_t s
 void launch_emit(const FrameArgs &fa, hipStream_t s);
 void launch_raster(const FrameArgs &fa, hipStream_t s);
 void launch_shade(const FrameArgs &fa, hipStream_t s);
+void launch_export_counters(const FrameArgs &fa, hipStream_t s);
 void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr,
                   float *depth, uint32_t *ids, unsigned long long *vis,
                   hipStream_t s);
 void launch_vis_from_depth(int npx, const float *depth,
                            unsigned long long *vis, hipStream_t s);
-void launch_resolve(int npx, const float *hdr, const uint32_t *ids,
-                    uint8_t *color, float exposure, hipStream_t s);
-void launch_post(int w, int h, const uint8_t *src, uint8_t *dst,
-                 const float *depth, uint32_t effects,
-                 const MopHipFogParams *fog, hipStream_t s);
+/* resolve (mode 0) or post (mode 1) over the tiles of n views */
+void launch_resolve_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
+                          hipStream_t s);
+void launch_post_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
+                       hipStream_t s);
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s);

@@ -130,8 +130,22 @@ struct MopHipFramebuffer {
   std::vector<PendingDraw> draws;
   std::vector<ShadeState> states;
   bool executed = false;
-  bool resolved = false;
+  bool checked = false;
+  /* resolve / post requested for this frame (replayed on re-execution) */
+  bool resolve_set = false;
+  float resolve_exposure = 1.0f;
+  uint32_t post_effects = 0;
+  bool has_fog = false;
+  MopHipFogParams fog = {};
+  /* per-segment counters written by the device into pinned host memory */
+  uint32_t *h_cnt = nullptr; /* MOPH_MAX_SEGS * CNT_N, host pointer */
+  uint32_t *d_cnt = nullptr; /* device alias of h_cnt */
+  int nseg = 0;
+  hipEvent_t done = nullptr;
+  size_t skipped = 0;
 };
+
+#define MOPH_MAX_SEGS 32
 
 template <class T> struct DevArr {
   T *p = nullptr;
@@ -190,9 +204,19 @@ struct StageTimers {
   size_t next = 0;
 };
 
+#define NSTAGE 16
+struct StageSlot {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool used = false;
+};
+
 struct MopHipDevice {
   hipStream_t stream = nullptr;
   StageTimers tm;
+  StageSlot ring[NSTAGE];
+  int ring_next = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::mutex mu;
   std::unordered_map<const void *, MopHipBuffer *> host2buf;
@@ -204,10 +228,9 @@ struct MopHipDevice {
       big_queue;
   DevArr<SubTri> subtris;
   uint32_t clip_cap_hint = 0;
-  /* pinned staging */
-  void *h_stage = nullptr;
-  size_t h_stage_cap = 0;
   uint32_t *h_counters = nullptr;
+  DevArr<ViewGPU> views;
+  DevArr<PostViewGPU> pviews;
   /* shadow: the map the hook API renders into, and the recorded casters */
   MopHipShadowMap *def_shadow = nullptr;
   MopHipShadowMap *rec_shadow = nullptr; /* map the recorded casters target */
@@ -253,6 +276,8 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   }
   HIPCHK(hipEventCreate(&d->ev0));
   HIPCHK(hipEventCreate(&d->ev1));
+  for (int k = 0; k < NSTAGE; k++)
+    HIPCHK(hipEventCreateWithFlags(&d->ring[k].ev, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&d->h_counters, CNT_N * sizeof(uint32_t), 0));
   return d;
 }
@@ -283,8 +308,13 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
     (void)hipFree(b->dev);
     delete b;
   }
-  if (d->h_stage)
-    (void)hipHostFree(d->h_stage);
+  for (int k = 0; k < NSTAGE; k++) {
+    if (d->ring[k].p)
+      (void)hipHostFree(d->ring[k].p);
+    (void)hipEventDestroy(d->ring[k].ev);
+  }
+  d->views.release();
+  d->pviews.release();
   if (d->h_counters)
     (void)hipHostFree(d->h_counters);
   (void)hipEventDestroy(d->ev0);
@@ -293,24 +323,34 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   delete d;
 }
 
-static void *stage(MopHipDevice *d, size_t bytes) {
-  if (bytes > d->h_stage_cap) {
-    /* the previous staging buffer may still feed an in-flight copy */
-    (void)hipStreamSynchronize(d->stream);
-    if (d->h_stage)
-      (void)hipHostFree(d->h_stage);
+/* Pinned staging ring: each upload takes the next slot, waiting only for
+ * the copy that used the slot NSTAGE uploads ago. */
+static uint8_t *stage_acquire(MopHipDevice *d, size_t bytes, int *slot) {
+  int k = d->ring_next;
+  d->ring_next = (k + 1) % NSTAGE;
+  StageSlot &sl = d->ring[k];
+  if (sl.used)
+    HIPCHK(hipEventSynchronize(sl.ev));
+  sl.used = false;
+  if (bytes > sl.cap) {
+    if (sl.p)
+      (void)hipHostFree(sl.p);
     size_t nc = bytes + bytes / 2 + 4096;
-    if (hipHostMalloc(&d->h_stage, nc, 0) != hipSuccess) {
-      d->h_stage = nullptr;
-      d->h_stage_cap = 0;
+    if (hipHostMalloc(&sl.p, nc, 0) != hipSuccess) {
+      sl.p = nullptr;
+      sl.cap = 0;
       set_err("hipHostMalloc(%zu) failed", nc);
       return nullptr;
     }
-    d->h_stage_cap = nc;
-  } else {
-    (void)hipStreamSynchronize(d->stream);
+    sl.cap = nc;
   }
-  return d->h_stage;
+  *slot = k;
+  return (uint8_t *)sl.p;
+}
+
+static void stage_release(MopHipDevice *d, int k) {
+  HIPCHK(hipEventRecord(d->ring[k].ev, d->stream));
+  d->ring[k].used = true;
 }
 
 static void tm_collect(MopHipDevice *d);
@@ -514,6 +554,10 @@ static void fb_free(MopHipFramebuffer *fb) {
   if (fb->h_depth) (void)hipHostFree(fb->h_depth);
   if (fb->h_ids) (void)hipHostFree(fb->h_ids);
   if (fb->h_hdr) (void)hipHostFree(fb->h_hdr);
+  if (fb->h_cnt) (void)hipHostFree(fb->h_cnt);
+  if (fb->done) (void)hipEventDestroy(fb->done);
+  fb->h_cnt = fb->d_cnt = nullptr;
+  fb->done = nullptr;
   fb->color[0] = fb->color[1] = nullptr;
   fb->hdr = fb->depth = nullptr;
   fb->ids = nullptr;
@@ -542,7 +586,11 @@ static bool fb_alloc(MopHipDevice *d, MopHipFramebuffer *fb, int w, int h) {
             hipHostMalloc((void **)&fb->h_color, n * 4, 0) == hipSuccess &&
             hipHostMalloc((void **)&fb->h_depth, n * 4, 0) == hipSuccess &&
             hipHostMalloc((void **)&fb->h_ids, n * 4, 0) == hipSuccess &&
-            hipHostMalloc((void **)&fb->h_hdr, n * 16, 0) == hipSuccess;
+            hipHostMalloc((void **)&fb->h_hdr, n * 16, 0) == hipSuccess &&
+            hipHostMalloc((void **)&fb->h_cnt, MOPH_MAX_SEGS * CNT_N * 4,
+                          hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+            hipHostGetDevicePointer((void **)&fb->d_cnt, fb->h_cnt, 0) == hipSuccess &&
+            hipEventCreateWithFlags(&fb->done, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     set_err("framebuffer allocation %dx%d failed", w, h);
     fb_free(fb);
@@ -587,6 +635,7 @@ extern "C" void mop_hip_framebuffer_resize(MopHipDevice *d, MopHipFramebuffer *f
   fb->draws.clear();
   fb->states.clear();
   fb->executed = false;
+  fb->checked = false;
   fb_alloc(d, fb, w, h);
 }
 
@@ -609,7 +658,10 @@ extern "C" void mop_hip_frame_begin(MopHipDevice *d, MopHipFramebuffer *fb, MopH
   fb->draws.clear();
   fb->states.clear();
   fb->executed = false;
-  fb->resolved = false;
+  fb->checked = false;
+  fb->resolve_set = false;
+  fb->post_effects = 0;
+  fb->has_fog = false;
   d->state_dirty = true;
   invalidate_host(fb);
 }
@@ -726,8 +778,8 @@ extern "C" void mop_hip_draw_instanced(MopHipDevice *d, MopHipFramebuffer *fb,
 /* ------------------------------------------------------------------------ */
 
 static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_tris,
-                                 uint32_t ndraws, uint32_t nlights, uint32_t &clip_cap,
-                                 uint32_t &subtri_cap) {
+                                 uint32_t ndraws, uint32_t nlights, uint32_t nviews,
+                                 uint32_t &clip_cap, uint32_t &subtri_cap) {
   clip_cap = n_tris / 16;
   if (clip_cap < 4096)
     clip_cap = 4096;
@@ -738,7 +790,8 @@ static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_t
     sc = 0xffffff00ull;
   subtri_cap = (uint32_t)sc;
   return d->draws.ensure(ndraws ? ndraws : 1) && d->lights.ensure(nlights ? nlights : 1) &&
-         d->vpos.ensure(n_verts ? n_verts : 1) && d->vattr.ensure((size_t)(n_verts ? n_verts : 1) * 3) &&
+         d->views.ensure(nviews ? nviews : 1) && d->vpos.ensure(n_verts ? n_verts : 1) &&
+         d->vattr.ensure((size_t)(n_verts ? n_verts : 1) * 3) &&
          d->cpos.ensure((size_t)clip_cap * MOPH_MAX_POLY) &&
          d->cattr.ensure((size_t)clip_cap * MOPH_MAX_POLY * 3) && d->cnt_poly.ensure(clip_cap) &&
          d->tri_counts.ensure((size_t)n_tris + 1) && d->clip_queue.ensure(clip_cap) &&
@@ -747,84 +800,116 @@ static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_t
          d->big_queue.ensure(subtri_cap);
 }
 
-static void fill_state(const ShadeState &st, FrameArgs &fa) {
-  fa.shadow_on = (st.shadow_on && st.shadow_map) ? 1 : 0;
-  fa.shadow_map = st.shadow_map;
-  fa.shadow_w = st.shadow_w;
-  fa.shadow_h = st.shadow_h;
-  memcpy(fa.shadow_vp, st.shadow_vp, sizeof(fa.shadow_vp));
-  fa.ibl_on = st.ibl_on ? 1 : 0;
-  fa.irr = st.irr;
-  fa.pf = st.pf;
-  fa.lut = st.lut;
-  fa.irr_w = st.irr_w;
-  fa.irr_h = st.irr_h;
-  fa.pf_w = st.pf_w;
-  fa.pf_h = st.pf_h;
-  fa.pf_levels = st.pf_levels;
-  fa.lut_size = st.lut_size;
-  fa.ibl_rot = st.ibl_rot;
-  fa.ibl_int = st.ibl_int;
+static uint32_t view_tiles(int w, int h) {
+  return (uint32_t)(((w + 15) / 16) * ((h + 15) / 16));
 }
 
-/* Runs draws [b, e) (one order-free class, one shading state). */
-static bool run_segment(MopHipDevice *d, MopHipFramebuffer *fb, size_t b, size_t e,
-                        bool first_segment) {
-  uint32_t n_verts = 0, n_tris = 0, nlights = 0;
+static void fill_view(MopHipFramebuffer *fb, const ShadeState &st, int seg, ViewGPU &v) {
+  memset(&v, 0, sizeof(v));
+  v.w = fb->w;
+  v.h = fb->h;
+  v.vis = fb->vis;
+  v.wbuf = fb->wbuf;
+  v.hdr = fb->hdr;
+  v.depth = fb->depth;
+  v.ids = fb->ids;
+  v.cnt_out = fb->d_cnt + (size_t)(seg < MOPH_MAX_SEGS ? seg : MOPH_MAX_SEGS - 1) * CNT_N;
+  v.shadow_on = (st.shadow_on && st.shadow_map) ? 1 : 0;
+  v.shadow_map = st.shadow_map;
+  v.shadow_w = st.shadow_w;
+  v.shadow_h = st.shadow_h;
+  memcpy(v.shadow_vp, st.shadow_vp, sizeof(v.shadow_vp));
+  v.ibl_on = st.ibl_on ? 1 : 0;
+  v.irr = st.irr;
+  v.pf = st.pf;
+  v.lut = st.lut;
+  v.irr_w = st.irr_w;
+  v.irr_h = st.irr_h;
+  v.pf_w = st.pf_w;
+  v.pf_h = st.pf_h;
+  v.pf_levels = st.pf_levels;
+  v.lut_size = st.lut_size;
+  v.ibl_rot = st.ibl_rot;
+  v.ibl_int = st.ibl_int;
+}
+
+struct SegJob {
+  MopHipFramebuffer *fb;
+  size_t b, e;
+  int seg;
+};
+
+/* One order-free segment of each job's framebuffer, all in one set of
+ * launches (views are independent: own visibility buffer, own state). */
+static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs) {
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
-  gd.reserve(e - b);
-  for (size_t i = b; i < e; i++) {
-    PendingDraw &pd = fb->draws[i];
-    DrawGPU g = pd.g;
-    g.vtx_base = n_verts;
-    g.tri_base = n_tris;
-    /* share the light array with the previous draw when identical */
-    if (!lights.empty() && !gd.empty() && gd.back().light_count == pd.lights.size() &&
-        memcmp(&lights[gd.back().light_off], pd.lights.data(),
-               pd.lights.size() * sizeof(MopHipLight)) == 0) {
-      g.light_off = gd.back().light_off;
-    } else {
-      g.light_off = (uint32_t)lights.size();
-      lights.insert(lights.end(), pd.lights.begin(), pd.lights.end());
+  std::vector<ViewGPU> views;
+  uint32_t n_verts = 0, n_tris = 0, tiles = 0;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    MopHipFramebuffer *fb = jobs[j].fb;
+    ViewGPU v;
+    fill_view(fb, fb->states[fb->draws[jobs[j].b].state], jobs[j].seg, v);
+    v.tile_base = tiles;
+    tiles += view_tiles(fb->w, fb->h);
+    views.push_back(v);
+    for (size_t i = jobs[j].b; i < jobs[j].e; i++) {
+      PendingDraw &pd = fb->draws[i];
+      DrawGPU g = pd.g;
+      g.view = (uint32_t)j;
+      g.vtx_base = n_verts;
+      g.tri_base = n_tris;
+      if (!gd.empty() && gd.back().light_count == pd.lights.size() &&
+          (pd.lights.empty() ||
+           memcmp(&lights[gd.back().light_off], pd.lights.data(),
+                  pd.lights.size() * sizeof(MopHipLight)) == 0)) {
+        g.light_off = gd.back().light_off;
+      } else {
+        g.light_off = (uint32_t)lights.size();
+        lights.insert(lights.end(), pd.lights.begin(), pd.lights.end());
+      }
+      if ((uint64_t)n_verts + g.vertex_count > 0xffffffffull ||
+          (uint64_t)n_tris + g.tri_count > 0x7fffffffull) {
+        set_err("batch exceeds 2^31 triangles; remaining draws skipped");
+        break;
+      }
+      n_verts += g.vertex_count;
+      n_tris += g.tri_count;
+      gd.push_back(g);
     }
-    if ((uint64_t)n_verts + g.vertex_count > 0xffffffffull ||
-        (uint64_t)n_tris + g.tri_count > 0x7fffffffull) {
-      set_err("frame exceeds 2^31 triangles; remaining draws skipped");
-      break;
-    }
-    n_verts += g.vertex_count;
-    n_tris += g.tri_count;
-    gd.push_back(g);
   }
-  nlights = (uint32_t)lights.size();
-  if (gd.empty())
-    return true;
   uint32_t clip_cap, subtri_cap;
-  if (!ensure_frame_scratch(d, n_verts, n_tris, (uint32_t)gd.size(), nlights, clip_cap,
-                            subtri_cap))
+  if (!ensure_frame_scratch(d, n_verts, n_tris, (uint32_t)gd.size(), (uint32_t)lights.size(),
+                            (uint32_t)views.size(), clip_cap, subtri_cap))
     return false;
-  size_t bytes_d = gd.size() * sizeof(DrawGPU), bytes_l = lights.size() * sizeof(MopHipLight);
-  uint8_t *stg = (uint8_t *)stage(d, bytes_d + bytes_l + 64);
+  size_t bd = gd.size() * sizeof(DrawGPU), bl = lights.size() * sizeof(MopHipLight),
+         bv = views.size() * sizeof(ViewGPU);
+  int slot;
+  uint8_t *stg = stage_acquire(d, bd + bl + bv + 64, &slot);
   if (!stg)
     return false;
-  memcpy(stg, gd.data(), bytes_d);
-  if (bytes_l)
-    memcpy(stg + bytes_d, lights.data(), bytes_l);
-  HIPCHK(hipMemcpyAsync(d->draws.p, stg, bytes_d, hipMemcpyHostToDevice, d->stream));
-  if (bytes_l)
-    HIPCHK(hipMemcpyAsync(d->lights.p, stg + bytes_d, bytes_l, hipMemcpyHostToDevice, d->stream));
+  memcpy(stg, gd.data(), bd);
+  if (bl)
+    memcpy(stg + bd, lights.data(), bl);
+  memcpy(stg + bd + bl, views.data(), bv);
+  if (bd)
+    HIPCHK(hipMemcpyAsync(d->draws.p, stg, bd, hipMemcpyHostToDevice, d->stream));
+  if (bl)
+    HIPCHK(hipMemcpyAsync(d->lights.p, stg + bd, bl, hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipMemcpyAsync(d->views.p, stg + bd + bl, bv, hipMemcpyHostToDevice, d->stream));
+  stage_release(d, slot);
 
   FrameArgs fa;
   memset(&fa, 0, sizeof(fa));
   fa.draws = d->draws.p;
+  fa.views = d->views.p;
   fa.ndraws = (uint32_t)gd.size();
+  fa.nviews = (uint32_t)views.size();
   fa.n_verts = n_verts;
   fa.n_tris = n_tris;
   fa.clip_cap = clip_cap;
   fa.subtri_cap = subtri_cap;
-  fa.width = fb->w;
-  fa.height = fb->h;
+  fa.n_tiles = tiles;
   fa.vpos = d->vpos.p;
   fa.vattr = d->vattr.p;
   fa.cpos = d->cpos.p;
@@ -836,124 +921,244 @@ static bool run_segment(MopHipDevice *d, MopHipFramebuffer *fb, size_t b, size_t
   fa.counters = d->counters.p;
   fa.subtris = d->subtris.p;
   fa.big_queue = d->big_queue.p;
-  fa.vis = fb->vis;
-  fa.wbuf = fb->wbuf;
-  fa.hdr = fb->hdr;
-  fa.depth = fb->depth;
-  fa.ids = fb->ids;
   fa.lights = d->lights.p;
-  fill_state(fb->states[fb->draws[b].state], fa);
 
-  int npx = fb->w * fb->h;
-  if (!first_segment)
-    launch_vis_from_depth(npx, fb->depth, fb->vis, d->stream);
+  for (const SegJob &j : jobs)
+    if (j.seg > 0)
+      launch_vis_from_depth(j.fb->w * j.fb->h, j.fb->depth, j.fb->vis, d->stream);
   launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
-  {
-    TmScope t(d, TM_VERTEX);
-    launch_vertex(fa, d->stream);
+  if (fa.ndraws) {
+    {
+      TmScope t(d, TM_VERTEX);
+      launch_vertex(fa, d->stream);
+    }
+    {
+      TmScope t(d, TM_SETUP);
+      launch_classify(fa, d->stream);
+      launch_clip(fa, d->stream);
+      launch_scan_counts(fa, d->block_sums.p, d->stream);
+      launch_emit(fa, d->stream);
+    }
+    {
+      TmScope t(d, TM_RASTER);
+      launch_raster(fa, d->stream);
+    }
+    {
+      TmScope t(d, TM_SHADE);
+      launch_shade(fa, d->stream);
+    }
   }
-  {
-    TmScope t(d, TM_SETUP);
-    launch_classify(fa, d->stream);
-    launch_clip(fa, d->stream);
-    launch_scan_counts(fa, d->block_sums.p, d->stream);
-    launch_emit(fa, d->stream);
-  }
-  {
-    TmScope t(d, TM_RASTER);
-    launch_raster(fa, d->stream);
-  }
-  {
-    TmScope t(d, TM_SHADE);
-    launch_shade(fa, d->stream);
-  }
-  HIPCHK(hipMemcpyAsync(d->h_counters, d->counters.p, CNT_N * sizeof(uint32_t),
-                        hipMemcpyDeviceToHost, d->stream));
-  d->stats.triangles_in += n_tris;
+  launch_export_counters(fa, d->stream);
   return true;
 }
 
-static void execute_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
-  if (fb->executed)
+/* resolve every frame's HDR buffer into colour[0] (one launch) */
+static void run_resolve(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fbs) {
+  if (fbs.empty())
     return;
-  for (int attempt = 0; attempt < 4; attempt++) {
-    d->stats = MopHipFrameStats();
-    HIPCHK(hipEventRecord(d->ev0, d->stream));
-    int npx = fb->w * fb->h;
-    if (fb->has_clear)
-      launch_clear(npx, fb->clear, fb->color[0], fb->hdr, fb->depth, fb->ids, fb->vis,
-                   d->stream);
-    bool first = true;
-    bool overflow = false;
-    uint32_t agg[4] = {0, 0, 0, 0};
-    size_t i = 0, n = fb->draws.size();
-    size_t skipped = 0;
-    while (i < n) {
-      size_t j = i + 1;
-      while (j < n && fb->draws[j].cls == fb->draws[i].cls &&
-             fb->draws[j].state == fb->draws[i].state)
-        j++;
-      if (fb->draws[i].cls == 0) {
-        run_segment(d, fb, i, j, first);
-        first = false;
-        /* counters of this segment */
-        HIPCHK(hipStreamSynchronize(d->stream));
-        uint32_t *c = d->h_counters;
-        if (c[CNT_OVERFLOW] & 3u) {
-          overflow = true;
-          uint32_t need = c[CNT_CLIP];
-          if (need > d->clip_cap_hint)
-            d->clip_cap_hint = need + need / 4;
-        }
-        agg[0] += c[CNT_SUBTRI];
-        agg[1] += c[CNT_CLIP];
-        agg[2] += c[CNT_BIG];
-      } else if (fb->draws[i].cls == 1) {
-        skipped += j - i;
-      }
-      i = j;
-      if (overflow)
-        break;
-    }
-    if (overflow)
-      continue;
-    if (skipped)
-      set_err("%zu draw(s) of a class this backend does not render yet were skipped", skipped);
-    {
-      TmScope t(d, TM_RESOLVE);
-      launch_resolve(npx, fb->hdr, fb->ids, fb->color[0], d->exposure, d->stream);
-    }
+  std::vector<PostViewGPU> pv(fbs.size());
+  uint32_t tiles = 0;
+  for (size_t i = 0; i < fbs.size(); i++) {
+    MopHipFramebuffer *fb = fbs[i];
+    PostViewGPU &v = pv[i];
+    memset(&v, 0, sizeof(v));
+    v.w = fb->w;
+    v.h = fb->h;
+    v.tile_base = tiles;
+    tiles += view_tiles(fb->w, fb->h);
+    v.hdr = fb->hdr;
+    v.ids = fb->ids;
+    v.dst = fb->color[0];
+    v.exposure = fb->resolve_set ? fb->resolve_exposure : d->exposure;
     fb->cur = 0;
-    HIPCHK(hipEventRecord(d->ev1, d->stream));
-    d->stats.subtris = agg[0];
-    d->stats.clipped = agg[1];
-    d->stats.big_subtris = agg[2];
-    d->timing_pending = true;
+    fb->v_color = false;
+  }
+  if (!d->pviews.ensure(pv.size()))
+    return;
+  int slot;
+  uint8_t *stg = stage_acquire(d, pv.size() * sizeof(PostViewGPU), &slot);
+  if (!stg)
+    return;
+  memcpy(stg, pv.data(), pv.size() * sizeof(PostViewGPU));
+  HIPCHK(hipMemcpyAsync(d->pviews.p, stg, pv.size() * sizeof(PostViewGPU),
+                        hipMemcpyHostToDevice, d->stream));
+  stage_release(d, slot);
+  TmScope t(d, TM_RESOLVE);
+  launch_resolve_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
+}
+
+/* FXAA + per-pixel effects of every frame with a recorded post request */
+static void run_post(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fbs) {
+  std::vector<PostViewGPU> pv;
+  uint32_t tiles = 0;
+  for (MopHipFramebuffer *fb : fbs) {
+    if (!fb->post_effects)
+      continue;
+    PostViewGPU v;
+    memset(&v, 0, sizeof(v));
+    v.w = fb->w;
+    v.h = fb->h;
+    v.tile_base = tiles;
+    tiles += view_tiles(fb->w, fb->h);
+    v.effects = fb->post_effects;
+    v.depth = fb->depth;
+    v.src = fb->color[fb->cur];
+    v.dst = fb->color[1 - fb->cur];
+    if (fb->has_fog) {
+      v.has_fog = 1;
+      v.fog_r = fb->fog.color.r;
+      v.fog_g = fb->fog.color.g;
+      v.fog_b = fb->fog.color.b;
+      v.fog_near = fb->fog.near_dist;
+      v.fog_far = fb->fog.far_dist;
+    }
+    fb->cur = 1 - fb->cur;
+    fb->v_color = false;
+    pv.push_back(v);
+  }
+  if (pv.empty() || !d->pviews.ensure(pv.size()))
+    return;
+  int slot;
+  uint8_t *stg = stage_acquire(d, pv.size() * sizeof(PostViewGPU), &slot);
+  if (!stg)
+    return;
+  memcpy(stg, pv.data(), pv.size() * sizeof(PostViewGPU));
+  HIPCHK(hipMemcpyAsync(d->pviews.p, stg, pv.size() * sizeof(PostViewGPU),
+                        hipMemcpyHostToDevice, d->stream));
+  stage_release(d, slot);
+  TmScope t(d, TM_POST);
+  launch_post_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
+}
+
+/* Execute the recorded frames of `fbs` that are not executed yet:
+ * clear, then the k-th order-free segment of every frame in one batch for
+ * k = 0, 1, ..., then one resolve launch and one post launch. */
+static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_t n) {
+  std::vector<MopHipFramebuffer *> todo;
+  for (size_t i = 0; i < n; i++)
+    if (fbs[i] && !fbs[i]->executed)
+      todo.push_back(fbs[i]);
+  if (todo.empty())
+    return;
+  HIPCHK(hipEventRecord(d->ev0, d->stream));
+  std::vector<std::vector<std::pair<size_t, size_t>>> segs(todo.size());
+  size_t rounds = 0;
+  for (size_t i = 0; i < todo.size(); i++) {
+    MopHipFramebuffer *fb = todo[i];
+    if (fb->has_clear)
+      launch_clear(fb->w * fb->h, fb->clear, fb->color[0], fb->hdr, fb->depth, fb->ids, fb->vis,
+                   d->stream);
+    fb->skipped = 0;
+    size_t a = 0, m = fb->draws.size();
+    while (a < m) {
+      size_t b = a + 1;
+      while (b < m && fb->draws[b].cls == fb->draws[a].cls &&
+             fb->draws[b].state == fb->draws[a].state)
+        b++;
+      if (fb->draws[a].cls == 0)
+        segs[i].push_back({a, b});
+      else if (fb->draws[a].cls == 1)
+        fb->skipped += b - a;
+      a = b;
+    }
+    fb->nseg = (int)segs[i].size();
+    if (fb->nseg > MOPH_MAX_SEGS)
+      set_err("frame has %d draw segments; counters of segments past %d are merged",
+              fb->nseg, MOPH_MAX_SEGS);
+    memset(fb->h_cnt, 0, MOPH_MAX_SEGS * CNT_N * 4);
+    if (segs[i].size() > rounds)
+      rounds = segs[i].size();
+    if (fb->skipped)
+      set_err("%zu draw(s) of a class this backend does not render yet were skipped",
+              fb->skipped);
+  }
+  for (size_t r = 0; r < rounds; r++) {
+    std::vector<SegJob> jobs;
+    for (size_t i = 0; i < todo.size(); i++)
+      if (r < segs[i].size())
+        jobs.push_back({todo[i], segs[i][r].first, segs[i][r].second, (int)r});
+    run_batch(d, jobs);
+  }
+  run_resolve(d, todo);
+  run_post(d, todo);
+  for (MopHipFramebuffer *fb : todo) {
+    HIPCHK(hipEventRecord(fb->done, d->stream));
     fb->executed = true;
-    fb->resolved = true;
+    fb->checked = false;
     invalidate_host(fb);
+  }
+  HIPCHK(hipEventRecord(d->ev1, d->stream));
+  d->timing_pending = true;
+}
+
+/* Wait for fb's frame, re-run it when a scratch capacity overflowed, and
+ * take its statistics. */
+static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
+  for (int attempt = 0; attempt < 4; attempt++) {
+    if (!fb->executed)
+      execute_frames(d, &fb, 1);
+    if (fb->checked)
+      return;
+    HIPCHK(hipEventSynchronize(fb->done));
+    uint32_t ovf = 0, need_clip = 0;
+    MopHipFrameStats st = {};
+    int ns = fb->nseg < MOPH_MAX_SEGS ? fb->nseg : MOPH_MAX_SEGS;
+    for (int k = 0; k < ns; k++) {
+      const uint32_t *c = fb->h_cnt + (size_t)k * CNT_N;
+      ovf |= c[CNT_OVERFLOW];
+      if (c[CNT_CLIP] > need_clip)
+        need_clip = c[CNT_CLIP];
+      st.subtris += c[CNT_SUBTRI];
+      st.clipped += c[CNT_CLIP];
+      st.big_subtris += c[CNT_BIG];
+    }
+    if (ovf & 8u)
+      set_err("a clipped polygon exceeded %d vertices and was dropped", MOPH_MAX_POLY);
+    if (ovf & 3u) {
+      /* clip queue or screen-triangle store too small: grow and re-run */
+      d->clip_cap_hint = need_clip + need_clip / 4 + 1024;
+      fb->executed = false;
+      continue;
+    }
+    fb->checked = true;
+    st.triangles_in = 0;
+    for (const PendingDraw &pd : fb->draws)
+      if (pd.cls == 0)
+        st.triangles_in += pd.g.tri_count;
+    st.shadow_tris = d->stats.shadow_tris;
+    st.gpu_ms = d->stats.gpu_ms;
+    d->stats = st;
     return;
   }
-  set_err("frame scratch overflow persisted; frame not rendered");
+  set_err("frame scratch overflow persisted; frame left incomplete");
+  fb->checked = true;
 }
 
 extern "C" void mop_hip_frame_end(MopHipDevice *d, MopHipFramebuffer *fb) {
   if (!d || !fb)
     return;
-  execute_frame(d, fb);
+  execute_frames(d, &fb, 1);
 }
 
 extern "C" void mop_hip_frame_submit(MopHipDevice *d, MopHipFramebuffer *fb) {
   if (!d || !fb)
     return;
-  execute_frame(d, fb);
+  execute_frames(d, &fb, 1);
 }
 
 extern "C" void mop_hip_frame_wait_readback(MopHipDevice *d, MopHipFramebuffer *fb) {
   if (!d || !fb)
     return;
-  execute_frame(d, fb);
-  HIPCHK(hipStreamSynchronize(d->stream));
+  check_frame(d, fb);
+}
+
+/* Execute the recorded frames of many framebuffers as one batch (the
+ * batched form: one set of launches for all of them). */
+extern "C" void mop_hip_frames_execute(MopHipDevice *d, MopHipFramebuffer *const *fbs,
+                                       size_t n) {
+  if (!d || !fbs)
+    return;
+  execute_frames(d, fbs, n);
 }
 
 extern "C" void mop_hip_set_exposure(MopHipDevice *d, float e) {
@@ -985,9 +1190,7 @@ extern "C" void mop_hip_get_frame_stats(MopHipDevice *d, MopHipFrameStats *out) 
 /* readbacks                                                                 */
 /* ------------------------------------------------------------------------ */
 
-static void ensure_ready(MopHipDevice *d, MopHipFramebuffer *fb) {
-  execute_frame(d, fb);
-}
+static void ensure_ready(MopHipDevice *d, MopHipFramebuffer *fb) { check_frame(d, fb); }
 
 extern "C" const uint8_t *mop_hip_framebuffer_read_color(MopHipDevice *d, MopHipFramebuffer *fb,
                                                          int *ow, int *oh) {
@@ -1162,41 +1365,30 @@ static void run_shadow(MopHipDevice *d) {
     gd.push_back(g);
   }
   if (!gd.empty() && n_tris) {
-    for (int attempt = 0; attempt < 3; attempt++) {
-      uint32_t big_cap = d->shadow_big.cap ? (uint32_t)d->shadow_big.cap : 4096;
-      if (!d->shadow_big.ensure(big_cap) || !d->shadow_draws_dev.ensure(gd.size()) ||
-          !d->counters.ensure(CNT_N))
-        return;
-      big_cap = (uint32_t)d->shadow_big.cap;
-      size_t bytes = gd.size() * sizeof(ShadowDrawGPU);
-      void *stg = stage(d, bytes);
-      if (!stg)
-        return;
-      memcpy(stg, gd.data(), bytes);
-      HIPCHK(hipMemcpyAsync(d->shadow_draws_dev.p, stg, bytes, hipMemcpyHostToDevice, d->stream));
-      launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
-      ShadowArgsGPU sa;
-      sa.draws = d->shadow_draws_dev.p;
-      sa.ndraws = (uint32_t)gd.size();
-      sa.n_tris = n_tris;
-      sa.size = size;
-      sa.big_cap = big_cap;
-      sa.keys = map->keys.p;
-      sa.big_queue = d->shadow_big.p;
-      sa.counters = d->counters.p;
-      launch_shadow_raster(sa, d->stream);
-      HIPCHK(hipMemcpyAsync(d->h_counters, d->counters.p, CNT_N * sizeof(uint32_t),
-                            hipMemcpyDeviceToHost, d->stream));
-      HIPCHK(hipStreamSynchronize(d->stream));
-      if (d->h_counters[CNT_OVERFLOW] & 4u) {
-        /* more big triangles than queue slots: grow and redo from a clear map */
-        d->shadow_big.ensure((size_t)d->h_counters[CNT_SHADOW_BIG] + 1);
-        launch_shadow_clear((int)npx, map->keys.p, d->stream);
-        continue;
-      }
-      d->stats.shadow_tris = n_tris;
-      break;
-    }
+    /* the big-triangle queue holds every caster in the worst case */
+    if (!d->shadow_big.ensure(n_tris) || !d->shadow_draws_dev.ensure(gd.size()) ||
+        !d->counters.ensure(CNT_N))
+      return;
+    size_t bytes = gd.size() * sizeof(ShadowDrawGPU);
+    int slot;
+    uint8_t *stg = stage_acquire(d, bytes, &slot);
+    if (!stg)
+      return;
+    memcpy(stg, gd.data(), bytes);
+    HIPCHK(hipMemcpyAsync(d->shadow_draws_dev.p, stg, bytes, hipMemcpyHostToDevice, d->stream));
+    stage_release(d, slot);
+    launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
+    ShadowArgsGPU sa;
+    sa.draws = d->shadow_draws_dev.p;
+    sa.ndraws = (uint32_t)gd.size();
+    sa.n_tris = n_tris;
+    sa.size = size;
+    sa.big_cap = (uint32_t)d->shadow_big.cap;
+    sa.keys = map->keys.p;
+    sa.big_queue = d->shadow_big.p;
+    sa.counters = d->counters.p;
+    launch_shadow_raster(sa, d->stream);
+    d->stats.shadow_tris = n_tris;
   }
   launch_shadow_finalize((int)npx, map->keys.p, map->depth.p, d->stream);
 }
@@ -1441,21 +1633,46 @@ extern "C" int mop_hip_ibl_download(MopHipDevice *d, const MopHipIbl *ibl, float
 extern "C" void mop_hip_hdr_resolve(MopHipDevice *d, MopHipFramebuffer *fb, float exposure) {
   if (!d || !fb)
     return;
-  execute_frame(d, fb);
-  TmScope t(d, TM_RESOLVE);
-  launch_resolve(fb->w * fb->h, fb->hdr, fb->ids, fb->color[0], exposure, d->stream);
-  fb->cur = 0;
-  fb->v_color = false;
+  fb->resolve_set = true;
+  fb->resolve_exposure = exposure;
+  if (!fb->executed) {
+    execute_frames(d, &fb, 1);
+    return;
+  }
+  std::vector<MopHipFramebuffer *> one(1, fb);
+  run_resolve(d, one);
+  run_post(d, one);
+  HIPCHK(hipEventRecord(fb->done, d->stream));
+}
+
+/* Record the resolve and post-process requests of the current frame without
+ * executing anything (frames executed later by mop_hip_frames_execute). */
+extern "C" void mop_hip_frame_request_post(MopHipDevice *d, MopHipFramebuffer *fb,
+                                           float exposure, uint32_t effects,
+                                           const MopHipFogParams *fog) {
+  if (!d || !fb)
+    return;
+  fb->resolve_set = true;
+  fb->resolve_exposure = exposure;
+  fb->post_effects = effects;
+  fb->has_fog = fog != nullptr;
+  if (fog)
+    fb->fog = *fog;
 }
 
 extern "C" void mop_hip_postprocess_apply(MopHipDevice *d, MopHipFramebuffer *fb,
                                           uint32_t effects, const MopHipFogParams *fog) {
   if (!d || !fb || effects == 0)
     return;
-  execute_frame(d, fb);
-  TmScope t(d, TM_POST);
-  launch_post(fb->w, fb->h, fb->color[fb->cur], fb->color[1 - fb->cur], fb->depth, effects, fog,
-              d->stream);
-  fb->cur = 1 - fb->cur;
-  fb->v_color = false;
+  fb->post_effects = effects;
+  fb->has_fog = fog != nullptr;
+  if (fog)
+    fb->fog = *fog;
+  if (!fb->executed) {
+    execute_frames(d, &fb, 1);
+    return;
+  }
+  std::vector<MopHipFramebuffer *> one(1, fb);
+  run_post(d, one);
+  HIPCHK(hipEventRecord(fb->done, d->stream));
 }

@@ -416,7 +416,7 @@ static int mesh_outside(const SceneMesh *m, const MopHipMat4 *vpm) {
   return 0;
 }
 
-uint32_t mop_hip_scene_render(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,
+uint32_t mop_hip_scene_record(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,
                               int width, int height) {
   if (!s || !fb || !cam)
     return 0;
@@ -499,14 +499,20 @@ uint32_t mop_hip_scene_render(MopHipScene *s, MopHipFramebuffer *fb, const MopHi
     mop_hip_draw(dev, fb, &c);
     tris += m->index_count / 3;
   }
-  mop_hip_frame_end(dev, fb);
-  mop_hip_hdr_resolve(dev, fb, s->exposure);
+  /* rg_frame_end + rg_cleanup: resolve with the exposure, then post */
+  mop_hip_frame_request_post(dev, fb, s->exposure, s->post_effects, NULL);
   if (shadowed)
     mop_hip_shadow_clear(dev);
   if (s->ibl)
     mop_hip_ibl_clear(dev);
-  if (s->post_effects)
-    mop_hip_postprocess_apply(dev, fb, s->post_effects, NULL);
   s->last_triangles = tris;
+  return tris;
+}
+
+uint32_t mop_hip_scene_render(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,
+                              int width, int height) {
+  uint32_t tris = mop_hip_scene_record(s, fb, cam, width, height);
+  if (s && fb)
+    mop_hip_frames_execute(s->dev, &fb, 1);
   return tris;
 }

@@ -426,3 +426,71 @@ def small_scene(seed=7, width=160, height=120, n=12, with_shadow=True, with_env=
     if with_env:
         sc.env = sky_env(256, 128)
     return sc
+
+
+def _subdiv_for(tris):
+    s = 0
+    while s < 4 and 20 * 4 ** (s + 1) <= tris:
+        s += 1
+    return s
+
+
+def batch_scene(rng, width, height, target_tris, name):
+    """One viewport of the batched workload: icospheres (subdivision 1-4)
+    and, for large triangle budgets, a noise-displaced grid; 1-4 lights, the
+    first a shadowed sun; random PBR materials; own camera."""
+    sc = Scene(name, width, height, fov=rng.uniform(35, 60), znear=0.1, zfar=400.0,
+               shadow_size=1024)
+    remaining = int(target_tris)
+    oid = 1
+    if remaining > 40000:
+        n = max(8, int(math.sqrt(remaining / 2 / 2)))
+        v, i, _ = terrain(n, 60.0, 3.0, rng.next() & 0x7FFFFFFF,
+                          (rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7), 1.0))
+        sc.meshes.append(Mesh(v, i, oid, colmajor(mat_translate((0.0, -6.0, 0.0))), True,
+                              0.0, rng.uniform(0.5, 1.0)))
+        oid += 1
+        remaining -= len(i) // 3
+    while remaining > 0:
+        s = _subdiv_for(remaining)
+        pos, idx = icosphere_base(s)
+        col = (rng.uniform(), rng.uniform(), rng.uniform(), 1.0)
+        v = _verts(pos, pos, col)
+        p = (rng.uniform(-15, 15), rng.uniform(-4, 8), rng.uniform(-15, 15))
+        sc.meshes.append(Mesh(v, idx.copy(), oid, trs(p, random_quat(rng), rng.uniform(0.5, 3.0)),
+                              True, 1.0 if rng.uniform() < 0.5 else 0.0, rng.uniform(0.05, 1.0)))
+        oid += 1
+        remaining -= len(idx) // 3
+    nl = 1 + int(rng.uniform() * 4) % 4
+    sc.lights.append(Light(0, direction=(rng.uniform(-1, 1), -1.0, rng.uniform(-1, 1)),
+                           color=(1.0, 0.95, 0.9, 1.0), intensity=rng.uniform(1.5, 4.0),
+                           cast_shadows=True))
+    for k in range(nl - 1):
+        p = (rng.uniform(-15, 15), rng.uniform(2, 12), rng.uniform(-15, 15))
+        if k % 2 == 0:
+            sc.lights.append(Light(1, position=p, color=(1.0, 0.85, 0.7, 1.0),
+                                   intensity=rng.log_uniform(5, 200)))
+        else:
+            nrm = math.sqrt(sum(c * c for c in p))
+            sc.lights.append(Light(2, position=p, direction=tuple(-c / nrm for c in p),
+                                   color=(0.7, 0.85, 1.0, 1.0), intensity=rng.log_uniform(5, 200),
+                                   inner_cos=math.cos(math.radians(20)),
+                                   outer_cos=math.cos(math.radians(35))))
+    yaw, pitch, dist = rng.uniform(0, 2 * math.pi), rng.uniform(0.2, 0.8), rng.uniform(30, 45)
+    sc.eye = (dist * math.cos(pitch) * math.sin(yaw), dist * math.sin(pitch),
+              dist * math.cos(pitch) * math.cos(yaw))
+    sc.target = (0.0, 0.0, 0.0)
+    return sc
+
+
+def config4(n_views=64, seed=0x4D4F5004, tri_lo=1e3, tri_hi=2e6, max_w=1920, max_h=1080):
+    """Batched workload: n viewports of uneven resolution (W ~ U[256, 1920],
+    H ~ U[256, 1080]) and triangle count (log-uniform [1e3, 2e6])."""
+    rng = Xoshiro(seed)
+    views = []
+    for k in range(n_views):
+        w = int(rng.uniform(256, max_w + 1))
+        h = int(rng.uniform(256, max_h + 1))
+        t = rng.log_uniform(tri_lo, tri_hi)
+        views.append(batch_scene(rng, w, h, t, f"cfg4_view{k}_{w}x{h}"))
+    return views

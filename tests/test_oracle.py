@@ -1,0 +1,58 @@
+"""Pins the CPU restatement (oracle/mop_oracle.c) against the reference
+build: the unmodified reference viewport core drives both the reference CPU
+backend (single-threaded, submission order) and the restatement, on the same
+scenes; every buffer must be bit-identical.  A planted mistake in the
+restatement's arithmetic (a dropped term, a swapped operand) changes depth
+or colour bits somewhere in these scenes, so each stage is pinned here."""
+import numpy as np
+import pytest
+
+import refcore
+from mop_amd import scenes
+
+pytestmark = pytest.mark.skipif(not refcore.available(), reason="oracle/_ref not built")
+
+
+def _same(scene):
+    a = refcore.render(scene, 3)
+    b = refcore.render(scene, 1)
+    cmp = refcore.compare(a, b)
+    print(scene.name, cmp)
+    assert cmp["depth_diff_px"] == 0
+    assert cmp["id_diff_px"] == 0
+    assert cmp["hdr_bitdiff_px"] == 0
+    assert cmp["color_diff_px"] == 0
+    return a, b
+
+
+def test_config1_full_size():
+    a, _ = _same(scenes.config1())
+    ids = np.unique(a["ids"])
+    assert set(ids.tolist()) == {0, 1, 2}
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_small_scenes_shadow_spot_point(seed):
+    _same(scenes.small_scene(seed=seed))
+
+
+def test_small_scene_ibl_and_shadow():
+    _same(scenes.small_scene(seed=5, with_env=True))
+
+
+def test_config2_scaled():
+    _same(scenes.config2(320, 180, n_instances=24, subdiv=2, shadow_size=1024,
+                         env_size=(256, 128)))
+
+
+def test_config3_scaled_near_plane_clipping():
+    _same(scenes.config3(320, 180, n_quads=96, n_spheres=8, shadow_size=1024))
+
+
+def test_background_invariants():
+    # test_torture_depth.c:360-378: empty pixels keep depth 1.0 and id 0
+    a = refcore.render(scenes.config1(160, 120), 3)
+    bg = a["ids"] == 0
+    assert bg.any() and (a["depth"][bg] == 1.0).all()
+    fg = ~bg
+    assert (a["depth"][fg] < 1.0).all() and (a["depth"][fg] >= 0.0).all()
