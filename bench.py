@@ -128,6 +128,63 @@ print(json.dumps({{"frames": len(times), "median_ms": times[len(times)//2], "tri
                       f"1 warm-up, mop_viewport_render + read_color wall time"}
 
 
+def batched_form(mod, rounds):
+    """The 64-viewport workload (uneven resolutions, 1e3..2e6 triangles each):
+    plain = one viewport per launch set, each finished before the next;
+    batched = all 64 recorded, then executed in one set of launches.  Each
+    round ends with every frame finished on the device; per-frame latency of
+    the batched stream is submission of the batch to that frame's finish."""
+    import numpy as np
+    from mop_amd import driver, scenes
+    dev = driver.Device()
+    views = scenes.config4()
+    drv = [driver.SceneDriver(dev, sc, shadow_cache=True) for sc in views]
+    fbs = [dev.framebuffer(sc.width, sc.height) for sc in views]
+    tris = sum(sc.triangle_count for sc in views)
+    # warm-up (also renders the cached shadow maps)
+    for d, fb in zip(drv, fbs):
+        d.render(fb)
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(rounds):
+        for d, fb in zip(drv, fbs):
+            d.render(fb)
+            dev.sync()
+    plain_s = (time.perf_counter() - t0) / rounds
+    lat = []
+    t0 = time.perf_counter()
+    for r in range(rounds):
+        tb = time.perf_counter()
+        for d, fb in zip(drv, fbs):
+            d.record(fb, driver.camera_of(d.sc, yaw_delta=0.5 * r))
+        dev.execute(fbs)
+        pending = list(range(len(fbs)))
+        while pending:
+            still = []
+            for k in pending:
+                if fbs[k].done():
+                    lat.append(time.perf_counter() - tb)
+                else:
+                    still.append(k)
+            pending = still
+    batch_s = (time.perf_counter() - t0) / rounds
+    lat = np.array(lat) * 1e3
+    out = {"views": len(views), "triangles_per_batch": tris,
+           "pixels_per_batch": sum(sc.width * sc.height for sc in views),
+           "plain_frames_per_s": round(len(views) / plain_s, 2),
+           "batched_frames_per_s": round(len(views) / batch_s, 2),
+           "batched_over_plain": round(plain_s / batch_s, 3),
+           "batched_mtri_per_s": round(tris / batch_s / 1e6, 2),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+    for fb in fbs:
+        fb.close()
+    for d in drv:
+        d.close()
+    dev.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +193,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-rounds", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -206,6 +265,7 @@ def main():
                 "kernel": "k_shade", "launch_ms": shade_ms,
                 "algorithmic_bytes_per_launch": algo_bytes}
 
+    batch = None if args.no_batch else batched_form(mod, args.batch_rounds)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, args.cpu_cores)
 
     out = {
@@ -231,6 +291,7 @@ def main():
         "frame_stats": {"subtris": st.subtris, "clipped": st.clipped,
                         "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris},
         "roofline": roof,
+        "batched_64_viewports": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

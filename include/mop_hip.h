@@ -247,6 +247,10 @@ void mop_hip_frame_request_post(MopHipDevice *dev, MopHipFramebuffer *fb, float 
                                 uint32_t effects, const MopHipFogParams *fog);
 /* Execute every recorded, not yet executed frame of fbs[0..n) as one batch. */
 void mop_hip_frames_execute(MopHipDevice *dev, MopHipFramebuffer *const *fbs, size_t n);
+/* 1 when fb's frame has finished on the device (non-blocking). */
+int mop_hip_frame_done(MopHipDevice *dev, MopHipFramebuffer *fb);
+/* Wait until everything queued on the device has finished. */
+void mop_hip_device_sync(MopHipDevice *dev);
 
 /* ------------------------------------------------------------------------
  * Device-resident shadow maps and IBL tables (no counterpart in the

@@ -468,9 +468,11 @@ __global__ __launch_bounds__(WG) void k_scan_apply(uint32_t *io, uint32_t n,
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == WG - 1) {
     io[n] = run; /* total */
-    counters[CNT_SUBTRI] = run;
-    if (run > cap)
-      atomicOr(&counters[CNT_OVERFLOW], 2u);
+    if (counters) {
+      counters[CNT_SUBTRI] = run;
+      if (run > cap)
+        atomicOr(&counters[CNT_OVERFLOW], 2u);
+    }
   }
 }
 
@@ -646,7 +648,8 @@ __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
     SubTri st = fa.subtris[s];
     const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
     RasterTri r;
-    load_raster_tri(vw.w, vw.h, st, r); /* succeeded when queued */
+    if (!load_raster_tri(vw.w, vw.h, st, r)) /* succeeded when queued; uniform */
+      continue;
     unsigned long long seq = (unsigned long long)(s + 1);
     int nrows = r.es.max_y - r.es.min_y + 1;
     if (threadIdx.x == 0) {
@@ -991,65 +994,16 @@ DEV void load_attr(const FrameArgs &fa, uint32_t slot, float4 &a0, float4 &a1, f
   a2 = p[2];
 }
 
-DEV uint32_t find_view_by_tile(const ViewGPU *v, uint32_t n, uint32_t t) {
-  uint32_t lo = 0, hi = n - 1;
-  while (lo < hi) {
-    uint32_t mid = (lo + hi + 1) >> 1;
-    if (v[mid].tile_base <= t)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  return lo;
-}
-
-/* One workgroup per 16x16 tile of one view; tiles of all views of the batch
- * are numbered consecutively. */
-__global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
-  uint32_t vi = find_view_by_tile(frame.views, frame.nviews, blockIdx.x);
-  const ViewGPU &fa = frame.views[vi];
-  uint32_t t = blockIdx.x - fa.tile_base;
-  uint32_t tiles_x = ((uint32_t)fa.w + 15) / 16;
-  int px = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
-  int py = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
-  if (px >= fa.w || py >= fa.h)
-    return;
-  size_t idx = (size_t)py * (size_t)fa.w + (size_t)px;
-  unsigned long long key = fa.vis[idx];
-  uint32_t s1 = (uint32_t)key;
-  if (s1 == 0)
-    return;
-  uint32_t s = s1 - 1;
-  SubTri st = frame.subtris[s];
-  RasterTri r;
-  load_raster_tri(fa.w, fa.h, st, r);
-  float w0, w1, w2;
-  if (bbox_area(r.es) > MOPH_SMALL_AREA) {
-    float4 w = fa.wbuf[idx];
-    w0 = w.x;
-    w1 = w.y;
-    w2 = w.z;
-  } else {
-    /* replay the reference's incremental walk up to this pixel */
-    float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
-    for (int y = r.es.min_y; y < py; y++) {
-      w0r += r.es.e_dy[0];
-      w1r += r.es.e_dy[1];
-      w2r += r.es.e_dy[2];
-    }
-    w0 = w0r;
-    w1 = w1r;
-    w2 = w2r;
-    for (int x = r.es.min_x; x < px; x++) {
-      w0 += r.es.e_dx[0];
-      w1 += r.es.e_dx[1];
-      w2 += r.es.e_dx[2];
-    }
-  }
+/* Colour of one fragment of screen triangle st with edge values w0..w2:
+ * perspective-correct interpolation, multi-light Lambert + GGX + PCF + IBL,
+ * composition and clamp (rasterizer.c:1950-2092).  Also returns z. */
+DEV void shade_fragment(const FrameArgs &frame, const ViewGPU &fa, const SubTri &st,
+                        const RasterTri &r, float w0, float w1, float w2, float &z,
+                        float &pr, float &pg, float &pb) {
   float b0 = w0 * r.es.inv_area;
   float b1 = w1 * r.es.inv_area;
   float b2 = w2 * r.es.inv_area;
-  float z = b0 * st.sz[0] + b1 * st.sz[1] + b2 * st.sz[2];
+  z = b0 * st.sz[0] + b1 * st.sz[1] + b2 * st.sz[2];
 
   const DrawGPU &d = frame.draws[st.draw];
   float pc0 = b0 * st.iw[0];
@@ -1131,7 +1085,6 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
       env_b = he_b * fe_b * d.ambient * 3.0f;
     }
   }
-  float pr, pg, pb;
   if (has_irr) {
     pr = fr * (lit[0] + irr[0]) * diffuse_scale + env_r + spec[0];
     pg = fg * (lit[1] + irr[1]) * diffuse_scale + env_g + spec[1];
@@ -1147,6 +1100,67 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
     pg = 0.0f;
   if (pb < 0.0f)
     pb = 0.0f;
+}
+
+DEV uint32_t find_view_by_tile(const ViewGPU *v, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (v[mid].tile_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* One workgroup per 16x16 tile of one view; tiles of all views of the batch
+ * are numbered consecutively. */
+__global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
+  uint32_t vi = find_view_by_tile(frame.views, frame.nviews, blockIdx.x);
+  const ViewGPU &fa = frame.views[vi];
+  uint32_t t = blockIdx.x - fa.tile_base;
+  uint32_t tiles_x = ((uint32_t)fa.w + 15) / 16;
+  int px = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
+  int py = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  if (px >= fa.w || py >= fa.h)
+    return;
+  size_t idx = (size_t)py * (size_t)fa.w + (size_t)px;
+  unsigned long long key = fa.vis[idx];
+  uint32_t s1 = (uint32_t)key;
+  if (s1 == 0)
+    return;
+  uint32_t s = s1 - 1;
+  SubTri st = frame.subtris[s];
+  RasterTri r;
+  if (!load_raster_tri(fa.w, fa.h, st, r))
+    return;
+  float w0, w1, w2;
+  if (bbox_area(r.es) > MOPH_SMALL_AREA) {
+    float4 w = fa.wbuf[idx];
+    w0 = w.x;
+    w1 = w.y;
+    w2 = w.z;
+  } else {
+    /* replay the reference's incremental walk up to this pixel */
+    float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+    for (int y = r.es.min_y; y < py; y++) {
+      w0r += r.es.e_dy[0];
+      w1r += r.es.e_dy[1];
+      w2r += r.es.e_dy[2];
+    }
+    w0 = w0r;
+    w1 = w1r;
+    w2 = w2r;
+    for (int x = r.es.min_x; x < px; x++) {
+      w0 += r.es.e_dx[0];
+      w1 += r.es.e_dx[1];
+      w2 += r.es.e_dx[2];
+    }
+  }
+  float z, pr, pg, pb;
+  shade_fragment(frame, fa, st, r, w0, w1, w2, z, pr, pg, pb);
+  const DrawGPU &d = frame.draws[st.draw];
   reinterpret_cast<float4 *>(fa.hdr)[idx] = make_float4(pr, pg, pb, 1.0f);
   fa.depth[idx] = z;
   fa.ids[idx] = d.object_id;
@@ -1547,7 +1561,8 @@ __global__ __launch_bounds__(WG) void k_shadow_big(ShadowArgsGPU sa) {
     nbig = sa.big_cap;
   for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
     ShadowTri st;
-    shadow_setup(sa, sa.big_queue[b], st);
+    if (!shadow_setup(sa, sa.big_queue[b], st)) /* succeeded when queued; uniform */
+      continue;
     int nrows = st.es.max_y - st.es.min_y + 1;
     if (threadIdx.x == 0) {
       carry[0] = st.es.w_row[0];
@@ -1621,4 +1636,285 @@ __global__ void k_copy_u8(size_t n, const uint8_t *src, uint8_t *dst) {
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s) {
   size_t blocks = (n + WG * 16 - 1) / (WG * 16);
   hipLaunchKernelGGL(k_copy_u8, dim3((unsigned)blocks), dim3(WG), 0, s, n, src, dst);
+}
+
+/* --------------------------------------------------------------------- */
+/* ordered path: draws whose result depends on submission order          */
+/* (blending, depth write off -- the core's instanced draws --, opacity  */
+/* below 1).  Screen triangles are binned to 16x16 tiles (count, scan,   */
+/* fill), each tile's list is sorted by submission index in LDS, and one */
+/* workgroup per tile replays the CPU's per-triangle loop with the tile's */
+/* colour, depth and object-ID held in LDS (rasterizer.c:1926-2140,      */
+/* cpu_backend.c:539-545,622-628 for depth_write = false).               */
+/* --------------------------------------------------------------------- */
+
+#define OT 16
+#define OT_CAP 4096
+
+DEV bool tile_span(const EdgeSetup &es, int &tx0, int &ty0, int &tx1, int &ty1) {
+  tx0 = es.min_x / OT;
+  ty0 = es.min_y / OT;
+  tx1 = es.max_x / OT;
+  ty1 = es.max_y / OT;
+  return true;
+}
+
+__global__ __launch_bounds__(WG) void k_bin_count(FrameArgs fa, uint32_t *tile_counts) {
+  uint32_t total = fa.counters[CNT_SUBTRI];
+  if (total > fa.subtri_cap)
+    total = fa.subtri_cap;
+  for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
+    SubTri st = fa.subtris[s];
+    const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
+    EdgeSetup es;
+    if (!edge_setup(st.sx, st.sy, vw.w, vw.h, es))
+      continue;
+    int tx0, ty0, tx1, ty1;
+    tile_span(es, tx0, ty0, tx1, ty1);
+    int tiles_x = (vw.w + OT - 1) / OT;
+    for (int ty = ty0; ty <= ty1; ty++)
+      for (int tx = tx0; tx <= tx1; tx++)
+        atomicAdd(&tile_counts[vw.tile_base + (uint32_t)(ty * tiles_x + tx)], 1u);
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_bin_fill(FrameArgs fa, const uint32_t *tile_start,
+                                                 uint32_t *tile_cursor, uint32_t *bins,
+                                                 uint32_t bin_cap) {
+  uint32_t total = fa.counters[CNT_SUBTRI];
+  if (total > fa.subtri_cap)
+    total = fa.subtri_cap;
+  for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
+    SubTri st = fa.subtris[s];
+    const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
+    EdgeSetup es;
+    if (!edge_setup(st.sx, st.sy, vw.w, vw.h, es))
+      continue;
+    int tx0, ty0, tx1, ty1;
+    tile_span(es, tx0, ty0, tx1, ty1);
+    int tiles_x = (vw.w + OT - 1) / OT;
+    for (int ty = ty0; ty <= ty1; ty++)
+      for (int tx = tx0; tx <= tx1; tx++) {
+        uint32_t t = vw.tile_base + (uint32_t)(ty * tiles_x + tx);
+        uint32_t pos = tile_start[t] + atomicAdd(&tile_cursor[t], 1u);
+        if (pos < bin_cap)
+          bins[pos] = s;
+        else
+          atomicOr(&fa.counters[CNT_OVERFLOW], 16u);
+      }
+  }
+}
+
+/* in-LDS bitonic sort of n (<= OT_CAP) keys, ascending; pads with ~0 */
+DEV void lds_sort(uint32_t *k, int n) {
+  int m = 1;
+  while (m < n)
+    m <<= 1;
+  for (int i = n + (int)threadIdx.x; i < m; i += WG)
+    k[i] = 0xffffffffu;
+  __syncthreads();
+  for (int size = 2; size <= m; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = threadIdx.x; i < m; i += WG) {
+        int j = i ^ stride;
+        if (j > i) {
+          bool up = ((i & size) == 0);
+          uint32_t a = k[i], b = k[j];
+          if ((a > b) == up) {
+            k[i] = b;
+            k[j] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct OrderedLds {
+  float4 hdr[OT * OT];
+  float depth[OT * OT];
+  float saved[OT * OT];
+  uint32_t ids[OT * OT];
+  uint32_t list[OT_CAP];
+  uint32_t next_min;
+};
+
+/* One screen triangle against this lane's pixel, the way one iteration of
+ * the smooth multi-light loop treats it (rasterizer.c:1929-2140). */
+DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &st,
+                          const RasterTri &r, int px, int py, int li, OrderedLds &L) {
+  float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+  for (int y = r.es.min_y; y < py; y++) {
+    w0r += r.es.e_dy[0];
+    w1r += r.es.e_dy[1];
+    w2r += r.es.e_dy[2];
+  }
+  float w0 = w0r, w1 = w1r, w2 = w2r;
+  for (int x = r.es.min_x; x < px; x++) {
+    w0 += r.es.e_dx[0];
+    w1 += r.es.e_dx[1];
+    w2 += r.es.e_dx[2];
+  }
+  if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f))
+    return;
+  float d0 = w0 * r.inv_e[0], d1 = w1 * r.inv_e[1], d2 = w2 * r.inv_e[2];
+  float min_d = d0;
+  if (d1 < min_d)
+    min_d = d1;
+  if (d2 < min_d)
+    min_d = d2;
+  if (!(min_d >= 0.0f))
+    return;
+  const DrawGPU &d = frame.draws[st.draw];
+  float b0 = w0 * r.es.inv_area, b1 = w1 * r.es.inv_area, b2 = w2 * r.es.inv_area;
+  float zt = b0 * r.sz[0] + b1 * r.sz[1] + b2 * r.sz[2];
+  bool depth_test = (d.flags & MOPH_DRAW_DEPTH_TEST) != 0;
+  if (depth_test && !(zt < L.depth[li]))
+    return;
+  float z, pr, pg, pb;
+  shade_fragment(frame, vw, st, r, w0, w1, w2, z, pr, pg, pb);
+  float op = d.opacity < 0.0f ? 0.0f : (d.opacity > 1.0f ? 1.0f : d.opacity);
+  int blend = (int)((d.flags >> MOPH_DRAW_BLEND_SHIFT) & 3u);
+  float cov = (min_d < 1.0f && blend != MOP_HIP_BLEND_OPAQUE) ? min_d : 1.0f;
+  float fa_ = op * cov;
+  if (blend == MOP_HIP_BLEND_OPAQUE && fa_ >= 1.0f) {
+    L.hdr[li] = make_float4(pr, pg, pb, 1.0f);
+    L.depth[li] = z;
+    L.ids[li] = d.object_id;
+  } else {
+    float inv_fa = 1.0f - fa_;
+    float4 dc = L.hdr[li];
+    if (blend == MOP_HIP_BLEND_ADDITIVE) {
+      pr = dc.x + pr * fa_;
+      pg = dc.y + pg * fa_;
+      pb = dc.z + pb * fa_;
+    } else if (blend == MOP_HIP_BLEND_MULTIPLY) {
+      pr = dc.x * pr;
+      pg = dc.y * pg;
+      pb = dc.z * pb;
+    } else {
+      pr = pr * fa_ + dc.x * inv_fa;
+      pg = pg * fa_ + dc.y * inv_fa;
+      pb = pb * fa_ + dc.z * inv_fa;
+    }
+    L.hdr[li] = make_float4(pr, pg, pb, 1.0f);
+    if (fa_ > 0.5f) {
+      L.depth[li] = z;
+      L.ids[li] = d.object_id;
+    }
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint32_t *tile_start,
+                                                     const uint32_t *bins) {
+  __shared__ OrderedLds L;
+  uint32_t vi = find_view_by_tile(frame.views, frame.nviews, blockIdx.x);
+  const ViewGPU &vw = frame.views[vi];
+  uint32_t t = blockIdx.x - vw.tile_base;
+  uint32_t tiles_x = ((uint32_t)vw.w + OT - 1) / OT;
+  int px = (int)(t % tiles_x) * OT + (int)(threadIdx.x % OT);
+  int py = (int)(t / tiles_x) * OT + (int)(threadIdx.x / OT);
+  bool inside = px < vw.w && py < vw.h;
+  int li = threadIdx.x;
+  size_t idx = (size_t)py * (size_t)vw.w + (size_t)px;
+  uint32_t b0 = tile_start[blockIdx.x], n = tile_start[blockIdx.x + 1] - b0;
+  if (n == 0)
+    return; /* uniform */
+  if (inside) {
+    L.hdr[li] = reinterpret_cast<const float4 *>(vw.hdr)[idx];
+    L.depth[li] = vw.depth[idx];
+    L.ids[li] = vw.ids[idx];
+  }
+  uint32_t cur_draw = 0xffffffffu;
+  bool saved_live = false;
+  uint32_t processed = 0, last = 0;
+  bool first = true;
+  while (processed < n) {
+    int m;
+    if (n <= OT_CAP) {
+      for (uint32_t i = threadIdx.x; i < n; i += WG)
+        L.list[i] = bins[b0 + i];
+      __syncthreads();
+      lds_sort(L.list, (int)n);
+      m = (int)n;
+    } else {
+      /* long list: take entries one at a time in submission order */
+      if (threadIdx.x == 0)
+        L.next_min = 0xffffffffu;
+      __syncthreads();
+      uint32_t mine = 0xffffffffu;
+      for (uint32_t i = threadIdx.x; i < n; i += WG) {
+        uint32_t v = bins[b0 + i];
+        if ((first || v > last) && v < mine)
+          mine = v;
+      }
+      atomicMin(&L.next_min, mine);
+      __syncthreads();
+      L.list[0] = L.next_min;
+      m = 1;
+      __syncthreads();
+    }
+    for (int e = 0; e < m; e++) {
+      uint32_t s = L.list[e];
+      uint32_t draw = frame.subtris[s].draw;
+      if (draw != cur_draw) {
+        /* end of a draw with depth_write = false: restore its depth */
+        if (saved_live && inside)
+          L.depth[li] = L.saved[li];
+        saved_live = false;
+        cur_draw = draw;
+        if (!(frame.draws[draw].flags & MOPH_DRAW_DEPTH_WRITE)) {
+          if (inside)
+            L.saved[li] = L.depth[li];
+          saved_live = true;
+        }
+      }
+      SubTri st = frame.subtris[s];
+      RasterTri r;
+      bool ok = load_raster_tri(vw.w, vw.h, st, r);
+      bool in_bbox = ok && inside && px >= r.es.min_x && px <= r.es.max_x &&
+                     py >= r.es.min_y && py <= r.es.max_y;
+      /* coverage mask of the wave: skip the triangle when no lane is in its bbox */
+      if (__ballot(in_bbox) == 0ull)
+        continue;
+      if (in_bbox)
+        ordered_fragment(frame, vw, st, r, px, py, li, L);
+    }
+    processed += (uint32_t)m;
+    last = L.list[m - 1];
+    first = false;
+    __syncthreads();
+  }
+  if (saved_live && inside)
+    L.depth[li] = L.saved[li];
+  if (inside) {
+    reinterpret_cast<float4 *>(vw.hdr)[idx] = L.hdr[li];
+    vw.depth[idx] = L.depth[li];
+    vw.ids[idx] = L.ids[li];
+  }
+}
+
+void launch_ordered(const FrameArgs &fa, uint32_t *tile_counts, uint32_t *tile_cursor,
+                    uint32_t *bins, uint32_t bin_cap, uint32_t *block_sums, hipStream_t s) {
+  uint32_t nt = fa.n_tiles;
+  if (!nt)
+    return;
+  launch_fill_u32((size_t)nt + 1, tile_counts, 0u, s);
+  launch_fill_u32(nt, tile_cursor, 0u, s);
+  uint32_t grid = (fa.subtri_cap + WG - 1) / WG;
+  if (grid > 4096)
+    grid = 4096;
+  if (grid == 0)
+    grid = 1;
+  hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(WG), 0, s, fa, tile_counts);
+  /* exclusive scan of tile counts in place (total lands in tile_counts[nt]) */
+  uint32_t nb = (nt + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(WG), 0, s, tile_counts, nt, block_sums);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(WG), 0, s, block_sums, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(WG), 0, s, tile_counts, nt, block_sums,
+                     (uint32_t *)nullptr, 0u);
+  hipLaunchKernelGGL(k_bin_fill, dim3(grid), dim3(WG), 0, s, fa, tile_counts, tile_cursor, bins,
+                     bin_cap);
+  hipLaunchKernelGGL(k_tile_ordered, dim3(nt), dim3(WG), 0, s, fa, tile_counts, bins);
 }

@@ -41,6 +41,9 @@
 enum {
   MOPH_DRAW_CULL = 1u << 0,
   MOPH_DRAW_FLEX = 1u << 1,
+  MOPH_DRAW_DEPTH_TEST = 1u << 2,
+  MOPH_DRAW_DEPTH_WRITE = 1u << 3,
+  MOPH_DRAW_BLEND_SHIFT = 4, /* 2 bits: MOP_HIP_BLEND_* */
 };
 
 /* One draw (or one instance of an instanced draw) as the kernels see it. */
@@ -67,6 +70,8 @@ struct DrawGPU {
   uint32_t light_off;
   uint32_t light_count;
   uint32_t view; /* index into the batch's views */
+  float opacity;
+  uint32_t pad1_[3];
 };
 
 /* Screen-space triangle: exactly the MopSwScreenVertex positions the CPU
@@ -83,7 +88,7 @@ enum {
   CNT_BIG = 1,      /* big screen triangles queued */
   CNT_SUBTRI = 2,   /* total screen triangles (written by the scan) */
   CNT_OVERFLOW = 3, /* bit0 clip queue, bit1 subtri store, bit2 shadow big queue,
-                       bit3 clipped polygon longer than MOPH_MAX_POLY */
+                       bit3 clipped polygon longer than MOPH_MAX_POLY, bit4 tile bins */
   CNT_SHADOW_BIG = 4,
   CNT_N = 8
 };
@@ -188,6 +193,10 @@ void launch_emit(const FrameArgs &fa, hipStream_t s);
 void launch_raster(const FrameArgs &fa, hipStream_t s);
 void launch_shade(const FrameArgs &fa, hipStream_t s);
 void launch_export_counters(const FrameArgs &fa, hipStream_t s);
+/* ordered path: bin screen triangles to 16x16 tiles, sort each tile's list
+ * by submission index, replay the CPU loop per tile in LDS */
+void launch_ordered(const FrameArgs &fa, uint32_t *tile_counts, uint32_t *tile_cursor,
+                    uint32_t *bins, uint32_t bin_cap, uint32_t *block_sums, hipStream_t s);
 void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr,
                   float *depth, uint32_t *ids, unsigned long long *vis,
                   hipStream_t s);

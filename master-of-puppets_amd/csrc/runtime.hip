@@ -228,6 +228,8 @@ struct MopHipDevice {
       big_queue;
   DevArr<SubTri> subtris;
   uint32_t clip_cap_hint = 0;
+  DevArr<uint32_t> tile_counts, tile_cursor, bins;
+  uint32_t bin_cap_hint = 0;
   uint32_t *h_counters = nullptr;
   DevArr<ViewGPU> views;
   DevArr<PostViewGPU> pviews;
@@ -315,6 +317,9 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   }
   d->views.release();
   d->pviews.release();
+  d->tile_counts.release();
+  d->tile_cursor.release();
+  d->bins.release();
   if (d->h_counters)
     (void)hipHostFree(d->h_counters);
   (void)hipEventDestroy(d->ev0);
@@ -740,15 +745,24 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
   if (c->lights && c->light_count)
     pd.lights.assign(c->lights, c->lights + c->light_count);
   g.light_count = c->light_count;
-  bool order_free = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
-                    c->light_count > 0 && c->depth_test && c->depth_write &&
+  if (c->depth_test)
+    g.flags |= MOPH_DRAW_DEPTH_TEST;
+  if (c->depth_write)
+    g.flags |= MOPH_DRAW_DEPTH_WRITE;
+  g.flags |= ((uint32_t)c->blend_mode & 3u) << MOPH_DRAW_BLEND_SHIFT;
+  g.opacity = c->opacity;
+  bool lit_smooth = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
+                    c->light_count > 0;
+  bool order_free = lit_smooth && c->depth_test && c->depth_write &&
                     c->blend_mode == MOP_HIP_BLEND_OPAQUE && c->opacity >= 1.0f;
-  /* An alpha/additive draw at opacity <= 0 changes nothing: every fragment
+  /* An alpha/additive draw at opacity <= 0 changes no colour: every fragment
    * blends as dst*1 + src*0 and writes depth/id only above alpha 0.5
-   * (rasterizer.c:2109-2140).  The core draws its hidden grid that way. */
+   * (rasterizer.c:2109-2140).  The core draws its hidden grid (unlit, so not
+   * renderable here otherwise) that way; it is dropped. */
   bool invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
                     c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
-  pd.cls = order_free ? 0 : (invisible ? 2 : 1);
+  /* 0 order-free, 3 ordered (tile path), 2 no-op, 1 not rendered */
+  pd.cls = order_free ? 0 : (lit_smooth ? 3 : (invisible ? 2 : 1));
   pd.state = current_state(d, fb);
   fb->draws.push_back(std::move(pd));
   fb->executed = false;
@@ -841,7 +855,7 @@ struct SegJob {
 
 /* One order-free segment of each job's framebuffer, all in one set of
  * launches (views are independent: own visibility buffer, own state). */
-static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs) {
+static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ordered) {
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
   std::vector<ViewGPU> views;
@@ -923,9 +937,22 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs) {
   fa.big_queue = d->big_queue.p;
   fa.lights = d->lights.p;
 
-  for (const SegJob &j : jobs)
-    if (j.seg > 0)
-      launch_vis_from_depth(j.fb->w * j.fb->h, j.fb->depth, j.fb->vis, d->stream);
+  if (!ordered)
+    for (const SegJob &j : jobs)
+      if (j.seg > 0)
+        launch_vis_from_depth(j.fb->w * j.fb->h, j.fb->depth, j.fb->vis, d->stream);
+  uint32_t bin_cap = 0;
+  if (ordered) {
+    uint64_t bc = 2ull * subtri_cap + 65536;
+    if (d->bin_cap_hint > bc)
+      bc = d->bin_cap_hint;
+    if (bc > 0xffffff00ull)
+      bc = 0xffffff00ull;
+    bin_cap = (uint32_t)bc;
+    if (!d->tile_counts.ensure((size_t)tiles + 1) || !d->tile_cursor.ensure(tiles ? tiles : 1) ||
+        !d->bins.ensure(bin_cap) || !d->block_sums.ensure(tiles / (256 * 16) + 2))
+      return false;
+  }
   launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
   if (fa.ndraws) {
     {
@@ -939,13 +966,19 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs) {
       launch_scan_counts(fa, d->block_sums.p, d->stream);
       launch_emit(fa, d->stream);
     }
-    {
-      TmScope t(d, TM_RASTER);
-      launch_raster(fa, d->stream);
-    }
-    {
+    if (ordered) {
       TmScope t(d, TM_SHADE);
-      launch_shade(fa, d->stream);
+      launch_ordered(fa, d->tile_counts.p, d->tile_cursor.p, d->bins.p, bin_cap,
+                     d->block_sums.p, d->stream);
+    } else {
+      {
+        TmScope t(d, TM_RASTER);
+        launch_raster(fa, d->stream);
+      }
+      {
+        TmScope t(d, TM_SHADE);
+        launch_shade(fa, d->stream);
+      }
     }
   }
   launch_export_counters(fa, d->stream);
@@ -1055,7 +1088,7 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
       while (b < m && fb->draws[b].cls == fb->draws[a].cls &&
              fb->draws[b].state == fb->draws[a].state)
         b++;
-      if (fb->draws[a].cls == 0)
+      if (fb->draws[a].cls == 0 || fb->draws[a].cls == 3)
         segs[i].push_back({a, b});
       else if (fb->draws[a].cls == 1)
         fb->skipped += b - a;
@@ -1073,11 +1106,16 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
               fb->skipped);
   }
   for (size_t r = 0; r < rounds; r++) {
-    std::vector<SegJob> jobs;
+    std::vector<SegJob> jobs[2];
     for (size_t i = 0; i < todo.size(); i++)
-      if (r < segs[i].size())
-        jobs.push_back({todo[i], segs[i][r].first, segs[i][r].second, (int)r});
-    run_batch(d, jobs);
+      if (r < segs[i].size()) {
+        bool ord = todo[i]->draws[segs[i][r].first].cls == 3;
+        jobs[ord ? 1 : 0].push_back({todo[i], segs[i][r].first, segs[i][r].second, (int)r});
+      }
+    if (!jobs[0].empty())
+      run_batch(d, jobs[0], false);
+    if (!jobs[1].empty())
+      run_batch(d, jobs[1], true);
   }
   run_resolve(d, todo);
   run_post(d, todo);
@@ -1114,16 +1152,20 @@ static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
     }
     if (ovf & 8u)
       set_err("a clipped polygon exceeded %d vertices and was dropped", MOPH_MAX_POLY);
-    if (ovf & 3u) {
-      /* clip queue or screen-triangle store too small: grow and re-run */
-      d->clip_cap_hint = need_clip + need_clip / 4 + 1024;
+    if (ovf & 19u) {
+      /* clip queue, screen-triangle store or tile bins too small: grow, re-run */
+      if (ovf & 3u)
+        d->clip_cap_hint = need_clip + need_clip / 4 + 1024;
+      if (ovf & 16u)
+        d->bin_cap_hint = (uint32_t)(d->bins.cap * 2 < 0xffffff00ull ? d->bins.cap * 2
+                                                                     : 0xffffff00ull);
       fb->executed = false;
       continue;
     }
     fb->checked = true;
     st.triangles_in = 0;
     for (const PendingDraw &pd : fb->draws)
-      if (pd.cls == 0)
+      if (pd.cls == 0 || pd.cls == 3)
         st.triangles_in += pd.g.tri_count;
     st.shadow_tris = d->stats.shadow_tris;
     st.gpu_ms = d->stats.gpu_ms;
@@ -1159,6 +1201,19 @@ extern "C" void mop_hip_frames_execute(MopHipDevice *d, MopHipFramebuffer *const
   if (!d || !fbs)
     return;
   execute_frames(d, fbs, n);
+}
+
+/* 1 when fb's frame has executed on the device (non-blocking), else 0. */
+extern "C" int mop_hip_frame_done(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (!d || !fb || !fb->executed)
+    return 0;
+  return hipEventQuery(fb->done) == hipSuccess ? 1 : 0;
+}
+
+/* Wait for all work queued on the device. */
+extern "C" void mop_hip_device_sync(MopHipDevice *d) {
+  if (d)
+    HIPCHK(hipStreamSynchronize(d->stream));
 }
 
 extern "C" void mop_hip_set_exposure(MopHipDevice *d, float e) {

@@ -72,6 +72,16 @@ class Mesh:
     has_material: bool = True
     metallic: float = 0.0
     roughness: float = 0.5
+    blend: int = 0                # MopBlendMode
+    opacity: float = 1.0
+
+
+@dataclass
+class InstancedMesh:
+    vertices: np.ndarray
+    indices: np.ndarray
+    object_id: int
+    transforms: np.ndarray        # (n, 16) float32 column-major
 
 
 @dataclass
@@ -93,6 +103,7 @@ class Scene:
     width: int
     height: int
     meshes: list = field(default_factory=list)
+    instanced: list = field(default_factory=list)
     lights: list = field(default_factory=list)
     eye: tuple = (0.0, 0.0, 5.0)
     target: tuple = (0.0, 0.0, 0.0)
@@ -108,7 +119,8 @@ class Scene:
 
     @property
     def triangle_count(self):
-        return sum(len(m.indices) // 3 for m in self.meshes)
+        return (sum(len(m.indices) // 3 for m in self.meshes) +
+                sum(len(m.indices) // 3 * len(m.transforms) for m in self.instanced))
 
 
 # ---------------------------------------------------------------------------
@@ -494,3 +506,25 @@ def config4(n_views=64, seed=0x4D4F5004, tri_lo=1e3, tri_hi=2e6, max_w=1920, max
         t = rng.log_uniform(tri_lo, tri_hi)
         views.append(batch_scene(rng, w, h, t, f"cfg4_view{k}_{w}x{h}"))
     return views
+
+
+def ordered_scene(seed=9, width=160, height=120, with_env=False):
+    """Draws whose result depends on submission order: transparent meshes
+    (alpha, additive, multiply; opacity below 1), drawn back to front by the
+    core's transparent pass, and an instanced mesh (the core's instanced pass
+    draws with depth writes off), over an opaque scene."""
+    sc = small_scene(seed=seed, width=width, height=height, n=6, with_env=with_env)
+    sc.name = f"ordered_{seed}_{width}x{height}"
+    rng = Xoshiro(seed * 7919 + 1)
+    for k in range(6):
+        col = (rng.uniform(), rng.uniform(), rng.uniform(), 1.0)
+        v, i = icosphere(2, col)
+        m = trs((rng.uniform(-3, 3), rng.uniform(-1, 2), rng.uniform(-3, 3)), random_quat(rng),
+                rng.uniform(0.6, 1.6))
+        sc.meshes.append(Mesh(v, i, 100 + k, m, True, 0.0, rng.uniform(0.2, 0.9),
+                              blend=1 + (k % 3), opacity=rng.uniform(0.25, 0.8)))
+    v, i = icosphere(1, (0.9, 0.9, 0.3, 1.0))
+    xf = np.stack([trs((rng.uniform(-4, 4), rng.uniform(-1, 3), rng.uniform(-4, 4)),
+                       random_quat(rng), rng.uniform(0.3, 0.8)) for _ in range(5)])
+    sc.instanced.append(InstancedMesh(v, i, 200, xf.astype(F32)))
+    return sc

@@ -1,0 +1,151 @@
+"""Serving form: a request loop over resident scenes with continuous batching.
+
+Resident state: every scene's meshes, IBL tables and shadow map stay in HBM
+(the shadow map is rendered once and reused: it depends on the scene and its
+lights, not on the camera).  Requests arrive as a Poisson stream; each picks a
+scene, an orbit delta of the scene camera and a resolution class.
+
+Admission is by HBM budget: the budget is cut into framebuffer slots per
+resolution class (48 B/px on the device); a request is admitted when a slot of
+its class is free, waits in a bounded queue otherwise, and is rejected when the
+queue is full.  Continuous batching: while up to `inflight` batches run on the
+device, newly arrived requests are recorded into free slots and launched as
+the next batch; finished batches are read back (RGBA8 to host) and their
+slots freed.  Latency = arrival to host readback.  The plain mode serves the
+same stream one request per launch, synchronously.
+"""
+import collections
+import math
+import time
+
+import numpy as np
+
+from . import driver
+
+RES_CLASSES = [((640, 480), 0.25), ((1280, 720), 0.50), ((1920, 1080), 0.25)]
+BYTES_PER_PX = 48  # colour x2, HDR, depth, id, visibility, edge values
+
+
+def make_requests(rate, duration, n_scenes, seed=0x4D4F5005):
+    rng = np.random.default_rng(seed)
+    n = int(rate * duration * 1.2) + 16
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    t = t[t < duration]
+    k = len(t)
+    scene = rng.integers(0, n_scenes, k)
+    yaw = rng.uniform(-5.0, 5.0, k)
+    pitch = rng.uniform(-5.0, 5.0, k)
+    cls = rng.choice(len(RES_CLASSES), k, p=[p for _, p in RES_CLASSES])
+    return [(float(t[i]), int(scene[i]), float(yaw[i]), float(pitch[i]), int(cls[i]))
+            for i in range(k)]
+
+
+class Server:
+    def __init__(self, device, scenes, budget_bytes=8 << 30, max_batch=32, inflight=2,
+                 max_slots_per_class=64):
+        self.d = device
+        self.scenes = scenes  # list of (SceneDriver, Scene)
+        self.max_batch = max_batch
+        self.inflight = inflight
+        self.free = []
+        self.slots = []
+        for (w, h), share in RES_CLASSES:
+            n = int(budget_bytes * share // (w * h * BYTES_PER_PX))
+            n = max(1, min(n, max_slots_per_class))
+            fbs = [device.framebuffer(w, h) for _ in range(n)]
+            self.slots.append(fbs)
+            self.free.append(list(fbs))
+        self.queue_cap = 4 * sum(len(s) for s in self.slots)
+
+    def close(self):
+        for fbs in self.slots:
+            for fb in fbs:
+                fb.close()
+
+    def _camera(self, req):
+        _, si, yaw, pitch, _ = req
+        return driver.camera_of(self.scenes[si][1], yaw, pitch)
+
+    def run(self, requests, plain=False, time_limit=None):
+        lat = []
+        rejected = 0
+        max_queue = 0
+        queue = collections.deque()
+        inflight = collections.deque()  # (t_launch, [(req, fb)])
+        i = 0
+        n = len(requests)
+        t0 = time.perf_counter()
+        done_count = 0
+        last_done = t0
+        batches = 0
+        batch_sizes = []
+        while done_count + rejected < n:
+            now = time.perf_counter() - t0
+            if time_limit and now > time_limit:
+                break
+            while i < n and requests[i][0] <= now:
+                if len(queue) >= self.queue_cap:
+                    rejected += 1
+                else:
+                    queue.append(requests[i])
+                i += 1
+            max_queue = max(max_queue, len(queue))
+            # completions
+            progressed = False
+            while inflight and all(fb.done() for _, fb in inflight[0][1]):
+                _, items = inflight.popleft()
+                for req, fb in items:
+                    fb.read_color_raw()
+                    tdone = time.perf_counter() - t0
+                    lat.append(tdone - req[0])
+                    self.free[req[4]].append(fb)
+                    done_count += 1
+                    last_done = tdone
+                progressed = True
+            # launch
+            if queue and len(inflight) < self.inflight:
+                batch = []
+                skipped = collections.deque()
+                limit = 1 if plain else self.max_batch
+                while queue and len(batch) < limit:
+                    req = queue.popleft()
+                    if self.free[req[4]]:
+                        fb = self.free[req[4]].pop()
+                        batch.append((req, fb))
+                    else:
+                        skipped.append(req)
+                        if plain:
+                            break
+                queue.extendleft(reversed(skipped))
+                if batch:
+                    for req, fb in batch:
+                        self.scenes[req[1]][0].record(fb, self._camera(req))
+                    self.d.execute([fb for _, fb in batch])
+                    inflight.append((time.perf_counter() - t0, batch))
+                    batches += 1
+                    batch_sizes.append(len(batch))
+                    progressed = True
+                    if plain:
+                        # one request per launch, synchronously
+                        for req, fb in batch:
+                            fb.read_color_raw()
+                    continue
+            if not progressed:
+                if i < n and not queue and not inflight:
+                    wait = requests[i][0] - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.002))
+                else:
+                    time.sleep(20e-6)
+        lat = np.array(lat) * 1e3
+        span = max(last_done, 1e-9)
+        return {
+            "completed": done_count,
+            "rejected": rejected,
+            "max_queue": max_queue,
+            "batches": batches,
+            "mean_batch": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
+            "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+            "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+            "sustained_fps": done_count / span,
+        }

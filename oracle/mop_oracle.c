@@ -537,12 +537,23 @@ static int cur_state(MopHipDevice *d) {
   return (int)d->nstates - 1;
 }
 
+static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
+                         const MopHipMat4 *model, const MopHipMat4 *mvp);
+static void shade_px(const MopHipDevice *dev, const ShadeState *S, const DrawRec *dr,
+                     const SubTri *t, float w0, float w1, float w2, float inv_area,
+                     float out[3]);
+
 static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
                      const MopHipMat4 *model, const MopHipMat4 *mvp) {
-  int order_free = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
-                   c->light_count > 0 && c->depth_test && c->depth_write &&
+  int lit_smooth = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
+                   c->light_count > 0;
+  int order_free = lit_smooth && c->depth_test && c->depth_write &&
                    c->blend_mode == MOP_HIP_BLEND_OPAQUE && c->opacity >= 1.0f;
   if (!order_free) {
+    if (lit_smooth) {
+      draw_ordered(d, fb, c, model, mvp);
+      return;
+    }
     int invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
                      c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
     if (!invisible)
@@ -616,6 +627,137 @@ static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCal
     }
   }
   free(vc);
+}
+
+/* Draws whose result depends on order, restated as the reference's
+ * sequential loop: per triangle, per pixel, depth test against the current
+ * buffer, blend (rasterizer.c:1926-2140); depth_write = false saves and
+ * restores the depth buffer around the draw (cpu_backend.c:539-545,622-628). */
+static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
+                         const MopHipMat4 *model, const MopHipMat4 *mvp) {
+  shade_segment(d, fb);
+  size_t npx = (size_t)fb->w * fb->h;
+  float *saved = NULL;
+  if (c->depth_test && !c->depth_write) {
+    saved = malloc(npx * sizeof(float));
+    memcpy(saved, fb->depth, npx * sizeof(float));
+  }
+  DrawRec dr;
+  dr.object_id = c->object_id;
+  dr.ambient = c->ambient;
+  dr.metallic = c->metallic;
+  dr.roughness = c->roughness;
+  dr.eye = c->cam_eye;
+  dr.nlights = c->light_count;
+  dr.lights = (MopHipLight *)c->lights;
+  dr.state = cur_state(d);
+  float op = clamp01(c->opacity);
+  uint32_t stride = c->vertex_format ? c->vertex_format->stride : sizeof(MopHipVertex);
+  uint32_t nv = c->vertex_count;
+  ClipV *vc = malloc((size_t)(nv ? nv : 1) * sizeof(ClipV));
+  for (uint32_t i = 0; i < nv; i++)
+    xform_vertex(c, model, mvp, (const uint8_t *)c->vertex_buffer->data + (size_t)i * stride,
+                 &vc[i]);
+  const uint32_t *ib = (const uint32_t *)c->index_buffer->data;
+  int cull = c->backface_cull;
+  for (uint32_t i = 0; i + 2 < c->index_count; i += 3) {
+    uint32_t i0 = ib[i], i1 = ib[i + 1], i2 = ib[i + 2];
+    if (i0 >= nv || i1 >= nv || i2 >= nv)
+      continue;
+    const float *a = vc[i0].p, *b = vc[i1].p, *cc = vc[i2].p;
+    if ((a[0] < -a[3] && b[0] < -b[3] && cc[0] < -cc[3]) ||
+        (a[0] > a[3] && b[0] > b[3] && cc[0] > cc[3]) ||
+        (a[1] < -a[3] && b[1] < -b[3] && cc[1] < -cc[3]) ||
+        (a[1] > a[3] && b[1] > b[3] && cc[1] > cc[3]) ||
+        (a[2] < -a[3] && b[2] < -b[3] && cc[2] < -cc[3]) ||
+        (a[2] > a[3] && b[2] > b[3] && cc[2] > cc[3]))
+      continue;
+    if (cull && a[3] > 0.0f && b[3] > 0.0f && cc[3] > 0.0f) {
+      float ex = b[0] * a[3] - a[0] * b[3], ey = b[1] * a[3] - a[1] * b[3];
+      float fx = cc[0] * a[3] - a[0] * cc[3], fy = cc[1] * a[3] - a[1] * cc[3];
+      if (ex * fy - ey * fx <= 0.0f)
+        continue;
+    }
+    ClipV poly[24], tmp[24];
+    int inside = a[3] > 0.0f && b[3] > 0.0f && cc[3] > 0.0f;
+    for (int k = 0; k < 3 && inside; k++) {
+      const float *q = k == 0 ? a : k == 1 ? b : cc;
+      inside = q[0] >= -q[3] && q[0] <= q[3] && q[1] >= -q[3] && q[1] <= q[3] &&
+               q[2] >= -q[3] && q[2] <= q[3];
+    }
+    poly[0] = vc[i0];
+    poly[1] = vc[i1];
+    poly[2] = vc[i2];
+    int n = inside ? 3 : clip_poly(poly, 3, tmp);
+    if (n < 3)
+      continue;
+    for (int k = 1; k < n - 1; k++) {
+      SubTri st;
+      if (!fan_tri(&poly[0], &poly[k], &poly[k + 1], cull, fb->w, fb->h, &st))
+        continue;
+      Setup sp;
+      if (!setup_tri(st.sx, st.sy, fb->w, fb->h, &sp, 1))
+        continue;
+      float r0 = sp.wr[0], r1 = sp.wr[1], r2 = sp.wr[2];
+      for (int y = sp.y0; y <= sp.y1; y++) {
+        float w0 = r0, w1 = r1, w2 = r2;
+        for (int x = sp.x0; x <= sp.x1; x++) {
+          if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f) {
+            float d0 = w0 * sp.inv_e[0], d1 = w1 * sp.inv_e[1], d2 = w2 * sp.inv_e[2];
+            float md = d0;
+            if (d1 < md) md = d1;
+            if (d2 < md) md = d2;
+            if (md >= 0.0f) {
+              float z = w0 * sp.inv_area * st.sz[0] + w1 * sp.inv_area * st.sz[1] +
+                        w2 * sp.inv_area * st.sz[2];
+              size_t idx = (size_t)y * fb->w + x;
+              if (!c->depth_test || z < fb->depth[idx]) {
+                float rgb[3];
+                shade_px(d, &d->states[dr.state], &dr, &st, w0, w1, w2, sp.inv_area, rgb);
+                float cov = (md < 1.0f && c->blend_mode != MOP_HIP_BLEND_OPAQUE) ? md : 1.0f;
+                float fa = op * cov;
+                float *hp = fb->hdr + idx * 4;
+                if (c->blend_mode == MOP_HIP_BLEND_OPAQUE && fa >= 1.0f) {
+                  hp[0] = rgb[0];
+                  hp[1] = rgb[1];
+                  hp[2] = rgb[2];
+                  hp[3] = 1.0f;
+                  fb->depth[idx] = z;
+                  fb->ids[idx] = c->object_id;
+                } else {
+                  float inv = 1.0f - fa;
+                  for (int q = 0; q < 3; q++) {
+                    if (c->blend_mode == MOP_HIP_BLEND_ADDITIVE)
+                      hp[q] = hp[q] + rgb[q] * fa;
+                    else if (c->blend_mode == MOP_HIP_BLEND_MULTIPLY)
+                      hp[q] = hp[q] * rgb[q];
+                    else
+                      hp[q] = rgb[q] * fa + hp[q] * inv;
+                  }
+                  hp[3] = 1.0f;
+                  if (fa > 0.5f) {
+                    fb->depth[idx] = z;
+                    fb->ids[idx] = c->object_id;
+                  }
+                }
+              }
+            }
+          }
+          w0 += sp.edx[0];
+          w1 += sp.edx[1];
+          w2 += sp.edx[2];
+        }
+        r0 += sp.edy[0];
+        r1 += sp.edy[1];
+        r2 += sp.edy[2];
+      }
+    }
+  }
+  free(vc);
+  if (saved) {
+    memcpy(fb->depth, saved, npx * sizeof(float));
+    free(saved);
+  }
 }
 
 void mop_oracle_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c) {

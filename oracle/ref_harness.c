@@ -281,6 +281,20 @@ MopMesh *mh_add_mesh(MopViewport *vp, const MopVertex *verts, uint32_t nv,
   return m;
 }
 
+/* Instanced mesh (one geometry, many transforms; pass_instanced,
+ * viewport.c:2602-2646).  transforms: count x 16 floats column-major. */
+int mh_add_instanced(MopViewport *vp, const MopVertex *verts, uint32_t nv,
+                     const uint32_t *idx, uint32_t ni, uint32_t object_id,
+                     const float *transforms, uint32_t count) {
+  MopMeshDesc md = {.vertices = verts,
+                    .vertex_count = nv,
+                    .indices = idx,
+                    .index_count = ni,
+                    .object_id = object_id};
+  return mop_viewport_add_instanced_mesh(vp, &md, (const MopMat4 *)transforms, count) ? 0
+                                                                                       : -1;
+}
+
 int mh_add_light(MopViewport *vp, int type, const float *pos, const float *dir,
                  const float *color, float intensity, float range,
                  float inner_cos, float outer_cos, int cast_shadows) {

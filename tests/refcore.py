@@ -42,6 +42,8 @@ def harness():
                                   C.c_int, C.c_float]
         h.mh_add_light.argtypes = [VP, C.c_int, F, F, F, C.c_float, C.c_float, C.c_float,
                                    C.c_float, C.c_int]
+        h.mh_add_instanced.argtypes = [VP, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                       C.c_uint32, C.c_void_p, C.c_uint32]
         h.mh_set_camera.argtypes = [VP, F, F, F, C.c_float, C.c_float, C.c_float]
         h.mh_set_clear_color.argtypes = [VP, F]
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
@@ -122,9 +124,17 @@ class RefViewport:
                 model = mm.ctypes.data
             ok = h.mh_add_mesh(self.vp, v.ctypes.data, len(v), i.ctypes.data, len(i),
                                m.object_id, model, int(m.has_material), m.metallic,
-                               m.roughness, 0, 1.0)
+                               m.roughness, getattr(m, "blend", 0), getattr(m, "opacity", 1.0))
             if not ok:
                 raise RuntimeError("mop_viewport_add_mesh failed")
+        for im in getattr(scene, "instanced", []):
+            v = np.ascontiguousarray(im.vertices, np.float32)
+            i = np.ascontiguousarray(im.indices, np.uint32)
+            xf = np.ascontiguousarray(im.transforms, np.float32)
+            self._keep += [v, i, xf]
+            if h.mh_add_instanced(self.vp, v.ctypes.data, len(v), i.ctypes.data, len(i),
+                                  im.object_id, xf.ctypes.data, len(xf)) != 0:
+                raise RuntimeError("mop_viewport_add_instanced_mesh failed")
         for L in scene.lights:
             h.mh_add_light(self.vp, L.type, _f3(L.position), _f3(L.direction), _f3(L.color),
                            L.intensity, L.range, L.inner_cos, L.outer_cos, int(L.cast_shadows))

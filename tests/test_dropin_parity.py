@@ -58,3 +58,12 @@ def test_config2_scaled():
 def test_config3_scaled_near_plane():
     sc = scenes.config3(320, 180, n_quads=96, n_spheres=8, shadow_size=1024)
     _check(sc)
+
+
+@pytest.mark.parametrize("seed", [9, 10])
+def test_ordered_draws_transparent_and_instanced(seed):
+    _check(scenes.ordered_scene(seed, with_env=(seed == 10)))
+
+
+def test_ordered_draws_larger_view():
+    _check(scenes.ordered_scene(12, 480, 320))

@@ -56,3 +56,11 @@ def test_background_invariants():
     assert bg.any() and (a["depth"][bg] == 1.0).all()
     fg = ~bg
     assert (a["depth"][fg] < 1.0).all() and (a["depth"][fg] >= 0.0).all()
+
+
+@pytest.mark.parametrize("seed", [9, 10, 11])
+def test_ordered_draws_transparent_and_instanced(seed):
+    # alpha/additive/multiply at opacity < 1 (transparent pass, back to front)
+    # and an instanced mesh (depth writes off): order-dependent draws
+    a, _ = _same(scenes.ordered_scene(seed, with_env=(seed == 10)))
+    assert (a["ids"] == 200).any()
