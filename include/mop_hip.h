@@ -247,6 +247,12 @@ void mop_hip_frame_request_post(MopHipDevice *dev, MopHipFramebuffer *fb, float 
                                 uint32_t effects, const MopHipFogParams *fog);
 /* Execute every recorded, not yet executed frame of fbs[0..n) as one batch. */
 void mop_hip_frames_execute(MopHipDevice *dev, MopHipFramebuffer *const *fbs, size_t n);
+/* Launch-shape tunables (results never depend on them): bbox area up to
+ * which one lane walks a screen triangle (default 64), shading tile edge
+ * 8/16/32 (default 16), workgroups of the small-triangle raster (0 = auto).
+ * Returns 0, or -1 for an invalid value. */
+int mop_hip_set_tuning(MopHipDevice *dev, uint32_t small_area, uint32_t shade_tile,
+                       uint32_t raster_grid);
 /* 1 when fb's frame has finished on the device (non-blocking). */
 int mop_hip_frame_done(MopHipDevice *dev, MopHipFramebuffer *fb);
 /* Wait until everything queued on the device has finished. */

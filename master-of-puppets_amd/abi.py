@@ -141,6 +141,7 @@ SIGS = {
     "mop_hip_frame_request_post": (None, [VP, VP, C.c_float, C.c_uint32, C.POINTER(FogParams)]),
     "mop_hip_frames_execute": (None, [VP, C.POINTER(VP), C.c_size_t]),
     "mop_hip_frame_done": (C.c_int, [VP, VP]),
+    "mop_hip_set_tuning": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32]),
     "mop_hip_device_sync": (None, [VP]),
 }
 

@@ -608,7 +608,7 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
     RasterTri r;
     if (!load_raster_tri(vw.w, vw.h, st, r))
       continue;
-    if (bbox_area(r.es) > MOPH_SMALL_AREA) {
+    if (bbox_area(r.es) > (long long)fa.small_area) {
       uint32_t q = atomicAdd(&fa.counters[CNT_BIG], 1u);
       fa.big_queue[q] = s;
       continue;
@@ -703,8 +703,9 @@ __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
 void launch_raster(const FrameArgs &fa, hipStream_t s) {
   uint32_t cap = fa.subtri_cap;
   uint32_t grid = (cap + WG - 1) / WG;
-  if (grid > 8192)
-    grid = 8192;
+  uint32_t lim = fa.raster_grid ? fa.raster_grid : 8192;
+  if (grid > lim)
+    grid = lim;
   if (grid == 0)
     grid = 1;
   hipLaunchKernelGGL(k_raster_small, dim3(grid), dim3(WG), 0, s, fa);
@@ -1114,15 +1115,28 @@ DEV uint32_t find_view_by_tile(const ViewGPU *v, uint32_t n, uint32_t t) {
   return lo;
 }
 
+DEV uint32_t find_view_by_stile(const ViewGPU *v, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (v[mid].stile_base <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 /* One workgroup per 16x16 tile of one view; tiles of all views of the batch
  * are numbered consecutively. */
-__global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
-  uint32_t vi = find_view_by_tile(frame.views, frame.nviews, blockIdx.x);
+__global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
+  uint32_t vi = find_view_by_stile(frame.views, frame.nviews, blockIdx.x);
   const ViewGPU &fa = frame.views[vi];
-  uint32_t t = blockIdx.x - fa.tile_base;
-  uint32_t tiles_x = ((uint32_t)fa.w + 15) / 16;
-  int px = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
-  int py = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  uint32_t T = frame.shade_tile;
+  uint32_t t = blockIdx.x - fa.stile_base;
+  uint32_t tiles_x = ((uint32_t)fa.w + T - 1) / T;
+  int px = (int)((t % tiles_x) * T + threadIdx.x % T);
+  int py = (int)((t / tiles_x) * T + threadIdx.x / T);
   if (px >= fa.w || py >= fa.h)
     return;
   size_t idx = (size_t)py * (size_t)fa.w + (size_t)px;
@@ -1136,7 +1150,7 @@ __global__ __launch_bounds__(WG) void k_shade(FrameArgs frame) {
   if (!load_raster_tri(fa.w, fa.h, st, r))
     return;
   float w0, w1, w2;
-  if (bbox_area(r.es) > MOPH_SMALL_AREA) {
+  if (bbox_area(r.es) > (long long)frame.small_area) {
     float4 w = fa.wbuf[idx];
     w0 = w.x;
     w1 = w.y;
@@ -1183,9 +1197,9 @@ void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
 }
 
 void launch_shade(const FrameArgs &fa, hipStream_t s) {
-  if (!fa.n_tiles)
+  if (!fa.n_stiles)
     return;
-  hipLaunchKernelGGL(k_shade, dim3(fa.n_tiles), dim3(WG), 0, s, fa);
+  hipLaunchKernelGGL(k_shade, dim3(fa.n_stiles), dim3(fa.shade_tile * fa.shade_tile), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */

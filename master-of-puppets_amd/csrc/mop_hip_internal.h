@@ -98,8 +98,8 @@ enum {
  * set of launches; a single frame is a batch of one view. */
 struct ViewGPU {
   int32_t w, h;
-  uint32_t px_base;   /* first pixel of this view in the batch's pixel space */
-  uint32_t tile_base; /* first 16x16 tile of this view in the batch's tiles */
+  uint32_t stile_base; /* first shading tile of this view in the batch */
+  uint32_t tile_base;  /* first 16x16 tile of this view in the batch's tiles */
   unsigned long long *vis;
   float4 *wbuf;
   float *hdr;
@@ -130,7 +130,11 @@ struct FrameArgs {
   uint32_t n_tris;
   uint32_t clip_cap;
   uint32_t subtri_cap;
-  uint32_t n_tiles;   /* 16x16 shading tiles over all views */
+  uint32_t n_tiles;   /* 16x16 tiles over all views (ordered path) */
+  uint32_t n_stiles;  /* shade_tile^2 tiles over all views (shading) */
+  uint32_t small_area;  /* bbox pixels up to which one lane walks a triangle */
+  uint32_t shade_tile;  /* shading tile edge: 8, 16 or 32 */
+  uint32_t raster_grid; /* workgroups of the small-triangle raster (0 = auto) */
   uint32_t pad0_;
   float4 *vpos;   /* n_verts clip positions */
   float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */

@@ -228,6 +228,8 @@ struct MopHipDevice {
       big_queue;
   DevArr<SubTri> subtris;
   uint32_t clip_cap_hint = 0;
+  /* tunables (mop_hip_set_tuning) */
+  uint32_t small_area = 64, shade_tile = 16, raster_grid = 0;
   DevArr<uint32_t> tile_counts, tile_cursor, bins;
   uint32_t bin_cap_hint = 0;
   uint32_t *h_counters = nullptr;
@@ -859,13 +861,16 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
   std::vector<ViewGPU> views;
-  uint32_t n_verts = 0, n_tris = 0, tiles = 0;
+  uint32_t n_verts = 0, n_tris = 0, tiles = 0, stiles = 0;
+  int T = (int)d->shade_tile;
   for (size_t j = 0; j < jobs.size(); j++) {
     MopHipFramebuffer *fb = jobs[j].fb;
     ViewGPU v;
     fill_view(fb, fb->states[fb->draws[jobs[j].b].state], jobs[j].seg, v);
     v.tile_base = tiles;
     tiles += view_tiles(fb->w, fb->h);
+    v.stile_base = stiles;
+    stiles += (uint32_t)(((fb->w + T - 1) / T) * ((fb->h + T - 1) / T));
     views.push_back(v);
     for (size_t i = jobs[j].b; i < jobs[j].e; i++) {
       PendingDraw &pd = fb->draws[i];
@@ -924,6 +929,10 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.clip_cap = clip_cap;
   fa.subtri_cap = subtri_cap;
   fa.n_tiles = tiles;
+  fa.n_stiles = stiles;
+  fa.small_area = d->small_area;
+  fa.shade_tile = d->shade_tile;
+  fa.raster_grid = d->raster_grid;
   fa.vpos = d->vpos.p;
   fa.vattr = d->vattr.p;
   fa.cpos = d->cpos.p;
@@ -1201,6 +1210,20 @@ extern "C" void mop_hip_frames_execute(MopHipDevice *d, MopHipFramebuffer *const
   if (!d || !fbs)
     return;
   execute_frames(d, fbs, n);
+}
+
+/* Launch-shape tunables of the frame kernels: bbox area up to which one lane
+ * walks a screen triangle, shading tile edge (8, 16 or 32 pixels) and the
+ * workgroup count of the small-triangle raster (0 = one lane per triangle).
+ * Results do not depend on them; speed does. */
+extern "C" int mop_hip_set_tuning(MopHipDevice *d, uint32_t small_area, uint32_t shade_tile,
+                                  uint32_t raster_grid) {
+  if (!d || (shade_tile != 8 && shade_tile != 16 && shade_tile != 32) || small_area < 1)
+    return -1;
+  d->small_area = small_area;
+  d->shade_tile = shade_tile;
+  d->raster_grid = raster_grid;
+  return 0;
 }
 
 /* 1 when fb's frame has executed on the device (non-blocking), else 0. */
