@@ -570,6 +570,50 @@ DEV bool load_raster_tri(int W, int H, const SubTri &st, RasterTri &r) {
   return true;
 }
 
+DEV int draw_sm(const DrawGPU &d) { return (int)((d.flags >> MOPH_DRAW_SM_SHIFT) & 3u); }
+DEV bool sm_flat(int sm) { return sm == MOPH_SM_LIT_FLAT || sm == MOPH_SM_UNLIT_FLAT; }
+
+/* Incremental depth of the flat path, z_row += z_dy per row and z += z_dx
+ * per pixel (rasterizer.c:973-976, 1019-1022). */
+struct FlatZ {
+  float row, dx, dy;
+};
+
+DEV FlatZ flat_z_setup(const RasterTri &r) {
+  float ia = r.es.inv_area;
+  FlatZ f;
+  f.dx = (r.es.e_dx[0] * r.sz[0] + r.es.e_dx[1] * r.sz[1] + r.es.e_dx[2] * r.sz[2]) * ia;
+  f.dy = (r.es.e_dy[0] * r.sz[0] + r.es.e_dy[1] * r.sz[1] + r.es.e_dy[2] * r.sz[2]) * ia;
+  f.row = (r.es.w_row[0] * r.sz[0] + r.es.w_row[1] * r.sz[1] + r.es.w_row[2] * r.sz[2]) * ia;
+  return f;
+}
+
+/* Edge values (and the flat path's z) at pixel (px, py), replaying the
+ * serial increments from the bbox corner in the reference's order. */
+DEV void replay_walk(const RasterTri &r, bool flat, const FlatZ &fz, int px, int py, float &w0,
+                     float &w1, float &w2, float &z) {
+  float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+  float zr = fz.row;
+  for (int y = r.es.min_y; y < py; y++) {
+    w0r += r.es.e_dy[0];
+    w1r += r.es.e_dy[1];
+    w2r += r.es.e_dy[2];
+    if (flat)
+      zr += fz.dy;
+  }
+  w0 = w0r;
+  w1 = w1r;
+  w2 = w2r;
+  z = zr;
+  for (int x = r.es.min_x; x < px; x++) {
+    w0 += r.es.e_dx[0];
+    w1 += r.es.e_dx[1];
+    w2 += r.es.e_dx[2];
+    if (flat)
+      z += fz.dx;
+  }
+}
+
 /* Coverage + min_d test + barycentric depth of one pixel; false if the
  * pixel is not drawn (edge test or AA distance test fails or z is NaN). */
 DEV bool frag_depth(const RasterTri &r, float w0, float w1, float w2, float &z) {
@@ -615,6 +659,29 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
     }
     unsigned long long seq = (unsigned long long)(s + 1);
     float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+    if (sm_flat(draw_sm(fa.draws[st.draw]))) {
+      /* flat path: edge test only, incremental z (rasterizer.c:1000-1026) */
+      FlatZ fz = flat_z_setup(r);
+      float zr = fz.row;
+      for (int y = r.es.min_y; y <= r.es.max_y; y++) {
+        float w0 = w0r, w1 = w1r, w2 = w2r, z = zr;
+        for (int x = r.es.min_x; x <= r.es.max_x; x++) {
+          if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z) {
+            size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
+            vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+          }
+          w0 += r.es.e_dx[0];
+          w1 += r.es.e_dx[1];
+          w2 += r.es.e_dx[2];
+          z += fz.dx;
+        }
+        w0r += r.es.e_dy[0];
+        w1r += r.es.e_dy[1];
+        w2r += r.es.e_dy[2];
+        zr += fz.dy;
+      }
+      continue;
+    }
     for (int y = r.es.min_y; y <= r.es.max_y; y++) {
       float w0 = w0r, w1 = w1r, w2 = w2r;
       for (int x = r.es.min_x; x <= r.es.max_x; x++) {
@@ -640,59 +707,78 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
  * buffer; mode 1 (after all updates) stores the winner's edge values. */
 template <int MODE>
 __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
-  __shared__ float rows[3][WG];
-  __shared__ float carry[3];
+  __shared__ float rows[4][WG];
+  __shared__ float carry[4];
   uint32_t nbig = fa.counters[CNT_BIG];
   for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
     uint32_t s = fa.big_queue[b];
     SubTri st = fa.subtris[s];
-    const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
+    const DrawGPU &dr = fa.draws[st.draw];
+    const ViewGPU &vw = fa.views[dr.view];
     RasterTri r;
     if (!load_raster_tri(vw.w, vw.h, st, r)) /* succeeded when queued; uniform */
       continue;
+    bool flat = sm_flat(draw_sm(dr)); /* uniform */
+    FlatZ fz = {0.0f, 0.0f, 0.0f};
+    if (flat)
+      fz = flat_z_setup(r);
     unsigned long long seq = (unsigned long long)(s + 1);
     int nrows = r.es.max_y - r.es.min_y + 1;
     if (threadIdx.x == 0) {
       carry[0] = r.es.w_row[0];
       carry[1] = r.es.w_row[1];
       carry[2] = r.es.w_row[2];
+      carry[3] = fz.row;
     }
     for (int c0 = 0; c0 < nrows; c0 += WG) {
       __syncthreads();
       if (threadIdx.x == 0) {
-        float a0 = carry[0], a1 = carry[1], a2 = carry[2];
+        float a0 = carry[0], a1 = carry[1], a2 = carry[2], a3 = carry[3];
         int lim = nrows - c0 < WG ? nrows - c0 : WG;
         for (int k = 0; k < lim; k++) {
           rows[0][k] = a0;
           rows[1][k] = a1;
           rows[2][k] = a2;
+          rows[3][k] = a3;
           a0 += r.es.e_dy[0];
           a1 += r.es.e_dy[1];
           a2 += r.es.e_dy[2];
+          if (flat)
+            a3 += fz.dy;
         }
         carry[0] = a0;
         carry[1] = a1;
         carry[2] = a2;
+        carry[3] = a3;
       }
       __syncthreads();
       int row = c0 + (int)threadIdx.x;
       if (row < nrows) {
         int y = r.es.min_y + row;
         float w0 = rows[0][threadIdx.x], w1 = rows[1][threadIdx.x], w2 = rows[2][threadIdx.x];
+        float zf = rows[3][threadIdx.x];
         size_t rowbase = (size_t)y * (size_t)vw.w;
         for (int x = r.es.min_x; x <= r.es.max_x; x++) {
           float z;
-          if (frag_depth(r, w0, w1, w2, z)) {
+          bool hit;
+          if (flat) {
+            z = zf;
+            hit = w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z;
+          } else {
+            hit = frag_depth(r, w0, w1, w2, z);
+          }
+          if (hit) {
             unsigned long long key = ((unsigned long long)depth_key(z) << 32) | seq;
             if (MODE == 0) {
               vis_update(vw.vis, rowbase + x, key);
             } else if (vw.vis[rowbase + x] == key) {
-              vw.wbuf[rowbase + x] = make_float4(w0, w1, w2, 0.0f);
+              vw.wbuf[rowbase + x] = make_float4(w0, w1, w2, z);
             }
           }
           w0 += r.es.e_dx[0];
           w1 += r.es.e_dx[1];
           w2 += r.es.e_dx[2];
+          zf += fz.dx;
         }
       }
     }
@@ -1103,6 +1189,161 @@ DEV void shade_fragment(const FrameArgs &frame, const ViewGPU &fa, const SubTri 
     pb = 0.0f;
 }
 
+DEV float clamp01f(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
+
+/* compute_multi_light (rasterizer.c:1498-1589): scalar intensity of the flat
+ * path at one point; spot lights fall off as 1/(1+d^2). */
+DEV float multi_light_scalar(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d,
+                             V3 normal, V3 wp) {
+  float total = d.ambient;
+  const MopHipLight *L = frame.lights + d.light_off;
+  for (uint32_t i = 0; i < d.light_count; i++) {
+    const MopHipLight &l = L[i];
+    if (!l.active)
+      continue;
+    float ndotl = 0.0f, att = 1.0f, spot = 1.0f;
+    if (l.type == MOP_HIP_LIGHT_DIRECTIONAL) {
+      V3 dir = v3norm(light_dir(l));
+      ndotl = -v3dot(normal, dir);
+    } else if (l.type == MOP_HIP_LIGHT_POINT || l.type == MOP_HIP_LIGHT_SPOT) {
+      V3 to_light = v3sub(light_pos(l), wp);
+      float dist = v3len(to_light);
+      if (dist < 1e-6f)
+        dist = 1e-6f;
+      V3 dir = v3scale(to_light, 1.0f / dist);
+      ndotl = v3dot(normal, dir);
+      if (l.type == MOP_HIP_LIGHT_SPOT) {
+        V3 spot_dir = v3norm(light_dir(l));
+        float cos_angle = -v3dot(dir, spot_dir);
+        if (cos_angle < l.spot_outer_cos) {
+          spot = 0.0f;
+        } else if (cos_angle < l.spot_inner_cos) {
+          float range_s = l.spot_inner_cos - l.spot_outer_cos;
+          if (range_s > 1e-6f) {
+            float t = (cos_angle - l.spot_outer_cos) / range_s;
+            spot = t * t * (3.0f - 2.0f * t);
+          }
+        }
+        if (l.range > 0.0f) {
+          float r = dist / l.range;
+          att = 1.0f - r;
+          if (att < 0.0f)
+            att = 0.0f;
+          att *= att;
+        } else {
+          att = 1.0f / (1.0f + dist * dist);
+        }
+      } else {
+        att = atten(l, dist);
+      }
+    }
+    if (ndotl < 0.0f)
+      ndotl = 0.0f;
+    float shadow = 1.0f;
+    if (l.type == MOP_HIP_LIGHT_DIRECTIONAL && fa.shadow_on)
+      shadow = shadow_pcf(fa, wp);
+    total += ndotl * l.intensity * att * spot * shadow;
+  }
+  return clamp01f(total);
+}
+
+DEV V3 draw_light_dir(const DrawGPU &d) { return V3{d.light_dir[0], d.light_dir[1], d.light_dir[2]}; }
+
+/* Constant colour of a flat-shaded screen triangle: face normal from the sum
+ * of the fan vertices' normals, colour averaged; lit draws evaluate
+ * compute_multi_light at the centroid and add the metallic term
+ * (rasterizer.c:2556-2596), unlit draws use light_dir Lambert
+ * (rasterizer.c:1199-1220). */
+DEV void flat_color(const FrameArgs &frame, const ViewGPU &fa, const SubTri &st,
+                    const DrawGPU &d, int sm, float &cr, float &cg, float &cb) {
+  float4 A0[3], A1[3], A2[3];
+  load_attr(frame, st.a[0], A0[0], A1[0], A2[0]);
+  load_attr(frame, st.a[1], A0[1], A1[1], A2[1]);
+  load_attr(frame, st.a[2], A0[2], A1[2], A2[2]);
+  V3 fn = v3norm(V3{(A0[0].x + A0[1].x + A0[2].x), (A0[0].y + A0[1].y + A0[2].y),
+                    (A0[0].z + A0[1].z + A0[2].z)});
+  float avg_r = (A1[0].z + A1[1].z + A1[2].z) * (1.0f / 3.0f);
+  float avg_g = (A1[0].w + A1[1].w + A1[2].w) * (1.0f / 3.0f);
+  float avg_b = (A2[0].x + A2[1].x + A2[2].x) * (1.0f / 3.0f);
+  if (sm == MOPH_SM_LIT_FLAT) {
+    V3 wp = V3{(A0[0].w + A0[1].w + A0[2].w) / 3.0f, (A1[0].x + A1[1].x + A1[2].x) / 3.0f,
+               (A1[0].y + A1[1].y + A1[2].y) / 3.0f};
+    float lighting = multi_light_scalar(frame, fa, d, fn, wp);
+    float diffuse_scale = 1.0f - d.metallic;
+    float env_flat = d.ambient * d.metallic * 0.3f;
+    cr = avg_r * (lighting * diffuse_scale + env_flat);
+    cg = avg_g * (lighting * diffuse_scale + env_flat);
+    cb = avg_b * (lighting * diffuse_scale + env_flat);
+  } else {
+    float ndotl = v3dot(fn, v3norm(draw_light_dir(d)));
+    if (ndotl < 0.0f)
+      ndotl = 0.0f;
+    float lighting = clamp01f(d.ambient + (1.0f - d.ambient) * ndotl);
+    cr = avg_r * lighting;
+    cg = avg_g * lighting;
+    cb = avg_b * lighting;
+  }
+  if (cr < 0.0f)
+    cr = 0.0f;
+  if (cg < 0.0f)
+    cg = 0.0f;
+  if (cb < 0.0f)
+    cb = 0.0f;
+}
+
+/* Colour of one fragment of the unlit smooth path: perspective-correct
+ * normal and colour, light_dir Lambert plus the fixed Blinn-Phong term
+ * (rasterizer.c:1368-1419). */
+DEV void unlit_smooth_color(const FrameArgs &frame, const SubTri &st, const RasterTri &r,
+                            const DrawGPU &d, float w0, float w1, float w2, float &pr, float &pg,
+                            float &pb) {
+  float b0 = w0 * r.es.inv_area;
+  float b1 = w1 * r.es.inv_area;
+  float b2 = w2 * r.es.inv_area;
+  float pc0 = b0 * st.iw[0];
+  float pc1 = b1 * st.iw[1];
+  float pc2 = b2 * st.iw[2];
+  float inv_pc = 1.0f / (pc0 + pc1 + pc2);
+  pc0 *= inv_pc;
+  pc1 *= inv_pc;
+  pc2 *= inv_pc;
+  float4 A0[3], A1[3], A2[3];
+  load_attr(frame, st.a[0], A0[0], A1[0], A2[0]);
+  load_attr(frame, st.a[1], A0[1], A1[1], A2[1]);
+  load_attr(frame, st.a[2], A0[2], A1[2], A2[2]);
+  V3 n = V3{pc0 * A0[0].x + pc1 * A0[1].x + pc2 * A0[2].x,
+            pc0 * A0[0].y + pc1 * A0[1].y + pc2 * A0[2].y,
+            pc0 * A0[0].z + pc1 * A0[1].z + pc2 * A0[2].z};
+  n = v3norm(n);
+  float fr = pc0 * A1[0].z + pc1 * A1[1].z + pc2 * A1[2].z;
+  float fg = pc0 * A1[0].w + pc1 * A1[1].w + pc2 * A1[2].w;
+  float fb_ = pc0 * A2[0].x + pc1 * A2[1].x + pc2 * A2[2].x;
+  V3 nl = v3norm(draw_light_dir(d));
+  float ndotl = v3dot(n, nl);
+  if (ndotl < 0.0f)
+    ndotl = 0.0f;
+  float lit = d.ambient + (1.0f - d.ambient) * ndotl;
+  V3 half_v = v3norm(V3{nl.x, nl.y + 1.0f, nl.z});
+  float ndoth = v3dot(n, half_v);
+  if (ndoth < 0.0f)
+    ndoth = 0.0f;
+  float spec = ndoth * ndoth;
+  spec *= spec;
+  spec *= spec;
+  spec *= spec;
+  spec *= spec;
+  spec *= 0.25f;
+  pr = fr * lit + spec;
+  pg = fg * lit + spec;
+  pb = fb_ * lit + spec;
+  if (pr < 0.0f)
+    pr = 0.0f;
+  if (pg < 0.0f)
+    pg = 0.0f;
+  if (pb < 0.0f)
+    pb = 0.0f;
+}
+
 DEV uint32_t find_view_by_tile(const ViewGPU *v, uint32_t n, uint32_t t) {
   uint32_t lo = 0, hi = n - 1;
   while (lo < hi) {
@@ -1149,32 +1390,30 @@ __global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
   RasterTri r;
   if (!load_raster_tri(fa.w, fa.h, st, r))
     return;
-  float w0, w1, w2;
+  const DrawGPU &d = frame.draws[st.draw];
+  int sm = draw_sm(d);
+  bool flat = sm_flat(sm);
+  float w0, w1, w2, zf;
   if (bbox_area(r.es) > (long long)frame.small_area) {
     float4 w = fa.wbuf[idx];
     w0 = w.x;
     w1 = w.y;
     w2 = w.z;
+    zf = w.w;
   } else {
     /* replay the reference's incremental walk up to this pixel */
-    float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
-    for (int y = r.es.min_y; y < py; y++) {
-      w0r += r.es.e_dy[0];
-      w1r += r.es.e_dy[1];
-      w2r += r.es.e_dy[2];
-    }
-    w0 = w0r;
-    w1 = w1r;
-    w2 = w2r;
-    for (int x = r.es.min_x; x < px; x++) {
-      w0 += r.es.e_dx[0];
-      w1 += r.es.e_dx[1];
-      w2 += r.es.e_dx[2];
-    }
+    FlatZ fz = {0.0f, 0.0f, 0.0f};
+    if (flat)
+      fz = flat_z_setup(r);
+    replay_walk(r, flat, fz, px, py, w0, w1, w2, zf);
   }
   float z, pr, pg, pb;
-  shade_fragment(frame, fa, st, r, w0, w1, w2, z, pr, pg, pb);
-  const DrawGPU &d = frame.draws[st.draw];
+  if (flat) {
+    z = zf;
+    flat_color(frame, fa, st, d, sm, pr, pg, pb);
+  } else {
+    shade_fragment(frame, fa, st, r, w0, w1, w2, z, pr, pg, pb);
+  }
   reinterpret_cast<float4 *>(fa.hdr)[idx] = make_float4(pr, pg, pb, 1.0f);
   fa.depth[idx] = z;
   fa.ids[idx] = d.object_id;
@@ -1758,20 +1997,45 @@ struct OrderedLds {
  * the smooth multi-light loop treats it (rasterizer.c:1929-2140). */
 DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &st,
                           const RasterTri &r, int px, int py, int li, OrderedLds &L) {
-  float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
-  for (int y = r.es.min_y; y < py; y++) {
-    w0r += r.es.e_dy[0];
-    w1r += r.es.e_dy[1];
-    w2r += r.es.e_dy[2];
-  }
-  float w0 = w0r, w1 = w1r, w2 = w2r;
-  for (int x = r.es.min_x; x < px; x++) {
-    w0 += r.es.e_dx[0];
-    w1 += r.es.e_dx[1];
-    w2 += r.es.e_dx[2];
-  }
+  const DrawGPU &d = frame.draws[st.draw];
+  int sm = draw_sm(d);
+  bool flat = sm_flat(sm);
+  FlatZ fz = {0.0f, 0.0f, 0.0f};
+  if (flat)
+    fz = flat_z_setup(r);
+  float w0, w1, w2, zf;
+  replay_walk(r, flat, fz, px, py, w0, w1, w2, zf);
   if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f))
     return;
+  bool depth_test = (d.flags & MOPH_DRAW_DEPTH_TEST) != 0;
+  int blend = (int)((d.flags >> MOPH_DRAW_BLEND_SHIFT) & 3u);
+  if (flat) {
+    /* rasterize_filled_triangle (rasterizer.c:1000-1085): the blended
+     * branch writes colour only, never depth or id */
+    if (depth_test && !(zf < L.depth[li]))
+      return;
+    float cr, cg, cb;
+    flat_color(frame, vw, st, d, sm, cr, cg, cb);
+    float ca = clamp01f(d.opacity);
+    if (sm == MOPH_SM_UNLIT_FLAT)
+      ca = (float)cvt_u8(ca * 255.0f) / 255.0f;
+    if (blend == MOP_HIP_BLEND_OPAQUE && ca >= 1.0f) {
+      L.hdr[li] = make_float4(cr, cg, cb, 1.0f);
+      L.depth[li] = zf;
+      L.ids[li] = d.object_id;
+    } else {
+      float4 dc = L.hdr[li];
+      float inv_a = 1.0f - ca;
+      if (blend == MOP_HIP_BLEND_ADDITIVE)
+        L.hdr[li] = make_float4(dc.x + cr * ca, dc.y + cg * ca, dc.z + cb * ca, 1.0f);
+      else if (blend == MOP_HIP_BLEND_MULTIPLY)
+        L.hdr[li] = make_float4(dc.x * cr, dc.y * cg, dc.z * cb, 1.0f);
+      else
+        L.hdr[li] = make_float4(cr * ca + dc.x * inv_a, cg * ca + dc.y * inv_a,
+                                cb * ca + dc.z * inv_a, 1.0f);
+    }
+    return;
+  }
   float d0 = w0 * r.inv_e[0], d1 = w1 * r.inv_e[1], d2 = w2 * r.inv_e[2];
   float min_d = d0;
   if (d1 < min_d)
@@ -1780,17 +2044,23 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
     min_d = d2;
   if (!(min_d >= 0.0f))
     return;
-  const DrawGPU &d = frame.draws[st.draw];
   float b0 = w0 * r.es.inv_area, b1 = w1 * r.es.inv_area, b2 = w2 * r.es.inv_area;
   float zt = b0 * r.sz[0] + b1 * r.sz[1] + b2 * r.sz[2];
-  bool depth_test = (d.flags & MOPH_DRAW_DEPTH_TEST) != 0;
   if (depth_test && !(zt < L.depth[li]))
     return;
-  float z, pr, pg, pb;
-  shade_fragment(frame, vw, st, r, w0, w1, w2, z, pr, pg, pb);
+  float z, pr, pg, pb, cov;
   float op = d.opacity < 0.0f ? 0.0f : (d.opacity > 1.0f ? 1.0f : d.opacity);
-  int blend = (int)((d.flags >> MOPH_DRAW_BLEND_SHIFT) & 3u);
-  float cov = (min_d < 1.0f && blend != MOP_HIP_BLEND_OPAQUE) ? min_d : 1.0f;
+  if (sm == MOPH_SM_UNLIT_SMOOTH) {
+    /* silhouette AA against the depth already in the buffer
+     * (rasterizer.c:1421-1426) */
+    z = zt;
+    unlit_smooth_color(frame, st, r, d, w0, w1, w2, pr, pg, pb);
+    bool silhouette = (L.depth[li] - z) > 0.005f;
+    cov = (min_d < 1.0f && silhouette) ? min_d : 1.0f;
+  } else {
+    shade_fragment(frame, vw, st, r, w0, w1, w2, z, pr, pg, pb);
+    cov = (min_d < 1.0f && blend != MOP_HIP_BLEND_OPAQUE) ? min_d : 1.0f;
+  }
   float fa_ = op * cov;
   if (blend == MOP_HIP_BLEND_OPAQUE && fa_ >= 1.0f) {
     L.hdr[li] = make_float4(pr, pg, pb, 1.0f);

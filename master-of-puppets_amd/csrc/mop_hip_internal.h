@@ -44,6 +44,16 @@ enum {
   MOPH_DRAW_DEPTH_TEST = 1u << 2,
   MOPH_DRAW_DEPTH_WRITE = 1u << 3,
   MOPH_DRAW_BLEND_SHIFT = 4, /* 2 bits: MOP_HIP_BLEND_* */
+  MOPH_DRAW_SM_SHIFT = 6,    /* 2 bits: MOPH_SM_* */
+};
+
+/* Shading model of a draw, as cpu_backend.c:608-619 and
+ * rasterizer.c:2540-2610 / 1158-1243 dispatch it. */
+enum {
+  MOPH_SM_LIT_SMOOTH = 0,   /* lights, smooth: mop_sw_rasterize_triangle_smooth_ml */
+  MOPH_SM_LIT_FLAT = 1,     /* lights, flat: compute_multi_light + filled triangle */
+  MOPH_SM_UNLIT_FLAT = 2,   /* no lights, flat: light_dir Lambert + filled triangle */
+  MOPH_SM_UNLIT_SMOOTH = 3, /* no lights, smooth: mop_sw_rasterize_triangle_smooth */
 };
 
 /* One draw (or one instance of an instanced draw) as the kernels see it. */
@@ -71,7 +81,7 @@ struct DrawGPU {
   uint32_t light_count;
   uint32_t view; /* index into the batch's views */
   float opacity;
-  uint32_t pad1_[3];
+  float light_dir[3]; /* the call's light_dir (unlit models) */
 };
 
 /* Screen-space triangle: exactly the MopSwScreenVertex positions the CPU

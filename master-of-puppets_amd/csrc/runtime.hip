@@ -753,18 +753,30 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
     g.flags |= MOPH_DRAW_DEPTH_WRITE;
   g.flags |= ((uint32_t)c->blend_mode & 3u) << MOPH_DRAW_BLEND_SHIFT;
   g.opacity = c->opacity;
-  bool lit_smooth = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
-                    c->light_count > 0;
-  bool order_free = lit_smooth && c->depth_test && c->depth_write &&
-                    c->blend_mode == MOP_HIP_BLEND_OPAQUE && c->opacity >= 1.0f;
+  g.light_dir[0] = c->light_dir.x;
+  g.light_dir[1] = c->light_dir.y;
+  g.light_dir[2] = c->light_dir.z;
+  /* shading model: lit draws go through mop_sw_rasterize_triangle_full, the
+   * rest through mop_sw_rasterize_triangle (cpu_backend.c:608-619) */
+  bool lit = c->lights && c->light_count > 0;
+  bool smooth = c->shading_mode == MOP_HIP_SHADING_SMOOTH;
+  int sm = lit ? (smooth ? MOPH_SM_LIT_SMOOTH : MOPH_SM_LIT_FLAT)
+               : (smooth ? MOPH_SM_UNLIT_SMOOTH : MOPH_SM_UNLIT_FLAT);
+  g.flags |= (uint32_t)sm << MOPH_DRAW_SM_SHIFT;
+  /* Order-free: opaque writes under a strict depth test, so the result is
+   * the (depth, submission) minimum.  The unlit smooth model blends at
+   * silhouettes against the depth already drawn, so it is always ordered. */
+  bool order_free = !c->wireframe && sm != MOPH_SM_UNLIT_SMOOTH && c->depth_test &&
+                    c->depth_write && c->blend_mode == MOP_HIP_BLEND_OPAQUE &&
+                    c->opacity >= 1.0f;
   /* An alpha/additive draw at opacity <= 0 changes no colour: every fragment
    * blends as dst*1 + src*0 and writes depth/id only above alpha 0.5
-   * (rasterizer.c:2109-2140).  The core draws its hidden grid (unlit, so not
-   * renderable here otherwise) that way; it is dropped. */
+   * (rasterizer.c:1080-1083, 1453-1470, 2109-2140).  The core draws its
+   * hidden grid that way; it is dropped. */
   bool invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
                     c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
-  /* 0 order-free, 3 ordered (tile path), 2 no-op, 1 not rendered */
-  pd.cls = order_free ? 0 : (lit_smooth ? 3 : (invisible ? 2 : 1));
+  /* 0 order-free, 3 ordered (tile path), 2 no-op, 1 not rendered (wireframe) */
+  pd.cls = order_free ? 0 : (invisible ? 2 : (c->wireframe ? 1 : 3));
   pd.state = current_state(d, fb);
   fb->draws.push_back(std::move(pd));
   fb->executed = false;

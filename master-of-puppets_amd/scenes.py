@@ -74,6 +74,7 @@ class Mesh:
     roughness: float = 0.5
     blend: int = 0                # MopBlendMode
     opacity: float = 1.0
+    shading: int = -1             # MopShadingMode override (0 flat, 1 smooth), -1 none
 
 
 @dataclass
@@ -116,6 +117,9 @@ class Scene:
     env_intensity: float = 1.0
     shadow_size: int = 1024
     post_effects: int = 1 | 16     # MOP_POST_GAMMA | MOP_POST_FXAA (viewport.c:646)
+    shading: int = 1              # viewport shading mode (viewport.c:645: smooth)
+    light_dir: tuple = (0.3, 1.0, 0.5)  # viewport light_dir (viewport.c:643)
+    ambient: float = None         # None keeps the viewport default (0.25)
 
     @property
     def triangle_count(self):
@@ -527,4 +531,32 @@ def ordered_scene(seed=9, width=160, height=120, with_env=False):
     xf = np.stack([trs((rng.uniform(-4, 4), rng.uniform(-1, 3), rng.uniform(-4, 4)),
                        random_quat(rng), rng.uniform(0.3, 0.8)) for _ in range(5)])
     sc.instanced.append(InstancedMesh(v, i, 200, xf.astype(F32)))
+    return sc
+
+
+def shading_scene(seed=21, width=160, height=120, lit=True, with_shadow=True):
+    """Every shading model the CPU backend dispatches: flat and smooth meshes,
+    opaque and blended, with lights (compute_multi_light / smooth_ml) or
+    without (light_dir Lambert and the legacy smooth path with silhouette
+    AA).  Some meshes override the viewport's shading mode per mesh."""
+    sc = small_scene(seed=seed, width=width, height=height, n=5, with_shadow=with_shadow)
+    sc.name = f"shading_{'lit' if lit else 'unlit'}_{seed}_{width}x{height}"
+    rng = Xoshiro(seed * 104729 + 3)
+    if not lit:
+        sc.lights = []
+        sc.light_dir = (rng.uniform(-1, 1), 1.0, rng.uniform(-1, 1))
+        sc.ambient = 0.2
+    sc.shading = 0  # viewport default flat; some meshes override to smooth
+    for k, m in enumerate(sc.meshes):
+        m.shading = 1 if k % 3 == 2 else -1
+    for k in range(5):
+        col = (rng.uniform(), rng.uniform(), rng.uniform(), 1.0)
+        v, i = icosphere(2, col)
+        m = trs((rng.uniform(-3, 3), rng.uniform(-1, 2), rng.uniform(-3, 3)), random_quat(rng),
+                rng.uniform(0.6, 1.6))
+        blend = [0, 1, 2, 3, 0][k]
+        sc.meshes.append(Mesh(v, i, 300 + k, m, True, rng.uniform(0.0, 1.0),
+                              rng.uniform(0.2, 0.9), blend=blend,
+                              opacity=1.0 if k == 0 else rng.uniform(0.3, 0.8),
+                              shading=[0, 0, 1, 0, 1][k]))
     return sc

@@ -93,8 +93,13 @@ typedef struct SubTri {
   uint32_t draw;
 } SubTri;
 
+/* shading models (cpu_backend.c:608-619; rasterizer.c:2540-2610, 1158-1243) */
+enum { SM_LIT_SMOOTH = 0, SM_LIT_FLAT = 1, SM_UNLIT_FLAT = 2, SM_UNLIT_SMOOTH = 3 };
+
 typedef struct DrawRec {
   uint32_t object_id;
+  int sm;
+  V3 light_dir;
   float ambient, metallic, roughness;
   V3 eye;
   MopHipLight *lights;
@@ -469,6 +474,43 @@ static int setup_tri(const float *sx, const float *sy, int W, int H, Setup *s, i
   return 1;
 }
 
+/* flat path depth increments (rasterizer.c:973-976) */
+static void flat_z(const Setup *s, const SubTri *t, float *zr, float *zdx, float *zdy) {
+  *zdx = (s->edx[0] * t->sz[0] + s->edx[1] * t->sz[1] + s->edx[2] * t->sz[2]) * s->inv_area;
+  *zdy = (s->edy[0] * t->sz[0] + s->edy[1] * t->sz[1] + s->edy[2] * t->sz[2]) * s->inv_area;
+  *zr = (s->wr[0] * t->sz[0] + s->wr[1] * t->sz[1] + s->wr[2] * t->sz[2]) * s->inv_area;
+}
+
+/* rasterize_filled_triangle's opaque loop (rasterizer.c:1000-1026) as a
+ * visibility update: edge test only, incremental z. */
+static void walk_vis_flat(MopHipFramebuffer *fb, const SubTri *t, uint64_t seq) {
+  Setup s;
+  if (!setup_tri(t->sx, t->sy, fb->w, fb->h, &s, 0))
+    return;
+  float zr, zdx, zdy;
+  flat_z(&s, t, &zr, &zdx, &zdy);
+  float r0 = s.wr[0], r1 = s.wr[1], r2 = s.wr[2];
+  for (int y = s.y0; y <= s.y1; y++) {
+    float w0 = r0, w1 = r1, w2 = r2, z = zr;
+    for (int x = s.x0; x <= s.x1; x++) {
+      if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z) {
+        uint64_t key = ((uint64_t)dkey(z) << 32) | seq;
+        size_t i = (size_t)y * fb->w + x;
+        if (key < fb->vis[i])
+          fb->vis[i] = key;
+      }
+      w0 += s.edx[0];
+      w1 += s.edx[1];
+      w2 += s.edx[2];
+      z += zdx;
+    }
+    r0 += s.edy[0];
+    r1 += s.edy[1];
+    r2 += s.edy[2];
+    zr += zdy;
+  }
+}
+
 static void walk_vis(MopHipFramebuffer *fb, const SubTri *t, uint64_t seq) {
   Setup s;
   if (!setup_tri(t->sx, t->sy, fb->w, fb->h, &s, 1))
@@ -538,26 +580,31 @@ static int cur_state(MopHipDevice *d) {
 }
 
 static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
-                         const MopHipMat4 *model, const MopHipMat4 *mvp);
+                         const MopHipMat4 *model, const MopHipMat4 *mvp, int sm);
+static void flat_color(const ShadeState *S, const DrawRec *dr, const SubTri *t, float out[3]);
+static void unlit_smooth_px(const DrawRec *dr, const SubTri *t, float w0, float w1, float w2,
+                            float inv_area, float out[3]);
 static void shade_px(const MopHipDevice *dev, const ShadeState *S, const DrawRec *dr,
                      const SubTri *t, float w0, float w1, float w2, float inv_area,
                      float out[3]);
 
 static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
                      const MopHipMat4 *model, const MopHipMat4 *mvp) {
-  int lit_smooth = c->shading_mode == MOP_HIP_SHADING_SMOOTH && !c->wireframe && c->lights &&
-                   c->light_count > 0;
-  int order_free = lit_smooth && c->depth_test && c->depth_write &&
+  int lit = c->lights && c->light_count > 0;
+  int smooth = c->shading_mode == MOP_HIP_SHADING_SMOOTH;
+  int sm = lit ? (smooth ? SM_LIT_SMOOTH : SM_LIT_FLAT) : (smooth ? SM_UNLIT_SMOOTH : SM_UNLIT_FLAT);
+  int order_free = !c->wireframe && sm != SM_UNLIT_SMOOTH && c->depth_test && c->depth_write &&
                    c->blend_mode == MOP_HIP_BLEND_OPAQUE && c->opacity >= 1.0f;
   if (!order_free) {
-    if (lit_smooth) {
-      draw_ordered(d, fb, c, model, mvp);
-      return;
-    }
     int invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
                      c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
-    if (!invisible)
+    if (invisible)
+      return;
+    if (c->wireframe) {
       d->skipped++;
+      return;
+    }
+    draw_ordered(d, fb, c, model, mvp, sm);
     return;
   }
   open_segment(fb);
@@ -567,6 +614,8 @@ static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCal
   }
   DrawRec *dr = &fb->draws[fb->ndraws];
   dr->object_id = c->object_id;
+  dr->sm = sm;
+  dr->light_dir = c->light_dir;
   dr->ambient = c->ambient;
   dr->metallic = c->metallic;
   dr->roughness = c->roughness;
@@ -623,7 +672,10 @@ static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCal
         continue;
       st.draw = di;
       push_tri(fb, &st);
-      walk_vis(fb, &fb->tris[fb->ntris - 1], (uint64_t)fb->ntris);
+      if (sm == SM_LIT_FLAT || sm == SM_UNLIT_FLAT)
+        walk_vis_flat(fb, &fb->tris[fb->ntris - 1], (uint64_t)fb->ntris);
+      else
+        walk_vis(fb, &fb->tris[fb->ntris - 1], (uint64_t)fb->ntris);
     }
   }
   free(vc);
@@ -634,7 +686,7 @@ static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCal
  * buffer, blend (rasterizer.c:1926-2140); depth_write = false saves and
  * restores the depth buffer around the draw (cpu_backend.c:539-545,622-628). */
 static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
-                         const MopHipMat4 *model, const MopHipMat4 *mvp) {
+                         const MopHipMat4 *model, const MopHipMat4 *mvp, int sm) {
   shade_segment(d, fb);
   size_t npx = (size_t)fb->w * fb->h;
   float *saved = NULL;
@@ -644,6 +696,8 @@ static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDra
   }
   DrawRec dr;
   dr.object_id = c->object_id;
+  dr.sm = sm;
+  dr.light_dir = c->light_dir;
   dr.ambient = c->ambient;
   dr.metallic = c->metallic;
   dr.roughness = c->roughness;
@@ -698,6 +752,56 @@ static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDra
       Setup sp;
       if (!setup_tri(st.sx, st.sy, fb->w, fb->h, &sp, 1))
         continue;
+      if (sm == SM_LIT_FLAT || sm == SM_UNLIT_FLAT) {
+        /* rasterize_filled_triangle (rasterizer.c:890-1087) */
+        float col[3];
+        flat_color(&d->states[dr.state], &dr, &st, col);
+        float ca = clamp01(c->opacity);
+        if (sm == SM_UNLIT_FLAT)
+          ca = (float)(uint8_t)(ca * 255.0f) / 255.0f;
+        int opaque = c->blend_mode == MOP_HIP_BLEND_OPAQUE && ca >= 1.0f;
+        float zr, zdx, zdy;
+        flat_z(&sp, &st, &zr, &zdx, &zdy);
+        float r0 = sp.wr[0], r1 = sp.wr[1], r2 = sp.wr[2];
+        for (int y = sp.y0; y <= sp.y1; y++) {
+          float w0 = r0, w1 = r1, w2 = r2, z = zr;
+          for (int x = sp.x0; x <= sp.x1; x++) {
+            if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f) {
+              size_t idx = (size_t)y * fb->w + x;
+              if (!c->depth_test || z < fb->depth[idx]) {
+                float *hp = fb->hdr + idx * 4;
+                if (opaque) {
+                  hp[0] = col[0];
+                  hp[1] = col[1];
+                  hp[2] = col[2];
+                  hp[3] = 1.0f;
+                  fb->depth[idx] = z;
+                  fb->ids[idx] = c->object_id;
+                } else {
+                  for (int q = 0; q < 3; q++) {
+                    if (c->blend_mode == MOP_HIP_BLEND_ADDITIVE)
+                      hp[q] = hp[q] + col[q] * ca;
+                    else if (c->blend_mode == MOP_HIP_BLEND_MULTIPLY)
+                      hp[q] = hp[q] * col[q];
+                    else
+                      hp[q] = col[q] * ca + hp[q] * (1.0f - ca);
+                  }
+                  hp[3] = 1.0f;
+                }
+              }
+            }
+            w0 += sp.edx[0];
+            w1 += sp.edx[1];
+            w2 += sp.edx[2];
+            z += zdx;
+          }
+          r0 += sp.edy[0];
+          r1 += sp.edy[1];
+          r2 += sp.edy[2];
+          zr += zdy;
+        }
+        continue;
+      }
       float r0 = sp.wr[0], r1 = sp.wr[1], r2 = sp.wr[2];
       for (int y = sp.y0; y <= sp.y1; y++) {
         float w0 = r0, w1 = r1, w2 = r2;
@@ -712,9 +816,16 @@ static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDra
                         w2 * sp.inv_area * st.sz[2];
               size_t idx = (size_t)y * fb->w + x;
               if (!c->depth_test || z < fb->depth[idx]) {
-                float rgb[3];
-                shade_px(d, &d->states[dr.state], &dr, &st, w0, w1, w2, sp.inv_area, rgb);
-                float cov = (md < 1.0f && c->blend_mode != MOP_HIP_BLEND_OPAQUE) ? md : 1.0f;
+                float rgb[3], cov;
+                if (sm == SM_UNLIT_SMOOTH) {
+                  /* silhouette AA (rasterizer.c:1421-1426) */
+                  unlit_smooth_px(&dr, &st, w0, w1, w2, sp.inv_area, rgb);
+                  int sil = (fb->depth[idx] - z) > 0.005f;
+                  cov = (md < 1.0f && sil) ? md : 1.0f;
+                } else {
+                  shade_px(d, &d->states[dr.state], &dr, &st, w0, w1, w2, sp.inv_area, rgb);
+                  cov = (md < 1.0f && c->blend_mode != MOP_HIP_BLEND_OPAQUE) ? md : 1.0f;
+                }
                 float fa = op * cov;
                 float *hp = fb->hdr + idx * 4;
                 if (c->blend_mode == MOP_HIP_BLEND_OPAQUE && fa >= 1.0f) {
@@ -1027,6 +1138,122 @@ static void shade_px(const MopHipDevice *dev, const ShadeState *S, const DrawRec
   }
 }
 
+/* compute_multi_light (rasterizer.c:1498-1589) */
+static float multi_light(const ShadeState *S, const DrawRec *dr, V3 n, V3 wp) {
+  float total = dr->ambient;
+  for (uint32_t i = 0; i < dr->nlights; i++) {
+    const MopHipLight *L = &dr->lights[i];
+    if (!L->active)
+      continue;
+    float ndotl = 0.0f, att = 1.0f, spot = 1.0f;
+    if (L->type == MOP_HIP_LIGHT_DIRECTIONAL) {
+      ndotl = -vdot(n, vnorm(L->direction));
+    } else if (L->type == MOP_HIP_LIGHT_POINT || L->type == MOP_HIP_LIGHT_SPOT) {
+      V3 tl = vsub(L->position, wp);
+      float dist = vlen(tl);
+      if (dist < 1e-6f)
+        dist = 1e-6f;
+      V3 dir = vscale(tl, 1.0f / dist);
+      ndotl = vdot(n, dir);
+      if (L->type == MOP_HIP_LIGHT_SPOT) {
+        float ca = -vdot(dir, vnorm(L->direction));
+        if (ca < L->spot_outer_cos) {
+          spot = 0.0f;
+        } else if (ca < L->spot_inner_cos) {
+          float rs = L->spot_inner_cos - L->spot_outer_cos;
+          if (rs > 1e-6f) {
+            float tt = (ca - L->spot_outer_cos) / rs;
+            spot = tt * tt * (3.0f - 2.0f * tt);
+          }
+        }
+      }
+      if (L->range > 0.0f) {
+        float r = dist / L->range;
+        att = 1.0f - r;
+        if (att < 0.0f)
+          att = 0.0f;
+        att *= att;
+      } else if (L->type == MOP_HIP_LIGHT_POINT) {
+        float d2 = dist * dist;
+        if (d2 < 0.01f)
+          d2 = 0.01f;
+        att = 1.0f / d2;
+      } else {
+        att = 1.0f / (1.0f + dist * dist);
+      }
+    }
+    if (ndotl < 0.0f)
+      ndotl = 0.0f;
+    float sh = 1.0f;
+    if (L->type == MOP_HIP_LIGHT_DIRECTIONAL && S->shadow_on)
+      sh = pcf(S, wp);
+    total += ndotl * L->intensity * att * spot * sh;
+  }
+  return clamp01(total);
+}
+
+/* Constant colour of a flat screen triangle (rasterizer.c:2556-2596 lit,
+ * 1199-1220 unlit). */
+static void flat_color(const ShadeState *S, const DrawRec *dr, const SubTri *t, float out[3]) {
+  const ClipV *a = &t->v[0], *b = &t->v[1], *c = &t->v[2];
+  V3 fn = vnorm(v3(a->n[0] + b->n[0] + c->n[0], a->n[1] + b->n[1] + c->n[1],
+                   a->n[2] + b->n[2] + c->n[2]));
+  float avg[3];
+  for (int k = 0; k < 3; k++)
+    avg[k] = (a->c[k] + b->c[k] + c->c[k]) * (1.0f / 3.0f);
+  if (dr->sm == SM_LIT_FLAT) {
+    V3 wp = v3((a->wp[0] + b->wp[0] + c->wp[0]) / 3.0f, (a->wp[1] + b->wp[1] + c->wp[1]) / 3.0f,
+               (a->wp[2] + b->wp[2] + c->wp[2]) / 3.0f);
+    float lighting = multi_light(S, dr, fn, wp);
+    float ds = 1.0f - dr->metallic, envf = dr->ambient * dr->metallic * 0.3f;
+    for (int k = 0; k < 3; k++)
+      out[k] = avg[k] * (lighting * ds + envf);
+  } else {
+    float ndotl = vdot(fn, vnorm(dr->light_dir));
+    if (ndotl < 0.0f)
+      ndotl = 0.0f;
+    float lighting = clamp01(dr->ambient + (1.0f - dr->ambient) * ndotl);
+    for (int k = 0; k < 3; k++)
+      out[k] = avg[k] * lighting;
+  }
+  for (int k = 0; k < 3; k++)
+    if (out[k] < 0.0f)
+      out[k] = 0.0f;
+}
+
+/* mop_sw_rasterize_triangle_smooth's colour (rasterizer.c:1368-1419) */
+static void unlit_smooth_px(const DrawRec *dr, const SubTri *t, float w0, float w1, float w2,
+                            float inv_area, float out[3]) {
+  float b0 = w0 * inv_area, b1 = w1 * inv_area, b2 = w2 * inv_area;
+  float pc0 = b0 * t->iw[0], pc1 = b1 * t->iw[1], pc2 = b2 * t->iw[2];
+  float ip = 1.0f / (pc0 + pc1 + pc2);
+  pc0 *= ip;
+  pc1 *= ip;
+  pc2 *= ip;
+  const ClipV *A = &t->v[0], *B = &t->v[1], *Cc = &t->v[2];
+#define IP(f, k) (pc0 * A->f[k] + pc1 * B->f[k] + pc2 * Cc->f[k])
+  V3 n = vnorm(v3(IP(n, 0), IP(n, 1), IP(n, 2)));
+  float col[3] = {IP(c, 0), IP(c, 1), IP(c, 2)};
+#undef IP
+  V3 nl = vnorm(dr->light_dir);
+  float ndotl = vdot(n, nl);
+  if (ndotl < 0.0f)
+    ndotl = 0.0f;
+  float lit = dr->ambient + (1.0f - dr->ambient) * ndotl;
+  V3 hv = vnorm(v3(nl.x, nl.y + 1.0f, nl.z));
+  float ndh = vdot(n, hv);
+  if (ndh < 0.0f)
+    ndh = 0.0f;
+  float sp = ndh * ndh;
+  for (int k = 0; k < 4; k++)
+    sp *= sp;
+  sp *= 0.25f;
+  for (int k = 0; k < 3; k++) {
+    float p = col[k] * lit + sp;
+    out[k] = p < 0.0f ? 0.0f : p;
+  }
+}
+
 /* Shade the winners of the open segment; replays each winner's incremental
  * walk to its pixel for the edge values. */
 static void shade_segment(MopHipDevice *d, MopHipFramebuffer *fb) {
@@ -1041,20 +1268,30 @@ static void shade_segment(MopHipDevice *d, MopHipFramebuffer *fb) {
     Setup s;
     setup_tri(t->sx, t->sy, fb->w, fb->h, &s, 0);
     int px = (int)(i % fb->w), py = (int)(i / fb->w);
-    float w[3];
-    for (int k = 0; k < 3; k++) {
-      float r = s.wr[k];
-      for (int y = s.y0; y < py; y++)
-        r += s.edy[k];
-      for (int x = s.x0; x < px; x++)
-        r += s.edx[k];
-      w[k] = r;
-    }
-    float b0 = w[0] * s.inv_area, b1 = w[1] * s.inv_area, b2 = w[2] * s.inv_area;
-    float z = b0 * t->sz[0] + b1 * t->sz[1] + b2 * t->sz[2];
     const DrawRec *dr = &fb->draws[t->draw];
-    float rgb[3];
-    shade_px(d, &d->states[dr->state], dr, t, w[0], w[1], w[2], s.inv_area, rgb);
+    float rgb[3], z;
+    if (dr->sm == SM_LIT_FLAT || dr->sm == SM_UNLIT_FLAT) {
+      float zdx, zdy;
+      flat_z(&s, t, &z, &zdx, &zdy);
+      for (int y = s.y0; y < py; y++)
+        z += zdy;
+      for (int x = s.x0; x < px; x++)
+        z += zdx;
+      flat_color(&d->states[dr->state], dr, t, rgb);
+    } else {
+      float w[3];
+      for (int k = 0; k < 3; k++) {
+        float r = s.wr[k];
+        for (int y = s.y0; y < py; y++)
+          r += s.edy[k];
+        for (int x = s.x0; x < px; x++)
+          r += s.edx[k];
+        w[k] = r;
+      }
+      float b0 = w[0] * s.inv_area, b1 = w[1] * s.inv_area, b2 = w[2] * s.inv_area;
+      z = b0 * t->sz[0] + b1 * t->sz[1] + b2 * t->sz[2];
+      shade_px(d, &d->states[dr->state], dr, t, w[0], w[1], w[2], s.inv_area, rgb);
+    }
     fb->hdr[i * 4] = rgb[0];
     fb->hdr[i * 4 + 1] = rgb[1];
     fb->hdr[i * 4 + 2] = rgb[2];

@@ -336,6 +336,17 @@ void mh_set_shading(MopViewport *vp, int mode) {
 
 void mh_set_exposure(MopViewport *vp, float e) { mop_viewport_set_exposure(vp, e); }
 
+/* per-mesh shading override (mop_mesh_set_shading); mode < 0 keeps the
+ * viewport's mode */
+void mh_mesh_set_shading(MopMesh *m, int mode) {
+  if (m && mode >= 0)
+    mop_mesh_set_shading(m, (MopShadingMode)mode);
+}
+
+void mh_set_light_dir(MopViewport *vp, const float *d) {
+  mop_viewport_set_light_dir(vp, (MopVec3){d[0], d[1], d[2]});
+}
+
 /* Writes the RGBA32F equirect to `path` as float EXR (lossless) and loads it
  * as the viewport environment (MOP_ENV_HDRI, environment.c:730-846). */
 int mh_set_environment(MopViewport *vp, const float *rgba, int w, int h,

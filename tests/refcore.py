@@ -49,6 +49,9 @@ def harness():
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
         h.mh_set_ambient.argtypes = [VP, C.c_float]
         h.mh_set_exposure.argtypes = [VP, C.c_float]
+        h.mh_set_shading.argtypes = [VP, C.c_int]
+        h.mh_mesh_set_shading.argtypes = [VP, C.c_int]
+        h.mh_set_light_dir.argtypes = [VP, F]
         h.mh_set_environment.argtypes = [VP, C.c_void_p, C.c_int, C.c_int, C.c_char_p,
                                          C.c_float, C.c_float]
         h.mh_get_exposure.restype = C.c_float
@@ -127,6 +130,7 @@ class RefViewport:
                                m.roughness, getattr(m, "blend", 0), getattr(m, "opacity", 1.0))
             if not ok:
                 raise RuntimeError("mop_viewport_add_mesh failed")
+            h.mh_mesh_set_shading(ok, getattr(m, "shading", -1))
         for im in getattr(scene, "instanced", []):
             v = np.ascontiguousarray(im.vertices, np.float32)
             i = np.ascontiguousarray(im.indices, np.uint32)
@@ -135,12 +139,18 @@ class RefViewport:
             if h.mh_add_instanced(self.vp, v.ctypes.data, len(v), i.ctypes.data, len(i),
                                   im.object_id, xf.ctypes.data, len(xf)) != 0:
                 raise RuntimeError("mop_viewport_add_instanced_mesh failed")
+        # set_light_dir / set_ambient also rewrite lights[0] (viewport.c:1028-1049),
+        # so they go before the scene's lights are added
+        h.mh_set_light_dir(self.vp, _f3(getattr(scene, "light_dir", (0.3, 1.0, 0.5))))
+        if getattr(scene, "ambient", None) is not None:
+            h.mh_set_ambient(self.vp, scene.ambient)
         for L in scene.lights:
             h.mh_add_light(self.vp, L.type, _f3(L.position), _f3(L.direction), _f3(L.color),
                            L.intensity, L.range, L.inner_cos, L.outer_cos, int(L.cast_shadows))
         h.mh_set_camera(self.vp, _f3(scene.eye), _f3(scene.target), _f3(scene.up), scene.fov,
                         scene.znear, scene.zfar)
         h.mh_set_post_effects(self.vp, scene.post_effects)
+        h.mh_set_shading(self.vp, getattr(scene, "shading", 1))
         if scene.env is not None:
             env = np.ascontiguousarray(scene.env, np.float32)
             path = env_path or os.path.join(tempfile.gettempdir(),

@@ -67,3 +67,11 @@ def test_ordered_draws_transparent_and_instanced(seed):
 
 def test_ordered_draws_larger_view():
     _check(scenes.ordered_scene(12, 480, 320))
+
+
+@pytest.mark.parametrize("lit", [True, False])
+def test_flat_and_unlit_shading_models(lit):
+    # flat shading (incremental z, centroid lighting), unlit Lambert and the
+    # legacy smooth path's silhouette AA, opaque and blended
+    _check(scenes.shading_scene(21, lit=lit))
+    _check(scenes.shading_scene(22, 320, 200, lit=lit))

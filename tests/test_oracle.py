@@ -64,3 +64,12 @@ def test_ordered_draws_transparent_and_instanced(seed):
     # and an instanced mesh (depth writes off): order-dependent draws
     a, _ = _same(scenes.ordered_scene(seed, with_env=(seed == 10)))
     assert (a["ids"] == 200).any()
+
+
+@pytest.mark.parametrize("lit", [True, False])
+@pytest.mark.parametrize("seed", [21, 22])
+def test_shading_models_flat_unlit(seed, lit):
+    # flat (incremental z, centroid lighting), unlit Lambert and the legacy
+    # smooth path's silhouette AA, opaque and blended
+    a, _ = _same(scenes.shading_scene(seed, lit=lit))
+    assert len(np.unique(a["ids"])) > 4
