@@ -190,7 +190,11 @@ DEV bool inside_all(float4 a) {
 }
 
 /* One fan triangle survives: |w| guard, screen-space cull, bbox, |area|. */
-DEV bool fan_ok(float4 p0, float4 p1, float4 p2, bool cull, int W, int H) {
+/* A fan triangle that reaches the raster: |w| >= 1e-7, not culled by screen
+ * area and (filled draws only) with a non-empty clamped bbox and non-zero
+ * area.  Wireframe triangles draw their edges whatever their area
+ * (rasterizer.c:2515-2537, 2598-2606). */
+DEV bool fan_ok(float4 p0, float4 p1, float4 p2, bool cull, bool wire, int W, int H) {
   if (fabsf(p0.w) < 1e-7f || fabsf(p1.w) < 1e-7f || fabsf(p2.w) < 1e-7f)
     return false;
   float half_w = (float)W * 0.5f, half_h = (float)H * 0.5f;
@@ -202,6 +206,8 @@ DEV bool fan_ok(float4 p0, float4 p1, float4 p2, bool cull, int W, int H) {
       (sx[1] - sx[0]) * (sy[2] - sy[0]) - (sx[2] - sx[0]) * (sy[1] - sy[0]);
   if (cull && signed_area >= 0.0f)
     return false;
+  if (wire)
+    return true;
   EdgeSetup es;
   return edge_setup(sx, sy, W, H, es);
 }
@@ -233,7 +239,7 @@ __global__ __launch_bounds__(WG) void k_classify(FrameArgs fa) {
       if (a.w > 0.0f && b.w > 0.0f && c.w > 0.0f && inside_all(a) &&
           inside_all(b) && inside_all(c)) {
         const ViewGPU &vw = fa.views[d.view];
-        cnt = fan_ok(a, b, c, cull, vw.w, vw.h) ? 1u : 0u;
+        cnt = fan_ok(a, b, c, cull, (d.flags & MOPH_DRAW_WIRE) != 0, vw.w, vw.h) ? 1u : 0u;
       } else {
         uint32_t q = atomicAdd(&fa.counters[CNT_CLIP], 1u);
         if (q < fa.clip_cap) {
@@ -356,7 +362,7 @@ __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
         float4 p1 = make_float4(src[i].p[0], src[i].p[1], src[i].p[2], src[i].p[3]);
         float4 p2 = make_float4(src[i + 1].p[0], src[i + 1].p[1], src[i + 1].p[2],
                                 src[i + 1].p[3]);
-        if (fan_ok(p0, p1, p2, cull, vw.w, vw.h))
+        if (fan_ok(p0, p1, p2, cull, (d.flags & MOPH_DRAW_WIRE) != 0, vw.w, vw.h))
           cnt++;
       }
       size_t base = (size_t)q * MOPH_MAX_POLY;
@@ -534,7 +540,7 @@ __global__ __launch_bounds__(WG) void k_emit(FrameArgs fa) {
   uint32_t o = off;
   for (int i = 1; i < n - 1; i++) {
     float4 p1 = fa.cpos[base + i], p2 = fa.cpos[base + i + 1];
-    if (fan_ok(p0, p1, p2, cull, vw.w, vw.h)) {
+    if (fan_ok(p0, p1, p2, cull, (d.flags & MOPH_DRAW_WIRE) != 0, vw.w, vw.h)) {
       write_subtri(fa, o, di, p0, p1, p2, fa.n_verts + (uint32_t)base,
                    fa.n_verts + (uint32_t)(base + i), fa.n_verts + (uint32_t)(base + i + 1));
       o++;
@@ -1265,7 +1271,7 @@ DEV void flat_color(const FrameArgs &frame, const ViewGPU &fa, const SubTri &st,
   float avg_r = (A1[0].z + A1[1].z + A1[2].z) * (1.0f / 3.0f);
   float avg_g = (A1[0].w + A1[1].w + A1[2].w) * (1.0f / 3.0f);
   float avg_b = (A2[0].x + A2[1].x + A2[2].x) * (1.0f / 3.0f);
-  if (sm == MOPH_SM_LIT_FLAT) {
+  if (sm == MOPH_SM_LIT_FLAT || sm == MOPH_SM_LIT_SMOOTH) { /* lit (wireframe: any mode) */
     V3 wp = V3{(A0[0].w + A0[1].w + A0[2].w) / 3.0f, (A1[0].x + A1[1].x + A1[2].x) / 3.0f,
                (A1[0].y + A1[1].y + A1[2].y) / 3.0f};
     float lighting = multi_light_scalar(frame, fa, d, fn, wp);
@@ -1904,11 +1910,43 @@ void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s) {
 #define OT 16
 #define OT_CAP 4096
 
-DEV bool tile_span(const EdgeSetup &es, int &tx0, int &ty0, int &tx1, int &ty1) {
-  tx0 = es.min_x / OT;
-  ty0 = es.min_y / OT;
-  tx1 = es.max_x / OT;
-  ty1 = es.max_y / OT;
+/* Pixels a wireframe triangle's three lines may plot: Wu's algorithm plots
+ * at round(x) and floor(y), floor(y)+1 of points on the edges (the span's
+ * intercept is a serial float sum whose drift stays well under a pixel), so
+ * the vertex bbox widened by 4 px covers them.  Clamped to the view; false when
+ * empty (aa_plot drops off-screen pixels, rasterizer.c:691-692). */
+DEV bool wire_bbox(const SubTri &st, int W, int H, int &x0, int &y0, int &x1, int &y1) {
+  float mnx = fminf(fminf(st.sx[0], st.sx[1]), st.sx[2]);
+  float mny = fminf(fminf(st.sy[0], st.sy[1]), st.sy[2]);
+  float mxx = fmaxf(fmaxf(st.sx[0], st.sx[1]), st.sx[2]);
+  float mxy = fmaxf(fmaxf(st.sy[0], st.sy[1]), st.sy[2]);
+  if (!(mnx == mnx && mny == mny && mxx == mxx && mxy == mxy))
+    return false;
+  float lo_x = fmaxf(floorf(mnx) - 4.0f, 0.0f), lo_y = fmaxf(floorf(mny) - 4.0f, 0.0f);
+  float hi_x = fminf(ceilf(mxx) + 4.0f, (float)(W - 1));
+  float hi_y = fminf(ceilf(mxy) + 4.0f, (float)(H - 1));
+  if (lo_x > hi_x || lo_y > hi_y)
+    return false;
+  x0 = (int)lo_x;
+  y0 = (int)lo_y;
+  x1 = (int)hi_x;
+  y1 = (int)hi_y;
+  return true;
+}
+
+/* Screen bbox of an ordered primitive: the clamped triangle bbox, or the
+ * wireframe bbox above. */
+DEV bool prim_bbox(const FrameArgs &fa, const SubTri &st, int W, int H, int &x0, int &y0,
+                   int &x1, int &y1) {
+  if (fa.draws[st.draw].flags & MOPH_DRAW_WIRE)
+    return wire_bbox(st, W, H, x0, y0, x1, y1);
+  EdgeSetup es;
+  if (!edge_setup(st.sx, st.sy, W, H, es))
+    return false;
+  x0 = es.min_x;
+  y0 = es.min_y;
+  x1 = es.max_x;
+  y1 = es.max_y;
   return true;
 }
 
@@ -1919,11 +1957,10 @@ __global__ __launch_bounds__(WG) void k_bin_count(FrameArgs fa, uint32_t *tile_c
   for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
     SubTri st = fa.subtris[s];
     const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
-    EdgeSetup es;
-    if (!edge_setup(st.sx, st.sy, vw.w, vw.h, es))
+    int bx0, by0, bx1, by1;
+    if (!prim_bbox(fa, st, vw.w, vw.h, bx0, by0, bx1, by1))
       continue;
-    int tx0, ty0, tx1, ty1;
-    tile_span(es, tx0, ty0, tx1, ty1);
+    int tx0 = bx0 / OT, ty0 = by0 / OT, tx1 = bx1 / OT, ty1 = by1 / OT;
     int tiles_x = (vw.w + OT - 1) / OT;
     for (int ty = ty0; ty <= ty1; ty++)
       for (int tx = tx0; tx <= tx1; tx++)
@@ -1940,11 +1977,10 @@ __global__ __launch_bounds__(WG) void k_bin_fill(FrameArgs fa, const uint32_t *t
   for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
     SubTri st = fa.subtris[s];
     const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
-    EdgeSetup es;
-    if (!edge_setup(st.sx, st.sy, vw.w, vw.h, es))
+    int bx0, by0, bx1, by1;
+    if (!prim_bbox(fa, st, vw.w, vw.h, bx0, by0, bx1, by1))
       continue;
-    int tx0, ty0, tx1, ty1;
-    tile_span(es, tx0, ty0, tx1, ty1);
+    int tx0 = bx0 / OT, ty0 = by0 / OT, tx1 = bx1 / OT, ty1 = by1 / OT;
     int tiles_x = (vw.w + OT - 1) / OT;
     for (int ty = ty0; ty <= ty1; ty++)
       for (int tx = tx0; tx <= tx1; tx++) {
@@ -2090,6 +2126,110 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
   }
 }
 
+/* aa_plot (rasterizer.c:688-730): coverage blend of the line colour, depth
+ * and id written above coverage 0.5. */
+DEV void wire_plot(const DrawGPU &d, float z, float cov, float fr, float fg, float fb_, int li,
+                   OrderedLds &L) {
+  if (cov < 0.004f)
+    return;
+  if ((d.flags & MOPH_DRAW_DEPTH_TEST) && z >= L.depth[li])
+    return;
+  float4 dc = L.hdr[li];
+  float inv = 1.0f - cov;
+  L.hdr[li] = make_float4(fr * cov + dc.x * inv, fg * cov + dc.y * inv, fb_ * cov + dc.z * inv,
+                          1.0f);
+  if (cov > 0.5f) {
+    L.depth[li] = z;
+    L.ids[li] = d.object_id;
+  }
+}
+
+DEV float fpartf(float x) { return x - floorf(x); }
+DEV float rfpartf(float x) { return 1.0f - fpartf(x); }
+
+/* The plots one Xiaolin Wu line (mop_sw_draw_line_aa, thickness 1,
+ * rasterizer.c:779-871) makes at pixel (px, py), in the line's order:
+ * first endpoint, second endpoint, then the span between them, whose
+ * intercept is the serial sum intery += gradient. */
+DEV void wire_line(const DrawGPU &d, float x0, float y0, float z0, float x1, float y1, float z1,
+                   int px, int py, float fr, float fg, float fb_, int li, OrderedLds &L) {
+  bool steep = fabsf(y1 - y0) > fabsf(x1 - x0);
+  if (steep) {
+    float t = x0;
+    x0 = y0;
+    y0 = t;
+    t = x1;
+    x1 = y1;
+    y1 = t;
+  }
+  if (x0 > x1) {
+    float t = x0;
+    x0 = x1;
+    x1 = t;
+    t = y0;
+    y0 = y1;
+    y1 = t;
+    t = z0;
+    z0 = z1;
+    z1 = t;
+  }
+  int X = steep ? py : px, Y = steep ? px : py;
+  float dx = x1 - x0, dy = y1 - y0;
+  float gradient = (dx < 0.001f) ? 1.0f : dy / dx;
+  float xend = roundf(x0);
+  float yend1 = y0 + gradient * (xend - x0);
+  float xgap = rfpartf(x0 + 0.5f);
+  int xp1 = cvt_i32(xend), yp1 = cvt_i32(floorf(yend1));
+  if (X == xp1) {
+    if (Y == yp1)
+      wire_plot(d, z0, rfpartf(yend1) * xgap, fr, fg, fb_, li, L);
+    else if (Y == yp1 + 1)
+      wire_plot(d, z0, fpartf(yend1) * xgap, fr, fg, fb_, li, L);
+  }
+  xend = roundf(x1);
+  float yend2 = y1 + gradient * (xend - x1);
+  xgap = fpartf(x1 + 0.5f);
+  int xp2 = cvt_i32(xend), yp2 = cvt_i32(floorf(yend2));
+  if (X == xp2) {
+    if (Y == yp2)
+      wire_plot(d, z1, rfpartf(yend2) * xgap, fr, fg, fb_, li, L);
+    else if (Y == yp2 + 1)
+      wire_plot(d, z1, fpartf(yend2) * xgap, fr, fg, fb_, li, L);
+  }
+  if (X > xp1 && X < xp2) {
+    float total_steps = (float)(xp2 - xp1);
+    if (total_steps < 1.0f)
+      total_steps = 1.0f;
+    float intery = yend1 + gradient;
+    for (int x = xp1 + 1; x < X; x++)
+      intery += gradient;
+    int iy = cvt_i32(floorf(intery));
+    if (Y == iy || Y == iy + 1) {
+      float t = (float)(X - xp1) / total_steps;
+      float z = z0 + t * (z1 - z0);
+      wire_plot(d, z, Y == iy ? rfpartf(intery) : fpartf(intery), fr, fg, fb_, li, L);
+    }
+  }
+}
+
+/* A wireframe triangle: its flat colour truncated to 8 bits, then edges
+ * v0-v1, v1-v2, v2-v0 (rasterizer.c:2590-2603, 1222-1233). */
+DEV void wire_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &st, int px,
+                       int py, int li, OrderedLds &L) {
+  const DrawGPU &d = frame.draws[st.draw];
+  float cr, cg, cb;
+  flat_color(frame, vw, st, d, draw_sm(d), cr, cg, cb);
+  float fr = (float)cvt_u8((cr > 1.0f ? 1.0f : cr) * 255.0f) / 255.0f;
+  float fg = (float)cvt_u8((cg > 1.0f ? 1.0f : cg) * 255.0f) / 255.0f;
+  float fb_ = (float)cvt_u8((cb > 1.0f ? 1.0f : cb) * 255.0f) / 255.0f;
+  wire_line(d, st.sx[0], st.sy[0], st.sz[0], st.sx[1], st.sy[1], st.sz[1], px, py, fr, fg, fb_,
+            li, L);
+  wire_line(d, st.sx[1], st.sy[1], st.sz[1], st.sx[2], st.sy[2], st.sz[2], px, py, fr, fg, fb_,
+            li, L);
+  wire_line(d, st.sx[2], st.sy[2], st.sz[2], st.sx[0], st.sy[0], st.sz[0], px, py, fr, fg, fb_,
+            li, L);
+}
+
 __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint32_t *tile_start,
                                                      const uint32_t *bins) {
   __shared__ OrderedLds L;
@@ -2155,6 +2295,16 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
         }
       }
       SubTri st = frame.subtris[s];
+      if (frame.draws[draw].flags & MOPH_DRAW_WIRE) {
+        int bx0, by0, bx1, by1;
+        bool ok = wire_bbox(st, vw.w, vw.h, bx0, by0, bx1, by1);
+        bool in_bbox = ok && inside && px >= bx0 && px <= bx1 && py >= by0 && py <= by1;
+        if (__ballot(in_bbox) == 0ull)
+          continue;
+        if (in_bbox)
+          wire_fragment(frame, vw, st, px, py, li, L);
+        continue;
+      }
       RasterTri r;
       bool ok = load_raster_tri(vw.w, vw.h, st, r);
       bool in_bbox = ok && inside && px >= r.es.min_x && px <= r.es.max_x &&

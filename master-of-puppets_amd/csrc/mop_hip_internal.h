@@ -45,6 +45,7 @@ enum {
   MOPH_DRAW_DEPTH_WRITE = 1u << 3,
   MOPH_DRAW_BLEND_SHIFT = 4, /* 2 bits: MOP_HIP_BLEND_* */
   MOPH_DRAW_SM_SHIFT = 6,    /* 2 bits: MOPH_SM_* */
+  MOPH_DRAW_WIRE = 1u << 8,  /* wireframe: three anti-aliased lines per triangle */
 };
 
 /* Shading model of a draw, as cpu_backend.c:608-619 and

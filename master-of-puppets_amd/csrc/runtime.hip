@@ -763,6 +763,8 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
   int sm = lit ? (smooth ? MOPH_SM_LIT_SMOOTH : MOPH_SM_LIT_FLAT)
                : (smooth ? MOPH_SM_UNLIT_SMOOTH : MOPH_SM_UNLIT_FLAT);
   g.flags |= (uint32_t)sm << MOPH_DRAW_SM_SHIFT;
+  if (c->wireframe)
+    g.flags |= MOPH_DRAW_WIRE;
   /* Order-free: opaque writes under a strict depth test, so the result is
    * the (depth, submission) minimum.  The unlit smooth model blends at
    * silhouettes against the depth already drawn, so it is always ordered. */
@@ -775,8 +777,10 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
    * hidden grid that way; it is dropped. */
   bool invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
                     c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
-  /* 0 order-free, 3 ordered (tile path), 2 no-op, 1 not rendered (wireframe) */
-  pd.cls = order_free ? 0 : (invisible ? 2 : (c->wireframe ? 1 : 3));
+  /* 0 order-free, 3 ordered (tile path), 2 no-op */
+  if (c->wireframe)
+    invisible = false; /* lines ignore blend mode and opacity */
+  pd.cls = order_free ? 0 : (invisible ? 2 : 3);
   pd.state = current_state(d, fb);
   fb->draws.push_back(std::move(pd));
   fb->executed = false;

@@ -118,6 +118,7 @@ class Scene:
     shadow_size: int = 1024
     post_effects: int = 1 | 16     # MOP_POST_GAMMA | MOP_POST_FXAA (viewport.c:646)
     shading: int = 1              # viewport shading mode (viewport.c:645: smooth)
+    render_mode: int = 0          # MopRenderMode: 0 solid, 1 wireframe
     light_dir: tuple = (0.3, 1.0, 0.5)  # viewport light_dir (viewport.c:643)
     ambient: float = None         # None keeps the viewport default (0.25)
 
@@ -559,4 +560,14 @@ def shading_scene(seed=21, width=160, height=120, lit=True, with_shadow=True):
                               rng.uniform(0.2, 0.9), blend=blend,
                               opacity=1.0 if k == 0 else rng.uniform(0.3, 0.8),
                               shading=[0, 0, 1, 0, 1][k]))
+    return sc
+
+
+def wireframe_scene(seed=31, width=160, height=120, lit=True):
+    """The viewport in wireframe render mode: every mesh with a non-zero id
+    is drawn as anti-aliased lines along its screen triangles' edges, over
+    whatever is already in the buffer (lines blend, so order matters)."""
+    sc = shading_scene(seed, width, height, lit=lit)
+    sc.name = f"wire_{'lit' if lit else 'unlit'}_{seed}_{width}x{height}"
+    sc.render_mode = 1
     return sc

@@ -545,8 +545,8 @@ static void walk_vis(MopHipFramebuffer *fb, const SubTri *t, uint64_t seq) {
   }
 }
 
-static int fan_tri(const ClipV *a, const ClipV *b, const ClipV *c, int cull, int W, int H,
-                   SubTri *o) {
+static int fan_tri(const ClipV *a, const ClipV *b, const ClipV *c, int cull, int wire, int W,
+                   int H, SubTri *o) {
   if (fabsf(a->p[3]) < 1e-7f || fabsf(b->p[3]) < 1e-7f || fabsf(c->p[3]) < 1e-7f)
     return 0;
   const ClipV *v[3] = {a, b, c};
@@ -563,6 +563,8 @@ static int fan_tri(const ClipV *a, const ClipV *b, const ClipV *c, int cull, int
              (o->sx[2] - o->sx[0]) * (o->sy[1] - o->sy[0]);
   if (cull && sa >= 0.0f)
     return 0;
+  if (wire)
+    return 1;
   Setup s;
   return setup_tri(o->sx, o->sy, W, H, &s, 0);
 }
@@ -588,6 +590,85 @@ static void shade_px(const MopHipDevice *dev, const ShadeState *S, const DrawRec
                      const SubTri *t, float w0, float w1, float w2, float inv_area,
                      float out[3]);
 
+/* aa_plot (rasterizer.c:688-730) */
+static void aa_plot(MopHipFramebuffer *fb, int x, int y, float z, const uint8_t *c8, float cov,
+                    uint32_t id, int depth_test) {
+  if (x < 0 || x >= fb->w || y < 0 || y >= fb->h)
+    return;
+  if (cov < 0.004f)
+    return;
+  size_t i = (size_t)y * fb->w + x;
+  if (depth_test && z >= fb->depth[i])
+    return;
+  float *hp = fb->hdr + i * 4;
+  float inv = 1.0f - cov;
+  for (int q = 0; q < 3; q++)
+    hp[q] = ((float)c8[q] / 255.0f) * cov + hp[q] * inv;
+  hp[3] = 1.0f;
+  if (cov > 0.5f) {
+    fb->depth[i] = z;
+    fb->ids[i] = id;
+  }
+}
+
+static float fpart(float x) { return x - floorf(x); }
+static float rfpart(float x) { return 1.0f - fpart(x); }
+
+/* Xiaolin Wu line, thickness 1 (mop_sw_draw_line_aa, rasterizer.c:779-871) */
+static void line_aa(MopHipFramebuffer *fb, float x0, float y0, float z0, float x1, float y1,
+                    float z1, const uint8_t *c8, uint32_t id, int dt) {
+  int steep = fabsf(y1 - y0) > fabsf(x1 - x0);
+  float t;
+  if (steep) {
+    t = x0; x0 = y0; y0 = t;
+    t = x1; x1 = y1; y1 = t;
+  }
+  if (x0 > x1) {
+    t = x0; x0 = x1; x1 = t;
+    t = y0; y0 = y1; y1 = t;
+    t = z0; z0 = z1; z1 = t;
+  }
+  float dx = x1 - x0, dy = y1 - y0;
+  float g = (dx < 0.001f) ? 1.0f : dy / dx;
+  float xe = roundf(x0), ye = y0 + g * (xe - x0), xg = rfpart(x0 + 0.5f);
+  int xp1 = (int)xe, yp1 = (int)floorf(ye);
+  if (steep) {
+    aa_plot(fb, yp1, xp1, z0, c8, rfpart(ye) * xg, id, dt);
+    aa_plot(fb, yp1 + 1, xp1, z0, c8, fpart(ye) * xg, id, dt);
+  } else {
+    aa_plot(fb, xp1, yp1, z0, c8, rfpart(ye) * xg, id, dt);
+    aa_plot(fb, xp1, yp1 + 1, z0, c8, fpart(ye) * xg, id, dt);
+  }
+  float iy_f = ye + g;
+  xe = roundf(x1);
+  ye = y1 + g * (xe - x1);
+  xg = fpart(x1 + 0.5f);
+  int xp2 = (int)xe, yp2 = (int)floorf(ye);
+  if (steep) {
+    aa_plot(fb, yp2, xp2, z1, c8, rfpart(ye) * xg, id, dt);
+    aa_plot(fb, yp2 + 1, xp2, z1, c8, fpart(ye) * xg, id, dt);
+  } else {
+    aa_plot(fb, xp2, yp2, z1, c8, rfpart(ye) * xg, id, dt);
+    aa_plot(fb, xp2, yp2 + 1, z1, c8, fpart(ye) * xg, id, dt);
+  }
+  float steps = (float)(xp2 - xp1);
+  if (steps < 1.0f)
+    steps = 1.0f;
+  for (int x = xp1 + 1; x < xp2; x++) {
+    float tt = (float)(x - xp1) / steps;
+    float z = z0 + tt * (z1 - z0);
+    int iy = (int)floorf(iy_f);
+    if (steep) {
+      aa_plot(fb, iy, x, z, c8, rfpart(iy_f), id, dt);
+      aa_plot(fb, iy + 1, x, z, c8, fpart(iy_f), id, dt);
+    } else {
+      aa_plot(fb, x, iy, z, c8, rfpart(iy_f), id, dt);
+      aa_plot(fb, x, iy + 1, z, c8, fpart(iy_f), id, dt);
+    }
+    iy_f += g;
+  }
+}
+
 static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCall *c,
                      const MopHipMat4 *model, const MopHipMat4 *mvp) {
   int lit = c->lights && c->light_count > 0;
@@ -596,14 +677,12 @@ static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCal
   int order_free = !c->wireframe && sm != SM_UNLIT_SMOOTH && c->depth_test && c->depth_write &&
                    c->blend_mode == MOP_HIP_BLEND_OPAQUE && c->opacity >= 1.0f;
   if (!order_free) {
-    int invisible = (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
-                     c->blend_mode == MOP_HIP_BLEND_ADDITIVE) && !(c->opacity > 0.0f);
+    /* lines ignore blend mode and opacity */
+    int invisible = !c->wireframe && (c->blend_mode == MOP_HIP_BLEND_ALPHA ||
+                                      c->blend_mode == MOP_HIP_BLEND_ADDITIVE) &&
+                    !(c->opacity > 0.0f);
     if (invisible)
       return;
-    if (c->wireframe) {
-      d->skipped++;
-      return;
-    }
     draw_ordered(d, fb, c, model, mvp, sm);
     return;
   }
@@ -668,7 +747,7 @@ static void draw_one(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDrawCal
       continue;
     for (int k = 1; k < n - 1; k++) {
       SubTri st;
-      if (!fan_tri(&poly[0], &poly[k], &poly[k + 1], cull, fb->w, fb->h, &st))
+      if (!fan_tri(&poly[0], &poly[k], &poly[k + 1], cull, c->wireframe, fb->w, fb->h, &st))
         continue;
       st.draw = di;
       push_tri(fb, &st);
@@ -747,8 +826,23 @@ static void draw_ordered(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDra
       continue;
     for (int k = 1; k < n - 1; k++) {
       SubTri st;
-      if (!fan_tri(&poly[0], &poly[k], &poly[k + 1], cull, fb->w, fb->h, &st))
+      if (!fan_tri(&poly[0], &poly[k], &poly[k + 1], cull, c->wireframe, fb->w, fb->h, &st))
         continue;
+      if (c->wireframe) {
+        /* flat colour truncated to 8 bits, three Wu lines
+         * (rasterizer.c:2590-2603, 1222-1233) */
+        float col[3];
+        flat_color(&d->states[dr.state], &dr, &st, col);
+        uint8_t c8[3];
+        for (int q = 0; q < 3; q++)
+          c8[q] = (uint8_t)((col[q] > 1.0f ? 1.0f : col[q]) * 255.0f);
+        for (int e = 0; e < 3; e++) {
+          int a0 = e, a1 = (e + 1) % 3;
+          line_aa(fb, st.sx[a0], st.sy[a0], st.sz[a0], st.sx[a1], st.sy[a1], st.sz[a1], c8,
+                  c->object_id, c->depth_test);
+        }
+        continue;
+      }
       Setup sp;
       if (!setup_tri(st.sx, st.sy, fb->w, fb->h, &sp, 1))
         continue;
@@ -1201,7 +1295,7 @@ static void flat_color(const ShadeState *S, const DrawRec *dr, const SubTri *t, 
   float avg[3];
   for (int k = 0; k < 3; k++)
     avg[k] = (a->c[k] + b->c[k] + c->c[k]) * (1.0f / 3.0f);
-  if (dr->sm == SM_LIT_FLAT) {
+  if (dr->sm == SM_LIT_FLAT || dr->sm == SM_LIT_SMOOTH) { /* lit (wireframe: any mode) */
     V3 wp = v3((a->wp[0] + b->wp[0] + c->wp[0]) / 3.0f, (a->wp[1] + b->wp[1] + c->wp[1]) / 3.0f,
                (a->wp[2] + b->wp[2] + c->wp[2]) / 3.0f);
     float lighting = multi_light(S, dr, fn, wp);

@@ -343,6 +343,10 @@ void mh_mesh_set_shading(MopMesh *m, int mode) {
     mop_mesh_set_shading(m, (MopShadingMode)mode);
 }
 
+void mh_set_render_mode(MopViewport *vp, int mode) {
+  mop_viewport_set_render_mode(vp, (MopRenderMode)mode);
+}
+
 void mh_set_light_dir(MopViewport *vp, const float *d) {
   mop_viewport_set_light_dir(vp, (MopVec3){d[0], d[1], d[2]});
 }

@@ -52,6 +52,7 @@ def harness():
         h.mh_set_shading.argtypes = [VP, C.c_int]
         h.mh_mesh_set_shading.argtypes = [VP, C.c_int]
         h.mh_set_light_dir.argtypes = [VP, F]
+        h.mh_set_render_mode.argtypes = [VP, C.c_int]
         h.mh_set_environment.argtypes = [VP, C.c_void_p, C.c_int, C.c_int, C.c_char_p,
                                          C.c_float, C.c_float]
         h.mh_get_exposure.restype = C.c_float
@@ -151,6 +152,7 @@ class RefViewport:
                         scene.znear, scene.zfar)
         h.mh_set_post_effects(self.vp, scene.post_effects)
         h.mh_set_shading(self.vp, getattr(scene, "shading", 1))
+        h.mh_set_render_mode(self.vp, getattr(scene, "render_mode", 0))
         if scene.env is not None:
             env = np.ascontiguousarray(scene.env, np.float32)
             path = env_path or os.path.join(tempfile.gettempdir(),

@@ -75,3 +75,10 @@ def test_flat_and_unlit_shading_models(lit):
     # legacy smooth path's silhouette AA, opaque and blended
     _check(scenes.shading_scene(21, lit=lit))
     _check(scenes.shading_scene(22, 320, 200, lit=lit))
+
+
+@pytest.mark.parametrize("lit", [True, False])
+def test_wireframe_render_mode(lit):
+    # Xiaolin Wu lines, blended in submission order, over the core's grid
+    _check(scenes.wireframe_scene(31, lit=lit))
+    _check(scenes.wireframe_scene(32, 320, 200, lit=lit))

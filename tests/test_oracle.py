@@ -73,3 +73,10 @@ def test_shading_models_flat_unlit(seed, lit):
     # smooth path's silhouette AA, opaque and blended
     a, _ = _same(scenes.shading_scene(seed, lit=lit))
     assert len(np.unique(a["ids"])) > 4
+
+
+@pytest.mark.parametrize("lit", [True, False])
+def test_wireframe_lines(lit):
+    # Xiaolin Wu lines of every screen triangle, coverage-blended in order
+    a, _ = _same(scenes.wireframe_scene(31, lit=lit))
+    assert len(np.unique(a["ids"])) > 3
