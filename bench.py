@@ -27,6 +27,15 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(REPO, "master-of-puppets_amd")
 
+T_START = time.time()
+
+
+def log(msg):
+    """Progress line on stderr (the JSON result alone goes to stdout)."""
+    sys.stderr.write(f"[bench {time.time() - T_START:7.1f}s] {msg}\n")
+    sys.stderr.flush()
+
+
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # non-packed FP32 lane-ops/s, TOP/s
 
@@ -96,11 +105,13 @@ import conftest, refcore
 from mop_amd import scenes
 sc = scenes.config2(shadow_size=1024)
 vp = refcore.RefViewport(sc, 0)
+sys.stderr.write("[cpu baseline] scene loaded\n"); sys.stderr.flush()
 vp.render()                      # warm-up frame
 times = []
 t_end = time.time() + {seconds}
 while True:
     times.append(vp.render())
+    sys.stderr.write(f"[cpu baseline] frame {{len(times)}}: {{times[-1]:.1f}} ms\n"); sys.stderr.flush()
     if time.time() > t_end and len(times) >= 1:
         break
 tris = vp.triangle_count()
@@ -108,11 +119,10 @@ times.sort()
 print(json.dumps({{"frames": len(times), "median_ms": times[len(times)//2], "tris": tris}}))
 """
     try:
-        out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+        out = subprocess.run([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True,
                              timeout=seconds * 6 + 240, cwd=REPO)
         line = [l for l in out.stdout.splitlines() if l.startswith("{")]
         if not line:
-            sys.stderr.write(out.stderr[-2000:])
             return None
         r = json.loads(line[-1])
     except Exception as e:  # baseline is reported, never required
@@ -138,18 +148,22 @@ def batched_form(mod, rounds):
     from mop_amd import driver, scenes
     dev = driver.Device()
     views = scenes.config4()
+    log(f"batched: {len(views)} views generated")
     drv = [driver.SceneDriver(dev, sc, shadow_cache=True) for sc in views]
+    log("batched: scenes resident")
     fbs = [dev.framebuffer(sc.width, sc.height) for sc in views]
     tris = sum(sc.triangle_count for sc in views)
     # warm-up (also renders the cached shadow maps)
     for d, fb in zip(drv, fbs):
         d.render(fb)
     dev.sync()
+    log("batched: warm-up done")
     t0 = time.perf_counter()
-    for _ in range(rounds):
+    for r in range(rounds):
         for d, fb in zip(drv, fbs):
             d.render(fb)
             dev.sync()
+        log(f"batched: plain round {r} done")
     plain_s = (time.perf_counter() - t0) / rounds
     lat = []
     t0 = time.perf_counter()
@@ -167,6 +181,7 @@ def batched_form(mod, rounds):
                 else:
                     still.append(k)
             pending = still
+        log(f"batched: batched round {r} done")
     batch_s = (time.perf_counter() - t0) / rounds
     lat = np.array(lat) * 1e3
     out = {"views": len(views), "triangles_per_batch": tris,
@@ -213,12 +228,16 @@ def main():
 
     sc = scenes.config2()
     W, H = sc.width, sc.height
+    log("config 2 scene generated")
     drv, cam = make_scene_driver(lib, dev, sc)
     fb = lib.mop_hip_framebuffer_create(dev, C.byref(abi.FramebufferDesc(W, H)))
-
-    for _ in range(args.warmup):
-        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
     torch.cuda.synchronize()
+    log("scene resident (meshes, IBL tables)")
+
+    for k in range(args.warmup):
+        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
+        torch.cuda.synchronize()
+        log(f"warm-up frame {k} done")
     lib.mop_hip_timers_enable(dev, 1)
     tris = 0
     torch.cuda.synchronize()
@@ -228,6 +247,7 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame")
     fps = 1e3 / ms_per_step
     mtri = tris / args.steps / (ms_per_step * 1e3)
 
@@ -237,8 +257,22 @@ def main():
         if lib.mop_hip_timers_read(dev, name.encode(), C.byref(tot), C.byref(cnt)) == 0 and cnt.value:
             stages[name] = round(tot.value / cnt.value, 4)
     lib.mop_hip_timers_enable(dev, 0)
+
+    # the timed frames must have drawn the scene: read the last one back
+    # (this also runs the backend's scratch-overflow check and frame stats)
+    import numpy as np
+    iw, ih = C.c_int(), C.c_int()
+    ids = np.ctypeslib.as_array(
+        lib.mop_hip_framebuffer_read_object_id(dev, fb, C.byref(iw), C.byref(ih)),
+        (W * H,)).copy()
     st = abi.FrameStats()
     lib.mop_hip_get_frame_stats(dev, C.byref(st))
+    visible_objects = int(len(np.unique(ids[ids != 0])))
+    covered = float((ids != 0).mean())
+    log(f"check: {visible_objects} objects visible, {covered:.3f} of pixels covered, "
+        f"{st.subtris} screen triangles")
+    if visible_objects == 0 or st.subtris == 0:
+        raise RuntimeError("timed frames drew nothing: " + lib.mop_hip_last_error().decode())
 
     # PCIe-inclusive: frame + RGBA8 + depth + ID on the host
     n_rb = max(5, args.steps // 5)
@@ -250,6 +284,7 @@ def main():
         lib.mop_hip_framebuffer_read_depth(dev, fb, C.byref(iw), C.byref(ih))
         lib.mop_hip_framebuffer_read_object_id(dev, fb, C.byref(iw), C.byref(ih))
     fps_rb = n_rb / (time.perf_counter() - t0)
+    log(f"readback: {fps_rb:.2f} frames/s")
 
     # roofline of the dominant stage: shading, per pixel reads the visibility
     # word (8 B) and writes HDR RGBA32F + depth + ID (24 B); algorithmic bytes
@@ -266,7 +301,13 @@ def main():
                 "algorithmic_bytes_per_launch": algo_bytes}
 
     batch = None if args.no_batch else batched_form(mod, args.batch_rounds)
+    if batch:
+        log(f"batched: {batch}")
+    if not args.no_cpu:
+        log("cpu baseline: starting")
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, args.cpu_cores)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']} frames/s on {cpu['cores']} cores")
 
     out = {
         "metric": "frames/s (Mtri/s alongside), 1080p PBR scene",
@@ -289,7 +330,9 @@ def main():
         "fps_with_readback": round(fps_rb, 3),
         "stage_ms": stages,
         "frame_stats": {"subtris": st.subtris, "clipped": st.clipped,
-                        "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris},
+                        "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris,
+                        "visible_objects": visible_objects,
+                        "covered_pixel_fraction": round(covered, 4)},
         "roofline": roof,
         "batched_64_viewports": batch,
         "cpu_baseline": cpu,

@@ -27,6 +27,7 @@
 #include <dlfcn.h>
 #include <mop/mop.h>
 #include <stddef.h>
+#include <sched.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -118,9 +119,21 @@ int mh_load_hip(const char *so_path) { return mh_load_backend(2, so_path, "mop_h
 /* ---- wrapped reference symbols ----------------------------------------- */
 
 MopSwThreadPool *__real_mop_sw_threadpool_create(int num_threads);
+/* Mode 1: no pool, so cpu_draw takes its single-threaded submission-order
+ * path.  Otherwise the reference's own rule (cores - 1 workers plus the
+ * calling thread, cpu_backend.c:60-71) applied to the cores this process may
+ * run on: sysconf counts every online CPU, which oversubscribes a process
+ * pinned to fewer cores. */
 MopSwThreadPool *__wrap_mop_sw_threadpool_create(int num_threads) {
   if (g_mode == 1)
     return NULL;
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) {
+    int n = CPU_COUNT(&set);
+    int workers = n > 1 ? n - 1 : 1;
+    if (workers < num_threads)
+      num_threads = workers;
+  }
   return __real_mop_sw_threadpool_create(num_threads);
 }
 

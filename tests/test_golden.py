@@ -43,8 +43,13 @@ def test_oracle_config1_full_size_digests():
 
 
 def render_driver(sc):
-    """Render a Scene through the HIP frame driver; returns the buffers."""
+    """Render a Scene through the HIP frame driver; returns the buffers.  The
+    shadow map is the reference's fixed 1024^2 (viewport.c:2286), which the
+    goldens were rendered with whatever size the scene asks for."""
+    import copy
     import mop_amd
+    sc = copy.copy(sc)
+    sc.shadow_size = 1024
     from mop_amd import abi
     import bench
     lib = mop_amd.lib()
