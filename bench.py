@@ -91,9 +91,16 @@ def make_scene_driver(lib, dev, sc):
     return s, cam
 
 
-def cpu_baseline(seconds, cores):
-    """Reference CPU backend (multithreaded as shipped) on the same scene, in
-    a child process pinned to `cores` host CPUs.  Returns a dict or None."""
+def cpu_baseline(seconds, cores=1):
+    """Reference CPU backend on the same scene, in a child process pinned to
+    `cores` host CPUs, through its single-threaded draw path (the path
+    cpu_draw takes without a thread pool, cpu_backend.c:588-620).  The
+    shipped pool (cores - 1 workers, rasterizer_mt.c:347-394) was measured
+    on the 16-core GPU host at 70 s for the 800x600 one-sphere frame (its
+    idle workers keep re-taking the finished work item under the pool mutex
+    while the caller waits to clear it), so it is not a usable baseline
+    there; on an 8-core development host it rendered this scene 12% faster
+    than the single-threaded path.  Returns a dict or None."""
     harness = os.path.join(REPO, "oracle", "_ref", "ref_harness.so")
     if not os.path.exists(harness):
         return None
@@ -104,14 +111,14 @@ sys.path.insert(0, {os.path.join(REPO, 'tests')!r})
 import conftest, refcore
 from mop_amd import scenes
 sc = scenes.config2(shadow_size=1024)
-vp = refcore.RefViewport(sc, 0)
-sys.stderr.write("[cpu baseline] scene loaded\n"); sys.stderr.flush()
+vp = refcore.RefViewport(sc, 1)
+sys.stderr.write("[cpu baseline] scene loaded\\n"); sys.stderr.flush()
 vp.render()                      # warm-up frame
 times = []
 t_end = time.time() + {seconds}
 while True:
     times.append(vp.render())
-    sys.stderr.write(f"[cpu baseline] frame {{len(times)}}: {{times[-1]:.1f}} ms\n"); sys.stderr.flush()
+    sys.stderr.write(f"[cpu baseline] frame {{len(times)}}: {{times[-1]:.1f}} ms\\n"); sys.stderr.flush()
     if time.time() > t_end and len(times) >= 1:
         break
 tris = vp.triangle_count()
@@ -132,10 +139,11 @@ print(json.dumps({{"frames": len(times), "median_ms": times[len(times)//2], "tri
     return {"value": round(fps, 4), "unit": "frames/s", "cores": cores,
             "kind": "reference",
             "mtri_per_s": round(r["tris"] / (r["median_ms"] * 1e3), 3),
-            "sample": f"reference CPU backend (oracle/_ref, -O2, thread pool as shipped) "
-                      f"rendering the same 1920x1080 PBR scene with a 1024^2 shadow map "
-                      f"(the reference's fixed size): median of {r['frames']} frames after "
-                      f"1 warm-up, mop_viewport_render + read_color wall time"}
+            "sample": f"reference CPU backend (oracle/_ref, gcc -O2) through its "
+                      f"single-threaded draw path on {cores} pinned core(s), rendering the "
+                      f"same 1920x1080 PBR scene with a 1024^2 shadow map (the reference's "
+                      f"fixed size): median of {r['frames']} frames after 1 warm-up, "
+                      f"mop_viewport_render + read_color wall time"}
 
 
 def batched_form(mod, rounds):
@@ -166,12 +174,19 @@ def batched_form(mod, rounds):
         log(f"batched: plain round {r} done")
     plain_s = (time.perf_counter() - t0) / rounds
     lat = []
+    rec_s, exe_s, gpu_ms = [], [], []
+    gpu_time = dev.lib.mop_hip_frame_gpu_time_ms
+    gpu_time.restype = C.c_float
     t0 = time.perf_counter()
     for r in range(rounds):
         tb = time.perf_counter()
         for d, fb in zip(drv, fbs):
             d.record(fb, driver.camera_of(d.sc, yaw_delta=0.5 * r))
+        tr = time.perf_counter()
         dev.execute(fbs)
+        te = time.perf_counter()
+        rec_s.append(tr - tb)
+        exe_s.append(te - tr)
         pending = list(range(len(fbs)))
         while pending:
             still = []
@@ -181,7 +196,9 @@ def batched_form(mod, rounds):
                 else:
                     still.append(k)
             pending = still
-        log(f"batched: batched round {r} done")
+        gpu_ms.append(gpu_time(dev.dev))
+        log(f"batched: batched round {r} done: record {rec_s[-1]*1e3:.2f} ms, launch "
+            f"{exe_s[-1]*1e3:.2f} ms, device span {gpu_ms[-1]:.2f} ms")
     batch_s = (time.perf_counter() - t0) / rounds
     lat = np.array(lat) * 1e3
     out = {"views": len(views), "triangles_per_batch": tris,
@@ -191,7 +208,10 @@ def batched_form(mod, rounds):
            "batched_over_plain": round(plain_s / batch_s, 3),
            "batched_mtri_per_s": round(tris / batch_s / 1e6, 2),
            "p50_ms": round(float(np.percentile(lat, 50)), 3),
-           "p99_ms": round(float(np.percentile(lat, 99)), 3)}
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "host_record_ms": round(float(np.median(rec_s)) * 1e3, 3),
+           "host_launch_ms": round(float(np.median(exe_s)) * 1e3, 3),
+           "device_span_ms": round(float(np.median(gpu_ms)), 3)}
     for fb in fbs:
         fb.close()
     for d in drv:
@@ -206,10 +226,13 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-cores", type=int, default=16)
+    ap.add_argument("--cpu-cores", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-rounds", type=int, default=8)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (the "
+                         "reference dispatch that scales rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -218,6 +241,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {n} bytes done")
+        del a, b
     mod = load_pkg()
     from mop_amd import scenes, abi
     lib = mod.lib()

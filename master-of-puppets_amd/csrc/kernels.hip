@@ -41,6 +41,28 @@ DEV uint32_t find_draw_by_tri(const DrawGPU *d, uint32_t n, uint32_t t) {
   return lo;
 }
 
+/* The same searches for one lane of a wave: the wave's first active lane
+ * searches with wave-uniform (scalar) loads, and a lane whose item lies
+ * outside that draw searches on its own.  Consecutive lanes nearly always
+ * share a draw. */
+DEV uint32_t find_draw_by_vtx_wave(const DrawGPU *d, uint32_t n, uint32_t v) {
+  uint32_t v0 = __builtin_amdgcn_readfirstlane(v);
+  uint32_t i = find_draw_by_vtx(d, n, v0);
+  uint32_t hi = (i + 1 < n) ? d[i + 1].vtx_base : 0xffffffffu;
+  if (v >= d[i].vtx_base && v < hi)
+    return i;
+  return find_draw_by_vtx(d, n, v);
+}
+
+DEV uint32_t find_draw_by_tri_wave(const DrawGPU *d, uint32_t n, uint32_t t) {
+  uint32_t t0 = __builtin_amdgcn_readfirstlane(t);
+  uint32_t i = find_draw_by_tri(d, n, t0);
+  uint32_t hi = (i + 1 < n) ? d[i + 1].tri_base : 0xffffffffu;
+  if (t >= d[i].tri_base && t < hi)
+    return i;
+  return find_draw_by_tri(d, n, t);
+}
+
 /* --------------------------------------------------------------------- */
 /* k_vertex: cpu_prepare_triangle (cpu_backend.c:224-324) and the flexible */
 /* format variant (cpu_backend.c:330-526), once per vertex.               */
@@ -104,7 +126,7 @@ __global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
   uint32_t v = blockIdx.x * WG + threadIdx.x;
   if (v >= fa.n_verts)
     return;
-  uint32_t di = find_draw_by_vtx(fa.draws, fa.ndraws, v);
+  uint32_t di = find_draw_by_vtx_wave(fa.draws, fa.ndraws, v);
   const DrawGPU &d = fa.draws[di];
   uint32_t i = v - d.vtx_base;
   const uint8_t *vb = d.vbuf + (size_t)i * d.stride;
@@ -159,6 +181,14 @@ __global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
     c[3] *= (float)t[3] / 255.0f;
   }
   fa.vpos[v] = clip;
+  if (fa.vscr) {
+    /* the fan-vertex screen transform (rasterizer.c:2520-2532); an unclipped
+     * triangle's fan vertices are its own vertices */
+    const ViewGPU &vw = fa.views[d.view];
+    float4 sc;
+    screen_xform(clip, (float)vw.w * 0.5f, (float)vw.h * 0.5f, sc.x, sc.y, sc.z, sc.w);
+    fa.vscr[v] = sc;
+  }
   float4 *a = fa.vattr + (size_t)v * 3;
   a[0] = make_float4(nx, ny, nz, wp.x);
   a[1] = make_float4(wp.y, wp.z, c[0], c[1]);
@@ -290,6 +320,12 @@ DEV void lerp_cv(const CV &a, const CV &b, float t, CV &r) {
     r.a[k] = a.a[k] + t * (b.a[k] - a.a[k]);
 }
 
+/* One Sutherland-Hodgman plane (clip_against_plane, rasterizer.c:553-593).
+ * A triangle clipped by the six planes never exceeds 9 vertices, so buffers
+ * of CLIP_CAP entries hold every polygon the reference's 24-entry buffers do;
+ * a longer one (impossible) is reported as n > MOPH_MAX_POLY. */
+#define CLIP_CAP 12
+
 DEV int clip_plane(const CV *in, int n, CV *out, int plane) {
   if (n < 1)
     return 0;
@@ -302,15 +338,18 @@ DEV int clip_plane(const CV *in, int n, CV *out, int plane) {
     if (curr_d >= 0.0f) {
       if (prev_d < 0.0f) {
         float t = prev_d / (prev_d - curr_d);
-        if (oc < 24)
-          lerp_cv(prev, curr, t, out[oc++]);
+        if (oc < CLIP_CAP)
+          lerp_cv(prev, curr, t, out[oc]);
+        oc++;
       }
-      if (oc < 24)
-        out[oc++] = curr;
+      if (oc < CLIP_CAP)
+        out[oc] = curr;
+      oc++;
     } else if (prev_d >= 0.0f) {
       float t = prev_d / (prev_d - curr_d);
-      if (oc < 24)
-        lerp_cv(prev, curr, t, out[oc++]);
+      if (oc < CLIP_CAP)
+        lerp_cv(prev, curr, t, out[oc]);
+      oc++;
     }
     prev = curr;
     prev_d = curr_d;
@@ -328,7 +367,10 @@ DEV void load_cv(const FrameArgs &fa, uint32_t v, CV &o) {
   o.a[6] = a1.z; o.a[7] = a1.w; o.a[8] = a2.x; o.a[9] = a2.y;
 }
 
+/* The two polygon buffers of each lane live in LDS (86 KB per 64 lanes), not
+ * in per-lane scratch. */
 __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
+  __shared__ CV pool[64][2][CLIP_CAP];
   uint32_t nq = fa.counters[CNT_CLIP];
   if (nq > fa.clip_cap)
     nq = fa.clip_cap;
@@ -337,7 +379,7 @@ __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
     uint32_t di = find_draw_by_tri(fa.draws, fa.ndraws, t);
     const DrawGPU &d = fa.draws[di];
     uint32_t k = t - d.tri_base;
-    CV bufA[24], bufB[24]; /* MAX_CLIP_VERTICES, rasterizer.c:596 */
+    CV *bufA = pool[threadIdx.x][0], *bufB = pool[threadIdx.x][1];
     load_cv(fa, d.vtx_base + d.ibuf[3 * k], bufA[0]);
     load_cv(fa, d.vtx_base + d.ibuf[3 * k + 1], bufA[1]);
     load_cv(fa, d.vtx_base + d.ibuf[3 * k + 2], bufA[2]);
@@ -345,6 +387,8 @@ __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
     CV *src = bufA, *dst = bufB;
     for (int pl = 0; pl < 6 && n > 0; pl++) {
       n = clip_plane(src, n, dst, pl);
+      if (n > CLIP_CAP)
+        break;
       CV *tmp = src;
       src = dst;
       dst = tmp;
@@ -377,16 +421,19 @@ __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
       n = 0;
     }
     fa.cnt_poly[q] = (uint32_t)n;
-    fa.tri_counts[t] = cnt;
+    if (!fa.fused)
+      fa.tri_counts[t] = cnt;
   }
 }
 
 void launch_clip(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris || !fa.clip_cap)
     return;
+  /* the queue length is known on the device only: a grid-stride loop over
+   * at most two workgroups per CU (each lane needs 2.7 KB of scratch) */
   uint32_t grid = (fa.clip_cap + 63) / 64;
-  if (grid > 4096)
-    grid = 4096;
+  if (grid > 512)
+    grid = 512;
   hipLaunchKernelGGL(k_clip, dim3(grid), dim3(64), 0, s, fa);
 }
 
@@ -642,10 +689,101 @@ DEV bool frag_depth(const RasterTri &r, float w0, float w1, float w2, float &z) 
   return z == z;
 }
 
+/* Visibility update: a 64-bit atomic min with no return value, so a lane
+ * walking a triangle issues its pixels back to back without waiting on the
+ * memory system (the min is resolved in L2). */
 DEV void vis_update(unsigned long long *vis, size_t idx, unsigned long long key) {
-  unsigned long long cur = __builtin_nontemporal_load(vis + idx);
-  if (key < cur)
-    atomicMin(vis + idx, key);
+  atomicMin(vis + idx, key);
+}
+
+/* Screen triangle e of the order-free path, e = (t << 4) | k: k = 15 is
+ * triangle t drawn whole, k < 15 the fan triangle k+1 of its clipped polygon
+ * (poly[0], poly[k+1], poly[k+2]; rasterizer.c:2510-2513).  e + 1 is its
+ * submission sequence number: triangles in submission order, fan triangles
+ * in fan order. */
+#define FUSED_WHOLE 15u
+
+DEV void fused_subtri(const FrameArgs &fa, uint32_t e, SubTri &st) {
+  uint32_t t = e >> 4, k = e & 15u;
+  uint4 rec = fa.tri_rec[t];
+  st.draw = rec.w;
+  if (k == FUSED_WHOLE) {
+    uint32_t v[3] = {rec.x, rec.y, rec.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      float4 sc = fa.vscr[v[i]];
+      st.sx[i] = sc.x;
+      st.sy[i] = sc.y;
+      st.sz[i] = sc.z;
+      st.iw[i] = sc.w;
+      st.a[i] = v[i];
+    }
+  } else {
+    const ViewGPU &vw = fa.views[fa.draws[rec.w].view];
+    float hw = (float)vw.w * 0.5f, hh = (float)vw.h * 0.5f;
+    uint32_t base = rec.x * MOPH_MAX_POLY;
+    uint32_t v[3] = {base, base + k + 1, base + k + 2};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      screen_xform(fa.cpos[v[i]], hw, hh, st.sx[i], st.sy[i], st.sz[i], st.iw[i]);
+      st.a[i] = fa.n_verts + v[i];
+    }
+  }
+}
+
+/* Screen triangle s of either path: an index into the screen-triangle store,
+ * or an order-free entry (above). */
+DEV void load_subtri(const FrameArgs &fa, uint32_t s, SubTri &st) {
+  if (fa.fused)
+    fused_subtri(fa, s, st);
+  else
+    st = fa.subtris[s];
+}
+
+/* The whole bbox walk of one small screen triangle by one lane, updating the
+ * visibility buffer: the smooth path's coverage and barycentric depth, or
+ * the flat path's edge test and incremental z (rasterizer.c:1000-1026). */
+DEV void walk_small_vis(const ViewGPU &vw, const RasterTri &r, bool flat,
+                        unsigned long long seq) {
+  float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
+  if (flat) {
+    FlatZ fz = flat_z_setup(r);
+    float zr = fz.row;
+    for (int y = r.es.min_y; y <= r.es.max_y; y++) {
+      float w0 = w0r, w1 = w1r, w2 = w2r, z = zr;
+      for (int x = r.es.min_x; x <= r.es.max_x; x++) {
+        if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z) {
+          size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
+          vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+        }
+        w0 += r.es.e_dx[0];
+        w1 += r.es.e_dx[1];
+        w2 += r.es.e_dx[2];
+        z += fz.dx;
+      }
+      w0r += r.es.e_dy[0];
+      w1r += r.es.e_dy[1];
+      w2r += r.es.e_dy[2];
+      zr += fz.dy;
+    }
+    return;
+  }
+  for (int y = r.es.min_y; y <= r.es.max_y; y++) {
+    float w0 = w0r, w1 = w1r, w2 = w2r;
+    for (int x = r.es.min_x; x <= r.es.max_x; x++) {
+      float z;
+      if (frag_depth(r, w0, w1, w2, z)) {
+        size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
+        vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+      }
+      w0 += r.es.e_dx[0];
+      w1 += r.es.e_dx[1];
+      w2 += r.es.e_dx[2];
+    }
+    w0r += r.es.e_dy[0];
+    w1r += r.es.e_dy[1];
+    w2r += r.es.e_dy[2];
+  }
 }
 
 __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
@@ -663,47 +801,7 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
       fa.big_queue[q] = s;
       continue;
     }
-    unsigned long long seq = (unsigned long long)(s + 1);
-    float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
-    if (sm_flat(draw_sm(fa.draws[st.draw]))) {
-      /* flat path: edge test only, incremental z (rasterizer.c:1000-1026) */
-      FlatZ fz = flat_z_setup(r);
-      float zr = fz.row;
-      for (int y = r.es.min_y; y <= r.es.max_y; y++) {
-        float w0 = w0r, w1 = w1r, w2 = w2r, z = zr;
-        for (int x = r.es.min_x; x <= r.es.max_x; x++) {
-          if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z) {
-            size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
-            vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
-          }
-          w0 += r.es.e_dx[0];
-          w1 += r.es.e_dx[1];
-          w2 += r.es.e_dx[2];
-          z += fz.dx;
-        }
-        w0r += r.es.e_dy[0];
-        w1r += r.es.e_dy[1];
-        w2r += r.es.e_dy[2];
-        zr += fz.dy;
-      }
-      continue;
-    }
-    for (int y = r.es.min_y; y <= r.es.max_y; y++) {
-      float w0 = w0r, w1 = w1r, w2 = w2r;
-      for (int x = r.es.min_x; x <= r.es.max_x; x++) {
-        float z;
-        if (frag_depth(r, w0, w1, w2, z)) {
-          size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
-          vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
-        }
-        w0 += r.es.e_dx[0];
-        w1 += r.es.e_dx[1];
-        w2 += r.es.e_dx[2];
-      }
-      w0r += r.es.e_dy[0];
-      w1r += r.es.e_dy[1];
-      w2r += r.es.e_dy[2];
-    }
+    walk_small_vis(vw, r, sm_flat(draw_sm(fa.draws[st.draw])), (unsigned long long)(s + 1));
   }
 }
 
@@ -718,7 +816,8 @@ __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
   uint32_t nbig = fa.counters[CNT_BIG];
   for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
     uint32_t s = fa.big_queue[b];
-    SubTri st = fa.subtris[s];
+    SubTri st;
+    load_subtri(fa, s, st);
     const DrawGPU &dr = fa.draws[st.draw];
     const ViewGPU &vw = fa.views[dr.view];
     RasterTri r;
@@ -801,6 +900,185 @@ void launch_raster(const FrameArgs &fa, hipStream_t s) {
   if (grid == 0)
     grid = 1;
   hipLaunchKernelGGL(k_raster_small, dim3(grid), dim3(WG), 0, s, fa);
+  hipLaunchKernelGGL(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
+  hipLaunchKernelGGL(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* order-free path: classification, setup and small-triangle raster fused */
+/* --------------------------------------------------------------------- */
+
+/* Queue a big screen triangle or walk a small one; true if it reached the
+ * raster (a non-empty clamped bbox and a non-zero area). */
+DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, bool flat,
+                      uint32_t e) {
+  RasterTri r;
+  if (!edge_setup(st.sx, st.sy, vw.w, vw.h, r.es))
+    return false;
+  if (bbox_area(r.es) > (long long)fa.small_area) {
+    uint32_t q = atomicAdd(&fa.counters[CNT_BIG], 1u);
+    fa.big_queue[q] = e;
+    return true;
+  }
+  r.inv_e[0] = inv_edge_len(r.es.e_dx[0], r.es.e_dy[0]);
+  r.inv_e[1] = inv_edge_len(r.es.e_dx[1], r.es.e_dy[1]);
+  r.inv_e[2] = inv_edge_len(r.es.e_dx[2], r.es.e_dy[2]);
+  r.sz[0] = st.sz[0];
+  r.sz[1] = st.sz[1];
+  r.sz[2] = st.sz[2];
+  walk_small_vis(vw, r, flat, (unsigned long long)e + 1ull);
+  return true;
+}
+
+/* Screen triangles that reached the raster, counted per workgroup into
+ * blk_counts (summed by k_sum_blocks): one same-address atomic per wave from
+ * every workgroup of the grid would serialize at the memory side. */
+DEV void count_subtris(uint32_t *blk_counts, bool emitted) {
+  __shared__ uint32_t wave_sum[WG / 64];
+  unsigned long long m = __ballot(emitted);
+  if ((threadIdx.x & 63u) == 0)
+    wave_sum[threadIdx.x >> 6] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t c = 0;
+    for (int w = 0; w < WG / 64; w++)
+      c += wave_sum[w];
+    blk_counts[blockIdx.x] = c;
+  }
+}
+
+__global__ __launch_bounds__(WG) void k_sum_blocks(const uint32_t *blk_counts, uint32_t n,
+                                                   uint32_t *counter) {
+  __shared__ uint32_t part[WG];
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < n; i += WG)
+    c += blk_counts[i];
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = WG / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    atomicAdd(counter, part[0]);
+}
+
+/* Per triangle of the batch: index checks, trivial reject and clip-space
+ * cull (rasterizer.c:2469-2485), trivial accept or a clip-queue entry
+ * (2492-2504), then for an accepted triangle the fan checks (2515-2537) and
+ * the visibility walk or a big-triangle queue entry. */
+__global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
+  uint32_t t = blockIdx.x * WG + threadIdx.x;
+  bool emitted = false;
+  if (t < fa.n_tris) {
+    uint32_t di = find_draw_by_tri_wave(fa.draws, fa.ndraws, t);
+    const DrawGPU &d = fa.draws[di];
+    uint32_t k = t - d.tri_base;
+    uint32_t i0 = d.ibuf[3 * k], i1 = d.ibuf[3 * k + 1], i2 = d.ibuf[3 * k + 2];
+    if (i0 < d.vertex_count && i1 < d.vertex_count && i2 < d.vertex_count) {
+      uint32_t v0 = d.vtx_base + i0, v1 = d.vtx_base + i1, v2 = d.vtx_base + i2;
+      float4 a = fa.vpos[v0], b = fa.vpos[v1], c = fa.vpos[v2];
+      bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
+      bool alive = !trivially_rejected(a, b, c);
+      if (alive && cull && a.w > 0.0f && b.w > 0.0f && c.w > 0.0f) {
+        float ex = b.x * a.w - a.x * b.w;
+        float ey = b.y * a.w - a.y * b.w;
+        float fx = c.x * a.w - a.x * c.w;
+        float fy = c.y * a.w - a.y * c.w;
+        if (ex * fy - ey * fx <= 0.0f)
+          alive = false;
+      }
+      if (alive) {
+        if (a.w > 0.0f && b.w > 0.0f && c.w > 0.0f && inside_all(a) && inside_all(b) &&
+            inside_all(c)) {
+          if (!(fabsf(a.w) < 1e-7f || fabsf(b.w) < 1e-7f || fabsf(c.w) < 1e-7f)) {
+            SubTri st;
+            uint32_t v[3] = {v0, v1, v2};
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+              float4 sc = fa.vscr[v[i]];
+              st.sx[i] = sc.x;
+              st.sy[i] = sc.y;
+              st.sz[i] = sc.z;
+              st.iw[i] = sc.w;
+            }
+            float signed_area = (st.sx[1] - st.sx[0]) * (st.sy[2] - st.sy[0]) -
+                                (st.sx[2] - st.sx[0]) * (st.sy[1] - st.sy[0]);
+            if (!(cull && signed_area >= 0.0f)) {
+              const ViewGPU &vw = fa.views[d.view];
+              fa.tri_rec[t] = make_uint4(v0, v1, v2, di);
+              emitted = fused_raster(fa, vw, st, sm_flat(draw_sm(d)), (t << 4) | FUSED_WHOLE);
+            }
+          }
+        } else {
+          uint32_t q = atomicAdd(&fa.counters[CNT_CLIP], 1u);
+          if (q < fa.clip_cap) {
+            fa.clip_queue[q] = t;
+            fa.tri_rec[t] = make_uint4(q, 0u, 0u, di);
+          } else {
+            atomicOr(&fa.counters[CNT_OVERFLOW], 1u);
+          }
+        }
+      }
+    }
+  }
+  count_subtris(fa.blk_counts, emitted);
+}
+
+/* Fan triangles of the clipped polygons (after k_clip): the same checks and
+ * raster as k_tri, one lane per polygon. */
+#define FAN_MAX (MOPH_MAX_POLY - 2)
+
+__global__ __launch_bounds__(WG) void k_clip_raster(FrameArgs fa, uint32_t *blk_counts) {
+  __shared__ uint32_t acc;
+  if (threadIdx.x == 0)
+    acc = 0;
+  __syncthreads();
+  uint32_t nq = fa.counters[CNT_CLIP];
+  if (nq > fa.clip_cap)
+    nq = fa.clip_cap;
+  uint32_t items = nq * FAN_MAX, emitted = 0;
+  for (uint32_t it = blockIdx.x * WG + threadIdx.x; it < items; it += gridDim.x * WG) {
+    uint32_t q = it / FAN_MAX;
+    int i = (int)(it % FAN_MAX) + 1; /* fan triangle poly[0], poly[i], poly[i+1] */
+    int n = (int)fa.cnt_poly[q];
+    if (i >= n - 1)
+      continue;
+    uint32_t t = fa.clip_queue[q];
+    const DrawGPU &d = fa.draws[fa.tri_rec[t].w];
+    const ViewGPU &vw = fa.views[d.view];
+    size_t base = (size_t)q * MOPH_MAX_POLY;
+    if (!fan_ok(fa.cpos[base], fa.cpos[base + i], fa.cpos[base + i + 1],
+                (d.flags & MOPH_DRAW_CULL) != 0, false, vw.w, vw.h))
+      continue;
+    uint32_t e = (t << 4) | (uint32_t)(i - 1);
+    SubTri st;
+    fused_subtri(fa, e, st);
+    if (fused_raster(fa, vw, st, sm_flat(draw_sm(d)), e))
+      emitted++;
+  }
+  if (emitted)
+    atomicAdd(&acc, emitted);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    blk_counts[blockIdx.x] = acc;
+}
+
+void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
+  if (!fa.n_tris)
+    return;
+  uint32_t nb = (fa.n_tris + WG - 1) / WG;
+  hipLaunchKernelGGL(k_tri, dim3(nb), dim3(WG), 0, s, fa);
+  launch_clip(fa, s);
+  /* blk_counts holds nb + CLIP_RASTER_GRID entries */
+  hipLaunchKernelGGL(k_clip_raster, dim3(CLIP_RASTER_GRID), dim3(WG), 0, s, fa,
+                     fa.blk_counts + nb);
+  hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(WG), 0, s, (const uint32_t *)fa.blk_counts,
+                     nb + CLIP_RASTER_GRID, fa.counters + CNT_SUBTRI);
+}
+
+void launch_raster_big(const FrameArgs &fa, hipStream_t s) {
   hipLaunchKernelGGL(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
   hipLaunchKernelGGL(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
 }
@@ -1392,7 +1670,10 @@ __global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
   if (s1 == 0)
     return;
   uint32_t s = s1 - 1;
-  SubTri st = frame.subtris[s];
+  if (frame.fused ? (s >> 4) >= frame.n_tris : s >= frame.subtri_cap)
+    return; /* not a key of this batch */
+  SubTri st;
+  load_subtri(frame, s, st);
   RasterTri r;
   if (!load_raster_tri(fa.w, fa.h, st, r))
     return;
@@ -1554,18 +1835,57 @@ DEV float srgb_enc(float c) {
   return c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
 }
 
+/* The colour effects of mop_postprocess_apply on one channel value when the
+ * result depends on that value alone (no fog, no vignette): tonemap, gamma,
+ * clamp, round (postprocess.c:45-116). */
+DEV uint8_t post_channel(uint8_t o, bool tonemap, bool gamma) {
+  float r = (float)o / 255.0f;
+  if (tonemap)
+    r = r / (r + 1.0f);
+  if (gamma)
+    r = srgb_enc(r > 0.0f ? r : 0.0f);
+  if (r > 1.0f)
+    r = 1.0f;
+  if (r < 0.0f)
+    r = 0.0f;
+  return cvt_u8(r * 255.0f + 0.5f);
+}
+
+/* 4 x 256 table: [tonemap | gamma << 1][value] */
+__global__ void k_post_lut(uint8_t *lut) {
+  lut[blockIdx.x * 256 + threadIdx.x] =
+      post_channel((uint8_t)threadIdx.x, (blockIdx.x & 1u) != 0, (blockIdx.x & 2u) != 0);
+}
+
+void launch_post_lut(uint8_t *lut, hipStream_t s) {
+  hipLaunchKernelGGL(k_post_lut, dim3(4), dim3(256), 0, s, lut);
+}
+
 /* mop_sw_fxaa (rasterizer.c:2625-2761) followed by the per-pixel effects of
  * mop_postprocess_apply (postprocess.c:36-116); FXAA reads only the copy of
- * the resolved image, so both fuse into one pass from src into dst. */
+ * the resolved image, so both fuse into one pass from src into dst.  The
+ * lumas of the 18x18 neighbourhood of the block's tile are computed once
+ * into LDS. */
 __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t nviews) {
+  __shared__ float lum[18][18];
   const PostViewGPU &v = views[find_post_view(views, nviews, blockIdx.x)];
   uint32_t t = blockIdx.x - v.tile_base;
   uint32_t tiles_x = ((uint32_t)v.w + 15) / 16;
-  int x = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
-  int y = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
+  int x0 = (int)(t % tiles_x) * 16, y0 = (int)(t / tiles_x) * 16;
+  int x = x0 + (int)(threadIdx.x & 15);
+  int y = y0 + (int)(threadIdx.x >> 4);
   const int w = v.w, h = v.h;
+  if (v.effects & MOP_HIP_POST_FXAA) {
+    for (int i = (int)threadIdx.x; i < 18 * 18; i += WG) {
+      int lx = x0 + i % 18 - 1, ly = y0 + i / 18 - 1;
+      lum[i / 18][i % 18] = (lx >= 0 && lx < w && ly >= 0 && ly < h) ? luma_at(v.src, w, lx, ly)
+                                                                     : 0.0f;
+    }
+    __syncthreads(); /* uniform: effects is per block */
+  }
   if (x >= w || y >= h)
     return;
+  const int tx = (int)(threadIdx.x & 15) + 1, ty = (int)(threadIdx.x >> 4) + 1;
   const uint8_t *src = v.src;
   uint8_t *dst = v.dst;
   const float *depth = v.depth;
@@ -1577,11 +1897,11 @@ __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t 
   if ((effects & MOP_HIP_POST_FXAA) && w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 &&
       y < h - 1) {
     const float EDGE_THRESHOLD = 0.0625f, EDGE_THRESHOLD_MIN = 0.004f, SUBPIX_QUALITY = 0.75f;
-    float lM = luma_at(src, w, x, y);
-    float lN = luma_at(src, w, x, y - 1);
-    float lS = luma_at(src, w, x, y + 1);
-    float lW = luma_at(src, w, x - 1, y);
-    float lE = luma_at(src, w, x + 1, y);
+    float lM = lum[ty][tx];
+    float lN = lum[ty - 1][tx];
+    float lS = lum[ty + 1][tx];
+    float lW = lum[ty][tx - 1];
+    float lE = lum[ty][tx + 1];
     float range_min = lM;
     if (lN < range_min) range_min = lN;
     if (lS < range_min) range_min = lS;
@@ -1594,10 +1914,10 @@ __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t 
     if (lE > range_max) range_max = lE;
     float range = range_max - range_min;
     if (!(range < EDGE_THRESHOLD * range_max + EDGE_THRESHOLD_MIN)) {
-      float lNW = luma_at(src, w, x - 1, y - 1);
-      float lNE = luma_at(src, w, x + 1, y - 1);
-      float lSW = luma_at(src, w, x - 1, y + 1);
-      float lSE = luma_at(src, w, x + 1, y + 1);
+      float lNW = lum[ty - 1][tx - 1];
+      float lNE = lum[ty - 1][tx + 1];
+      float lSW = lum[ty + 1][tx - 1];
+      float lSE = lum[ty + 1][tx + 1];
       float luma_avg = (lN + lS + lW + lE) * (2.0f / 12.0f) + (lNW + lNE + lSW + lSE) * (1.0f / 12.0f);
       float subpix = fabsf(luma_avg - lM);
       subpix = subpix / range;
@@ -1639,7 +1959,11 @@ __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t 
   }
   uint32_t rest = effects & (MOP_HIP_POST_GAMMA | MOP_HIP_POST_TONEMAP |
                              MOP_HIP_POST_VIGNETTE | MOP_HIP_POST_FOG);
-  if (effects != 0) {
+  if (effects != 0 && v.lut) {
+    o0 = v.lut[o0];
+    o1 = v.lut[o1];
+    o2 = v.lut[o2];
+  } else if (effects != 0) {
     /* postprocess.c runs its per-pixel loop whenever effects != 0 */
     float r = (float)o0 / 255.0f;
     float g = (float)o1 / 255.0f;
@@ -1744,28 +2068,73 @@ struct ShadowTri {
   float sz[3];
 };
 
-DEV bool shadow_setup(const ShadowArgsGPU &sa, uint32_t t, ShadowTri &st) {
-  uint32_t di = find_shadow_draw(sa.draws, sa.ndraws, t);
+DEV uint32_t find_shadow_draw_vtx(const ShadowDrawGPU *d, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (d[mid].vtx_base <= v)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* wave-uniform first search, as find_draw_by_tri_wave */
+DEV uint32_t find_shadow_draw_wave(const ShadowDrawGPU *d, uint32_t n, uint32_t t) {
+  uint32_t t0 = __builtin_amdgcn_readfirstlane(t);
+  uint32_t i = find_shadow_draw(d, n, t0);
+  uint32_t hi = (i + 1 < n) ? d[i + 1].tri_base : 0xffffffffu;
+  if (t >= d[i].tri_base && t < hi)
+    return i;
+  return find_shadow_draw(d, n, t);
+}
+
+DEV uint32_t find_shadow_draw_vtx_wave(const ShadowDrawGPU *d, uint32_t n, uint32_t v) {
+  uint32_t v0 = __builtin_amdgcn_readfirstlane(v);
+  uint32_t i = find_shadow_draw_vtx(d, n, v0);
+  uint32_t hi = (i + 1 < n) ? d[i + 1].vtx_base : 0xffffffffu;
+  if (v >= d[i].vtx_base && v < hi)
+    return i;
+  return find_shadow_draw_vtx(d, n, v);
+}
+
+/* Per caster vertex, once: light clip position and shadow-map position
+ * (the per-vertex part of mop_sw_shadow_render_mesh, rasterizer.c:383-420;
+ * a vertex shared by several triangles gets the same values each time). */
+__global__ __launch_bounds__(WG) void k_shadow_vertex(ShadowArgsGPU sa) {
+  uint32_t v = blockIdx.x * WG + threadIdx.x;
+  if (v >= sa.n_verts)
+    return;
+  const ShadowDrawGPU &d = sa.draws[find_shadow_draw_vtx_wave(sa.draws, sa.ndraws, v)];
+  const float *p = (const float *)(d.vbuf + (size_t)(v - d.vtx_base) * sizeof(MopHipVertex));
+  float4 c = mat4_mul_vec4(d.mvp, p[0], p[1], p[2], 1.0f);
+  float half = (float)sa.size * 0.5f;
+  float4 sc;
+  screen_xform(c, half, half, sc.x, sc.y, sc.z, sc.w);
+  sa.lclip[v] = c;
+  sa.lscr[v] = sc;
+}
+
+DEV bool shadow_setup(const ShadowArgsGPU &sa, uint32_t t, ShadowTri &st, bool wave = false) {
+  uint32_t di = wave ? find_shadow_draw_wave(sa.draws, sa.ndraws, t)
+                     : find_shadow_draw(sa.draws, sa.ndraws, t);
   const ShadowDrawGPU &d = sa.draws[di];
   uint32_t k = t - d.tri_base;
   uint32_t i0 = d.ibuf[3 * k], i1 = d.ibuf[3 * k + 1], i2 = d.ibuf[3 * k + 2];
   if (i0 >= d.vertex_count || i1 >= d.vertex_count || i2 >= d.vertex_count)
     return false;
-  const float *v0 = (const float *)(d.vbuf + (size_t)i0 * sizeof(MopHipVertex));
-  const float *v1 = (const float *)(d.vbuf + (size_t)i1 * sizeof(MopHipVertex));
-  const float *v2 = (const float *)(d.vbuf + (size_t)i2 * sizeof(MopHipVertex));
-  float4 c0 = mat4_mul_vec4(d.mvp, v0[0], v0[1], v0[2], 1.0f);
-  float4 c1 = mat4_mul_vec4(d.mvp, v1[0], v1[1], v1[2], 1.0f);
-  float4 c2 = mat4_mul_vec4(d.mvp, v2[0], v2[1], v2[2], 1.0f);
+  uint32_t v0 = d.vtx_base + i0, v1 = d.vtx_base + i1, v2 = d.vtx_base + i2;
+  float4 c0 = sa.lclip[v0], c1 = sa.lclip[v1], c2 = sa.lclip[v2];
   if (trivially_rejected(c0, c1, c2))
     return false;
   if (c0.w <= 1e-6f || c1.w <= 1e-6f || c2.w <= 1e-6f)
     return false;
-  float half = (float)sa.size * 0.5f;
-  float sx[3], sy[3], iw[3];
-  screen_xform(c0, half, half, sx[0], sy[0], st.sz[0], iw[0]);
-  screen_xform(c1, half, half, sx[1], sy[1], st.sz[1], iw[1]);
-  screen_xform(c2, half, half, sx[2], sy[2], st.sz[2], iw[2]);
+  float4 s0 = sa.lscr[v0], s1 = sa.lscr[v1], s2 = sa.lscr[v2];
+  float sx[3] = {s0.x, s1.x, s2.x}, sy[3] = {s0.y, s1.y, s2.y};
+  st.sz[0] = s0.z;
+  st.sz[1] = s1.z;
+  st.sz[2] = s2.z;
   return edge_setup(sx, sy, sa.size, sa.size, st.es);
 }
 
@@ -1776,9 +2145,7 @@ DEV void shadow_frag(const ShadowArgsGPU &sa, const ShadowTri &st, int x, int y,
   if (z != z)
     return;
   uint32_t k = depth_key(z);
-  uint32_t *p = sa.keys + (size_t)y * (size_t)sa.size + (size_t)x;
-  if (k < __builtin_nontemporal_load(p))
-    atomicMin(p, k);
+  atomicMin(sa.keys + (size_t)y * (size_t)sa.size + (size_t)x, k);
 }
 
 __global__ __launch_bounds__(WG) void k_shadow_small(ShadowArgsGPU sa) {
@@ -1786,7 +2153,7 @@ __global__ __launch_bounds__(WG) void k_shadow_small(ShadowArgsGPU sa) {
   if (t >= sa.n_tris)
     return;
   ShadowTri st;
-  if (!shadow_setup(sa, t, st))
+  if (!shadow_setup(sa, t, st, true))
     return;
   if (bbox_area(st.es) > MOPH_SMALL_AREA) {
     uint32_t q = atomicAdd(&sa.counters[CNT_SHADOW_BIG], 1u);
@@ -1866,6 +2233,7 @@ __global__ __launch_bounds__(WG) void k_shadow_big(ShadowArgsGPU sa) {
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s) {
   if (!sa.n_tris)
     return;
+  hipLaunchKernelGGL(k_shadow_vertex, dim3((sa.n_verts + WG - 1) / WG), dim3(WG), 0, s, sa);
   hipLaunchKernelGGL(k_shadow_small, dim3((sa.n_tris + WG - 1) / WG), dim3(WG), 0, s, sa);
   hipLaunchKernelGGL(k_shadow_big, dim3(2048), dim3(WG), 0, s, sa);
 }

@@ -146,8 +146,12 @@ struct FrameArgs {
   uint32_t small_area;  /* bbox pixels up to which one lane walks a triangle */
   uint32_t shade_tile;  /* shading tile edge: 8, 16 or 32 */
   uint32_t raster_grid; /* workgroups of the small-triangle raster (0 = auto) */
-  uint32_t pad0_;
+  uint32_t fused;       /* 1: order-free path (no screen-triangle store) */
   float4 *vpos;   /* n_verts clip positions */
+  float4 *vscr;   /* n_verts screen positions (sx, sy, sz, 1/w) in the draw's view */
+  uint4 *tri_rec; /* order-free path, per triangle: (v0, v1, v2, draw) when drawn
+                     whole, (clip slot, 0, 0, draw) when queued for clipping */
+  uint32_t *blk_counts; /* order-free path: screen triangles per k_tri workgroup */
   float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */
   float4 *cpos;   /* clip_cap * MOPH_MAX_POLY */
   float4 *cattr;  /* clip_cap * MOPH_MAX_POLY * 3 */
@@ -175,6 +179,7 @@ struct PostViewGPU {
   float fog_r, fog_g, fog_b, fog_near, fog_far;
   int32_t has_fog;
   int32_t pad_;
+  const uint8_t *lut; /* per-channel table of the colour effects, or null */
 };
 
 /* One shadow caster mesh (mop_sw_shadow_render_mesh arguments). */
@@ -185,7 +190,7 @@ struct ShadowDrawGPU {
   uint32_t vertex_count;
   uint32_t tri_count;
   uint32_t tri_base;
-  uint32_t pad_;
+  uint32_t vtx_base; /* first slot in the light-space vertex cache */
 };
 
 struct ShadowArgsGPU {
@@ -194,6 +199,10 @@ struct ShadowArgsGPU {
   uint32_t n_tris;
   int32_t size;
   uint32_t big_cap;
+  uint32_t n_verts;
+  uint32_t pad_;
+  float4 *lclip; /* n_verts light clip positions */
+  float4 *lscr;  /* n_verts shadow-map positions (sx, sy, sz, -) */
   uint32_t *keys; /* size*size ordered depth keys */
   uint32_t *big_queue;
   uint32_t *counters;
@@ -203,6 +212,10 @@ struct ShadowArgsGPU {
 void launch_vertex(const FrameArgs &fa, hipStream_t s);
 void launch_classify(const FrameArgs &fa, hipStream_t s);
 void launch_clip(const FrameArgs &fa, hipStream_t s);
+/* workgroups of the order-free clipped-fan raster (k_clip_raster) */
+#define CLIP_RASTER_GRID 256
+void launch_fused_setup(const FrameArgs &fa, hipStream_t s);
+void launch_raster_big(const FrameArgs &fa, hipStream_t s);
 void launch_scan_counts(const FrameArgs &fa, uint32_t *block_sums, hipStream_t s);
 void launch_emit(const FrameArgs &fa, hipStream_t s);
 void launch_raster(const FrameArgs &fa, hipStream_t s);
@@ -220,6 +233,7 @@ void launch_vis_from_depth(int npx, const float *depth,
 /* resolve (mode 0) or post (mode 1) over the tiles of n views */
 void launch_resolve_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
                           hipStream_t s);
+void launch_post_lut(uint8_t *lut, hipStream_t s);
 void launch_post_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
                        hipStream_t s);
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);

@@ -223,7 +223,9 @@ struct MopHipDevice {
   /* frame scratch */
   DevArr<DrawGPU> draws;
   DevArr<MopHipLight> lights;
-  DevArr<float4> vpos, vattr, cpos, cattr;
+  DevArr<float4> vpos, vscr, vattr, cpos, cattr;
+  DevArr<uint4> tri_rec;
+  DevArr<uint32_t> blk_counts;
   DevArr<uint32_t> cnt_poly, tri_counts, clip_queue, clip_slot, counters, block_sums,
       big_queue;
   DevArr<SubTri> subtris;
@@ -235,10 +237,12 @@ struct MopHipDevice {
   uint32_t *h_counters = nullptr;
   DevArr<ViewGPU> views;
   DevArr<PostViewGPU> pviews;
+  DevArr<uint8_t> post_lut; /* 4 x 256 colour-effect tables */
   /* shadow: the map the hook API renders into, and the recorded casters */
   MopHipShadowMap *def_shadow = nullptr;
   MopHipShadowMap *rec_shadow = nullptr; /* map the recorded casters target */
   DevArr<uint32_t> shadow_big;
+  DevArr<float4> shadow_lclip, shadow_lscr;
   DevArr<ShadowDrawGPU> shadow_draws_dev;
   std::vector<ShadowDrawHost> shadow_draws;
   std::vector<MopHipBuffer *> shadow_temp;
@@ -283,6 +287,8 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   for (int k = 0; k < NSTAGE; k++)
     HIPCHK(hipEventCreateWithFlags(&d->ring[k].ev, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&d->h_counters, CNT_N * sizeof(uint32_t), 0));
+  if (d->post_lut.ensure(4 * 256))
+    launch_post_lut(d->post_lut.p, d->stream);
   return d;
 }
 
@@ -306,6 +312,12 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->subtris.release();
   d->shadow_big.release();
   d->shadow_draws_dev.release();
+  d->shadow_lclip.release();
+  d->shadow_lscr.release();
+  d->vscr.release();
+  d->tri_rec.release();
+  d->blk_counts.release();
+  d->post_lut.release();
   mop_hip_shadow_map_destroy(d, d->def_shadow);
   mop_hip_ibl_destroy(d, d->def_ibl);
   for (MopHipBuffer *b : d->shadow_temp) {
@@ -949,6 +961,17 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.small_area = d->small_area;
   fa.shade_tile = d->shade_tile;
   fa.raster_grid = d->raster_grid;
+  /* order-free batches use the fused path while (triangle << 4 | fan index)
+   * fits the 32-bit sequence half of the visibility key */
+  fa.fused = (!ordered && n_tris < (1u << 27)) ? 1u : 0u;
+  if (fa.fused) {
+    if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
+        !d->blk_counts.ensure(n_tris / 256 + 1 + CLIP_RASTER_GRID))
+      return false;
+    fa.vscr = d->vscr.p;
+    fa.tri_rec = d->tri_rec.p;
+    fa.blk_counts = d->blk_counts.p;
+  }
   fa.vpos = d->vpos.p;
   fa.vattr = d->vattr.p;
   fa.cpos = d->cpos.p;
@@ -964,7 +987,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
 
   if (!ordered)
     for (const SegJob &j : jobs)
-      if (j.seg > 0)
+      if (j.seg > 0 || !j.fb->has_clear)
         launch_vis_from_depth(j.fb->w * j.fb->h, j.fb->depth, j.fb->vis, d->stream);
   uint32_t bin_cap = 0;
   if (ordered) {
@@ -983,6 +1006,22 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     {
       TmScope t(d, TM_VERTEX);
       launch_vertex(fa, d->stream);
+    }
+    if (fa.fused) {
+      {
+        TmScope t(d, TM_SETUP);
+        launch_fused_setup(fa, d->stream);
+      }
+      {
+        TmScope t(d, TM_RASTER);
+        launch_raster_big(fa, d->stream);
+      }
+      {
+        TmScope t(d, TM_SHADE);
+        launch_shade(fa, d->stream);
+      }
+      launch_export_counters(fa, d->stream);
+      return true;
     }
     {
       TmScope t(d, TM_SETUP);
@@ -1069,6 +1108,14 @@ static void run_post(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fb
       v.fog_b = fb->fog.color.b;
       v.fog_near = fb->fog.near_dist;
       v.fog_far = fb->fog.far_dist;
+    }
+    /* without fog or vignette the colour effects map each channel value on
+     * its own: a 256-entry table per (tonemap, gamma) */
+    bool fog_on = (fb->post_effects & MOP_HIP_POST_FOG) && fb->has_fog;
+    if (!fog_on && !(fb->post_effects & MOP_HIP_POST_VIGNETTE) && d->post_lut.p) {
+      int variant = ((fb->post_effects & MOP_HIP_POST_TONEMAP) ? 1 : 0) |
+                    ((fb->post_effects & MOP_HIP_POST_GAMMA) ? 2 : 0);
+      v.lut = d->post_lut.p + variant * 256;
     }
     fb->cur = 1 - fb->cur;
     fb->v_color = false;
@@ -1450,18 +1497,21 @@ static void run_shadow(MopHipDevice *d) {
   int size = map->size;
   size_t npx = (size_t)size * size;
   launch_shadow_clear((int)npx, map->keys.p, d->stream);
-  uint32_t n_tris = 0;
+  uint32_t n_tris = 0, n_verts = 0;
   std::vector<ShadowDrawGPU> gd;
   for (ShadowDrawHost &sh : d->shadow_draws) {
     ShadowDrawGPU g = sh.g;
     g.tri_base = n_tris;
+    g.vtx_base = n_verts;
     n_tris += g.tri_count;
+    n_verts += g.vertex_count;
     gd.push_back(g);
   }
   if (!gd.empty() && n_tris) {
     /* the big-triangle queue holds every caster in the worst case */
     if (!d->shadow_big.ensure(n_tris) || !d->shadow_draws_dev.ensure(gd.size()) ||
-        !d->counters.ensure(CNT_N))
+        !d->counters.ensure(CNT_N) || !d->shadow_lclip.ensure(n_verts ? n_verts : 1) ||
+        !d->shadow_lscr.ensure(n_verts ? n_verts : 1))
       return;
     size_t bytes = gd.size() * sizeof(ShadowDrawGPU);
     int slot;
@@ -1481,6 +1531,10 @@ static void run_shadow(MopHipDevice *d) {
     sa.keys = map->keys.p;
     sa.big_queue = d->shadow_big.p;
     sa.counters = d->counters.p;
+    sa.n_verts = n_verts;
+    sa.pad_ = 0;
+    sa.lclip = d->shadow_lclip.p;
+    sa.lscr = d->shadow_lscr.p;
     launch_shadow_raster(sa, d->stream);
     d->stats.shadow_tris = n_tris;
   }
@@ -1549,6 +1603,11 @@ extern "C" void mop_hip_shadow_draw(MopHipDevice *d, MopHipBuffer *vb, uint32_t 
                                     const MopHipMat4 *model, const MopHipMat4 *light_vp) {
   if (!d || !vb || !ib || !model || !light_vp || !d->rec_shadow)
     return;
+  /* the kernels read every vertex and index counted here: clamp to the buffers */
+  if ((size_t)vertex_count * sizeof(MopHipVertex) > vb->size)
+    vertex_count = (uint32_t)(vb->size / sizeof(MopHipVertex));
+  if ((size_t)index_count * sizeof(uint32_t) > ib->size)
+    index_count = (uint32_t)(ib->size / sizeof(uint32_t));
   shadow_add(d, vb->dev, vertex_count, ib->dev, index_count, *model, *light_vp);
 }
 
