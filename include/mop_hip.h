@@ -66,7 +66,7 @@ typedef struct MopHipLight {
 /* vertex_format.h:25-70 */
 enum {
   MOP_HIP_ATTRIB_POSITION = 0, MOP_HIP_ATTRIB_NORMAL = 1, MOP_HIP_ATTRIB_COLOR = 2,
-  MOP_HIP_ATTRIB_TEXCOORD0 = 3
+  MOP_HIP_ATTRIB_TEXCOORD0 = 3, MOP_HIP_ATTRIB_JOINTS = 6, MOP_HIP_ATTRIB_WEIGHTS = 7
 };
 enum {
   MOP_HIP_FORMAT_FLOAT = 0, MOP_HIP_FORMAT_FLOAT2 = 1, MOP_HIP_FORMAT_FLOAT3 = 2,
@@ -247,6 +247,26 @@ void mop_hip_frame_request_post(MopHipDevice *dev, MopHipFramebuffer *fb, float 
                                 uint32_t effects, const MopHipFogParams *fog);
 /* Execute every recorded, not yet executed frame of fbs[0..n) as one batch. */
 void mop_hip_frames_execute(MopHipDevice *dev, MopHipFramebuffer *const *fbs, size_t n);
+/* ---- skinning and morph targets (device side of a dispatch hook) ----
+ * mop_skin_apply (viewport.c:1572-1680) and mop_morph_apply
+ * (viewport.c:1763-1805) compute the deformed vertices on the host and upload
+ * them with buffer_update.  These entry points take the same inputs -- the
+ * mesh's bind-pose snapshot, its vertex format and the bone matrices / morph
+ * targets -- and write the deformed vertices straight into the draw's vertex
+ * buffer on the device, with the reference's arithmetic; buffer_read returns
+ * the deformed bytes afterwards.  The bind pose is copied to the device on
+ * the first call for a buffer (and again if the pointer changes).
+ * skin: fmt must hold POSITION (float3), JOINTS (ubyte4), WEIGHTS (float4);
+ * NORMAL is skinned and renormalized when present.  morph: fmt NULL means
+ * MopVertex; targets are target_count x vertex_count x 3 floats.
+ * Both return 0, or -1 with mop_hip_last_error() set. */
+int mop_hip_skin_apply(MopHipDevice *dev, MopHipBuffer *vb, const void *bind_pose,
+                       uint32_t vertex_count, const MopHipVertexFormat *fmt,
+                       const MopHipMat4 *bones, uint32_t bone_count);
+int mop_hip_morph_apply(MopHipDevice *dev, MopHipBuffer *vb, const void *bind_pose,
+                        uint32_t vertex_count, const MopHipVertexFormat *fmt,
+                        const float *targets, const float *weights, uint32_t target_count);
+
 /* Launch-shape tunables (results never depend on them): bbox area up to
  * which one lane walks a screen triangle (default 64), shading tile edge
  * 8/16/32 (default 16), workgroups of the small-triangle raster (0 = auto).

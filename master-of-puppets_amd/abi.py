@@ -34,6 +34,25 @@ class BufferDesc(C.Structure):
     _fields_ = [("data", C.c_void_p), ("size", C.c_size_t)]
 
 
+class VertexAttrib(C.Structure):
+    _fields_ = [("semantic", C.c_int32), ("format", C.c_int32), ("offset", C.c_uint32)]
+
+
+class VertexFormat(C.Structure):
+    _fields_ = [("attribs", VertexAttrib * 12), ("attrib_count", C.c_uint32),
+                ("stride", C.c_uint32)]
+
+
+def vertex_format(attrs, stride):
+    """attrs: [(semantic, format, offset)] -> VertexFormat"""
+    f = VertexFormat()
+    for i, (sem, fmt, off) in enumerate(attrs):
+        f.attribs[i] = VertexAttrib(sem, fmt, off)
+    f.attrib_count = len(attrs)
+    f.stride = stride
+    return f
+
+
 class FramebufferDesc(C.Structure):
     _fields_ = [("width", C.c_int), ("height", C.c_int)]
 
@@ -142,6 +161,8 @@ SIGS = {
     "mop_hip_frames_execute": (None, [VP, C.POINTER(VP), C.c_size_t]),
     "mop_hip_frame_done": (C.c_int, [VP, VP]),
     "mop_hip_set_tuning": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "mop_hip_skin_apply": (C.c_int, [VP, VP, VP, C.c_uint32, VP, VP, C.c_uint32]),
+    "mop_hip_morph_apply": (C.c_int, [VP, VP, VP, C.c_uint32, VP, VP, VP, C.c_uint32]),
     "mop_hip_device_sync": (None, [VP]),
 }
 

@@ -212,6 +212,33 @@ struct ShadowArgsGPU {
 void launch_vertex(const FrameArgs &fa, hipStream_t s);
 void launch_classify(const FrameArgs &fa, hipStream_t s);
 void launch_clip(const FrameArgs &fa, hipStream_t s);
+/* Skinning: bind-pose records in, deformed records out (same layout). */
+struct SkinArgs {
+  const uint8_t *bind;
+  uint8_t *out;
+  const float *bones; /* bone_count column-major 4x4 */
+  uint32_t vertex_count;
+  uint32_t stride;
+  uint32_t bone_count;
+  int32_t pos_off, normal_off, joints_off, weights_off; /* normal_off < 0: none */
+  int32_t aligned; /* stride and buffers dword aligned */
+};
+
+struct MorphArgs {
+  const uint8_t *bind;
+  uint8_t *out;
+  const float *targets; /* target_count x vertex_count x 3 */
+  const float *weights;
+  uint32_t vertex_count;
+  uint32_t stride;
+  uint32_t target_count;
+  int32_t pos_off;
+  int32_t aligned;
+};
+
+void launch_skin(const SkinArgs &a, hipStream_t s);
+void launch_morph(const MorphArgs &a, hipStream_t s);
+
 /* workgroups of the order-free clipped-fan raster (k_clip_raster) */
 #define CLIP_RASTER_GRID 256
 void launch_fused_setup(const FrameArgs &fa, hipStream_t s);

@@ -83,6 +83,12 @@ struct MopHipBuffer {
   void *host = nullptr;
   void *dev = nullptr;
   size_t size = 0;
+  MopHipDevice *owner = nullptr;
+  bool host_stale = false; /* device side rewritten by skinning/morphing */
+  /* device copy of the bind pose handed to the skinning/morph entry points */
+  void *bind_dev = nullptr;
+  const void *bind_src = nullptr;
+  size_t bind_size = 0;
 };
 
 struct MopHipTexture {
@@ -226,6 +232,7 @@ struct MopHipDevice {
   DevArr<float4> vpos, vscr, vattr, cpos, cattr;
   DevArr<uint4> tri_rec;
   DevArr<uint32_t> blk_counts;
+  DevArr<float> skin_bones, morph_targets, morph_weights;
   DevArr<uint32_t> cnt_poly, tri_counts, clip_queue, clip_slot, counters, block_sums,
       big_queue;
   DevArr<SubTri> subtris;
@@ -317,6 +324,9 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->vscr.release();
   d->tri_rec.release();
   d->blk_counts.release();
+  d->skin_bones.release();
+  d->morph_targets.release();
+  d->morph_weights.release();
   d->post_lut.release();
   mop_hip_shadow_map_destroy(d, d->def_shadow);
   mop_hip_ibl_destroy(d, d->def_ibl);
@@ -472,6 +482,7 @@ extern "C" MopHipBuffer *mop_hip_buffer_create(MopHipDevice *d, const MopHipBuff
   }
   if (desc->size)
     HIPCHK(hipMemcpy(b->dev, b->host, desc->size, hipMemcpyHostToDevice));
+  b->owner = d;
   std::lock_guard<std::mutex> lk(d->mu);
   d->host2buf[b->host] = b;
   return b;
@@ -486,6 +497,8 @@ extern "C" void mop_hip_buffer_destroy(MopHipDevice *d, MopHipBuffer *b) {
     d->host2buf.erase(b->host);
   }
   (void)hipFree(b->dev);
+  if (b->bind_dev)
+    (void)hipFree(b->bind_dev);
   free(b->host);
   delete b;
 }
@@ -501,7 +514,131 @@ extern "C" void mop_hip_buffer_update(MopHipDevice *d, MopHipBuffer *b, const vo
                    hipMemcpyHostToDevice));
 }
 
-extern "C" const void *mop_hip_buffer_read(MopHipBuffer *b) { return b ? b->host : nullptr; }
+/* Host-visible bytes (the shadow pass and skinning read them,
+ * viewport.c:2426-2429, 1539); refreshed from the device after skinning. */
+extern "C" const void *mop_hip_buffer_read(MopHipBuffer *b) {
+  if (!b)
+    return nullptr;
+  if (b->host_stale && b->owner) {
+    HIPCHK(hipStreamSynchronize(b->owner->stream));
+    HIPCHK(hipMemcpy(b->host, b->dev, b->size, hipMemcpyDeviceToHost));
+    b->host_stale = false;
+  }
+  return b->host;
+}
+
+/* ------------------------------------------------------------------------ */
+/* skinning and morph targets                                                */
+/* ------------------------------------------------------------------------ */
+
+static const MopHipVertexAttrib *find_attr(const MopHipVertexFormat *f, int sem);
+
+/* the bind pose of vb on the device (uploaded once per bind-pose pointer) */
+static bool bind_pose_on_device(MopHipDevice *d, MopHipBuffer *vb, const void *bind,
+                                size_t bytes) {
+  if (vb->bind_dev && vb->bind_src == bind && vb->bind_size == bytes)
+    return true;
+  HIPCHK(hipStreamSynchronize(d->stream));
+  if (vb->bind_dev)
+    (void)hipFree(vb->bind_dev);
+  vb->bind_dev = nullptr;
+  if (hipMalloc(&vb->bind_dev, bytes ? bytes : 4) != hipSuccess) {
+    set_err("hipMalloc(%zu) for a bind pose failed", bytes);
+    return false;
+  }
+  HIPCHK(hipMemcpy(vb->bind_dev, bind, bytes, hipMemcpyHostToDevice));
+  vb->bind_src = bind;
+  vb->bind_size = bytes;
+  return true;
+}
+
+extern "C" int mop_hip_skin_apply(MopHipDevice *d, MopHipBuffer *vb, const void *bind_pose,
+                                  uint32_t vertex_count, const MopHipVertexFormat *fmt,
+                                  const MopHipMat4 *bones, uint32_t bone_count) {
+  if (!d || !vb || !bind_pose || !fmt || !bones || bone_count == 0) {
+    set_err("skin_apply: missing argument");
+    return -1;
+  }
+  const MopHipVertexAttrib *pa = find_attr(fmt, MOP_HIP_ATTRIB_POSITION);
+  const MopHipVertexAttrib *na = find_attr(fmt, MOP_HIP_ATTRIB_NORMAL);
+  const MopHipVertexAttrib *ja = find_attr(fmt, MOP_HIP_ATTRIB_JOINTS);
+  const MopHipVertexAttrib *wa = find_attr(fmt, MOP_HIP_ATTRIB_WEIGHTS);
+  if (!pa || !ja || !wa) {
+    set_err("skin_apply: format lacks POSITION, JOINTS or WEIGHTS");
+    return -1;
+  }
+  size_t stride = fmt->stride, bytes = (size_t)vertex_count * stride;
+  if (stride == 0 || bytes > vb->size || pa->offset + 12 > stride || ja->offset + 4 > stride ||
+      wa->offset + 16 > stride || (na && na->offset + 12 > stride)) {
+    set_err("skin_apply: vertex format does not fit the buffer");
+    return -1;
+  }
+  if (!bind_pose_on_device(d, vb, bind_pose, bytes) || !d->skin_bones.ensure((size_t)bone_count * 16))
+    return -1;
+  HIPCHK(hipMemcpyAsync(d->skin_bones.p, bones, (size_t)bone_count * sizeof(MopHipMat4),
+                        hipMemcpyHostToDevice, d->stream));
+  SkinArgs a;
+  a.bind = (const uint8_t *)vb->bind_dev;
+  a.out = (uint8_t *)vb->dev;
+  a.bones = d->skin_bones.p;
+  a.vertex_count = vertex_count;
+  a.stride = (uint32_t)stride;
+  a.bone_count = bone_count;
+  a.pos_off = (int32_t)pa->offset;
+  a.normal_off = na ? (int32_t)na->offset : -1;
+  a.joints_off = (int32_t)ja->offset;
+  a.weights_off = (int32_t)wa->offset;
+  a.aligned = (stride % 4) == 0;
+  launch_skin(a, d->stream);
+  vb->host_stale = true;
+  return 0;
+}
+
+extern "C" int mop_hip_morph_apply(MopHipDevice *d, MopHipBuffer *vb, const void *bind_pose,
+                                   uint32_t vertex_count, const MopHipVertexFormat *fmt,
+                                   const float *targets, const float *weights,
+                                   uint32_t target_count) {
+  if (!d || !vb || !bind_pose || !targets || !weights || target_count == 0) {
+    set_err("morph_apply: missing argument");
+    return -1;
+  }
+  size_t stride = fmt ? fmt->stride : sizeof(MopHipVertex);
+  int32_t pos_off = 0; /* position is at offset 0 in MopVertex */
+  if (fmt) {
+    const MopHipVertexAttrib *pa = find_attr(fmt, MOP_HIP_ATTRIB_POSITION);
+    if (!pa) {
+      set_err("morph_apply: format lacks POSITION");
+      return -1;
+    }
+    pos_off = (int32_t)pa->offset;
+  }
+  size_t bytes = (size_t)vertex_count * stride;
+  size_t tcount = (size_t)target_count * vertex_count * 3;
+  if (stride == 0 || bytes > vb->size || (size_t)pos_off + 12 > stride) {
+    set_err("morph_apply: vertex format does not fit the buffer");
+    return -1;
+  }
+  if (!bind_pose_on_device(d, vb, bind_pose, bytes) || !d->morph_targets.ensure(tcount) ||
+      !d->morph_weights.ensure(target_count))
+    return -1;
+  HIPCHK(hipMemcpyAsync(d->morph_targets.p, targets, tcount * sizeof(float),
+                        hipMemcpyHostToDevice, d->stream));
+  HIPCHK(hipMemcpyAsync(d->morph_weights.p, weights, (size_t)target_count * sizeof(float),
+                        hipMemcpyHostToDevice, d->stream));
+  MorphArgs a;
+  a.bind = (const uint8_t *)vb->bind_dev;
+  a.out = (uint8_t *)vb->dev;
+  a.targets = d->morph_targets.p;
+  a.weights = d->morph_weights.p;
+  a.vertex_count = vertex_count;
+  a.stride = (uint32_t)stride;
+  a.target_count = target_count;
+  a.pos_off = pos_off;
+  a.aligned = (stride % 4) == 0;
+  launch_morph(a, d->stream);
+  vb->host_stale = true;
+  return 0;
+}
 
 extern "C" MopHipTexture *mop_hip_texture_create(MopHipDevice *d, int w, int h,
                                                  const uint8_t *rgba) {

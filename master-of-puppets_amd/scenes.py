@@ -86,6 +86,21 @@ class InstancedMesh:
 
 
 @dataclass
+class SkinnedMesh:
+    """A flexible-format mesh (mop_viewport_add_mesh_ex) with bone matrices
+    and optional morph targets (mop_mesh_set_bone_matrices / _morph_targets)."""
+    data: np.ndarray              # (n, stride) uint8 interleaved vertex records
+    indices: np.ndarray
+    object_id: int
+    stride: int
+    attrs: list                   # [(semantic, format, offset)]
+    bones: np.ndarray = None      # (nb, 16) float32 column-major, bind -> current
+    morph_targets: np.ndarray = None  # (nt, n, 3) float32
+    morph_weights: np.ndarray = None  # (nt,) float32
+    model: np.ndarray = None
+
+
+@dataclass
 class Light:
     type: int                     # 0 directional, 1 point, 2 spot
     position: tuple = (0.0, 0.0, 0.0)
@@ -105,6 +120,7 @@ class Scene:
     height: int
     meshes: list = field(default_factory=list)
     instanced: list = field(default_factory=list)
+    skinned: list = field(default_factory=list)
     lights: list = field(default_factory=list)
     eye: tuple = (0.0, 0.0, 5.0)
     target: tuple = (0.0, 0.0, 0.0)
@@ -570,4 +586,88 @@ def wireframe_scene(seed=31, width=160, height=120, lit=True):
     sc = shading_scene(seed, width, height, lit=lit)
     sc.name = f"wire_{'lit' if lit else 'unlit'}_{seed}_{width}x{height}"
     sc.render_mode = 1
+    return sc
+
+
+# flexible-format layout of the skinned meshes: position, normal, colour,
+# joints (ubyte4), weights (float4) -- MOP_ATTRIB_* / MOP_FORMAT_* values
+SKIN_STRIDE = 60
+SKIN_ATTRS = [(0, 2, 0), (1, 2, 12), (2, 3, 24), (6, 4, 40), (7, 3, 44)]
+
+
+def skinned_tube(rng, n_bones=4, rings=24, segs=24, radius=0.35, length=3.0, color=None):
+    """A tube along +y skinned to a chain of bones (each vertex weighted to its
+    two nearest joints), in the bind pose; returns (records, indices)."""
+    col = color or (rng.uniform(0.2, 1.0), rng.uniform(0.2, 1.0), rng.uniform(0.2, 1.0), 1.0)
+    n = (rings + 1) * segs
+    rec = np.zeros((n, SKIN_STRIDE), np.uint8)
+    f = np.zeros((n, 10), np.float32)
+    joints = np.zeros((n, 4), np.uint8)
+    weights = np.zeros((n, 4), np.float32)
+    k = 0
+    for r in range(rings + 1):
+        y = length * r / rings
+        b = y / length * (n_bones - 1)
+        b0 = min(int(b), n_bones - 2)
+        t = np.float32(b - b0)
+        for s_ in range(segs):
+            a = 2.0 * math.pi * s_ / segs
+            nx, nz = math.cos(a), math.sin(a)
+            f[k] = (radius * nx, y, radius * nz, nx, 0.0, nz, *col)
+            joints[k] = (b0, b0 + 1, 0, 0)
+            weights[k] = (1.0 - t, t, 0.0, 0.0)
+            k += 1
+    rec[:, 0:40] = f.view(np.uint8).reshape(n, 40)
+    rec[:, 40:44] = joints
+    rec[:, 44:60] = weights.view(np.uint8).reshape(n, 16)
+    idx = []
+    for r in range(rings):
+        for s_ in range(segs):
+            a = r * segs + s_
+            b = r * segs + (s_ + 1) % segs
+            c = a + segs
+            d = b + segs
+            idx += [a, c, b, b, c, d]
+    return rec, np.asarray(idx, np.uint32)
+
+
+def bend_bones(n_bones, angle, length=3.0):
+    """Bind-to-current matrices bending the chain about z by `angle` per joint."""
+    out = []
+    acc = np.eye(4, dtype=np.float64)
+    seg = length / (n_bones - 1)
+    for b in range(n_bones):
+        y0 = b * seg
+        to = np.eye(4); to[1, 3] = -y0
+        back = np.eye(4); back[1, 3] = y0
+        c, s_ = math.cos(angle), math.sin(angle)
+        rot = np.array([[c, -s_, 0, 0], [s_, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]])
+        if b > 0:
+            acc = acc @ back @ rot @ to
+        out.append(acc.T.astype(np.float32).reshape(16))  # column-major
+    return np.stack(out)
+
+
+def skinned_scene(seed=51, width=160, height=120, n_tubes=3, morph=False):
+    """Skinned tubes (and, with morph, morph-targeted ones) over the small
+    scene: the core skins them on its side and uploads the deformed vertices;
+    the drop-in renders the flexible-format draws."""
+    sc = small_scene(seed=seed, width=width, height=height, n=4)
+    sc.name = f"skinned_{seed}_{'morph' if morph else 'bones'}_{width}x{height}"
+    rng = Xoshiro(seed * 31337 + 7)
+    for k in range(n_tubes):
+        rec, idx = skinned_tube(rng)
+        nb = 4
+        bones = bend_bones(nb, rng.uniform(-0.5, 0.5))
+        m = trs((rng.uniform(-3, 3), rng.uniform(-1.5, 0.5), rng.uniform(-2, 2)), random_quat(rng),
+                1.0)
+        sm = SkinnedMesh(rec, idx, 400 + k, SKIN_STRIDE, SKIN_ATTRS, bones=bones, model=m)
+        if morph:
+            nv = len(rec)
+            sm.bones = None
+            sm.morph_targets = np.stack([
+                np.asarray([[rng.uniform(-0.2, 0.2) for _ in range(3)] for _ in range(nv)], np.float32)
+                for _ in range(2)])
+            sm.morph_weights = np.asarray([rng.uniform(0.2, 1.0), rng.uniform(-0.5, 0.5)], np.float32)
+        sc.skinned.append(sm)
     return sc

@@ -188,6 +188,101 @@ void mop_oracle_buffer_update(MopHipDevice *d, MopHipBuffer *b, const void *data
 }
 const void *mop_oracle_buffer_read(MopHipBuffer *b) { return b ? b->data : NULL; }
 
+static const MopHipVertexAttrib *fmt_attr(const MopHipVertexFormat *f, int sem) {
+  for (uint32_t i = 0; i < f->attrib_count && i < 12; i++)
+    if (f->attribs[i].semantic == sem)
+      return &f->attribs[i];
+  return NULL;
+}
+
+/* mop_skin_apply (viewport.c:1572-1680): linear-blend skinning of the bind
+ * pose into the vertex buffer. */
+int mop_oracle_skin_apply(MopHipDevice *d, MopHipBuffer *vb, const void *bind, uint32_t vc,
+                          const MopHipVertexFormat *fmt, const MopHipMat4 *bones, uint32_t nb) {
+  (void)d;
+  const MopHipVertexAttrib *pa = fmt_attr(fmt, MOP_HIP_ATTRIB_POSITION);
+  const MopHipVertexAttrib *na = fmt_attr(fmt, MOP_HIP_ATTRIB_NORMAL);
+  const MopHipVertexAttrib *ja = fmt_attr(fmt, MOP_HIP_ATTRIB_JOINTS);
+  const MopHipVertexAttrib *wa = fmt_attr(fmt, MOP_HIP_ATTRIB_WEIGHTS);
+  if (!pa || !ja || !wa || (size_t)vc * fmt->stride > vb->size)
+    return -1;
+  uint8_t *out = vb->data;
+  memcpy(out, bind, (size_t)vc * fmt->stride);
+  for (uint32_t v = 0; v < vc; v++) {
+    const uint8_t *src = (const uint8_t *)bind + (size_t)v * fmt->stride;
+    uint8_t *dst = out + (size_t)v * fmt->stride;
+    const uint8_t *j = src + ja->offset;
+    float w[4], p[3];
+    memcpy(w, src + wa->offset, 16);
+    memcpy(p, src + pa->offset, 12);
+    float acc[3] = {0, 0, 0};
+    for (int b = 0; b < 4; b++) {
+      if (w[b] <= 0.0f || j[b] >= nb)
+        continue;
+      const float *m = bones[j[b]].d;
+      float r[3] = {m[0] * p[0] + m[4] * p[1] + m[8] * p[2] + m[12],
+                    m[1] * p[0] + m[5] * p[1] + m[9] * p[2] + m[13],
+                    m[2] * p[0] + m[6] * p[1] + m[10] * p[2] + m[14]};
+      for (int k = 0; k < 3; k++)
+        acc[k] += w[b] * r[k];
+    }
+    memcpy(dst + pa->offset, acc, 12);
+    if (na) {
+      float n[3], sn[3] = {0, 0, 0};
+      memcpy(n, src + na->offset, 12);
+      for (int b = 0; b < 4; b++) {
+        if (w[b] <= 0.0f || j[b] >= nb)
+          continue;
+        const float *m = bones[j[b]].d;
+        float r[3] = {m[0] * n[0] + m[4] * n[1] + m[8] * n[2],
+                      m[1] * n[0] + m[5] * n[1] + m[9] * n[2],
+                      m[2] * n[0] + m[6] * n[1] + m[10] * n[2]};
+        for (int k = 0; k < 3; k++)
+          sn[k] += w[b] * r[k];
+      }
+      float len = sqrtf(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]);
+      if (len > 1e-8f)
+        for (int k = 0; k < 3; k++)
+          sn[k] /= len;
+      memcpy(dst + na->offset, sn, 12);
+    }
+  }
+  return 0;
+}
+
+/* mop_morph_apply (viewport.c:1763-1805) */
+int mop_oracle_morph_apply(MopHipDevice *d, MopHipBuffer *vb, const void *bind, uint32_t vc,
+                           const MopHipVertexFormat *fmt, const float *targets,
+                           const float *weights, uint32_t nt) {
+  (void)d;
+  size_t stride = fmt ? fmt->stride : sizeof(MopHipVertex), off = 0;
+  if (fmt) {
+    const MopHipVertexAttrib *pa = fmt_attr(fmt, MOP_HIP_ATTRIB_POSITION);
+    if (!pa)
+      return -1;
+    off = pa->offset;
+  }
+  if ((size_t)vc * stride > vb->size)
+    return -1;
+  uint8_t *out = vb->data;
+  memcpy(out, bind, (size_t)vc * stride);
+  for (uint32_t t = 0; t < nt; t++) {
+    float w = weights[t];
+    if (w == 0.0f)
+      continue;
+    const float *dl = targets + (size_t)t * vc * 3;
+    for (uint32_t v = 0; v < vc; v++) {
+      float p[3];
+      memcpy(p, out + v * stride + off, 12);
+      p[0] += w * dl[v * 3 + 0];
+      p[1] += w * dl[v * 3 + 1];
+      p[2] += w * dl[v * 3 + 2];
+      memcpy(out + v * stride + off, p, 12);
+    }
+  }
+  return 0;
+}
+
 static void fb_alloc(MopHipFramebuffer *fb, int w, int h) {
   size_t n = (size_t)w * h;
   fb->w = w;

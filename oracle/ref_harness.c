@@ -356,6 +356,60 @@ void mh_mesh_set_shading(MopMesh *m, int mode) {
     mop_mesh_set_shading(m, (MopShadingMode)mode);
 }
 
+/* Mesh with a flexible vertex format (mop_viewport_add_mesh_ex): raw
+ * interleaved bytes, nattr attributes (semantic, format, offset). */
+MopMesh *mh_add_mesh_flex(MopViewport *vp, const void *data, uint32_t nv, uint32_t stride,
+                          const uint32_t *idx, uint32_t ni, uint32_t object_id,
+                          uint32_t nattr, const int *sem, const int *fmt, const uint32_t *off,
+                          const float *model) {
+  MopVertexFormat vf;
+  memset(&vf, 0, sizeof(vf));
+  if (nattr > MOP_MAX_VERTEX_ATTRIBS)
+    return NULL;
+  for (uint32_t i = 0; i < nattr; i++) {
+    vf.attribs[i].semantic = (MopAttribSemantic)sem[i];
+    vf.attribs[i].format = (MopAttribFormat)fmt[i];
+    vf.attribs[i].offset = off[i];
+  }
+  vf.attrib_count = nattr;
+  vf.stride = stride;
+  MopMeshDescEx md = {.vertex_data = data,
+                      .vertex_count = nv,
+                      .indices = idx,
+                      .index_count = ni,
+                      .object_id = object_id,
+                      .vertex_format = &vf};
+  MopMesh *m = mop_viewport_add_mesh_ex(vp, &md);
+  if (m && model) {
+    MopMat4 t;
+    memcpy(t.d, model, sizeof(t.d));
+    mop_mesh_set_transform(m, &t);
+  }
+  return m;
+}
+
+void mh_mesh_set_bones(MopViewport *vp, MopMesh *m, const float *mats, uint32_t n) {
+  mop_mesh_set_bone_matrices(m, vp, (const MopMat4 *)mats, n);
+}
+
+void mh_mesh_set_morph(MopViewport *vp, MopMesh *m, const float *targets,
+                       const float *weights, uint32_t n) {
+  mop_mesh_set_morph_targets(m, vp, targets, weights, n);
+}
+
+/* The mesh's current vertex bytes as the backend returns them (after a
+ * render these are the core's skinned / morphed vertices) and its bind-pose
+ * snapshot. */
+int mh_mesh_vertex_bytes(MopViewport *vp, MopMesh *m, void *out, size_t bytes) {
+  const void *p = vp->rhi->buffer_read(m->vertex_buffer);
+  if (!p)
+    return -1;
+  memcpy(out, p, bytes);
+  return 0;
+}
+
+const void *mh_mesh_bind_pose(MopMesh *m) { return m->bind_pose_data; }
+
 void mh_set_render_mode(MopViewport *vp, int mode) {
   mop_viewport_set_render_mode(vp, (MopRenderMode)mode);
 }

@@ -53,6 +53,15 @@ def harness():
         h.mh_mesh_set_shading.argtypes = [VP, C.c_int]
         h.mh_set_light_dir.argtypes = [VP, F]
         h.mh_set_render_mode.argtypes = [VP, C.c_int]
+        h.mh_add_mesh_flex.restype = VP
+        h.mh_add_mesh_flex.argtypes = [VP, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
+        h.mh_mesh_set_bones.argtypes = [VP, VP, C.c_void_p, C.c_uint32]
+        h.mh_mesh_set_morph.argtypes = [VP, VP, C.c_void_p, C.c_void_p, C.c_uint32]
+        h.mh_mesh_vertex_bytes.argtypes = [VP, VP, C.c_void_p, C.c_size_t]
+        h.mh_mesh_bind_pose.restype = C.c_void_p
+        h.mh_mesh_bind_pose.argtypes = [VP]
         h.mh_set_environment.argtypes = [VP, C.c_void_p, C.c_int, C.c_int, C.c_char_p,
                                          C.c_float, C.c_float]
         h.mh_get_exposure.restype = C.c_float
@@ -140,6 +149,31 @@ class RefViewport:
             if h.mh_add_instanced(self.vp, v.ctypes.data, len(v), i.ctypes.data, len(i),
                                   im.object_id, xf.ctypes.data, len(xf)) != 0:
                 raise RuntimeError("mop_viewport_add_instanced_mesh failed")
+        self.skinned = []
+        for sm in getattr(scene, "skinned", []):
+            data = np.ascontiguousarray(sm.data, np.uint8)
+            idx = np.ascontiguousarray(sm.indices, np.uint32)
+            sem = np.asarray([a[0] for a in sm.attrs], np.int32)
+            fmt = np.asarray([a[1] for a in sm.attrs], np.int32)
+            off = np.asarray([a[2] for a in sm.attrs], np.uint32)
+            mm = None if sm.model is None else np.ascontiguousarray(sm.model, np.float32)
+            self._keep += [data, idx, sem, fmt, off, mm]
+            mesh = h.mh_add_mesh_flex(self.vp, data.ctypes.data, len(data), sm.stride,
+                                      idx.ctypes.data, len(idx), sm.object_id, len(sm.attrs),
+                                      sem.ctypes.data, fmt.ctypes.data, off.ctypes.data,
+                                      None if mm is None else mm.ctypes.data)
+            if not mesh:
+                raise RuntimeError("mop_viewport_add_mesh_ex failed")
+            if sm.morph_targets is not None:
+                tg = np.ascontiguousarray(sm.morph_targets, np.float32)
+                wt = np.ascontiguousarray(sm.morph_weights, np.float32)
+                self._keep += [tg, wt]
+                h.mh_mesh_set_morph(self.vp, mesh, tg.ctypes.data, wt.ctypes.data, len(wt))
+            if sm.bones is not None:
+                bn = np.ascontiguousarray(sm.bones, np.float32)
+                self._keep.append(bn)
+                h.mh_mesh_set_bones(self.vp, mesh, bn.ctypes.data, len(bn))
+            self.skinned.append((mesh, sm))
         # set_light_dir / set_ambient also rewrite lights[0] (viewport.c:1028-1049),
         # so they go before the scene's lights are added
         h.mh_set_light_dir(self.vp, _f3(getattr(scene, "light_dir", (0.3, 1.0, 0.5))))
@@ -182,6 +216,20 @@ class RefViewport:
         if rc != 0:
             raise RuntimeError(f"mh_read failed ({rc})")
         return {"color": color, "depth": depth, "ids": ids, "hdr": hdr_a}
+
+    def skinned_vertex_bytes(self, k):
+        """Current vertex bytes of skinned mesh k as the backend returns them,
+        and its bind-pose snapshot (both (n, stride) uint8)."""
+        mesh, sm = self.skinned[k]
+        out = np.zeros_like(np.ascontiguousarray(sm.data, np.uint8))
+        if self.h.mh_mesh_vertex_bytes(self.vp, mesh, out.ctypes.data, out.nbytes) != 0:
+            raise RuntimeError("buffer_read failed")
+        bp = self.h.mh_mesh_bind_pose(mesh)
+        bind = None
+        if bp:
+            bind = np.ctypeslib.as_array((C.c_uint8 * out.nbytes).from_address(bp)).reshape(
+                out.shape).copy()
+        return out, bind
 
     def exposure(self):
         return self.h.mh_get_exposure(self.vp)
