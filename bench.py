@@ -270,25 +270,42 @@ def main():
         lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
         torch.cuda.synchronize()
         log(f"warm-up frame {k} done")
-    lib.mop_hip_timers_enable(dev, 1)
     tris = 0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    host_s = 0.0
     for _ in range(args.steps):
+        th = time.perf_counter()
         tris += lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
+        host_s += time.perf_counter() - th
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame")
+    host_ms = host_s * 1e3 / args.steps
+    log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame "
+        f"(host record + launch {host_ms:.3f} ms/frame)")
     fps = 1e3 / ms_per_step
     mtri = tris / args.steps / (ms_per_step * 1e3)
 
+    rep, cap = C.c_uint64(), C.c_uint64()
+    lib.mop_hip_graph_stats(dev, C.byref(rep), C.byref(cap))
+    log(f"graph replays {rep.value}, captures {cap.value}")
+
+    # stage times: the same frames with HIP events around every stage on the
+    # backend's stream (events force direct launches, so this is a separate
+    # timed region of the same workload)
+    lib.mop_hip_timers_enable(dev, 1)
+    n_st = max(10, args.steps // 2)
+    for _ in range(n_st):
+        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
+    torch.cuda.synchronize()
     stages = {}
     for name in ("shadow", "vertex", "setup", "raster", "shade", "resolve", "post"):
         tot, cnt = C.c_double(), C.c_uint64()
         if lib.mop_hip_timers_read(dev, name.encode(), C.byref(tot), C.byref(cnt)) == 0 and cnt.value:
             stages[name] = round(tot.value / cnt.value, 4)
     lib.mop_hip_timers_enable(dev, 0)
+    log(f"stage ms over {n_st} frames: {stages}")
 
     # the timed frames must have drawn the scene: read the last one back
     # (this also runs the backend's scratch-overflow check and frame stats)
@@ -360,6 +377,9 @@ def main():
                    "parallelism": "single GPU, single viewport per frame"},
         "mtri_per_s": round(mtri, 3),
         "fps_with_readback": round(fps_rb, 3),
+        "host_ms_per_frame": round(host_ms, 4),
+        "graph_replays": rep.value,
+        "graph_captures": cap.value,
         "stage_ms": stages,
         "frame_stats": {"subtris": st.subtris, "clipped": st.clipped,
                         "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris,

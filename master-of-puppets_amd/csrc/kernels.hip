@@ -198,7 +198,7 @@ __global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
 void launch_vertex(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_verts)
     return;
-  hipLaunchKernelGGL(k_vertex, dim3((fa.n_verts + WG - 1) / WG), dim3(WG), 0, s, fa);
+  launch(k_vertex, dim3((fa.n_verts + WG - 1) / WG), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(WG) void k_classify(FrameArgs fa) {
 void launch_classify(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris)
     return;
-  hipLaunchKernelGGL(k_classify, dim3((fa.n_tris + WG - 1) / WG), dim3(WG), 0, s, fa);
+  launch(k_classify, dim3((fa.n_tris + WG - 1) / WG), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -367,14 +367,18 @@ DEV void load_cv(const FrameArgs &fa, uint32_t v, CV &o) {
   o.a[6] = a1.z; o.a[7] = a1.w; o.a[8] = a2.x; o.a[9] = a2.y;
 }
 
-/* The two polygon buffers of each lane live in LDS (86 KB per 64 lanes), not
- * in per-lane scratch. */
-__global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
-  __shared__ CV pool[64][2][CLIP_CAP];
+/* The two polygon buffers of each lane live in LDS (1344 B per lane), not in
+ * per-lane scratch.  A clip is a long serial chain (six planes over up to
+ * nine vertices), so workgroups are small: the few clipped triangles of a
+ * frame spread over many CUs instead of sharing a wave. */
+#define CLIP_WG 16
+
+__global__ __launch_bounds__(CLIP_WG) void k_clip(FrameArgs fa) {
+  __shared__ CV pool[CLIP_WG][2][CLIP_CAP];
   uint32_t nq = fa.counters[CNT_CLIP];
   if (nq > fa.clip_cap)
     nq = fa.clip_cap;
-  for (uint32_t q = blockIdx.x * 64 + threadIdx.x; q < nq; q += gridDim.x * 64) {
+  for (uint32_t q = blockIdx.x * CLIP_WG + threadIdx.x; q < nq; q += gridDim.x * CLIP_WG) {
     uint32_t t = fa.clip_queue[q];
     uint32_t di = find_draw_by_tri(fa.draws, fa.ndraws, t);
     const DrawGPU &d = fa.draws[di];
@@ -429,12 +433,11 @@ __global__ __launch_bounds__(64) void k_clip(FrameArgs fa) {
 void launch_clip(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris || !fa.clip_cap)
     return;
-  /* the queue length is known on the device only: a grid-stride loop over
-   * at most two workgroups per CU (each lane needs 2.7 KB of scratch) */
-  uint32_t grid = (fa.clip_cap + 63) / 64;
-  if (grid > 512)
-    grid = 512;
-  hipLaunchKernelGGL(k_clip, dim3(grid), dim3(64), 0, s, fa);
+  /* the queue length is known on the device only: a grid-stride loop */
+  uint32_t grid = (fa.clip_cap + CLIP_WG - 1) / CLIP_WG;
+  if (grid > 2048)
+    grid = 2048;
+  launch(k_clip, dim3(grid), dim3(CLIP_WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -534,9 +537,9 @@ void launch_scan_counts(const FrameArgs &fa, uint32_t *block_sums, hipStream_t s
   uint32_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
   if (nb == 0)
     nb = 1;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(WG), 0, s, fa.tri_counts, n, block_sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(WG), 0, s, block_sums, nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(WG), 0, s, fa.tri_counts, n, block_sums,
+  launch(k_scan_reduce, dim3(nb), dim3(WG), 0, s, fa.tri_counts, n, block_sums);
+  launch(k_scan_sums, dim3(1), dim3(WG), 0, s, block_sums, nb);
+  launch(k_scan_apply, dim3(nb), dim3(WG), 0, s, fa.tri_counts, n, block_sums,
                      fa.counters, fa.subtri_cap);
 }
 
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(WG) void k_emit(FrameArgs fa) {
 void launch_emit(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris)
     return;
-  hipLaunchKernelGGL(k_emit, dim3((fa.n_tris + WG - 1) / WG), dim3(WG), 0, s, fa);
+  launch(k_emit, dim3((fa.n_tris + WG - 1) / WG), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -899,9 +902,9 @@ void launch_raster(const FrameArgs &fa, hipStream_t s) {
     grid = lim;
   if (grid == 0)
     grid = 1;
-  hipLaunchKernelGGL(k_raster_small, dim3(grid), dim3(WG), 0, s, fa);
-  hipLaunchKernelGGL(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
-  hipLaunchKernelGGL(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
+  launch(k_raster_small, dim3(grid), dim3(WG), 0, s, fa);
+  launch(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
+  launch(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -931,7 +934,7 @@ DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, 
 }
 
 /* Screen triangles that reached the raster, counted per workgroup into
- * blk_counts (summed by k_sum_blocks): one same-address atomic per wave from
+ * blk_counts (summed by k_finish): one same-address atomic per wave from
  * every workgroup of the grid would serialize at the memory side. */
 DEV void count_subtris(uint32_t *blk_counts, bool emitted) {
   __shared__ uint32_t wave_sum[WG / 64];
@@ -945,23 +948,6 @@ DEV void count_subtris(uint32_t *blk_counts, bool emitted) {
       c += wave_sum[w];
     blk_counts[blockIdx.x] = c;
   }
-}
-
-__global__ __launch_bounds__(WG) void k_sum_blocks(const uint32_t *blk_counts, uint32_t n,
-                                                   uint32_t *counter) {
-  __shared__ uint32_t part[WG];
-  uint32_t c = 0;
-  for (uint32_t i = threadIdx.x; i < n; i += WG)
-    c += blk_counts[i];
-  part[threadIdx.x] = c;
-  __syncthreads();
-  for (int s = WG / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s)
-      part[threadIdx.x] += part[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0)
-    atomicAdd(counter, part[0]);
 }
 
 /* Per triangle of the batch: index checks, trivial reject and clip-space
@@ -1069,18 +1055,17 @@ void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris)
     return;
   uint32_t nb = (fa.n_tris + WG - 1) / WG;
-  hipLaunchKernelGGL(k_tri, dim3(nb), dim3(WG), 0, s, fa);
+  launch(k_tri, dim3(nb), dim3(WG), 0, s, fa);
   launch_clip(fa, s);
   /* blk_counts holds nb + CLIP_RASTER_GRID entries */
-  hipLaunchKernelGGL(k_clip_raster, dim3(CLIP_RASTER_GRID), dim3(WG), 0, s, fa,
+  launch(k_clip_raster, dim3(CLIP_RASTER_GRID), dim3(WG), 0, s, fa,
                      fa.blk_counts + nb);
-  hipLaunchKernelGGL(k_sum_blocks, dim3(1), dim3(WG), 0, s, (const uint32_t *)fa.blk_counts,
-                     nb + CLIP_RASTER_GRID, fa.counters + CNT_SUBTRI);
+  /* the per-workgroup counts are summed by k_finish */
 }
 
 void launch_raster_big(const FrameArgs &fa, hipStream_t s) {
-  hipLaunchKernelGGL(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
-  hipLaunchKernelGGL(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
+  launch(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
+  launch(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1111,7 +1096,9 @@ DEV float shadow_pcf(const ViewGPU &fa, V3 wp) {
         total++;
         continue;
       }
-      float shadow_z = fa.shadow_map[(size_t)sy * (size_t)fa.shadow_w + (size_t)sx];
+      /* the map holds the pass's ordered keys; key_depth returns the stored
+       * depth (a -0.0 comes back as +0.0, which compares the same here) */
+      float shadow_z = key_depth(fa.shadow_map[(size_t)sy * (size_t)fa.shadow_w + (size_t)sx]);
       if (ndcz <= shadow_z + bias)
         lit++;
       total++;
@@ -1706,26 +1693,43 @@ __global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
   fa.ids[idx] = d.object_id;
 }
 
-/* Copy the batch counters to every view's host-visible counter slot. */
-__global__ void k_export_counters(FrameArgs fa) {
-  uint32_t v = blockIdx.x * 64 + threadIdx.x;
-  if (v >= fa.nviews)
-    return;
-  uint32_t *o = fa.views[v].cnt_out;
-  for (int k = 0; k < CNT_N; k++)
-    o[k] = fa.counters[k];
+/* End of a batch, one workgroup: add the per-workgroup screen-triangle
+ * counts (order-free path) to the batch counters, then copy the counters to
+ * every view's host-visible counter slot. */
+__global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk) {
+  __shared__ uint32_t part[WG];
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < nblk; i += WG)
+    c += fa.blk_counts[i];
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = WG / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && nblk)
+    fa.counters[CNT_SUBTRI] += part[0];
+  __syncthreads();
+  __threadfence_block();
+  for (uint32_t v = threadIdx.x; v < fa.nviews; v += WG) {
+    uint32_t *o = fa.views[v].cnt_out;
+    for (int k = 0; k < CNT_N; k++)
+      o[k] = fa.counters[k];
+  }
 }
 
 void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
   if (!fa.nviews)
     return;
-  hipLaunchKernelGGL(k_export_counters, dim3((fa.nviews + 63) / 64), dim3(64), 0, s, fa);
+  uint32_t nblk = fa.fused ? (fa.n_tris + WG - 1) / WG + CLIP_RASTER_GRID : 0u;
+  launch(k_finish, dim3(1), dim3(WG), 0, s, fa, nblk);
 }
 
 void launch_shade(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_stiles)
     return;
-  hipLaunchKernelGGL(k_shade, dim3(fa.n_stiles), dim3(fa.shade_tile * fa.shade_tile), 0, s, fa);
+  launch(k_shade, dim3(fa.n_stiles), dim3(fa.shade_tile * fa.shade_tile), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1752,7 +1756,7 @@ __global__ __launch_bounds__(WG) void k_clear(int npx, float r, float g, float b
 
 void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr, float *depth,
                   uint32_t *ids, unsigned long long *vis, hipStream_t s) {
-  hipLaunchKernelGGL(k_clear, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, clear[0],
+  launch(k_clear, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, clear[0],
                      clear[1], clear[2], clear[3], color, hdr, depth, ids, vis);
 }
 
@@ -1769,7 +1773,7 @@ __global__ __launch_bounds__(WG) void k_vis_from_depth(int npx, const float *dep
 
 void launch_vis_from_depth(int npx, const float *depth, unsigned long long *vis,
                            hipStream_t s) {
-  hipLaunchKernelGGL(k_vis_from_depth, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, depth,
+  launch(k_vis_from_depth, dim3((npx + WG - 1) / WG), dim3(WG), 0, s, npx, depth,
                      vis);
 }
 
@@ -1791,44 +1795,30 @@ DEV uint32_t find_post_view(const PostViewGPU *v, uint32_t n, uint32_t t) {
   return lo;
 }
 
-__global__ __launch_bounds__(WG) void k_resolve(const PostViewGPU *views, uint32_t nviews) {
-  const PostViewGPU &v = views[find_post_view(views, nviews, blockIdx.x)];
-  uint32_t t = blockIdx.x - v.tile_base;
-  uint32_t tiles_x = ((uint32_t)v.w + 15) / 16;
-  int x = (int)(t % tiles_x) * 16 + (int)(threadIdx.x & 15);
-  int y = (int)(t / tiles_x) * 16 + (int)(threadIdx.x >> 4);
-  if (x >= v.w || y >= v.h)
-    return;
-  size_t i = (size_t)y * (size_t)v.w + (size_t)x;
+/* mop_sw_hdr_resolve for one pixel (rasterizer.c:2776-2803): background
+ * (id 0) clamped, geometry ACES-tonemapped with the exposure; packed RGBA8. */
+DEV uint32_t resolve_px(const PostViewGPU &v, size_t i) {
   float4 h = reinterpret_cast<const float4 *>(v.hdr)[i];
-  uchar4 c;
+  uint32_t r, g, b, a;
   if (v.ids[i] == 0) {
-    c.x = cvt_u8(fminf(h.x, 1.0f) * 255.0f + 0.5f);
-    c.y = cvt_u8(fminf(h.y, 1.0f) * 255.0f + 0.5f);
-    c.z = cvt_u8(fminf(h.z, 1.0f) * 255.0f + 0.5f);
-    c.w = 255;
+    r = cvt_u8(fminf(h.x, 1.0f) * 255.0f + 0.5f);
+    g = cvt_u8(fminf(h.y, 1.0f) * 255.0f + 0.5f);
+    b = cvt_u8(fminf(h.z, 1.0f) * 255.0f + 0.5f);
+    a = 255;
   } else {
-    float r = aces(h.x * v.exposure);
-    float g = aces(h.y * v.exposure);
-    float b = aces(h.z * v.exposure);
-    c.x = cvt_u8(r * 255.0f + 0.5f);
-    c.y = cvt_u8(g * 255.0f + 0.5f);
-    c.z = cvt_u8(b * 255.0f + 0.5f);
-    c.w = cvt_u8(h.w * 255.0f + 0.5f);
+    r = cvt_u8(aces(h.x * v.exposure) * 255.0f + 0.5f);
+    g = cvt_u8(aces(h.y * v.exposure) * 255.0f + 0.5f);
+    b = cvt_u8(aces(h.z * v.exposure) * 255.0f + 0.5f);
+    a = cvt_u8(h.w * 255.0f + 0.5f);
   }
-  reinterpret_cast<uchar4 *>(v.dst)[i] = c;
+  return r | (g << 8) | (b << 16) | (a << 24);
 }
 
-void launch_resolve_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
-                          hipStream_t s) {
-  if (!n_tiles)
-    return;
-  hipLaunchKernelGGL(k_resolve, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
-}
+DEV uint32_t ch(uint32_t c, int k) { return (c >> (8 * k)) & 255u; }
 
-DEV float luma_at(const uint8_t *src, int w, int x, int y) {
-  const uint8_t *p = src + ((size_t)y * (size_t)w + (size_t)x) * 4;
-  return 0.299f * (float)p[0] + 0.587f * (float)p[1] + 0.114f * (float)p[2];
+/* luma of a resolved RGBA8 pixel (rasterizer.c:2633-2636) */
+DEV float luma_px(uint32_t c) {
+  return 0.299f * (float)ch(c, 0) + 0.587f * (float)ch(c, 1) + 0.114f * (float)ch(c, 2);
 }
 
 DEV float srgb_enc(float c) {
@@ -1858,15 +1848,18 @@ __global__ void k_post_lut(uint8_t *lut) {
 }
 
 void launch_post_lut(uint8_t *lut, hipStream_t s) {
-  hipLaunchKernelGGL(k_post_lut, dim3(4), dim3(256), 0, s, lut);
+  launch(k_post_lut, dim3(4), dim3(256), 0, s, lut);
 }
 
-/* mop_sw_fxaa (rasterizer.c:2625-2761) followed by the per-pixel effects of
- * mop_postprocess_apply (postprocess.c:36-116); FXAA reads only the copy of
- * the resolved image, so both fuse into one pass from src into dst.  The
- * lumas of the 18x18 neighbourhood of the block's tile are computed once
- * into LDS. */
-__global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t nviews) {
+/* The frame tail of one view tile: mop_sw_hdr_resolve, then (when post
+ * effects are requested) mop_sw_fxaa (rasterizer.c:2625-2761) and the
+ * per-pixel effects of mop_postprocess_apply (postprocess.c:36-116).  FXAA
+ * reads only resolved colours of the pixel's 3x3 neighbourhood, so each
+ * workgroup resolves its 16x16 tile plus a one-pixel halo into LDS and never
+ * writes the intermediate image.  Without fog or vignette the colour effects
+ * are a per-channel table (v.lut). */
+__global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t nviews) {
+  __shared__ uint32_t col[18][18];
   __shared__ float lum[18][18];
   const PostViewGPU &v = views[find_post_view(views, nviews, blockIdx.x)];
   uint32_t t = blockIdx.x - v.tile_base;
@@ -1875,25 +1868,28 @@ __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t 
   int x = x0 + (int)(threadIdx.x & 15);
   int y = y0 + (int)(threadIdx.x >> 4);
   const int w = v.w, h = v.h;
-  if (v.effects & MOP_HIP_POST_FXAA) {
-    for (int i = (int)threadIdx.x; i < 18 * 18; i += WG) {
-      int lx = x0 + i % 18 - 1, ly = y0 + i / 18 - 1;
-      lum[i / 18][i % 18] = (lx >= 0 && lx < w && ly >= 0 && ly < h) ? luma_at(v.src, w, lx, ly)
-                                                                     : 0.0f;
+  const uint32_t effects = v.effects;
+  if (effects == 0) {
+    if (x < w && y < h) {
+      size_t i = (size_t)y * (size_t)w + (size_t)x;
+      reinterpret_cast<uint32_t *>(v.dst)[i] = resolve_px(v, i);
     }
-    __syncthreads(); /* uniform: effects is per block */
+    return; /* uniform: effects is per view */
   }
+  for (int i = (int)threadIdx.x; i < 18 * 18; i += WG) {
+    int lx = x0 + i % 18 - 1, ly = y0 + i / 18 - 1;
+    uint32_t c = 0;
+    if (lx >= 0 && lx < w && ly >= 0 && ly < h)
+      c = resolve_px(v, (size_t)ly * (size_t)w + (size_t)lx);
+    col[i / 18][i % 18] = c;
+    lum[i / 18][i % 18] = luma_px(c);
+  }
+  __syncthreads();
   if (x >= w || y >= h)
     return;
   const int tx = (int)(threadIdx.x & 15) + 1, ty = (int)(threadIdx.x >> 4) + 1;
-  const uint8_t *src = v.src;
-  uint8_t *dst = v.dst;
-  const float *depth = v.depth;
-  uint32_t effects = v.effects;
-  float fog_r = v.fog_r, fog_g = v.fog_g, fog_b = v.fog_b, fog_near = v.fog_near,
-        fog_far = v.fog_far;
-  size_t ci = ((size_t)y * (size_t)w + (size_t)x) * 4;
-  uint8_t o0 = src[ci], o1 = src[ci + 1], o2 = src[ci + 2], o3 = src[ci + 3];
+  uint32_t cM = col[ty][tx];
+  uint32_t o0 = ch(cM, 0), o1 = ch(cM, 1), o2 = ch(cM, 2), o3 = ch(cM, 3);
   if ((effects & MOP_HIP_POST_FXAA) && w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 &&
       y < h - 1) {
     const float EDGE_THRESHOLD = 0.0625f, EDGE_THRESHOLD_MIN = 0.004f, SUBPIX_QUALITY = 0.75f;
@@ -1945,41 +1941,38 @@ __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t 
         off_y = -step_y;
       }
       float blend = subpix;
-      int nx = x + off_x, ny = y + off_y;
-      if (nx < 0) nx = 0;
-      if (nx >= w) nx = w - 1;
-      if (ny < 0) ny = 0;
-      if (ny >= h) ny = h - 1;
-      size_t ni = ((size_t)ny * (size_t)w + (size_t)nx) * 4;
+      /* x, y lie in 1..w-2, 1..h-2 here, so the neighbour is in the image and in
+       * the tile's halo */
+      uint32_t cN = col[ty + off_y][tx + off_x];
       float inv_blend = 1.0f - blend;
-      o0 = cvt_u8((float)src[ci + 0] * inv_blend + (float)src[ni + 0] * blend);
-      o1 = cvt_u8((float)src[ci + 1] * inv_blend + (float)src[ni + 1] * blend);
-      o2 = cvt_u8((float)src[ci + 2] * inv_blend + (float)src[ni + 2] * blend);
+      o0 = cvt_u8((float)ch(cM, 0) * inv_blend + (float)ch(cN, 0) * blend);
+      o1 = cvt_u8((float)ch(cM, 1) * inv_blend + (float)ch(cN, 1) * blend);
+      o2 = cvt_u8((float)ch(cM, 2) * inv_blend + (float)ch(cN, 2) * blend);
     }
   }
-  uint32_t rest = effects & (MOP_HIP_POST_GAMMA | MOP_HIP_POST_TONEMAP |
-                             MOP_HIP_POST_VIGNETTE | MOP_HIP_POST_FOG);
-  if (effects != 0 && v.lut) {
+  if (v.lut) {
     o0 = v.lut[o0];
     o1 = v.lut[o1];
     o2 = v.lut[o2];
-  } else if (effects != 0) {
+  } else {
     /* postprocess.c runs its per-pixel loop whenever effects != 0 */
+    uint32_t rest = effects & (MOP_HIP_POST_GAMMA | MOP_HIP_POST_TONEMAP |
+                               MOP_HIP_POST_VIGNETTE | MOP_HIP_POST_FOG);
     float r = (float)o0 / 255.0f;
     float g = (float)o1 / 255.0f;
     float b = (float)o2 / 255.0f;
     if ((rest & MOP_HIP_POST_FOG) && v.has_fog) {
-      float dz = depth[(size_t)y * (size_t)w + (size_t)x];
-      float range = fog_far - fog_near;
+      float dz = v.depth[(size_t)y * (size_t)w + (size_t)x];
+      float range = v.fog_far - v.fog_near;
       float factor = 0.0f;
       if (range > 0.0f) {
-        factor = (dz - fog_near) / range;
+        factor = (dz - v.fog_near) / range;
         if (factor < 0.0f) factor = 0.0f;
         if (factor > 1.0f) factor = 1.0f;
       }
-      r = r + (fog_r - r) * factor;
-      g = g + (fog_g - g) * factor;
-      b = b + (fog_b - b) * factor;
+      r = r + (v.fog_r - r) * factor;
+      g = g + (v.fog_g - g) * factor;
+      b = b + (v.fog_b - b) * factor;
     }
     if (rest & MOP_HIP_POST_TONEMAP) {
       r = r / (r + 1.0f);
@@ -2012,17 +2005,15 @@ __global__ __launch_bounds__(WG) void k_post(const PostViewGPU *views, uint32_t 
     o1 = cvt_u8(g * 255.0f + 0.5f);
     o2 = cvt_u8(b * 255.0f + 0.5f);
   }
-  dst[ci] = o0;
-  dst[ci + 1] = o1;
-  dst[ci + 2] = o2;
-  dst[ci + 3] = o3;
+  reinterpret_cast<uint32_t *>(v.dst)[(size_t)y * (size_t)w + (size_t)x] =
+      o0 | (o1 << 8) | (o2 << 16) | (o3 << 24);
 }
 
-void launch_post_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
+void launch_tail_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
                        hipStream_t s) {
   if (!n_tiles)
     return;
-  hipLaunchKernelGGL(k_post, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
+  launch(k_tail, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
 }
 
 __global__ void k_fill_u32(size_t n, uint32_t *p, uint32_t v) {
@@ -2034,7 +2025,7 @@ __global__ void k_fill_u32(size_t n, uint32_t *p, uint32_t v) {
 void launch_fill_u32(size_t n, uint32_t *p, uint32_t v, hipStream_t s) {
   if (!n)
     return;
-  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, s, n, p, v);
+  launch(k_fill_u32, dim3((unsigned)((n + WG - 1) / WG)), dim3(WG), 0, s, n, p, v);
 }
 
 /* --------------------------------------------------------------------- */
@@ -2048,7 +2039,7 @@ __global__ __launch_bounds__(WG) void k_shadow_clear(int n, uint32_t *keys) {
 }
 
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow_clear, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, keys);
+  launch(k_shadow_clear, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, keys);
 }
 
 DEV uint32_t find_shadow_draw(const ShadowDrawGPU *d, uint32_t n, uint32_t t) {
@@ -2233,9 +2224,9 @@ __global__ __launch_bounds__(WG) void k_shadow_big(ShadowArgsGPU sa) {
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s) {
   if (!sa.n_tris)
     return;
-  hipLaunchKernelGGL(k_shadow_vertex, dim3((sa.n_verts + WG - 1) / WG), dim3(WG), 0, s, sa);
-  hipLaunchKernelGGL(k_shadow_small, dim3((sa.n_tris + WG - 1) / WG), dim3(WG), 0, s, sa);
-  hipLaunchKernelGGL(k_shadow_big, dim3(2048), dim3(WG), 0, s, sa);
+  launch(k_shadow_vertex, dim3((sa.n_verts + WG - 1) / WG), dim3(WG), 0, s, sa);
+  launch(k_shadow_small, dim3((sa.n_tris + WG - 1) / WG), dim3(WG), 0, s, sa);
+  launch(k_shadow_big, dim3(2048), dim3(WG), 0, s, sa);
 }
 
 __global__ __launch_bounds__(WG) void k_shadow_finalize(int n, const uint32_t *keys,
@@ -2246,7 +2237,7 @@ __global__ __launch_bounds__(WG) void k_shadow_finalize(int n, const uint32_t *k
 }
 
 void launch_shadow_finalize(int n, const uint32_t *keys, float *depth, hipStream_t s) {
-  hipLaunchKernelGGL(k_shadow_finalize, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, keys,
+  launch(k_shadow_finalize, dim3((n + WG - 1) / WG), dim3(WG), 0, s, n, keys,
                      depth);
 }
 
@@ -2262,7 +2253,7 @@ __global__ void k_copy_u8(size_t n, const uint8_t *src, uint8_t *dst) {
 
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s) {
   size_t blocks = (n + WG * 16 - 1) / (WG * 16);
-  hipLaunchKernelGGL(k_copy_u8, dim3((unsigned)blocks), dim3(WG), 0, s, n, src, dst);
+  launch(k_copy_u8, dim3((unsigned)blocks), dim3(WG), 0, s, n, src, dst);
 }
 
 /* --------------------------------------------------------------------- */
@@ -2709,14 +2700,14 @@ void launch_ordered(const FrameArgs &fa, uint32_t *tile_counts, uint32_t *tile_c
     grid = 4096;
   if (grid == 0)
     grid = 1;
-  hipLaunchKernelGGL(k_bin_count, dim3(grid), dim3(WG), 0, s, fa, tile_counts);
+  launch(k_bin_count, dim3(grid), dim3(WG), 0, s, fa, tile_counts);
   /* exclusive scan of tile counts in place (total lands in tile_counts[nt]) */
   uint32_t nb = (nt + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(WG), 0, s, tile_counts, nt, block_sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(WG), 0, s, block_sums, nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(WG), 0, s, tile_counts, nt, block_sums,
+  launch(k_scan_reduce, dim3(nb), dim3(WG), 0, s, tile_counts, nt, block_sums);
+  launch(k_scan_sums, dim3(1), dim3(WG), 0, s, block_sums, nb);
+  launch(k_scan_apply, dim3(nb), dim3(WG), 0, s, tile_counts, nt, block_sums,
                      (uint32_t *)nullptr, 0u);
-  hipLaunchKernelGGL(k_bin_fill, dim3(grid), dim3(WG), 0, s, fa, tile_counts, tile_cursor, bins,
+  launch(k_bin_fill, dim3(grid), dim3(WG), 0, s, fa, tile_counts, tile_cursor, bins,
                      bin_cap);
-  hipLaunchKernelGGL(k_tile_ordered, dim3(nt), dim3(WG), 0, s, fa, tile_counts, bins);
+  launch(k_tile_ordered, dim3(nt), dim3(WG), 0, s, fa, tile_counts, bins);
 }

@@ -29,7 +29,46 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+#include <vector>
+
 #include "mop_hip.h"
+
+/* ---- frame plans ---------------------------------------------------------
+ * While a plan is active (t_plan set by the runtime), frame launches and
+ * uploads are recorded instead of issued: the plan keeps a byte signature of
+ * every kernel (function, grid, block, argument bytes) and upload (device
+ * destination, size), and the operations themselves.  The runtime then
+ * issues the operations directly, or replays a hipGraph captured from an
+ * earlier plan with the same signature (upload sources live in a pinned
+ * buffer owned by that graph, refilled before each replay). */
+struct Plan {
+  std::vector<uint8_t> sig;
+  std::vector<std::function<void(hipStream_t, const uint8_t *)>> ops; /* arg: upload source */
+  std::vector<uint8_t> blob; /* host bytes of the plan's uploads */
+};
+extern thread_local Plan *t_plan;
+
+template <typename T> inline void sig_put(std::vector<uint8_t> &s, const T &v) {
+  const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
+  s.insert(s.end(), p, p + sizeof(T));
+}
+
+/* hipLaunchKernelGGL, or its record into the active plan */
+template <typename... P, typename... A>
+inline void launch(void (*k)(P...), dim3 g, dim3 b, uint32_t shmem, hipStream_t s, A... a) {
+  if (Plan *pl = t_plan) {
+    sig_put(pl->sig, (uintptr_t)k);
+    sig_put(pl->sig, g);
+    sig_put(pl->sig, b);
+    (sig_put(pl->sig, (P)a), ...);
+    pl->ops.push_back([=](hipStream_t st, const uint8_t *) {
+      hipLaunchKernelGGL(k, g, b, shmem, st, (P)a...);
+    });
+  } else {
+    hipLaunchKernelGGL(k, g, b, shmem, s, (P)a...);
+  }
+}
 
 /* Screen triangles whose clamped bbox has more pixels than this are walked
  * one row per lane instead of one triangle per lane. */
@@ -118,7 +157,7 @@ struct ViewGPU {
   uint32_t *ids;
   uint32_t *cnt_out; /* device alias of host-pinned counters of this segment */
   /* shadow lookup state (mop_sw_shadow_set) */
-  const float *shadow_map;
+  const uint32_t *shadow_map; /* ordered depth keys (key_depth gives the depth) */
   int32_t shadow_w, shadow_h;
   int32_t shadow_on;
   int32_t ibl_on;
@@ -258,10 +297,8 @@ void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr,
 void launch_vis_from_depth(int npx, const float *depth,
                            unsigned long long *vis, hipStream_t s);
 /* resolve (mode 0) or post (mode 1) over the tiles of n views */
-void launch_resolve_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
-                          hipStream_t s);
 void launch_post_lut(uint8_t *lut, hipStream_t s);
-void launch_post_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
+void launch_tail_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
                        hipStream_t s);
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);

@@ -100,8 +100,8 @@ struct MopHipTexture {
 
 struct PendingDraw {
   DrawGPU g;              /* filled except vtx/tri bases and light_off */
-  std::vector<MopHipLight> lights;
-  int cls;                /* 0 = order-free opaque, else unsupported */
+  uint32_t light_at;      /* the draw's lights: fb->light_pool[light_at ..] */
+  int cls;                /* 0 order-free, 3 ordered, 2 no-op */
   int state;              /* index into the frame's shading-state list */
 };
 
@@ -109,7 +109,7 @@ struct PendingDraw {
  * the time of the draw): device pointers of the bound tables. */
 struct ShadeState {
   bool shadow_on = false;
-  const float *shadow_map = nullptr;
+  const uint32_t *shadow_map = nullptr; /* ordered depth keys */
   int shadow_w = 0, shadow_h = 0;
   float shadow_vp[16] = {0};
   bool ibl_on = false;
@@ -134,6 +134,7 @@ struct MopHipFramebuffer {
   bool has_clear = false;
   float clear[4] = {0, 0, 0, 0};
   std::vector<PendingDraw> draws;
+  std::vector<MopHipLight> light_pool; /* the recorded draws' lights */
   std::vector<ShadeState> states;
   bool executed = false;
   bool checked = false;
@@ -153,12 +154,21 @@ struct MopHipFramebuffer {
 
 #define MOPH_MAX_SEGS 32
 
+/* set when a plan needed a buffer to grow: the frame is planned again
+ * without a plan (growing frees memory that recorded operations point at) */
+static thread_local bool t_plan_abort = false;
+thread_local Plan *t_plan = nullptr;
+
 template <class T> struct DevArr {
   T *p = nullptr;
   size_t cap = 0;
   bool ensure(size_t n) {
     if (n <= cap)
       return true;
+    if (t_plan) {
+      t_plan_abort = true;
+      return false;
+    }
     if (p)
       (void)hipFree(p);
     size_t nc = n + n / 4 + 256;
@@ -181,8 +191,8 @@ template <class T> struct DevArr {
 
 struct MopHipShadowMap {
   int size = 0;
-  DevArr<uint32_t> keys;
-  DevArr<float> depth;
+  DevArr<uint32_t> keys; /* ordered depth keys: the pass's atomic min target and
+                            what the PCF lookup reads */
 };
 
 struct MopHipIbl {
@@ -193,6 +203,24 @@ struct MopHipIbl {
 
 struct ShadowDrawHost {
   ShadowDrawGPU g;
+};
+
+struct ShadowJob {
+  MopHipShadowMap *map;
+  std::vector<ShadowDrawHost> casters;
+};
+
+/* A captured frame plan: two graph instances, each with its own pinned
+ * upload buffer, used alternately so the host refills one while the device
+ * may still read the other. */
+struct GraphEntry {
+  std::vector<uint8_t> sig;
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  uint8_t *blob[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  size_t blob_size = 0;
+  int next = 0;
+  uint64_t use = 0;
 };
 
 /* Optional per-stage device timers: hipEvents recorded on the device
@@ -232,6 +260,7 @@ struct MopHipDevice {
   DevArr<float4> vpos, vscr, vattr, cpos, cattr;
   DevArr<uint4> tri_rec;
   DevArr<uint32_t> blk_counts;
+  DevArr<uint8_t> arena; /* per-batch upload: counters, draws, lights, views */
   DevArr<float> skin_bones, morph_targets, morph_weights;
   DevArr<uint32_t> cnt_poly, tri_counts, clip_queue, clip_slot, counters, block_sums,
       big_queue;
@@ -250,8 +279,16 @@ struct MopHipDevice {
   MopHipShadowMap *rec_shadow = nullptr; /* map the recorded casters target */
   DevArr<uint32_t> shadow_big;
   DevArr<float4> shadow_lclip, shadow_lscr;
+  DevArr<uint8_t> shadow_arena; /* counters + caster draws of one pass */
   DevArr<ShadowDrawGPU> shadow_draws_dev;
   std::vector<ShadowDrawHost> shadow_draws;
+  std::vector<ShadowJob> shadow_jobs; /* queued shadow passes */
+  /* frame plans replayed from hipGraphs (mop_hip_set_graphs) */
+  bool graphs_on = true;
+  std::vector<GraphEntry> graphs;
+  std::vector<uint8_t> last_sig;
+  uint64_t plan_clock = 0;
+  uint64_t graph_replays = 0, graph_captures = 0;
   std::vector<MopHipBuffer *> shadow_temp;
   ShadeState cur_state;
   bool state_dirty = true;
@@ -319,11 +356,23 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->subtris.release();
   d->shadow_big.release();
   d->shadow_draws_dev.release();
+  for (GraphEntry &e : d->graphs)
+    for (int k = 0; k < 2; k++) {
+      if (e.exec[k])
+        (void)hipGraphExecDestroy(e.exec[k]);
+      if (e.blob[k])
+        (void)hipHostFree(e.blob[k]);
+      if (e.done[k])
+        (void)hipEventDestroy(e.done[k]);
+    }
+  d->graphs.clear();
   d->shadow_lclip.release();
+  d->shadow_arena.release();
   d->shadow_lscr.release();
   d->vscr.release();
   d->tri_rec.release();
   d->blk_counts.release();
+  d->arena.release();
   d->skin_bones.release();
   d->morph_targets.release();
   d->morph_weights.release();
@@ -355,6 +404,13 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
 /* Pinned staging ring: each upload takes the next slot, waiting only for
  * the copy that used the slot NSTAGE uploads ago. */
 static uint8_t *stage_acquire(MopHipDevice *d, size_t bytes, int *slot) {
+  if (Plan *pl = t_plan) {
+    /* staged into the plan's upload blob (256-byte aligned regions) */
+    size_t off = (pl->blob.size() + 255) & ~(size_t)255;
+    pl->blob.resize(off + bytes);
+    *slot = -1;
+    return pl->blob.data() + off;
+  }
   int k = d->ring_next;
   d->ring_next = (k + 1) % NSTAGE;
   StageSlot &sl = d->ring[k];
@@ -378,8 +434,25 @@ static uint8_t *stage_acquire(MopHipDevice *d, size_t bytes, int *slot) {
 }
 
 static void stage_release(MopHipDevice *d, int k) {
+  if (k < 0)
+    return;
   HIPCHK(hipEventRecord(d->ring[k].ev, d->stream));
   d->ring[k].used = true;
+}
+
+/* Host-to-device copy of staged bytes (from stage_acquire), or its record
+ * into the active plan (source = an offset into the plan's upload blob). */
+static void upload(MopHipDevice *d, void *dst, const uint8_t *stg, size_t n) {
+  if (Plan *pl = t_plan) {
+    size_t off = (size_t)(stg - pl->blob.data());
+    sig_put(pl->sig, (uintptr_t)dst);
+    sig_put(pl->sig, n);
+    pl->ops.push_back([=](hipStream_t st, const uint8_t *base) {
+      HIPCHK(hipMemcpyAsync(dst, base + off, n, hipMemcpyHostToDevice, st));
+    });
+    return;
+  }
+  HIPCHK(hipMemcpyAsync(dst, stg, n, hipMemcpyHostToDevice, d->stream));
 }
 
 static void tm_collect(MopHipDevice *d);
@@ -789,6 +862,7 @@ extern "C" void mop_hip_framebuffer_resize(MopHipDevice *d, MopHipFramebuffer *f
   (void)hipStreamSynchronize(d->stream);
   fb_free(fb);
   fb->draws.clear();
+  fb->light_pool.clear();
   fb->states.clear();
   fb->executed = false;
   fb->checked = false;
@@ -812,6 +886,7 @@ extern "C" void mop_hip_frame_begin(MopHipDevice *d, MopHipFramebuffer *fb, MopH
   fb->clear[2] = c.b;
   fb->clear[3] = c.a;
   fb->draws.clear();
+  fb->light_pool.clear();
   fb->states.clear();
   fb->executed = false;
   fb->checked = false;
@@ -893,9 +968,20 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
   g.cam_eye[0] = c->cam_eye.x;
   g.cam_eye[1] = c->cam_eye.y;
   g.cam_eye[2] = c->cam_eye.z;
-  if (c->lights && c->light_count)
-    pd.lights.assign(c->lights, c->lights + c->light_count);
-  g.light_count = c->light_count;
+  /* lights go to the frame's pool; a draw whose lights equal the previous
+   * draw's reuses its entries (the core passes the viewport's array to every
+   * draw of a frame) */
+  pd.light_at = (uint32_t)fb->light_pool.size();
+  if (c->lights && c->light_count) {
+    size_t n = c->light_count;
+    if (!fb->draws.empty() && fb->draws.back().g.light_count == n &&
+        memcmp(fb->light_pool.data() + fb->draws.back().light_at, c->lights,
+               n * sizeof(MopHipLight)) == 0)
+      pd.light_at = fb->draws.back().light_at;
+    else
+      fb->light_pool.insert(fb->light_pool.end(), c->lights, c->lights + n);
+  }
+  g.light_count = c->lights ? c->light_count : 0;
   if (c->depth_test)
     g.flags |= MOPH_DRAW_DEPTH_TEST;
   if (c->depth_write)
@@ -970,8 +1056,10 @@ static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_t
   if (sc > 0xffffff00ull)
     sc = 0xffffff00ull;
   subtri_cap = (uint32_t)sc;
-  return d->draws.ensure(ndraws ? ndraws : 1) && d->lights.ensure(nlights ? nlights : 1) &&
-         d->views.ensure(nviews ? nviews : 1) && d->vpos.ensure(n_verts ? n_verts : 1) &&
+  (void)ndraws;
+  (void)nlights;
+  (void)nviews;
+  return d->vpos.ensure(n_verts ? n_verts : 1) &&
          d->vattr.ensure((size_t)(n_verts ? n_verts : 1) * 3) &&
          d->cpos.ensure((size_t)clip_cap * MOPH_MAX_POLY) &&
          d->cattr.ensure((size_t)clip_cap * MOPH_MAX_POLY * 3) && d->cnt_poly.ensure(clip_cap) &&
@@ -1043,14 +1131,18 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
       g.view = (uint32_t)j;
       g.vtx_base = n_verts;
       g.tri_base = n_tris;
-      if (!gd.empty() && gd.back().light_count == pd.lights.size() &&
-          (pd.lights.empty() ||
-           memcmp(&lights[gd.back().light_off], pd.lights.data(),
-                  pd.lights.size() * sizeof(MopHipLight)) == 0)) {
+      const MopHipLight *pl = fb->light_pool.data() + pd.light_at;
+      if (i > jobs[j].b && fb->draws[i - 1].light_at == pd.light_at &&
+          fb->draws[i - 1].g.light_count == g.light_count) {
+        g.light_off = gd.back().light_off; /* same pool entries as the previous draw */
+      } else if (!gd.empty() && gd.back().light_count == g.light_count &&
+                 (g.light_count == 0 ||
+                  memcmp(&lights[gd.back().light_off], pl,
+                         g.light_count * sizeof(MopHipLight)) == 0)) {
         g.light_off = gd.back().light_off;
       } else {
         g.light_off = (uint32_t)lights.size();
-        lights.insert(lights.end(), pd.lights.begin(), pd.lights.end());
+        lights.insert(lights.end(), pl, pl + g.light_count);
       }
       if ((uint64_t)n_verts + g.vertex_count > 0xffffffffull ||
           (uint64_t)n_tris + g.tri_count > 0x7fffffffull) {
@@ -1066,27 +1158,31 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (!ensure_frame_scratch(d, n_verts, n_tris, (uint32_t)gd.size(), (uint32_t)lights.size(),
                             (uint32_t)views.size(), clip_cap, subtri_cap))
     return false;
+  /* one upload per batch: [counters (zeroed) | draws | lights | views], each
+   * 256-byte aligned */
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t oc = 0, od = al(CNT_N * sizeof(uint32_t));
   size_t bd = gd.size() * sizeof(DrawGPU), bl = lights.size() * sizeof(MopHipLight),
          bv = views.size() * sizeof(ViewGPU);
+  size_t ol = od + al(bd), ov = ol + al(bl), total = ov + bv;
+  if (!d->arena.ensure(total))
+    return false;
   int slot;
-  uint8_t *stg = stage_acquire(d, bd + bl + bv + 64, &slot);
+  uint8_t *stg = stage_acquire(d, total, &slot);
   if (!stg)
     return false;
-  memcpy(stg, gd.data(), bd);
+  memset(stg + oc, 0, CNT_N * sizeof(uint32_t));
+  memcpy(stg + od, gd.data(), bd);
   if (bl)
-    memcpy(stg + bd, lights.data(), bl);
-  memcpy(stg + bd + bl, views.data(), bv);
-  if (bd)
-    HIPCHK(hipMemcpyAsync(d->draws.p, stg, bd, hipMemcpyHostToDevice, d->stream));
-  if (bl)
-    HIPCHK(hipMemcpyAsync(d->lights.p, stg + bd, bl, hipMemcpyHostToDevice, d->stream));
-  HIPCHK(hipMemcpyAsync(d->views.p, stg + bd + bl, bv, hipMemcpyHostToDevice, d->stream));
+    memcpy(stg + ol, lights.data(), bl);
+  memcpy(stg + ov, views.data(), bv);
+  upload(d, d->arena.p, stg, total);
   stage_release(d, slot);
 
   FrameArgs fa;
   memset(&fa, 0, sizeof(fa));
-  fa.draws = d->draws.p;
-  fa.views = d->views.p;
+  fa.draws = (const DrawGPU *)(d->arena.p + od);
+  fa.views = (const ViewGPU *)(d->arena.p + ov);
   fa.ndraws = (uint32_t)gd.size();
   fa.nviews = (uint32_t)views.size();
   fa.n_verts = n_verts;
@@ -1117,10 +1213,10 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.tri_counts = d->tri_counts.p;
   fa.clip_queue = d->clip_queue.p;
   fa.clip_slot = d->clip_slot.p;
-  fa.counters = d->counters.p;
+  fa.counters = (uint32_t *)(d->arena.p + oc);
   fa.subtris = d->subtris.p;
   fa.big_queue = d->big_queue.p;
-  fa.lights = d->lights.p;
+  fa.lights = (const MopHipLight *)(d->arena.p + ol);
 
   if (!ordered)
     for (const SegJob &j : jobs)
@@ -1138,7 +1234,6 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         !d->bins.ensure(bin_cap) || !d->block_sums.ensure(tiles / (256 * 16) + 2))
       return false;
   }
-  launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
   if (fa.ndraws) {
     {
       TmScope t(d, TM_VERTEX);
@@ -1186,8 +1281,11 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   return true;
 }
 
-/* resolve every frame's HDR buffer into colour[0] (one launch) */
-static void run_resolve(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fbs) {
+/* The frame tail of every executed frame in one launch: HDR resolve with the
+ * frame's exposure, then FXAA and the per-pixel effects when a post request
+ * was recorded (mop_hip_frame_request_post / the postprocess hook).  The
+ * final RGBA8 lands in color[0]. */
+static void run_tail(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fbs) {
   if (fbs.empty())
     return;
   std::vector<PostViewGPU> pv(fbs.size());
@@ -1202,42 +1300,10 @@ static void run_resolve(MopHipDevice *d, const std::vector<MopHipFramebuffer *> 
     tiles += view_tiles(fb->w, fb->h);
     v.hdr = fb->hdr;
     v.ids = fb->ids;
+    v.depth = fb->depth;
     v.dst = fb->color[0];
     v.exposure = fb->resolve_set ? fb->resolve_exposure : d->exposure;
-    fb->cur = 0;
-    fb->v_color = false;
-  }
-  if (!d->pviews.ensure(pv.size()))
-    return;
-  int slot;
-  uint8_t *stg = stage_acquire(d, pv.size() * sizeof(PostViewGPU), &slot);
-  if (!stg)
-    return;
-  memcpy(stg, pv.data(), pv.size() * sizeof(PostViewGPU));
-  HIPCHK(hipMemcpyAsync(d->pviews.p, stg, pv.size() * sizeof(PostViewGPU),
-                        hipMemcpyHostToDevice, d->stream));
-  stage_release(d, slot);
-  TmScope t(d, TM_RESOLVE);
-  launch_resolve_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
-}
-
-/* FXAA + per-pixel effects of every frame with a recorded post request */
-static void run_post(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fbs) {
-  std::vector<PostViewGPU> pv;
-  uint32_t tiles = 0;
-  for (MopHipFramebuffer *fb : fbs) {
-    if (!fb->post_effects)
-      continue;
-    PostViewGPU v;
-    memset(&v, 0, sizeof(v));
-    v.w = fb->w;
-    v.h = fb->h;
-    v.tile_base = tiles;
-    tiles += view_tiles(fb->w, fb->h);
     v.effects = fb->post_effects;
-    v.depth = fb->depth;
-    v.src = fb->color[fb->cur];
-    v.dst = fb->color[1 - fb->cur];
     if (fb->has_fog) {
       v.has_fog = 1;
       v.fog_r = fb->fog.color.r;
@@ -1249,32 +1315,138 @@ static void run_post(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fb
     /* without fog or vignette the colour effects map each channel value on
      * its own: a 256-entry table per (tonemap, gamma) */
     bool fog_on = (fb->post_effects & MOP_HIP_POST_FOG) && fb->has_fog;
-    if (!fog_on && !(fb->post_effects & MOP_HIP_POST_VIGNETTE) && d->post_lut.p) {
+    if (fb->post_effects && !fog_on && !(fb->post_effects & MOP_HIP_POST_VIGNETTE) &&
+        d->post_lut.p) {
       int variant = ((fb->post_effects & MOP_HIP_POST_TONEMAP) ? 1 : 0) |
                     ((fb->post_effects & MOP_HIP_POST_GAMMA) ? 2 : 0);
       v.lut = d->post_lut.p + variant * 256;
     }
-    fb->cur = 1 - fb->cur;
+    fb->cur = 0;
     fb->v_color = false;
-    pv.push_back(v);
   }
-  if (pv.empty() || !d->pviews.ensure(pv.size()))
+  if (!d->pviews.ensure(pv.size()))
     return;
   int slot;
   uint8_t *stg = stage_acquire(d, pv.size() * sizeof(PostViewGPU), &slot);
   if (!stg)
     return;
   memcpy(stg, pv.data(), pv.size() * sizeof(PostViewGPU));
-  HIPCHK(hipMemcpyAsync(d->pviews.p, stg, pv.size() * sizeof(PostViewGPU),
-                        hipMemcpyHostToDevice, d->stream));
+  upload(d, d->pviews.p, stg, pv.size() * sizeof(PostViewGPU));
   stage_release(d, slot);
   TmScope t(d, TM_POST);
-  launch_post_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
+  launch_tail_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
 }
 
-/* Execute the recorded frames of `fbs` that are not executed yet:
- * clear, then the k-th order-free segment of every frame in one batch for
- * k = 0, 1, ..., then one resolve launch and one post launch. */
+static void free_graph_entry(GraphEntry &e) {
+  for (int k = 0; k < 2; k++) {
+    if (e.done[k])
+      (void)hipEventSynchronize(e.done[k]);
+    if (e.exec[k])
+      (void)hipGraphExecDestroy(e.exec[k]);
+    if (e.blob[k])
+      (void)hipHostFree(e.blob[k]);
+    if (e.done[k])
+      (void)hipEventDestroy(e.done[k]);
+    e.exec[k] = nullptr;
+    e.blob[k] = nullptr;
+    e.done[k] = nullptr;
+  }
+}
+
+/* Issue a plan's operations on the stream, uploads from a staging slot. */
+static void issue_direct(MopHipDevice *d, const Plan &pl) {
+  const uint8_t *base = nullptr;
+  int slot = -1;
+  if (!pl.blob.empty()) {
+    uint8_t *stg = stage_acquire(d, pl.blob.size(), &slot);
+    if (!stg)
+      return;
+    memcpy(stg, pl.blob.data(), pl.blob.size());
+    base = stg;
+  }
+  for (const auto &op : pl.ops)
+    op(d->stream, base);
+  stage_release(d, slot);
+}
+
+/* Capture the plan twice (one graph per upload buffer). */
+static bool capture_entry(MopHipDevice *d, const Plan &pl, GraphEntry &e) {
+  e.sig = pl.sig;
+  e.blob_size = pl.blob.size();
+  for (int k = 0; k < 2; k++) {
+    if (hipHostMalloc((void **)&e.blob[k], e.blob_size ? e.blob_size : 16, 0) != hipSuccess ||
+        hipEventCreateWithFlags(&e.done[k], hipEventDisableTiming) != hipSuccess) {
+      free_graph_entry(e);
+      return false;
+    }
+    if (e.blob_size)
+      memcpy(e.blob[k], pl.blob.data(), e.blob_size);
+    if (hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      free_graph_entry(e);
+      return false;
+    }
+    for (const auto &op : pl.ops)
+      op(d->stream, e.blob[k]);
+    hipGraph_t g = nullptr;
+    hipError_t er = hipStreamEndCapture(d->stream, &g);
+    if (er != hipSuccess || !g ||
+        hipGraphInstantiate(&e.exec[k], g, nullptr, nullptr, 0) != hipSuccess) {
+      if (g)
+        (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      free_graph_entry(e);
+      return false;
+    }
+    (void)hipGraphDestroy(g);
+  }
+  return true;
+}
+
+static void replay_entry(MopHipDevice *d, GraphEntry &e, const Plan &pl) {
+  int k = e.next;
+  e.next ^= 1;
+  e.use = ++d->plan_clock;
+  HIPCHK(hipEventSynchronize(e.done[k])); /* its previous replay has read the buffer */
+  if (e.blob_size)
+    memcpy(e.blob[k], pl.blob.data(), e.blob_size);
+  HIPCHK(hipGraphLaunch(e.exec[k], d->stream));
+  HIPCHK(hipEventRecord(e.done[k], d->stream));
+  d->graph_replays++;
+}
+
+/* A plan seen for the second time in a row is captured; a captured plan is
+ * replayed; anything else is issued directly. */
+static void issue_plan(MopHipDevice *d, const Plan &pl) {
+  for (GraphEntry &e : d->graphs)
+    if (e.sig == pl.sig) {
+      replay_entry(d, e, pl);
+      return;
+    }
+  if (!pl.ops.empty() && pl.sig == d->last_sig) {
+    GraphEntry e;
+    if (capture_entry(d, pl, e)) {
+      if (d->graphs.size() >= 8) {
+        size_t lru = 0;
+        for (size_t i = 1; i < d->graphs.size(); i++)
+          if (d->graphs[i].use < d->graphs[lru].use)
+            lru = i;
+        free_graph_entry(d->graphs[lru]);
+        d->graphs.erase(d->graphs.begin() + (long)lru);
+      }
+      d->graphs.push_back(std::move(e));
+      d->graph_captures++;
+      replay_entry(d, d->graphs.back(), pl);
+      return;
+    }
+  }
+  d->last_sig = pl.sig;
+  issue_direct(d, pl);
+}
+
+static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &todo);
+static void run_shadow_job(MopHipDevice *d, MopHipShadowMap *map,
+                           const std::vector<ShadowDrawHost> &casters);
+
 static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_t n) {
   std::vector<MopHipFramebuffer *> todo;
   for (size_t i = 0; i < n; i++)
@@ -1283,6 +1455,38 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
   if (todo.empty())
     return;
   HIPCHK(hipEventRecord(d->ev0, d->stream));
+  bool planned = false;
+  if (d->graphs_on && !d->tm.on) {
+    Plan pl;
+    t_plan = &pl;
+    t_plan_abort = false;
+    frame_body(d, todo);
+    t_plan = nullptr;
+    if (!t_plan_abort) {
+      issue_plan(d, pl);
+      planned = true;
+    }
+    t_plan_abort = false;
+  }
+  if (!planned)
+    frame_body(d, todo); /* direct (timers on, graphs off, or a buffer had to grow) */
+  d->shadow_jobs.clear();
+  for (MopHipFramebuffer *fb : todo) {
+    HIPCHK(hipEventRecord(fb->done, d->stream));
+    fb->executed = true;
+    fb->checked = false;
+    invalidate_host(fb);
+  }
+  HIPCHK(hipEventRecord(d->ev1, d->stream));
+  d->timing_pending = true;
+}
+
+/* Everything a frame execution launches, in order: queued shadow passes,
+ * clears, the segments of every frame (order-free and ordered batches in
+ * rounds), the tail.  Runs under a plan or directly. */
+static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &todo) {
+  for (const ShadowJob &j : d->shadow_jobs)
+    run_shadow_job(d, j.map, j.casters);
   std::vector<std::vector<std::pair<size_t, size_t>>> segs(todo.size());
   size_t rounds = 0;
   for (size_t i = 0; i < todo.size(); i++) {
@@ -1307,7 +1511,6 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
     if (fb->nseg > MOPH_MAX_SEGS)
       set_err("frame has %d draw segments; counters of segments past %d are merged",
               fb->nseg, MOPH_MAX_SEGS);
-    memset(fb->h_cnt, 0, MOPH_MAX_SEGS * CNT_N * 4);
     if (segs[i].size() > rounds)
       rounds = segs[i].size();
     if (fb->skipped)
@@ -1326,16 +1529,7 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
     if (!jobs[1].empty())
       run_batch(d, jobs[1], true);
   }
-  run_resolve(d, todo);
-  run_post(d, todo);
-  for (MopHipFramebuffer *fb : todo) {
-    HIPCHK(hipEventRecord(fb->done, d->stream));
-    fb->executed = true;
-    fb->checked = false;
-    invalidate_host(fb);
-  }
-  HIPCHK(hipEventRecord(d->ev1, d->stream));
-  d->timing_pending = true;
+  run_tail(d, todo);
 }
 
 /* Wait for fb's frame, re-run it when a scratch capacity overflowed, and
@@ -1579,7 +1773,7 @@ extern "C" MopHipShadowMap *mop_hip_shadow_map_create(MopHipDevice *d, int size)
   MopHipShadowMap *m = new MopHipShadowMap();
   m->size = size;
   size_t n = (size_t)size * size;
-  if (!m->keys.ensure(n) || !m->depth.ensure(n)) {
+  if (!m->keys.ensure(n)) {
     delete m;
     return nullptr;
   }
@@ -1589,17 +1783,54 @@ extern "C" MopHipShadowMap *mop_hip_shadow_map_create(MopHipDevice *d, int size)
 extern "C" void mop_hip_shadow_map_destroy(MopHipDevice *d, MopHipShadowMap *m) {
   if (!m)
     return;
-  if (d)
+  if (d) {
+    /* drop queued passes into this map; their frames would read it anyway */
+    for (size_t i = 0; i < d->shadow_jobs.size();)
+      if (d->shadow_jobs[i].map == m)
+        d->shadow_jobs.erase(d->shadow_jobs.begin() + (long)i);
+      else
+        i++;
     (void)hipStreamSynchronize(d->stream);
+    /* graphs captured with this map's pointers must not be replayed */
+    for (GraphEntry &e : d->graphs)
+      free_graph_entry(e);
+    d->graphs.clear();
+    d->last_sig.clear();
+  }
   m->keys.release();
-  m->depth.release();
   delete m;
 }
 
+/* Frame plans replayed from captured hipGraphs: on by default.  Off, every
+ * frame's launches are issued one by one. */
+extern "C" void mop_hip_set_graphs(MopHipDevice *d, int on) {
+  if (!d)
+    return;
+  d->graphs_on = on != 0;
+  if (!d->graphs_on) {
+    for (GraphEntry &e : d->graphs)
+      free_graph_entry(e);
+    d->graphs.clear();
+    d->last_sig.clear();
+  }
+}
+
+extern "C" void mop_hip_graph_stats(MopHipDevice *d, uint64_t *replays, uint64_t *captures) {
+  if (!d)
+    return;
+  if (replays)
+    *replays = d->graph_replays;
+  if (captures)
+    *captures = d->graph_captures;
+}
+
 /* Begin a caster list for `map` (cleared to depth 1.0 when rendered). */
+static void flush_shadow_jobs(MopHipDevice *d);
+
 static void shadow_begin(MopHipDevice *d, MopHipShadowMap *map) {
   d->shadow_draws.clear();
   if (!d->shadow_temp.empty()) {
+    flush_shadow_jobs(d); /* queued passes still read the temporaries */
     (void)hipStreamSynchronize(d->stream);
     for (MopHipBuffer *b : d->shadow_temp) {
       (void)hipFree(b->dev);
@@ -1625,9 +1856,11 @@ static void shadow_add(MopHipDevice *d, const void *vdev, uint32_t vertex_count,
   d->shadow_draws.push_back(sh);
 }
 
-/* Rasterize the recorded casters into d->rec_shadow (keys -> float depth). */
-static void run_shadow(MopHipDevice *d) {
-  MopHipShadowMap *map = d->rec_shadow;
+/* Rasterize a caster list into its map (ordered depth keys).  Shadow passes
+ * are queued at shadow_end / shadow_set and issued at the start of the next
+ * frame execution (in the same plan), before any draw that reads them. */
+static void run_shadow_job(MopHipDevice *d, MopHipShadowMap *map,
+                           const std::vector<ShadowDrawHost> &casters) {
   if (!map)
     return;
   TmScope t(d, TM_SHADOW);
@@ -1636,7 +1869,7 @@ static void run_shadow(MopHipDevice *d) {
   launch_shadow_clear((int)npx, map->keys.p, d->stream);
   uint32_t n_tris = 0, n_verts = 0;
   std::vector<ShadowDrawGPU> gd;
-  for (ShadowDrawHost &sh : d->shadow_draws) {
+  for (const ShadowDrawHost &sh : casters) {
     ShadowDrawGPU g = sh.g;
     g.tri_base = n_tris;
     g.vtx_base = n_verts;
@@ -1646,28 +1879,30 @@ static void run_shadow(MopHipDevice *d) {
   }
   if (!gd.empty() && n_tris) {
     /* the big-triangle queue holds every caster in the worst case */
-    if (!d->shadow_big.ensure(n_tris) || !d->shadow_draws_dev.ensure(gd.size()) ||
-        !d->counters.ensure(CNT_N) || !d->shadow_lclip.ensure(n_verts ? n_verts : 1) ||
+    /* one upload: [counters (zeroed) | caster draws] */
+    size_t od = 256, bytes = gd.size() * sizeof(ShadowDrawGPU);
+    if (!d->shadow_big.ensure(n_tris) || !d->shadow_arena.ensure(od + bytes) ||
+        !d->shadow_lclip.ensure(n_verts ? n_verts : 1) ||
         !d->shadow_lscr.ensure(n_verts ? n_verts : 1))
       return;
-    size_t bytes = gd.size() * sizeof(ShadowDrawGPU);
     int slot;
-    uint8_t *stg = stage_acquire(d, bytes, &slot);
+    uint8_t *stg = stage_acquire(d, od + bytes, &slot);
     if (!stg)
       return;
-    memcpy(stg, gd.data(), bytes);
-    HIPCHK(hipMemcpyAsync(d->shadow_draws_dev.p, stg, bytes, hipMemcpyHostToDevice, d->stream));
+    memset(stg, 0, CNT_N * sizeof(uint32_t));
+    memcpy(stg + od, gd.data(), bytes);
+    upload(d, d->shadow_arena.p, stg, od + bytes);
     stage_release(d, slot);
-    launch_fill_u32(CNT_N, d->counters.p, 0u, d->stream);
     ShadowArgsGPU sa;
-    sa.draws = d->shadow_draws_dev.p;
+    memset(&sa, 0, sizeof(sa)); /* padding bytes enter plan signatures */
+    sa.draws = (const ShadowDrawGPU *)(d->shadow_arena.p + od);
     sa.ndraws = (uint32_t)gd.size();
     sa.n_tris = n_tris;
     sa.size = size;
     sa.big_cap = (uint32_t)d->shadow_big.cap;
     sa.keys = map->keys.p;
     sa.big_queue = d->shadow_big.p;
-    sa.counters = d->counters.p;
+    sa.counters = (uint32_t *)d->shadow_arena.p;
     sa.n_verts = n_verts;
     sa.pad_ = 0;
     sa.lclip = d->shadow_lclip.p;
@@ -1675,12 +1910,25 @@ static void run_shadow(MopHipDevice *d) {
     launch_shadow_raster(sa, d->stream);
     d->stats.shadow_tris = n_tris;
   }
-  launch_shadow_finalize((int)npx, map->keys.p, map->depth.p, d->stream);
+
+}
+
+static void queue_shadow(MopHipDevice *d) {
+  d->shadow_jobs.push_back({d->rec_shadow, d->shadow_draws});
+}
+
+/* issue the queued shadow passes now (outside a frame) */
+static void flush_shadow_jobs(MopHipDevice *d) {
+  if (d->shadow_jobs.empty())
+    return;
+  for (const ShadowJob &j : d->shadow_jobs)
+    run_shadow_job(d, j.map, j.casters);
+  d->shadow_jobs.clear();
 }
 
 static void shadow_enable(MopHipDevice *d, const MopHipShadowMap *map, const MopHipMat4 &lvp) {
   d->cur_state.shadow_on = true;
-  d->cur_state.shadow_map = map->depth.p;
+  d->cur_state.shadow_map = map->keys.p;
   d->cur_state.shadow_w = map->size;
   d->cur_state.shadow_h = map->size;
   memcpy(d->cur_state.shadow_vp, lvp.d, sizeof(d->cur_state.shadow_vp));
@@ -1717,7 +1965,7 @@ extern "C" void mop_hip_shadow_set(MopHipDevice *d, int w, int h, const MopHipMa
             d->rec_shadow->size);
     return;
   }
-  run_shadow(d);
+  queue_shadow(d);
   shadow_enable(d, d->rec_shadow, *light_vp);
 }
 
@@ -1751,7 +1999,7 @@ extern "C" void mop_hip_shadow_draw(MopHipDevice *d, MopHipBuffer *vb, uint32_t 
 extern "C" void mop_hip_shadow_end(MopHipDevice *d, const MopHipMat4 *light_vp) {
   if (!d || !light_vp || !d->rec_shadow)
     return;
-  run_shadow(d);
+  queue_shadow(d);
   shadow_enable(d, d->rec_shadow, *light_vp);
 }
 
@@ -1929,9 +2177,9 @@ extern "C" void mop_hip_hdr_resolve(MopHipDevice *d, MopHipFramebuffer *fb, floa
     execute_frames(d, &fb, 1);
     return;
   }
+  /* the tail recomputes from the HDR buffer, so re-running it is exact */
   std::vector<MopHipFramebuffer *> one(1, fb);
-  run_resolve(d, one);
-  run_post(d, one);
+  run_tail(d, one);
   HIPCHK(hipEventRecord(fb->done, d->stream));
 }
 
@@ -1962,7 +2210,8 @@ extern "C" void mop_hip_postprocess_apply(MopHipDevice *d, MopHipFramebuffer *fb
     execute_frames(d, &fb, 1);
     return;
   }
+  /* resolve + post from the HDR buffer: the same as post on the resolved image */
   std::vector<MopHipFramebuffer *> one(1, fb);
-  run_post(d, one);
+  run_tail(d, one);
   HIPCHK(hipEventRecord(fb->done, d->stream));
 }
