@@ -146,7 +146,7 @@ print(json.dumps({{"frames": len(times), "median_ms": times[len(times)//2], "tri
                       f"mop_viewport_render + read_color wall time"}
 
 
-def batched_form(mod, rounds):
+def batched_form(mod, rounds, tune=True):
     """The 64-viewport workload (uneven resolutions, 1e3..2e6 triangles each):
     plain = one viewport per launch set, each finished before the next;
     batched = all 64 recorded, then executed in one set of launches.  Each
@@ -166,6 +166,23 @@ def batched_form(mod, rounds):
         d.render(fb)
     dev.sync()
     log("batched: warm-up done")
+
+    # autotune the launch shapes on this batch (results never depend on them)
+    from mop_amd import autotune
+
+    def one_batch():
+        for d, fb in zip(drv, fbs):
+            d.record(fb)
+        dev.execute(fbs)
+        dev.sync()
+
+    if tune:
+        best, table = autotune.autotune(dev, one_batch)
+        log(f"batched: autotuned (small_area, shade_tile, raster_grid) = {best}: "
+            f"{table[best]:.3f} ms (default {table.get((64, 16, 8192), float('nan')):.3f} ms)")
+    else:
+        best, table = (64, 16, 8192), {}
+        autotune.set_tuning(dev, *best)
     t0 = time.perf_counter()
     for r in range(rounds):
         for d, fb in zip(drv, fbs):
@@ -211,7 +228,9 @@ def batched_form(mod, rounds):
            "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "host_record_ms": round(float(np.median(rec_s)) * 1e3, 3),
            "host_launch_ms": round(float(np.median(exe_s)) * 1e3, 3),
-           "device_span_ms": round(float(np.median(gpu_ms)), 3)}
+           "device_span_ms": round(float(np.median(gpu_ms)), 3),
+           "tuning": {"small_area": best[0], "shade_tile": best[1], "raster_grid": best[2],
+                      "candidates_ms": {str(k): round(v, 3) for k, v in table.items()}}}
     for fb in fbs:
         fb.close()
     for d in drv:
@@ -229,6 +248,10 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the 64-viewport batched form (for profiling)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batched form with the default launch shapes")
     ap.add_argument("--batch-rounds", type=int, default=8)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the "
@@ -241,6 +264,12 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.batch_only:
+        mod = load_pkg()
+        torch.cuda.init()
+        print(json.dumps({"batched_64_viewports": batched_form(mod, args.batch_rounds,
+                                                              not args.no_tune)}))
+        return
     if args.calib_copy_mib:
         n = args.calib_copy_mib << 20
         a = torch.ones(n, dtype=torch.uint8, device="cuda")
@@ -349,7 +378,7 @@ def main():
                 "kernel": "k_shade", "launch_ms": shade_ms,
                 "algorithmic_bytes_per_launch": algo_bytes}
 
-    batch = None if args.no_batch else batched_form(mod, args.batch_rounds)
+    batch = None if args.no_batch else batched_form(mod, args.batch_rounds, not args.no_tune)
     if batch:
         log(f"batched: {batch}")
     if not args.no_cpu:

@@ -278,7 +278,9 @@ void mop_hip_graph_stats(MopHipDevice *dev, uint64_t *replays, uint64_t *capture
 
 /* Launch-shape tunables (results never depend on them): bbox area up to
  * which one lane walks a screen triangle (default 64), shading tile edge
- * 8/16/32 (default 16), workgroups of the small-triangle raster (0 = auto).
+ * 8/16/32 (default 16), waves working through the large-triangle queue
+ * (0 = 8192; the small-triangle pass of batches over 2^27 triangles takes
+ * it as its workgroup count).
  * Returns 0, or -1 for an invalid value. */
 int mop_hip_set_tuning(MopHipDevice *dev, uint32_t small_area, uint32_t shade_tile,
                        uint32_t raster_grid);

@@ -812,9 +812,12 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
  * starts are the serial chain w_row += e_dy (rasterizer.c:2148-2150); lane 0
  * extends it 256 rows at a time into LDS.  mode 0 updates the visibility
  * buffer; mode 1 (after all updates) stores the winner's edge values. */
+/* One wave per large triangle (rows in chunks of 64 lanes). */
+#define BIG_WG 64
+
 template <int MODE>
-__global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
-  __shared__ float rows[4][WG];
+__global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
+  __shared__ float rows[4][BIG_WG];
   __shared__ float carry[4];
   uint32_t nbig = fa.counters[CNT_BIG];
   for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
@@ -838,11 +841,11 @@ __global__ __launch_bounds__(WG) void k_raster_big(FrameArgs fa) {
       carry[2] = r.es.w_row[2];
       carry[3] = fz.row;
     }
-    for (int c0 = 0; c0 < nrows; c0 += WG) {
+    for (int c0 = 0; c0 < nrows; c0 += BIG_WG) {
       __syncthreads();
       if (threadIdx.x == 0) {
         float a0 = carry[0], a1 = carry[1], a2 = carry[2], a3 = carry[3];
-        int lim = nrows - c0 < WG ? nrows - c0 : WG;
+        int lim = nrows - c0 < BIG_WG ? nrows - c0 : BIG_WG;
         for (int k = 0; k < lim; k++) {
           rows[0][k] = a0;
           rows[1][k] = a1;
@@ -903,8 +906,8 @@ void launch_raster(const FrameArgs &fa, hipStream_t s) {
   if (grid == 0)
     grid = 1;
   launch(k_raster_small, dim3(grid), dim3(WG), 0, s, fa);
-  launch(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
-  launch(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
+  launch(k_raster_big<0>, dim3(8192), dim3(BIG_WG), 0, s, fa);
+  launch(k_raster_big<1>, dim3(8192), dim3(BIG_WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1063,9 +1066,12 @@ void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
   /* the per-workgroup counts are summed by k_finish */
 }
 
+/* raster_grid sets the number of waves working through the large-triangle
+ * queue (0 = 8192) */
 void launch_raster_big(const FrameArgs &fa, hipStream_t s) {
-  launch(k_raster_big<0>, dim3(2048), dim3(WG), 0, s, fa);
-  launch(k_raster_big<1>, dim3(2048), dim3(WG), 0, s, fa);
+  uint32_t g = fa.raster_grid ? fa.raster_grid : 8192;
+  launch(k_raster_big<0>, dim3(g), dim3(BIG_WG), 0, s, fa);
+  launch(k_raster_big<1>, dim3(g), dim3(BIG_WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1693,13 +1699,16 @@ __global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
   fa.ids[idx] = d.object_id;
 }
 
-/* End of a batch, one workgroup: add the per-workgroup screen-triangle
- * counts (order-free path) to the batch counters, then copy the counters to
- * every view's host-visible counter slot. */
-__global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk) {
+/* End of a batch: add the per-workgroup screen-triangle counts (order-free
+ * path) to the batch counters -- each workgroup sums a chunk and adds it
+ * once -- and the last workgroup to finish copies the counters to every
+ * view's host-visible counter slot. */
+__global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint32_t chunk) {
   __shared__ uint32_t part[WG];
+  __shared__ bool last;
+  uint32_t lo = blockIdx.x * chunk, hi = lo + chunk < nblk ? lo + chunk : nblk;
   uint32_t c = 0;
-  for (uint32_t i = threadIdx.x; i < nblk; i += WG)
+  for (uint32_t i = lo + threadIdx.x; i < hi; i += WG)
     c += fa.blk_counts[i];
   part[threadIdx.x] = c;
   __syncthreads();
@@ -1708,14 +1717,20 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk) {
       part[threadIdx.x] += part[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0 && nblk)
-    fa.counters[CNT_SUBTRI] += part[0];
+  if (threadIdx.x == 0) {
+    if (part[0])
+      atomicAdd(&fa.counters[CNT_SUBTRI], part[0]);
+    __threadfence();
+    last = atomicAdd(&fa.counters[CNT_TICKET], 1u) == gridDim.x - 1;
+  }
   __syncthreads();
-  __threadfence_block();
+  if (!last)
+    return;
+  __threadfence();
   for (uint32_t v = threadIdx.x; v < fa.nviews; v += WG) {
     uint32_t *o = fa.views[v].cnt_out;
     for (int k = 0; k < CNT_N; k++)
-      o[k] = fa.counters[k];
+      o[k] = __hip_atomic_load(&fa.counters[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1723,7 +1738,13 @@ void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
   if (!fa.nviews)
     return;
   uint32_t nblk = fa.fused ? (fa.n_tris + WG - 1) / WG + CLIP_RASTER_GRID : 0u;
-  launch(k_finish, dim3(1), dim3(WG), 0, s, fa, nblk);
+  uint32_t grid = (nblk + 4095) / 4096;
+  if (grid < 1)
+    grid = 1;
+  if (grid > 256)
+    grid = 256;
+  uint32_t chunk = (nblk + grid - 1) / grid;
+  launch(k_finish, dim3(grid), dim3(WG), 0, s, fa, nblk, chunk);
 }
 
 void launch_shade(const FrameArgs &fa, hipStream_t s) {
@@ -1752,6 +1773,41 @@ __global__ __launch_bounds__(WG) void k_clear(int npx, float r, float g, float b
   depth[i] = 1.0f;
   ids[i] = 0u;
   vis[i] = (unsigned long long)depth_key(1.0f) << 32;
+}
+
+DEV uint32_t find_clear_view(const ClearViewGPU *v, uint32_t n, uint32_t px) {
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) >> 1;
+    if (v[mid].px_base <= px)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+/* mop_sw_framebuffer_clear (rasterizer.c:114-134) for every view of a
+ * frame execution in one launch; the RGBA8 image is left alone (the tail
+ * writes every pixel of it). */
+__global__ __launch_bounds__(WG) void k_clear_views(const ClearViewGPU *views, uint32_t n,
+                                                    uint32_t total) {
+  uint32_t p = blockIdx.x * WG + threadIdx.x;
+  if (p >= total)
+    return;
+  const ClearViewGPU &v = views[find_clear_view(views, n, p)];
+  uint32_t i = p - v.px_base;
+  reinterpret_cast<float4 *>(v.hdr)[i] = make_float4(v.clear[0], v.clear[1], v.clear[2],
+                                                     v.clear[3]);
+  v.depth[i] = 1.0f;
+  v.ids[i] = 0u;
+  v.vis[i] = (unsigned long long)depth_key(1.0f) << 32;
+}
+
+void launch_clear_views(const ClearViewGPU *views, uint32_t n, uint32_t total, hipStream_t s) {
+  if (!n || !total)
+    return;
+  launch(k_clear_views, dim3((total + WG - 1) / WG), dim3(WG), 0, s, views, n, total);
 }
 
 void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr, float *depth,

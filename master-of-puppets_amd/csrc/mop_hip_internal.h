@@ -140,6 +140,7 @@ enum {
   CNT_OVERFLOW = 3, /* bit0 clip queue, bit1 subtri store, bit2 shadow big queue,
                        bit3 clipped polygon longer than MOPH_MAX_POLY, bit4 tile bins */
   CNT_SHADOW_BIG = 4,
+  CNT_TICKET = 7, /* k_finish: workgroups done */
   CNT_N = 8
 };
 
@@ -203,6 +204,18 @@ struct FrameArgs {
   uint32_t *big_queue;
   const MopHipLight *lights;
 };
+
+/* One view of the batched clear. */
+struct ClearViewGPU {
+  uint32_t px_base; /* first pixel of the view in the launch */
+  uint32_t pad_;
+  float clear[4];
+  float *hdr;
+  float *depth;
+  uint32_t *ids;
+  unsigned long long *vis;
+};
+void launch_clear_views(const ClearViewGPU *views, uint32_t n, uint32_t total, hipStream_t s);
 
 /* Per-view arguments of the resolve and post passes. */
 struct PostViewGPU {

@@ -273,6 +273,7 @@ struct MopHipDevice {
   uint32_t *h_counters = nullptr;
   DevArr<ViewGPU> views;
   DevArr<PostViewGPU> pviews;
+  DevArr<ClearViewGPU> clear_views;
   DevArr<uint8_t> post_lut; /* 4 x 256 colour-effect tables */
   /* shadow: the map the hook API renders into, and the recorded casters */
   MopHipShadowMap *def_shadow = nullptr;
@@ -390,6 +391,7 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   }
   d->views.release();
   d->pviews.release();
+  d->clear_views.release();
   d->tile_counts.release();
   d->tile_cursor.release();
   d->bins.release();
@@ -1487,13 +1489,36 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
 static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &todo) {
   for (const ShadowJob &j : d->shadow_jobs)
     run_shadow_job(d, j.map, j.casters);
+  /* the clears of every frame in one launch */
+  std::vector<ClearViewGPU> cv;
+  uint32_t cpx = 0;
+  for (MopHipFramebuffer *fb : todo)
+    if (fb->has_clear) {
+      ClearViewGPU c;
+      memset(&c, 0, sizeof(c));
+      c.px_base = cpx;
+      cpx += (uint32_t)(fb->w * fb->h);
+      memcpy(c.clear, fb->clear, sizeof(c.clear));
+      c.hdr = fb->hdr;
+      c.depth = fb->depth;
+      c.ids = fb->ids;
+      c.vis = fb->vis;
+      cv.push_back(c);
+    }
+  if (!cv.empty() && d->clear_views.ensure(cv.size())) {
+    int slot;
+    uint8_t *stg = stage_acquire(d, cv.size() * sizeof(ClearViewGPU), &slot);
+    if (stg) {
+      memcpy(stg, cv.data(), cv.size() * sizeof(ClearViewGPU));
+      upload(d, d->clear_views.p, stg, cv.size() * sizeof(ClearViewGPU));
+      stage_release(d, slot);
+      launch_clear_views(d->clear_views.p, (uint32_t)cv.size(), cpx, d->stream);
+    }
+  }
   std::vector<std::vector<std::pair<size_t, size_t>>> segs(todo.size());
   size_t rounds = 0;
   for (size_t i = 0; i < todo.size(); i++) {
     MopHipFramebuffer *fb = todo[i];
-    if (fb->has_clear)
-      launch_clear(fb->w * fb->h, fb->clear, fb->color[0], fb->hdr, fb->depth, fb->ids, fb->vis,
-                   d->stream);
     fb->skipped = 0;
     size_t a = 0, m = fb->draws.size();
     while (a < m) {
@@ -1608,8 +1633,8 @@ extern "C" void mop_hip_frames_execute(MopHipDevice *d, MopHipFramebuffer *const
 
 /* Launch-shape tunables of the frame kernels: bbox area up to which one lane
  * walks a screen triangle, shading tile edge (8, 16 or 32 pixels) and the
- * workgroup count of the small-triangle raster (0 = one lane per triangle).
- * Results do not depend on them; speed does. */
+ * number of waves of the large-triangle raster (0 = 8192).  Results do not
+ * depend on them; speed does. */
 extern "C" int mop_hip_set_tuning(MopHipDevice *d, uint32_t small_area, uint32_t shade_tile,
                                   uint32_t raster_grid) {
   if (!d || (shade_tile != 8 && shade_tile != 16 && shade_tile != 32) || small_area < 1)
