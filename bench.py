@@ -239,6 +239,50 @@ def batched_form(mod, rounds, tune=True):
     return out
 
 
+def serving_form(mod, rates=(100.0, 1000.0, 5000.0), duration=2.0, n_scenes=16,
+                 plain_rate=1000.0):
+    """Serving form: 16 resident raster-stress scenes (2048x2048-quad
+    displaced terrain, 8.4M triangles, plus 64 icospheres; one shadowed sun,
+    4096^2 shadow map rendered once and kept), Poisson request streams at
+    several rates, resolution mix 25% 640x480 / 50% 1280x720 / 25% 1920x1080,
+    continuous batching admitted by an HBM budget; p50/p99 from arrival to
+    RGBA8 on the host.  The plain mode serves the same stream one request per
+    launch."""
+    from mop_amd import driver, scenes, serve
+    dev = driver.Device()
+    pool = []
+    for k in range(n_scenes):
+        sc = scenes.config3(1280, 720, seed=0x4D4F5010 + k)
+        pool.append((driver.SceneDriver(dev, sc, shadow_cache=True), sc))
+        log(f"serving: scene {k} resident ({sc.triangle_count} triangles)")
+    srv = serve.Server(dev, pool, budget_bytes=16 << 30, max_batch=32, inflight=2)
+    # warm-up: every scene once per class (shadow maps, scratch, graphs)
+    warm = [(0.0, si, 0.0, 0.0, c) for si in range(n_scenes) for c in range(3)]
+    srv.run(warm, time_limit=60.0)
+    log("serving: warm-up done")
+
+    def clean(r):
+        r.pop("outputs", None)
+        return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
+
+    out = {"scenes": n_scenes, "triangles_per_scene": pool[0][1].triangle_count,
+           "duration_s": duration, "batched": {}}
+    for rate in rates:
+        reqs = serve.make_requests(rate, duration, n_scenes, seed=0x4D4F5005 + int(rate))
+        out["batched"][str(int(rate))] = r = clean(srv.run(reqs, time_limit=duration * 3 + 5.0))
+        log(f"serving batched {rate:.0f} req/s: {r}")
+    reqs = serve.make_requests(plain_rate, duration, n_scenes,
+                               seed=0x4D4F5005 + int(plain_rate))
+    out["plain"] = {str(int(plain_rate)): clean(srv.run(reqs, plain=True,
+                                                        time_limit=duration * 3 + 5.0))}
+    log(f"serving plain {plain_rate:.0f} req/s: {out['plain']}")
+    srv.close()
+    for d, _ in pool:
+        d.close()
+    dev.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -252,6 +296,8 @@ def main():
                     help="run only the 64-viewport batched form (for profiling)")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched form with the default launch shapes")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--batch-rounds", type=int, default=8)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the "
@@ -269,6 +315,11 @@ def main():
         torch.cuda.init()
         print(json.dumps({"batched_64_viewports": batched_form(mod, args.batch_rounds,
                                                               not args.no_tune)}))
+        return
+    if args.serve_only:
+        mod = load_pkg()
+        torch.cuda.init()
+        print(json.dumps({"serving": serving_form(mod)}))
         return
     if args.calib_copy_mib:
         n = args.calib_copy_mib << 20
@@ -381,6 +432,7 @@ def main():
     batch = None if args.no_batch else batched_form(mod, args.batch_rounds, not args.no_tune)
     if batch:
         log(f"batched: {batch}")
+    serving = None if args.no_serve else serving_form(mod)
     if not args.no_cpu:
         log("cpu baseline: starting")
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, args.cpu_cores)
@@ -416,6 +468,7 @@ def main():
                         "covered_pixel_fraction": round(covered, 4)},
         "roofline": roof,
         "batched_64_viewports": batch,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -626,6 +626,45 @@ DEV bool load_raster_tri(int W, int H, const SubTri &st, RasterTri &r) {
   return true;
 }
 
+/* When is the smooth walk's AA distance test (min_i w_i * inv_e_i >= 0,
+ * rasterizer.c:1929-1936) implied by its edge test (all w_i >= 0)?  With
+ * inv_e_i finite and positive, w_i * inv_e_i is >= 0 (or -0.0, or +inf)
+ * for every w_i >= 0, so the min passes too.  inv_e_i is 1/len (finite) when
+ * len > 1e-6 and 0 otherwise; a squared length above 1e-10 guarantees the
+ * former (and one below 1e30 keeps 1/len positive).  Only a triangle with a
+ * near-degenerate or enormous edge needs the full test
+ * (where 0 * inf could make it fail); the others skip the three square roots
+ * and divisions.  The order-free kernels use this; the ordered path also
+ * needs min_d as coverage and always computes it. */
+DEV bool edges_nondegenerate(const EdgeSetup &es) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float l2 = es.e_dx[k] * es.e_dx[k] + es.e_dy[k] * es.e_dy[k];
+    if (!(l2 > 1e-10f && l2 < 1e30f)) /* and 1/len stays a normal positive float */
+      return false;
+  }
+  return true;
+}
+
+/* load_raster_tri for the order-free kernels: inv_e only where the AA
+ * distance test is not implied (then inv_e holds the real values, else
+ * ones, which make frag_depth's min_d test the edge test's consequence). */
+DEV bool load_raster_tri_fast(int W, int H, const SubTri &st, RasterTri &r) {
+  if (!edge_setup(st.sx, st.sy, W, H, r.es))
+    return false;
+  if (edges_nondegenerate(r.es)) {
+    r.inv_e[0] = r.inv_e[1] = r.inv_e[2] = 1.0f;
+  } else {
+    r.inv_e[0] = inv_edge_len(r.es.e_dx[0], r.es.e_dy[0]);
+    r.inv_e[1] = inv_edge_len(r.es.e_dx[1], r.es.e_dy[1]);
+    r.inv_e[2] = inv_edge_len(r.es.e_dx[2], r.es.e_dy[2]);
+  }
+  r.sz[0] = st.sz[0];
+  r.sz[1] = st.sz[1];
+  r.sz[2] = st.sz[2];
+  return true;
+}
+
 DEV int draw_sm(const DrawGPU &d) { return (int)((d.flags >> MOPH_DRAW_SM_SHIFT) & 3u); }
 DEV bool sm_flat(int sm) { return sm == MOPH_SM_LIT_FLAT || sm == MOPH_SM_UNLIT_FLAT; }
 
@@ -827,7 +866,7 @@ __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
     const DrawGPU &dr = fa.draws[st.draw];
     const ViewGPU &vw = fa.views[dr.view];
     RasterTri r;
-    if (!load_raster_tri(vw.w, vw.h, st, r)) /* succeeded when queued; uniform */
+    if (!load_raster_tri_fast(vw.w, vw.h, st, r)) /* succeeded when queued; uniform */
       continue;
     bool flat = sm_flat(draw_sm(dr)); /* uniform */
     FlatZ fz = {0.0f, 0.0f, 0.0f};
@@ -926,9 +965,13 @@ DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, 
     fa.big_queue[q] = e;
     return true;
   }
-  r.inv_e[0] = inv_edge_len(r.es.e_dx[0], r.es.e_dy[0]);
-  r.inv_e[1] = inv_edge_len(r.es.e_dx[1], r.es.e_dy[1]);
-  r.inv_e[2] = inv_edge_len(r.es.e_dx[2], r.es.e_dy[2]);
+  if (edges_nondegenerate(r.es)) {
+    r.inv_e[0] = r.inv_e[1] = r.inv_e[2] = 1.0f; /* min_d implied, see above */
+  } else {
+    r.inv_e[0] = inv_edge_len(r.es.e_dx[0], r.es.e_dy[0]);
+    r.inv_e[1] = inv_edge_len(r.es.e_dx[1], r.es.e_dy[1]);
+    r.inv_e[2] = inv_edge_len(r.es.e_dx[2], r.es.e_dy[2]);
+  }
   r.sz[0] = st.sz[0];
   r.sz[1] = st.sz[1];
   r.sz[2] = st.sz[2];
@@ -1668,7 +1711,7 @@ __global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
   SubTri st;
   load_subtri(frame, s, st);
   RasterTri r;
-  if (!load_raster_tri(fa.w, fa.h, st, r))
+  if (!load_raster_tri_fast(fa.w, fa.h, st, r)) /* inv_e unused by shading */
     return;
   const DrawGPU &d = frame.draws[st.draw];
   int sm = draw_sm(d);

@@ -135,6 +135,7 @@ class Scene:
     post_effects: int = 1 | 16     # MOP_POST_GAMMA | MOP_POST_FXAA (viewport.c:646)
     shading: int = 1              # viewport shading mode (viewport.c:645: smooth)
     render_mode: int = 0          # MopRenderMode: 0 solid, 1 wireframe
+    ssaa: int = 1                 # MopViewportDesc.ssaa_factor (internal buffers ssaa x size)
     light_dir: tuple = (0.3, 1.0, 0.5)  # viewport light_dir (viewport.c:643)
     ambient: float = None         # None keeps the viewport default (0.25)
 
@@ -256,12 +257,46 @@ def value_noise(x, z, seed, octaves=6, base_freq=1.0 / 32.0):
     return total / norm
 
 
+def value_noise_grid(xs, zs, seed, octaves=6, base_freq=1.0 / 32.0):
+    """value_noise over the grid meshgrid(xs, zs) (rows follow zs), with the
+    same float64 operations per point: the lattice hash is evaluated once
+    per lattice point and gathered, and the smoothstep weights per row and
+    per column."""
+    xs = np.asarray(xs, np.float64)
+    zs = np.asarray(zs, np.float64)
+    total = np.zeros((len(zs), len(xs)))
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for o in range(octaves):
+        fx, fz = xs * freq, zs * freq
+        ix, iz = np.floor(fx).astype(np.int64), np.floor(fz).astype(np.int64)
+        tx, tz = fx - ix, fz - iz
+        sx = (tx * tx * (3 - 2 * tx))[None, :]
+        sz = (tz * tz * (3 - 2 * tz))[:, None]
+        x0, z0 = int(ix.min()), int(iz.min())
+        la = np.arange(x0, int(ix.max()) + 2)[None, :]
+        lb = np.arange(z0, int(iz.max()) + 2)[:, None]
+        h = (la * 374761393 + lb * 668265263 + (seed + o * 1013) * 2147483647) & 0xFFFFFFFF
+        h = ((h ^ (h >> 13)) * 1274126177) & 0xFFFFFFFF
+        h = h ^ (h >> 16)
+        table = (h & 0xFFFFFF) / float(0xFFFFFF) * 2.0 - 1.0  # [z, x]
+        cx, cz = (ix - x0)[None, :], (iz - z0)[:, None]
+        v00, v10 = table[cz, cx], table[cz, cx + 1]
+        v01, v11 = table[cz + 1, cx], table[cz + 1, cx + 1]
+        top = v00 + (v10 - v00) * sx
+        bot = v01 + (v11 - v01) * sx
+        total += amp * (top + (bot - top) * sz)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
 def terrain(n_quads, size, amplitude, seed, color):
     """(n+1)^2-vertex displaced grid over [-size/2, size/2]^2, 2n^2 triangles."""
     n = n_quads
     lin = np.linspace(-size / 2, size / 2, n + 1)
     gx, gz = np.meshgrid(lin, lin)  # row = z, col = x
-    h = value_noise(gx, gz, seed) * amplitude
+    h = value_noise_grid(lin, lin, seed) * amplitude
     step = size / n
     hx = np.gradient(h, step, axis=1)
     hz = np.gradient(h, step, axis=0)

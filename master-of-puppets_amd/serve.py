@@ -42,10 +42,14 @@ def make_requests(rate, duration, n_scenes, seed=0x4D4F5005):
 
 class Server:
     def __init__(self, device, scenes, budget_bytes=8 << 30, max_batch=32, inflight=2,
-                 max_slots_per_class=64):
+                 max_slots_per_class=64, max_batch_tris=(1 << 27) - 1):
         self.d = device
         self.scenes = scenes  # list of (SceneDriver, Scene)
         self.max_batch = max_batch
+        # a batch stays under 2^27 triangles: the fused order-free path keys
+        # fragments by (triangle << 4 | fan index) in 32 bits
+        self.max_batch_tris = max_batch_tris
+        self.tris = [sc.triangle_count for _, sc in scenes]
         self.inflight = inflight
         self.free = []
         self.slots = []
@@ -66,7 +70,11 @@ class Server:
         _, si, yaw, pitch, _ = req
         return driver.camera_of(self.scenes[si][1], yaw, pitch)
 
-    def run(self, requests, plain=False, time_limit=None):
+    def run(self, requests, plain=False, time_limit=None, keep=()):
+        """Serve `requests` ((arrival s, scene, yaw, pitch, class) tuples).
+        keep: request indices whose RGBA8 results are kept (in the result's
+        "outputs", index -> (h, w, 4) uint8) to compare with the same request
+        rendered alone."""
         lat = []
         rejected = 0
         max_queue = 0
@@ -79,6 +87,9 @@ class Server:
         last_done = t0
         batches = 0
         batch_sizes = []
+        keep = set(keep)
+        outputs = {}
+        index = {id(r): k for k, r in enumerate(requests)}
         while done_count + rejected < n:
             now = time.perf_counter() - t0
             if time_limit and now > time_limit:
@@ -95,7 +106,11 @@ class Server:
             while inflight and all(fb.done() for _, fb in inflight[0][1]):
                 _, items = inflight.popleft()
                 for req, fb in items:
-                    fb.read_color_raw()
+                    ptr = fb.read_color_raw()
+                    k = index.get(id(req))
+                    if k in keep:
+                        outputs[k] = np.ctypeslib.as_array(ptr, (fb.h * fb.w * 4,)).reshape(
+                            fb.h, fb.w, 4).copy()
                     tdone = time.perf_counter() - t0
                     lat.append(tdone - req[0])
                     self.free[req[4]].append(fb)
@@ -107,9 +122,14 @@ class Server:
                 batch = []
                 skipped = collections.deque()
                 limit = 1 if plain else self.max_batch
+                tris = 0
                 while queue and len(batch) < limit:
                     req = queue.popleft()
+                    if batch and tris + self.tris[req[1]] > self.max_batch_tris:
+                        queue.appendleft(req)
+                        break
                     if self.free[req[4]]:
+                        tris += self.tris[req[1]]
                         fb = self.free[req[4]].pop()
                         batch.append((req, fb))
                     else:
@@ -126,9 +146,9 @@ class Server:
                     batch_sizes.append(len(batch))
                     progressed = True
                     if plain:
-                        # one request per launch, synchronously
-                        for req, fb in batch:
-                            fb.read_color_raw()
+                        # one request per launch, synchronously: wait for it
+                        # here; the completion step above reads it back
+                        self.d.sync()
                     continue
             if not progressed:
                 if i < n and not queue and not inflight:
@@ -137,15 +157,25 @@ class Server:
                         time.sleep(min(wait, 0.002))
                 else:
                     time.sleep(20e-6)
+        # a run cut off by its time limit leaves batches on the device: wait
+        # for them and give their slots back (they are not counted)
+        if inflight:
+            self.d.sync()
+            for _, items in inflight:
+                for req, fb in items:
+                    self.free[req[4]].append(fb)
+        unfinished = n - done_count - rejected
         lat = np.array(lat) * 1e3
         span = max(last_done, 1e-9)
         return {
             "completed": done_count,
             "rejected": rejected,
+            "unfinished": unfinished,
             "max_queue": max_queue,
             "batches": batches,
             "mean_batch": float(np.mean(batch_sizes)) if batch_sizes else 0.0,
             "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
             "sustained_fps": done_count / span,
+            "outputs": outputs,
         }

@@ -239,11 +239,11 @@ void __wrap_mop_postprocess_apply(MopSwFramebuffer *fb, uint32_t effects,
 
 /* ---- scene construction through the public API ------------------------- */
 
-MopViewport *mh_viewport(int w, int h) {
+MopViewport *mh_viewport_ssaa(int w, int h, int ssaa) {
   MopViewportDesc d = {.width = w,
                        .height = h,
                        .backend = MOP_BACKEND_CPU,
-                       .ssaa_factor = 1};
+                       .ssaa_factor = ssaa};
   MopViewport *vp = mop_viewport_create(&d);
   if (!vp)
     return NULL;
@@ -252,6 +252,8 @@ MopViewport *mh_viewport(int w, int h) {
   mop_viewport_clear_lights(vp);
   return vp;
 }
+
+MopViewport *mh_viewport(int w, int h) { return mh_viewport_ssaa(w, h, 1); }
 
 void mh_destroy(MopViewport *vp) {
   if (g_cur_vp == vp)

@@ -35,6 +35,8 @@ def harness():
         h.mh_load_backend.argtypes = [C.c_int, C.c_char_p, C.c_char_p, C.c_char_p]
         h.mh_viewport.restype = VP
         h.mh_viewport.argtypes = [C.c_int, C.c_int]
+        h.mh_viewport_ssaa.restype = VP
+        h.mh_viewport_ssaa.argtypes = [C.c_int, C.c_int, C.c_int]
         h.mh_destroy.argtypes = [VP]
         h.mh_add_mesh.restype = VP
         h.mh_add_mesh.argtypes = [VP, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
@@ -123,7 +125,7 @@ class RefViewport:
         if h.mh_set_mode(mode) != 0:
             raise RuntimeError(f"mode {mode} unavailable")
         self.h, self.mode, self.scene = h, mode, scene
-        self.vp = h.mh_viewport(scene.width, scene.height)
+        self.vp = h.mh_viewport_ssaa(scene.width, scene.height, getattr(scene, "ssaa", 1))
         if not self.vp:
             raise RuntimeError("mop_viewport_create failed")
         self._keep = []
@@ -206,11 +208,14 @@ class RefViewport:
         return self.h.mh_render(self.vp)
 
     def read(self, hdr=True):
+        """RGBA8 at the viewport size (the core averages SSAA samples,
+        viewport.c:3385-3411); depth, ID and HDR at the internal size."""
         w, h = self.scene.width, self.scene.height
+        k = getattr(self.scene, "ssaa", 1)
         color = np.zeros((h, w, 4), np.uint8)
-        depth = np.zeros((h, w), np.float32)
-        ids = np.zeros((h, w), np.uint32)
-        hdr_a = np.zeros((h, w, 4), np.float32) if hdr else None
+        depth = np.zeros((h * k, w * k), np.float32)
+        ids = np.zeros((h * k, w * k), np.uint32)
+        hdr_a = np.zeros((h * k, w * k, 4), np.float32) if hdr else None
         rc = self.h.mh_read(self.vp, color.ctypes.data, depth.ctypes.data, ids.ctypes.data,
                             hdr_a.ctypes.data if hdr else None)
         if rc != 0:

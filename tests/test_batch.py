@@ -68,3 +68,36 @@ def test_serving_loop_completes_requests():
             d.close()
     finally:
         dev.close()
+
+
+def test_serving_results_equal_alone():
+    # continuous batching: requests recorded into free slots while earlier
+    # batches are still on the device; every kept result must equal the same
+    # request rendered alone, bit for bit
+    from mop_amd import driver, serve
+    dev = driver.Device()
+    try:
+        pool = []
+        for k in range(3):
+            sc = scenes.small_scene(seed=70 + k, width=64, height=48, n=5, with_env=(k == 1))
+            pool.append((driver.SceneDriver(dev, sc, shadow_cache=True), sc))
+        srv = serve.Server(dev, pool, budget_bytes=256 << 20, max_batch=8, inflight=2)
+        reqs = serve.make_requests(rate=600.0, duration=0.25, n_scenes=len(pool), seed=1234)
+        keep = list(range(0, min(len(reqs), 24), 3))
+        res = srv.run(reqs, time_limit=20.0, keep=keep)
+        print("serving:", {k: v for k, v in res.items() if k != "outputs"}, flush=True)
+        assert res["batches"] < res["completed"]  # requests really shared launches
+        assert len(res["outputs"]) == len(keep)
+        for k in keep:
+            t, si, yaw, pitch, cls = reqs[k]
+            (w, h), _ = serve.RES_CLASSES[cls]
+            fb = dev.framebuffer(w, h)
+            pool[si][0].render(fb, driver.camera_of(pool[si][1], yaw, pitch))
+            alone = fb.read(color=True, depth=False, ids=False)["color"]
+            fb.close()
+            assert np.array_equal(alone, res["outputs"][k]), k
+        srv.close()
+        for d, _ in pool:
+            d.close()
+    finally:
+        dev.close()

@@ -82,3 +82,17 @@ def test_wireframe_render_mode(lit):
     # Xiaolin Wu lines, blended in submission order, over the core's grid
     _check(scenes.wireframe_scene(31, lit=lit))
     _check(scenes.wireframe_scene(32, 320, 200, lit=lit))
+
+
+def test_ssaa_2_downsample_and_pick():
+    sc = scenes.small_scene(seed=25, width=128, height=96, with_env=True)
+    sc.ssaa = 2
+    _check(sc)
+    ref = refcore.RefViewport(sc, 1)
+    hip = refcore.RefViewport(sc, 2)
+    ref.render()
+    hip.render()
+    for x, y in [(10, 10), (64, 48), (100, 80), (5, 90)]:
+        assert ref.pick(x, y) == hip.pick(x, y), (x, y)
+    ref.close()
+    hip.close()

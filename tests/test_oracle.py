@@ -80,3 +80,12 @@ def test_wireframe_lines(lit):
     # Xiaolin Wu lines of every screen triangle, coverage-blended in order
     a, _ = _same(scenes.wireframe_scene(31, lit=lit))
     assert len(np.unique(a["ids"])) > 3
+
+
+def test_ssaa_downsample_and_pick():
+    # ssaa_factor 2: internal buffers at twice the size, the core averages the
+    # colour samples and scales pick coordinates (viewport.c:3385-3411, 3738-3745)
+    sc = scenes.small_scene(seed=24, width=96, height=72)
+    sc.ssaa = 2
+    a, b = _same(sc)
+    assert a["depth"].shape == (144, 192) and a["color"].shape == (72, 96, 4)
