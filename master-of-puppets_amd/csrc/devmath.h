@@ -173,4 +173,11 @@ DEV long long bbox_area(const EdgeSetup &s) {
   return (long long)(s.max_x - s.min_x + 1) * (long long)(s.max_y - s.min_y + 1);
 }
 
+/* checkpoints per bbox row of a big screen triangle: one per CHK_SPAN columns */
+#define CHK_SHIFT 4
+#define CHK_SPAN (1 << CHK_SHIFT)
+DEV uint32_t chk_per_row(const EdgeSetup &es) {
+  return (uint32_t)(es.max_x - es.min_x + CHK_SPAN) >> CHK_SHIFT;
+}
+
 #endif

@@ -854,6 +854,37 @@ __global__ __launch_bounds__(WG) void k_raster_small(FrameArgs fa) {
 /* One wave per large triangle (rows in chunks of 64 lanes). */
 #define BIG_WG 64
 
+/* Columns [xa, xb] of a bbox row outside which no pixel can pass the edge
+ * test.  Along the row edge i takes the serial values c(n) = fl(c(n-1) +
+ * e_dx[i]) from c(0) = w_i; each addition rounds by at most 2^-24 of its
+ * result, so |c(n) - (w_i + n e_dx[i])| <= n 2^-23 M with M = |w_i| + N |e_dx[i]|
+ * over a row of N columns.  A pixel whose exact value lies below minus that
+ * margin cannot reach 0; the margin is doubled and two columns are added for
+ * the float evaluation of the bound.  The walk still starts at min_x (the
+ * values are serial) but skips the tests before xa and stops after xb. */
+DEV void row_span(const RasterTri &r, const float w[3], int &xa, int &xb) {
+  int lo = 0, n = r.es.max_x - r.es.min_x, hi = n;
+  float N = (float)(n + 1);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    float e = r.es.e_dx[i];
+    float margin = N * (fabsf(w[i]) + N * fabsf(e)) * 2.384185791e-7f; /* 2^-22 */
+    if (e < 0.0f) {
+      float t = (w[i] + margin) / -e;
+      int m = t < -4.0f ? -4 : (t > (float)n ? n : (int)t + 2);
+      hi = m < hi ? m : hi;
+    } else if (e > 0.0f) {
+      float t = (-margin - w[i]) / e;
+      int m = t < 0.0f ? 0 : (t > (float)n ? n + 4 : (int)t - 2);
+      lo = m > lo ? m : lo;
+    } else if (w[i] < -margin) {
+      hi = -1;
+    }
+  }
+  xa = r.es.min_x + (lo > 0 ? lo : 0);
+  xb = r.es.min_x + hi;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
   __shared__ float rows[4][BIG_WG];
@@ -874,6 +905,10 @@ __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
       fz = flat_z_setup(r);
     unsigned long long seq = (unsigned long long)(s + 1);
     int nrows = r.es.max_y - r.es.min_y + 1;
+    /* order-free path: checkpoint store of this triangle (or none) */
+    uint32_t nchk = chk_per_row(r.es);
+    float4 *chk = (MODE == 0 && fa.fused && fa.big_chk[b] != ~0u) ? fa.chk + fa.big_chk[b]
+                                                                    : nullptr;
     if (threadIdx.x == 0) {
       carry[0] = r.es.w_row[0];
       carry[1] = r.es.w_row[1];
@@ -908,7 +943,26 @@ __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
         float w0 = rows[0][threadIdx.x], w1 = rows[1][threadIdx.x], w2 = rows[2][threadIdx.x];
         float zf = rows[3][threadIdx.x];
         size_t rowbase = (size_t)y * (size_t)vw.w;
-        for (int x = r.es.min_x; x <= r.es.max_x; x++) {
+        float4 *crow = chk ? chk + (size_t)row * nchk : nullptr;
+        int xa, xb;
+        {
+          float wr[3] = {w0, w1, w2};
+          row_span(r, wr, xa, xb);
+        }
+        if (xa > xb)
+          xb = r.es.min_x - 1; /* no pixel of this row passes */
+        int x = r.es.min_x;
+        for (; x < xa && x <= xb; x++) { /* no pixel of these passes the edge test */
+          if (crow && ((x - r.es.min_x) & (CHK_SPAN - 1)) == 0)
+            crow[(x - r.es.min_x) >> CHK_SHIFT] = make_float4(w0, w1, w2, zf);
+          w0 += r.es.e_dx[0];
+          w1 += r.es.e_dx[1];
+          w2 += r.es.e_dx[2];
+          zf += fz.dx;
+        }
+        for (; x <= xb; x++) {
+          if (crow && ((x - r.es.min_x) & (CHK_SPAN - 1)) == 0)
+            crow[(x - r.es.min_x) >> CHK_SHIFT] = make_float4(w0, w1, w2, zf);
           float z;
           bool hit;
           if (flat) {
@@ -956,13 +1010,21 @@ void launch_raster(const FrameArgs &fa, hipStream_t s) {
 /* Queue a big screen triangle or walk a small one; true if it reached the
  * raster (a non-empty clamped bbox and a non-zero area). */
 DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, bool flat,
-                      uint32_t e) {
+                      uint32_t e, uint32_t okey) {
   RasterTri r;
   if (!edge_setup(st.sx, st.sy, vw.w, vw.h, r.es))
     return false;
   if (bbox_area(r.es) > (long long)fa.small_area) {
     uint32_t q = atomicAdd(&fa.counters[CNT_BIG], 1u);
+    uint32_t n = (uint32_t)(r.es.max_y - r.es.min_y + 1) * chk_per_row(r.es);
+    uint32_t off = atomicAdd(&fa.counters[CNT_CHK], n);
+    if (off > fa.chk_cap || n > fa.chk_cap - off) {
+      atomicOr(&fa.counters[CNT_OVERFLOW], 32u);
+      off = ~0u;
+    }
     fa.big_queue[q] = e;
+    fa.big_chk[q] = off;
+    fa.big_off[okey] = off;
     return true;
   }
   if (edges_nondegenerate(r.es)) {
@@ -1040,7 +1102,8 @@ __global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
             if (!(cull && signed_area >= 0.0f)) {
               const ViewGPU &vw = fa.views[d.view];
               fa.tri_rec[t] = make_uint4(v0, v1, v2, di);
-              emitted = fused_raster(fa, vw, st, sm_flat(draw_sm(d)), (t << 4) | FUSED_WHOLE);
+              emitted = fused_raster(fa, vw, st, sm_flat(draw_sm(d)), (t << 4) | FUSED_WHOLE,
+                                     t);
             }
           }
         } else {
@@ -1087,7 +1150,8 @@ __global__ __launch_bounds__(WG) void k_clip_raster(FrameArgs fa, uint32_t *blk_
     uint32_t e = (t << 4) | (uint32_t)(i - 1);
     SubTri st;
     fused_subtri(fa, e, st);
-    if (fused_raster(fa, vw, st, sm_flat(draw_sm(d)), e))
+    if (fused_raster(fa, vw, st, sm_flat(draw_sm(d)), e,
+                     fa.n_tris + q * 16u + (uint32_t)(i - 1)))
       emitted++;
   }
   if (emitted)
@@ -1111,10 +1175,11 @@ void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
 
 /* raster_grid sets the number of waves working through the large-triangle
  * queue (0 = 8192) */
+/* Order-free path: one pass (visibility and checkpoints); the shading pass
+ * finds a big triangle's edge values from the checkpoints. */
 void launch_raster_big(const FrameArgs &fa, hipStream_t s) {
   uint32_t g = fa.raster_grid ? fa.raster_grid : 8192;
   launch(k_raster_big<0>, dim3(g), dim3(BIG_WG), 0, s, fa);
-  launch(k_raster_big<1>, dim3(g), dim3(BIG_WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1135,6 +1200,19 @@ DEV float shadow_pcf(const ViewGPU &fa, V3 wp) {
   int cx = cvt_i32(u);
   int cy = cvt_i32(v);
   const float bias = 0.005f;
+  if (cx >= 1 && cx < fa.shadow_w - 1 && cy >= 1 && cy < fa.shadow_h - 1) {
+    /* all nine taps inside the map: three rows of three neighbours */
+    const uint32_t *row = fa.shadow_map + (size_t)(cy - 1) * (size_t)fa.shadow_w + (cx - 1);
+    int lit = 0;
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) {
+#pragma unroll
+      for (int dx = 0; dx < 3; dx++)
+        lit += (ndcz <= key_depth(row[dx]) + bias) ? 1 : 0;
+      row += fa.shadow_w;
+    }
+    return (float)lit / 9.0f;
+  }
   int lit = 0, total = 0;
   for (int dy = -1; dy <= 1; dy++) {
     for (int dx = -1; dx <= 1; dx++) {
@@ -1267,11 +1345,33 @@ DEV void ibl_brdf(const ViewGPU &fa, float ndotv, float roughness, float &scale,
 DEV V3 light_pos(const MopHipLight &l) { return V3{l.position.x, l.position.y, l.position.z}; }
 DEV V3 light_dir(const MopHipLight &l) { return V3{l.direction.x, l.direction.y, l.direction.z}; }
 
+/* Well-conditioned arithmetic of the light loop, which only feeds colour
+ * (within 1/255 on the 8-bit image, 1e-4 on the HDR buffer): hardware
+ * reciprocal and square root (1 ulp) instead of the correctly rounded
+ * sequences, and contraction of multiply-adds.  Depth, coverage, the shadow
+ * lookup position, every value that selects a table entry and the
+ * ill-conditioned GGX terms keep the reference's IEEE operations. */
+DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+DEV float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+DEV V3 fast_norm(V3 v) {
+  float l2 = v.x * v.x + v.y * v.y + v.z * v.z;
+  if (!(l2 >= 1e-16f))
+    return V3{0.0f, 0.0f, 0.0f};
+  float inv = __builtin_amdgcn_rsqf(l2);
+  return V3{v.x * inv, v.y * inv, v.z * inv};
+}
+
+/* 1 - N.H^2 (1 - alpha^2) of the GGX distribution: it cancels to about
+ * alpha^2 at the highlight, so it, the half vector and the light direction
+ * keep the reference's IEEE operations (no contraction, correctly rounded
+ * division and square root). */
+DEV float ggx_denom(float ndoth2, float alpha2) { return ndoth2 * (alpha2 - 1.0f) + 1.0f; }
+
 /* distance attenuation shared by point/spot (rasterizer.c:1625-1635) */
 DEV float atten(const MopHipLight &l, float dist) {
   float a;
   if (l.range > 0.0f) {
-    float r = dist / l.range;
+    float r = fast_div(dist, l.range);
     a = 1.0f - r;
     if (a < 0.0f)
       a = 0.0f;
@@ -1280,7 +1380,7 @@ DEV float atten(const MopHipLight &l, float dist) {
     float d2 = dist * dist;
     if (d2 < 0.01f)
       d2 = 0.01f;
-    a = 1.0f / d2;
+    a = fast_rcp(d2);
   }
   return a;
 }
@@ -1293,26 +1393,11 @@ struct ShadeOut {
  * compute_multi_specular_ggx (rasterizer.c:1701-1814): the two loops touch the
  * same lights in the same order and accumulate into separate sums, so one
  * loop with the PCF lookup shared gives the same values. */
-DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d, V3 normal,
-                     V3 wp, V3 view_dir,
-                     float base_r, float base_g, float base_b, float lit[3], float spec[3]) {
-  lit[0] = lit[1] = lit[2] = d.ambient;
-  spec[0] = spec[1] = spec[2] = 0.0f;
-  float roughness = d.roughness, metallic = d.metallic;
-  float alpha = roughness * roughness;
-  float alpha2 = alpha * alpha;
-  if (alpha2 < 1e-7f)
-    alpha2 = 1e-7f;
-  float ndotv = v3dot(normal, view_dir);
-  if (ndotv < 1e-4f)
-    ndotv = 1e-4f;
-  float f0_r = 0.04f * (1.0f - metallic) + base_r * metallic;
-  float f0_g = 0.04f * (1.0f - metallic) + base_g * metallic;
-  float f0_b = 0.04f * (1.0f - metallic) + base_b * metallic;
-  float k = alpha * 0.5f;
-  float g1v = ndotv / (ndotv * (1.0f - k) + k);
-  const MopHipLight *L = frame.lights + d.light_off;
-  for (uint32_t i = 0; i < d.light_count; i++) {
+DEV void light_loop(const MopHipLight *L, uint32_t count, const ViewGPU &fa, V3 normal, V3 wp,
+                    V3 view_dir, float ndotv, float alpha2, float k, float g1v, float f0_r,
+                    float f0_g, float f0_b, float lit[3], float spec[3]) {
+#pragma clang fp contract(fast)
+  for (uint32_t i = 0; i < count; i++) {
     const MopHipLight &l = L[i];
     if (!l.active)
       continue;
@@ -1333,14 +1418,14 @@ DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d
       V3 dir = v3scale(to_light, 1.0f / dist);
       ndotl_d = v3dot(normal, dir);
       if (l.type == MOP_HIP_LIGHT_SPOT) {
-        V3 spot_dir = v3norm(light_dir(l));
+        V3 spot_dir = fast_norm(light_dir(l));
         float cos_angle = -v3dot(dir, spot_dir);
         if (cos_angle < l.spot_outer_cos) {
           spot = 0.0f;
         } else if (cos_angle < l.spot_inner_cos) {
           float range_s = l.spot_inner_cos - l.spot_outer_cos;
           if (range_s > 1e-6f) {
-            float t = (cos_angle - l.spot_outer_cos) / range_s;
+            float t = fast_div(cos_angle - l.spot_outer_cos, range_s);
             spot = t * t * (3.0f - 2.0f * t);
           }
         }
@@ -1371,9 +1456,9 @@ DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d
     if (vdoth < 0.0f)
       vdoth = 0.0f;
     float ndoth2 = ndoth * ndoth;
-    float denom_d = ndoth2 * (alpha2 - 1.0f) + 1.0f;
-    float D = alpha2 / (3.14159265358979323846f * denom_d * denom_d);
-    float g1l = ndotl / (ndotl * (1.0f - k) + k);
+    float denom_d = ggx_denom(ndoth2, alpha2);
+    float D = fast_div(alpha2, 3.14159265358979323846f * denom_d * denom_d);
+    float g1l = fast_div(ndotl, ndotl * (1.0f - k) + k);
     float G = g1v * g1l;
     float omv = 1.0f - vdoth;
     float omv2 = omv * omv;
@@ -1384,13 +1469,46 @@ DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d
     float spec_denom = 4.0f * ndotl * ndotv;
     if (spec_denom < 1e-6f)
       spec_denom = 1e-6f;
-    float spec_term = D * G / spec_denom;
+    float spec_term = fast_div(D * G, spec_denom);
     float weight = spec_term * l.intensity * att_s * ndotl * shadow *
                    3.14159265358979323846f;
     spec[0] += weight * fr_r * l.color.r;
     spec[1] += weight * fr_g * l.color.g;
     spec[2] += weight * fr_b * l.color.b;
   }
+}
+
+DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d, V3 normal,
+                     V3 wp, V3 view_dir,
+                     float base_r, float base_g, float base_b, float lit[3], float spec[3]) {
+#pragma clang fp contract(fast)
+  lit[0] = lit[1] = lit[2] = d.ambient;
+  spec[0] = spec[1] = spec[2] = 0.0f;
+  float roughness = d.roughness, metallic = d.metallic;
+  float alpha = roughness * roughness;
+  float alpha2 = alpha * alpha;
+  if (alpha2 < 1e-7f)
+    alpha2 = 1e-7f;
+  float ndotv = v3dot(normal, view_dir);
+  if (ndotv < 1e-4f)
+    ndotv = 1e-4f;
+  float f0_r = 0.04f * (1.0f - metallic) + base_r * metallic;
+  float f0_g = 0.04f * (1.0f - metallic) + base_g * metallic;
+  float f0_b = 0.04f * (1.0f - metallic) + base_b * metallic;
+  float k = alpha * 0.5f;
+  float g1v = fast_div(ndotv, ndotv * (1.0f - k) + k);
+  /* the draws of one view share one light list: when every lane of the wave
+   * has the same list, its address is wave-uniform and the light records are
+   * read with scalar loads */
+  uint32_t off = d.light_off, cnt = d.light_count;
+  uint32_t off_u = __builtin_amdgcn_readfirstlane(off);
+  uint32_t cnt_u = __builtin_amdgcn_readfirstlane(cnt);
+  if (__all(off == off_u && cnt == cnt_u))
+    light_loop(frame.lights + off_u, cnt_u, fa, normal, wp, view_dir, ndotv, alpha2, k, g1v,
+               f0_r, f0_g, f0_b, lit, spec);
+  else
+    light_loop(frame.lights + off, cnt, fa, normal, wp, view_dir, ndotv, alpha2, k, g1v, f0_r,
+               f0_g, f0_b, lit, spec);
 }
 
 DEV void load_attr(const FrameArgs &fa, uint32_t slot, float4 &a0, float4 &a1, float4 &a2) {
@@ -1688,36 +1806,88 @@ DEV uint32_t find_view_by_stile(const ViewGPU *v, uint32_t n, uint32_t t) {
   return lo;
 }
 
-/* One workgroup per 16x16 tile of one view; tiles of all views of the batch
- * are numbered consecutively. */
-__global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
-  uint32_t vi = find_view_by_stile(frame.views, frame.nviews, blockIdx.x);
-  const ViewGPU &fa = frame.views[vi];
-  uint32_t T = frame.shade_tile;
-  uint32_t t = blockIdx.x - fa.stile_base;
-  uint32_t tiles_x = ((uint32_t)fa.w + T - 1) / T;
-  int px = (int)((t % tiles_x) * T + threadIdx.x % T);
-  int py = (int)((t / tiles_x) * T + threadIdx.x / T);
-  if (px >= fa.w || py >= fa.h)
+/* Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+ * XCD b % 8), each with its own L2.  xcd_tile(b, n) permutes n tile
+ * numbers so that XCD x takes one contiguous run of tiles, in raster order:
+ * neighbouring tiles share the triangles, vertices and halo pixels they read
+ * and the partial cache lines they write in one L2. */
+DEV uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  uint32_t x = b & 7u, j = b >> 3, q = n >> 3, r = n & 7u;
+  return x * q + (x < r ? x : r) + j;
+}
+
+/* Pixel idx of a cleared frame that no triangle of its first segment
+ * covers: the clear values of mop_sw_framebuffer_clear (rasterizer.c:114-134). */
+DEV void shade_clear_px(const ViewGPU &fa, size_t idx) {
+  if (!fa.shade_clear)
     return;
-  size_t idx = (size_t)py * (size_t)fa.w + (size_t)px;
-  unsigned long long key = fa.vis[idx];
-  uint32_t s1 = (uint32_t)key;
-  if (s1 == 0)
-    return;
-  uint32_t s = s1 - 1;
+  reinterpret_cast<float4 *>(fa.hdr)[idx] = make_float4(fa.clear[0], fa.clear[1], fa.clear[2],
+                                                        fa.clear[3]);
+  fa.depth[idx] = 1.0f;
+  fa.ids[idx] = 0u;
+}
+
+/* Colour, depth and object id of the pixel (px, py) whose visibility key
+ * names screen triangle s; false when the key does not name a triangle of
+ * this batch that covers the pixel.  The visibility buffer holds only keys
+ * written by this batch's raster (it is clear at the start of every frame);
+ * the checks keep every read in bounds even for a key that breaks that: a
+ * clip-slot entry outside the clip store, a pixel outside the triangle's
+ * bbox, a checkpoint outside the store. */
+DEV bool shade_px(const FrameArgs &frame, const ViewGPU &fa, uint32_t s, int px, int py,
+                  size_t idx, float4 &out, float &z, uint32_t &id) {
   if (frame.fused ? (s >> 4) >= frame.n_tris : s >= frame.subtri_cap)
-    return; /* not a key of this batch */
+    return false; /* not a key of this batch */
+  bool ok = true;
+  if (frame.fused) {
+    uint4 rec = frame.tri_rec[s >> 4];
+    ok = rec.w < frame.ndraws &&
+         ((s & 15u) == FUSED_WHOLE
+              ? (rec.x < frame.n_verts && rec.y < frame.n_verts && rec.z < frame.n_verts)
+              : (rec.x < frame.clip_cap && (s & 15u) < MOPH_MAX_POLY - 2));
+  }
+  if (!ok)
+    return false;
   SubTri st;
   load_subtri(frame, s, st);
+  uint32_t na = frame.n_verts + frame.clip_cap * MOPH_MAX_POLY;
+  if (st.draw >= frame.ndraws || st.a[0] >= na || st.a[1] >= na || st.a[2] >= na)
+    return false;
   RasterTri r;
-  if (!load_raster_tri_fast(fa.w, fa.h, st, r)) /* inv_e unused by shading */
-    return;
+  if (!load_raster_tri_fast(fa.w, fa.h, st, r) || px < r.es.min_x || px > r.es.max_x ||
+      py < r.es.min_y || py > r.es.max_y) /* inv_e unused by shading */
+    return false;
   const DrawGPU &d = frame.draws[st.draw];
   int sm = draw_sm(d);
   bool flat = sm_flat(sm);
   float w0, w1, w2, zf;
-  if (bbox_area(r.es) > (long long)frame.small_area) {
+  if (bbox_area(r.es) > (long long)frame.small_area && frame.fused) {
+    /* big screen triangle: walk state at the last checkpoint column of the
+     * row, then the remaining serial steps */
+    uint32_t t = s >> 4, k = s & 15u;
+    uint32_t off = frame.big_off[k == FUSED_WHOLE ? t
+                                                  : frame.n_tris + frame.tri_rec[t].x * 16u + k];
+    if (off == ~0u) /* checkpoint store overflowed: the frame is run again */
+      return false;
+    uint32_t dx = (uint32_t)(px - r.es.min_x);
+    size_t ci = off + (size_t)(py - r.es.min_y) * chk_per_row(r.es) + (dx >> CHK_SHIFT);
+    if (ci >= frame.chk_cap)
+      return false;
+    float4 c = frame.chk[ci];
+    w0 = c.x;
+    w1 = c.y;
+    w2 = c.z;
+    zf = c.w;
+    FlatZ fz = {0.0f, 0.0f, 0.0f};
+    if (flat)
+      fz = flat_z_setup(r);
+    for (uint32_t j = dx & ~(CHK_SPAN - 1u); j < dx; j++) {
+      w0 += r.es.e_dx[0];
+      w1 += r.es.e_dx[1];
+      w2 += r.es.e_dx[2];
+      zf += fz.dx;
+    }
+  } else if (bbox_area(r.es) > (long long)frame.small_area) {
     float4 w = fa.wbuf[idx];
     w0 = w.x;
     w1 = w.y;
@@ -1730,16 +1900,47 @@ __global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
       fz = flat_z_setup(r);
     replay_walk(r, flat, fz, px, py, w0, w1, w2, zf);
   }
-  float z, pr, pg, pb;
+  float pr, pg, pb;
   if (flat) {
     z = zf;
     flat_color(frame, fa, st, d, sm, pr, pg, pb);
   } else {
     shade_fragment(frame, fa, st, r, w0, w1, w2, z, pr, pg, pb);
   }
-  reinterpret_cast<float4 *>(fa.hdr)[idx] = make_float4(pr, pg, pb, 1.0f);
-  fa.depth[idx] = z;
-  fa.ids[idx] = d.object_id;
+  out = make_float4(pr, pg, pb, 1.0f);
+  id = d.object_id;
+  return true;
+}
+
+/* One workgroup per shading tile of one view; tiles of all views of the
+ * batch are numbered consecutively.  Every pixel's visibility entry is read
+ * and, after the pixel is shaded, set back to the cleared key, so the buffer
+ * is clear for the next segment or frame without a clearing pass.  All
+ * stores come after all loads. */
+__global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
+  uint32_t vi = find_view_by_stile(frame.views, frame.nviews, blockIdx.x);
+  const ViewGPU &fa = frame.views[vi];
+  uint32_t T = frame.shade_tile;
+  uint32_t t = blockIdx.x - fa.stile_base;
+  uint32_t tiles_x = ((uint32_t)fa.w + T - 1) / T;
+  int px = (int)((t % tiles_x) * T + threadIdx.x % T);
+  int py = (int)((t / tiles_x) * T + threadIdx.x / T);
+  if (px >= fa.w || py >= fa.h)
+    return;
+  size_t idx = (size_t)py * (size_t)fa.w + (size_t)px;
+  uint32_t s1 = (uint32_t)fa.vis[idx];
+  float4 c;
+  float z;
+  uint32_t id;
+  bool hit = s1 != 0 && shade_px(frame, fa, s1 - 1, px, py, idx, c, z, id);
+  fa.vis[idx] = (unsigned long long)depth_key(1.0f) << 32;
+  if (hit) {
+    reinterpret_cast<float4 *>(fa.hdr)[idx] = c;
+    fa.depth[idx] = z;
+    fa.ids[idx] = id;
+  } else {
+    shade_clear_px(fa, idx); /* uncovered */
+  }
 }
 
 /* End of a batch: add the per-workgroup screen-triangle counts (order-free
@@ -1960,8 +2161,9 @@ void launch_post_lut(uint8_t *lut, hipStream_t s) {
 __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t nviews) {
   __shared__ uint32_t col[18][18];
   __shared__ float lum[18][18];
-  const PostViewGPU &v = views[find_post_view(views, nviews, blockIdx.x)];
-  uint32_t t = blockIdx.x - v.tile_base;
+  uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const PostViewGPU &v = views[find_post_view(views, nviews, tile)];
+  uint32_t t = tile - v.tile_base;
   uint32_t tiles_x = ((uint32_t)v.w + 15) / 16;
   int x0 = (int)(t % tiles_x) * 16, y0 = (int)(t / tiles_x) * 16;
   int x = x0 + (int)(threadIdx.x & 15);

@@ -138,8 +138,10 @@ enum {
   CNT_BIG = 1,      /* big screen triangles queued */
   CNT_SUBTRI = 2,   /* total screen triangles (written by the scan) */
   CNT_OVERFLOW = 3, /* bit0 clip queue, bit1 subtri store, bit2 shadow big queue,
-                       bit3 clipped polygon longer than MOPH_MAX_POLY, bit4 tile bins */
+                       bit3 clipped polygon longer than MOPH_MAX_POLY, bit4 tile bins,
+                       bit5 big-triangle checkpoint store */
   CNT_SHADOW_BIG = 4,
+  CNT_CHK = 5,    /* order-free path: row checkpoint entries of big triangles */
   CNT_TICKET = 7, /* k_finish: workgroups done */
   CNT_N = 8
 };
@@ -169,6 +171,10 @@ struct ViewGPU {
   const float *lut;
   int32_t irr_w, irr_h, pf_w, pf_h, pf_levels, lut_size;
   float ibl_rot, ibl_int;
+  /* 1: this segment opens a cleared frame, and the shading pass writes the
+   * clear values (colour, depth 1, id 0) to the pixels no triangle covers */
+  int32_t shade_clear;
+  float clear[4];
 };
 
 /* Everything the frame kernels read, passed by value. */
@@ -192,6 +198,15 @@ struct FrameArgs {
   uint4 *tri_rec; /* order-free path, per triangle: (v0, v1, v2, draw) when drawn
                      whole, (clip slot, 0, 0, draw) when queued for clipping */
   uint32_t *blk_counts; /* order-free path: screen triangles per k_tri workgroup */
+  /* order-free path, big screen triangles: the walk state (w0, w1, w2, flat z)
+   * at every 16th pixel of every bbox row, so shading replays at most 15
+   * steps.  big_chk[q] is the first entry of big-queue entry q, big_off the
+   * same indexed by triangle (whole) or n_tris + clip slot * 16 + fan index
+   * (clipped); ~0 when the store overflowed. */
+  float4 *chk;
+  uint32_t chk_cap;
+  uint32_t *big_chk;
+  uint32_t *big_off;
   float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */
   float4 *cpos;   /* clip_cap * MOPH_MAX_POLY */
   float4 *cattr;  /* clip_cap * MOPH_MAX_POLY * 3 */

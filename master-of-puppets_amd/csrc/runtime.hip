@@ -132,6 +132,7 @@ struct MopHipFramebuffer {
   bool v_color = false, v_depth = false, v_ids = false, v_hdr = false;
   /* recorded frame */
   bool has_clear = false;
+  bool shade_clear = false; /* the clear is left to the first segment's shading */
   float clear[4] = {0, 0, 0, 0};
   std::vector<PendingDraw> draws;
   std::vector<MopHipLight> light_pool; /* the recorded draws' lights */
@@ -260,6 +261,9 @@ struct MopHipDevice {
   DevArr<float4> vpos, vscr, vattr, cpos, cattr;
   DevArr<uint4> tri_rec;
   DevArr<uint32_t> blk_counts;
+  DevArr<float4> chk;          /* big-triangle row checkpoints */
+  DevArr<uint32_t> big_chk, big_off;
+  uint32_t chk_cap_hint = 0;
   DevArr<uint8_t> arena; /* per-batch upload: counters, draws, lights, views */
   DevArr<float> skin_bones, morph_targets, morph_weights;
   DevArr<uint32_t> cnt_poly, tri_counts, clip_queue, clip_slot, counters, block_sums,
@@ -373,6 +377,9 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->vscr.release();
   d->tri_rec.release();
   d->blk_counts.release();
+  d->chk.release();
+  d->big_chk.release();
+  d->big_off.release();
   d->arena.release();
   d->skin_bones.release();
   d->morph_targets.release();
@@ -1102,6 +1109,10 @@ static void fill_view(MopHipFramebuffer *fb, const ShadeState &st, int seg, View
   v.lut_size = st.lut_size;
   v.ibl_rot = st.ibl_rot;
   v.ibl_int = st.ibl_int;
+  if (seg == 0 && fb->shade_clear) {
+    v.shade_clear = 1;
+    memcpy(v.clear, fb->clear, sizeof(v.clear));
+  }
 }
 
 struct SegJob {
@@ -1200,12 +1211,19 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
    * fits the 32-bit sequence half of the visibility key */
   fa.fused = (!ordered && n_tris < (1u << 27)) ? 1u : 0u;
   if (fa.fused) {
+    uint32_t chk_cap = d->chk_cap_hint > (1u << 20) ? d->chk_cap_hint : (1u << 20);
     if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
-        !d->blk_counts.ensure(n_tris / 256 + 1 + CLIP_RASTER_GRID))
+        !d->blk_counts.ensure(n_tris / 256 + 1 + CLIP_RASTER_GRID) ||
+        !d->chk.ensure(chk_cap) || !d->big_chk.ensure(subtri_cap) ||
+        !d->big_off.ensure((size_t)n_tris + (size_t)clip_cap * 16 + 1))
       return false;
     fa.vscr = d->vscr.p;
     fa.tri_rec = d->tri_rec.p;
     fa.blk_counts = d->blk_counts.p;
+    fa.chk = d->chk.p;
+    fa.chk_cap = chk_cap;
+    fa.big_chk = d->big_chk.p;
+    fa.big_off = d->big_off.p;
   }
   fa.vpos = d->vpos.p;
   fa.vattr = d->vattr.p;
@@ -1278,6 +1296,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         launch_shade(fa, d->stream);
       }
     }
+  } else if (!ordered) {
+    launch_shade(fa, d->stream); /* clears and leaves the visibility buffers clear */
   }
   launch_export_counters(fa, d->stream);
   return true;
@@ -1489,32 +1509,6 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
 static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &todo) {
   for (const ShadowJob &j : d->shadow_jobs)
     run_shadow_job(d, j.map, j.casters);
-  /* the clears of every frame in one launch */
-  std::vector<ClearViewGPU> cv;
-  uint32_t cpx = 0;
-  for (MopHipFramebuffer *fb : todo)
-    if (fb->has_clear) {
-      ClearViewGPU c;
-      memset(&c, 0, sizeof(c));
-      c.px_base = cpx;
-      cpx += (uint32_t)(fb->w * fb->h);
-      memcpy(c.clear, fb->clear, sizeof(c.clear));
-      c.hdr = fb->hdr;
-      c.depth = fb->depth;
-      c.ids = fb->ids;
-      c.vis = fb->vis;
-      cv.push_back(c);
-    }
-  if (!cv.empty() && d->clear_views.ensure(cv.size())) {
-    int slot;
-    uint8_t *stg = stage_acquire(d, cv.size() * sizeof(ClearViewGPU), &slot);
-    if (stg) {
-      memcpy(stg, cv.data(), cv.size() * sizeof(ClearViewGPU));
-      upload(d, d->clear_views.p, stg, cv.size() * sizeof(ClearViewGPU));
-      stage_release(d, slot);
-      launch_clear_views(d->clear_views.p, (uint32_t)cv.size(), cpx, d->stream);
-    }
-  }
   std::vector<std::vector<std::pair<size_t, size_t>>> segs(todo.size());
   size_t rounds = 0;
   for (size_t i = 0; i < todo.size(); i++) {
@@ -1542,6 +1536,40 @@ static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &
       set_err("%zu draw(s) of a class this backend does not render yet were skipped",
               fb->skipped);
   }
+  /* the clears of every frame in one launch, except for frames whose first
+   * segment is order-free: that segment's shading pass visits every pixel,
+   * writes the clear values where nothing was drawn and leaves the
+   * visibility buffer cleared (it is cleared when allocated and after every
+   * order-free shading pass, so it is clear at the start of every frame) */
+  std::vector<ClearViewGPU> cv;
+  uint32_t cpx = 0;
+  for (size_t i = 0; i < todo.size(); i++) {
+    MopHipFramebuffer *fb = todo[i];
+    fb->shade_clear = fb->has_clear && !segs[i].empty() &&
+                      fb->draws[segs[i][0].first].cls == 0;
+    if (fb->has_clear && !fb->shade_clear) {
+      ClearViewGPU c;
+      memset(&c, 0, sizeof(c));
+      c.px_base = cpx;
+      cpx += (uint32_t)(fb->w * fb->h);
+      memcpy(c.clear, fb->clear, sizeof(c.clear));
+      c.hdr = fb->hdr;
+      c.depth = fb->depth;
+      c.ids = fb->ids;
+      c.vis = fb->vis;
+      cv.push_back(c);
+    }
+  }
+  if (!cv.empty() && d->clear_views.ensure(cv.size())) {
+    int slot;
+    uint8_t *stg = stage_acquire(d, cv.size() * sizeof(ClearViewGPU), &slot);
+    if (stg) {
+      memcpy(stg, cv.data(), cv.size() * sizeof(ClearViewGPU));
+      upload(d, d->clear_views.p, stg, cv.size() * sizeof(ClearViewGPU));
+      stage_release(d, slot);
+      launch_clear_views(d->clear_views.p, (uint32_t)cv.size(), cpx, d->stream);
+    }
+  }
   for (size_t r = 0; r < rounds; r++) {
     std::vector<SegJob> jobs[2];
     for (size_t i = 0; i < todo.size(); i++)
@@ -1566,7 +1594,7 @@ static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
     if (fb->checked)
       return;
     HIPCHK(hipEventSynchronize(fb->done));
-    uint32_t ovf = 0, need_clip = 0;
+    uint32_t ovf = 0, need_clip = 0, need_chk = 0;
     MopHipFrameStats st = {};
     int ns = fb->nseg < MOPH_MAX_SEGS ? fb->nseg : MOPH_MAX_SEGS;
     for (int k = 0; k < ns; k++) {
@@ -1574,16 +1602,23 @@ static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
       ovf |= c[CNT_OVERFLOW];
       if (c[CNT_CLIP] > need_clip)
         need_clip = c[CNT_CLIP];
+      if (c[CNT_CHK] > need_chk)
+        need_chk = c[CNT_CHK];
       st.subtris += c[CNT_SUBTRI];
       st.clipped += c[CNT_CLIP];
       st.big_subtris += c[CNT_BIG];
     }
     if (ovf & 8u)
       set_err("a clipped polygon exceeded %d vertices and was dropped", MOPH_MAX_POLY);
-    if (ovf & 19u) {
-      /* clip queue, screen-triangle store or tile bins too small: grow, re-run */
+    if (ovf & 51u) {
+      /* clip queue, screen-triangle store, tile bins or checkpoint store too
+       * small: grow, re-run */
       if (ovf & 3u)
         d->clip_cap_hint = need_clip + need_clip / 4 + 1024;
+      if (ovf & 32u) {
+        uint64_t c = (uint64_t)need_chk + need_chk / 4 + 65536;
+        d->chk_cap_hint = c > 0xffffff00ull ? 0xffffff00u : (uint32_t)c;
+      }
       if (ovf & 16u)
         d->bin_cap_hint = (uint32_t)(d->bins.cap * 2 < 0xffffff00ull ? d->bins.cap * 2
                                                                      : 0xffffff00ull);
