@@ -123,12 +123,12 @@ DEV void read_f2(const uint8_t *base, int off, int fmt, float o[2]) {
 }
 
 __global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
-  uint32_t v = blockIdx.x * WG + threadIdx.x;
-  if (v >= fa.n_verts)
+  /* draw-major grid: the draw is uniform over the workgroup (scalar loads) */
+  const DrawGPU &d = fa.draws[fa.wg_vtx[blockIdx.x]];
+  uint32_t i = (blockIdx.x - d.vtx_wg) * WG + threadIdx.x;
+  if (i >= d.vertex_count)
     return;
-  uint32_t di = find_draw_by_vtx_wave(fa.draws, fa.ndraws, v);
-  const DrawGPU &d = fa.draws[di];
-  uint32_t i = v - d.vtx_base;
+  uint32_t v = d.vtx_base + i;
   const uint8_t *vb = d.vbuf + (size_t)i * d.stride;
   float p[3], n[3], c[4], uv[2];
   bool has_n = true;
@@ -196,9 +196,9 @@ __global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
 }
 
 void launch_vertex(const FrameArgs &fa, hipStream_t s) {
-  if (!fa.n_verts)
+  if (!fa.n_wg_vtx)
     return;
-  launch(k_vertex, dim3((fa.n_verts + WG - 1) / WG), dim3(WG), 0, s, fa);
+  launch(k_vertex, dim3(fa.n_wg_vtx), dim3(WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -1009,9 +1009,9 @@ void launch_raster(const FrameArgs &fa, hipStream_t s) {
 
 /* Queue a big screen triangle or walk a small one; true if it reached the
  * raster (a non-empty clamped bbox and a non-zero area). */
-DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, bool flat,
-                      uint32_t e, uint32_t okey) {
-  RasterTri r;
+DEV bool fused_setup(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, uint32_t e,
+                     uint32_t okey, RasterTri &r, bool &walk) {
+  walk = false;
   if (!edge_setup(st.sx, st.sy, vw.w, vw.h, r.es))
     return false;
   if (bbox_area(r.es) > (long long)fa.small_area) {
@@ -1037,8 +1037,18 @@ DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, 
   r.sz[0] = st.sz[0];
   r.sz[1] = st.sz[1];
   r.sz[2] = st.sz[2];
-  walk_small_vis(vw, r, flat, (unsigned long long)e + 1ull);
+  walk = true;
   return true;
+}
+
+DEV bool fused_raster(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, bool flat,
+                      uint32_t e, uint32_t okey) {
+  RasterTri r;
+  bool walk;
+  bool emitted = fused_setup(fa, vw, st, e, okey, r, walk);
+  if (walk)
+    walk_small_vis(vw, r, flat, (unsigned long long)e + 1ull);
+  return emitted;
 }
 
 /* Screen triangles that reached the raster, counted per workgroup into
@@ -1063,16 +1073,23 @@ DEV void count_subtris(uint32_t *blk_counts, bool emitted) {
  * (2492-2504), then for an accepted triangle the fan checks (2515-2537) and
  * the visibility walk or a big-triangle queue entry. */
 __global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
-  uint32_t t = blockIdx.x * WG + threadIdx.x;
-  bool emitted = false;
-  if (t < fa.n_tris) {
-    uint32_t di = find_draw_by_tri_wave(fa.draws, fa.ndraws, t);
-    const DrawGPU &d = fa.draws[di];
-    uint32_t k = t - d.tri_base;
+  /* draw-major grid: the draw (and so the view) is uniform over the
+   * workgroup (scalar loads) */
+  uint32_t di = fa.wg_tri[blockIdx.x];
+  const DrawGPU &d = fa.draws[di];
+  const ViewGPU &vw = fa.views[d.view];
+  uint32_t k = (blockIdx.x - d.tri_wg) * WG + threadIdx.x;
+  uint32_t t = d.tri_base + k;
+  bool emitted = false, walk = false;
+  RasterTri r;
+  if (k < d.tri_count) {
     uint32_t i0 = d.ibuf[3 * k], i1 = d.ibuf[3 * k + 1], i2 = d.ibuf[3 * k + 2];
     if (i0 < d.vertex_count && i1 < d.vertex_count && i2 < d.vertex_count) {
       uint32_t v0 = d.vtx_base + i0, v1 = d.vtx_base + i1, v2 = d.vtx_base + i2;
       float4 a = fa.vpos[v0], b = fa.vpos[v1], c = fa.vpos[v2];
+      /* the screen positions are read with the clip positions (one memory
+       * round trip, not two); only accepted triangles use them */
+      float4 sc[3] = {fa.vscr[v0], fa.vscr[v1], fa.vscr[v2]};
       bool cull = (d.flags & MOPH_DRAW_CULL) != 0;
       bool alive = !trivially_rejected(a, b, c);
       if (alive && cull && a.w > 0.0f && b.w > 0.0f && c.w > 0.0f) {
@@ -1088,22 +1105,18 @@ __global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
             inside_all(c)) {
           if (!(fabsf(a.w) < 1e-7f || fabsf(b.w) < 1e-7f || fabsf(c.w) < 1e-7f)) {
             SubTri st;
-            uint32_t v[3] = {v0, v1, v2};
 #pragma unroll
             for (int i = 0; i < 3; i++) {
-              float4 sc = fa.vscr[v[i]];
-              st.sx[i] = sc.x;
-              st.sy[i] = sc.y;
-              st.sz[i] = sc.z;
-              st.iw[i] = sc.w;
+              st.sx[i] = sc[i].x;
+              st.sy[i] = sc[i].y;
+              st.sz[i] = sc[i].z;
+              st.iw[i] = sc[i].w;
             }
             float signed_area = (st.sx[1] - st.sx[0]) * (st.sy[2] - st.sy[0]) -
                                 (st.sx[2] - st.sx[0]) * (st.sy[1] - st.sy[0]);
             if (!(cull && signed_area >= 0.0f)) {
-              const ViewGPU &vw = fa.views[d.view];
               fa.tri_rec[t] = make_uint4(v0, v1, v2, di);
-              emitted = fused_raster(fa, vw, st, sm_flat(draw_sm(d)), (t << 4) | FUSED_WHOLE,
-                                     t);
+              emitted = fused_setup(fa, vw, st, (t << 4) | FUSED_WHOLE, t, r, walk);
             }
           }
         } else {
@@ -1118,6 +1131,8 @@ __global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
       }
     }
   }
+  if (walk)
+    walk_small_vis(vw, r, sm_flat(draw_sm(d)), ((unsigned long long)t << 4 | FUSED_WHOLE) + 1ull);
   count_subtris(fa.blk_counts, emitted);
 }
 
@@ -1164,7 +1179,7 @@ __global__ __launch_bounds__(WG) void k_clip_raster(FrameArgs fa, uint32_t *blk_
 void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris)
     return;
-  uint32_t nb = (fa.n_tris + WG - 1) / WG;
+  uint32_t nb = fa.n_wg_tri;
   launch(k_tri, dim3(nb), dim3(WG), 0, s, fa);
   launch_clip(fa, s);
   /* blk_counts holds nb + CLIP_RASTER_GRID entries */
@@ -1981,7 +1996,7 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint
 void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
   if (!fa.nviews)
     return;
-  uint32_t nblk = fa.fused ? (fa.n_tris + WG - 1) / WG + CLIP_RASTER_GRID : 0u;
+  uint32_t nblk = fa.fused ? fa.n_wg_tri + CLIP_RASTER_GRID : 0u;
   uint32_t grid = (nblk + 4095) / 4096;
   if (grid < 1)
     grid = 1;

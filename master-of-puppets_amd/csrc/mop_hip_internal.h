@@ -122,6 +122,8 @@ struct DrawGPU {
   uint32_t view; /* index into the batch's views */
   float opacity;
   float light_dir[3]; /* the call's light_dir (unlit models) */
+  uint32_t vtx_wg;    /* first workgroup of this draw in the per-vertex grid */
+  uint32_t tri_wg;    /* first workgroup of this draw in the per-triangle grid */
 };
 
 /* Screen-space triangle: exactly the MopSwScreenVertex positions the CPU
@@ -183,6 +185,12 @@ struct FrameArgs {
   const ViewGPU *views;
   uint32_t ndraws;
   uint32_t nviews;
+  /* draw-major grids of the per-vertex and per-triangle kernels: workgroup b
+   * works on 256 items of the single draw wg_vtx[b] / wg_tri[b] */
+  const uint32_t *wg_vtx;
+  const uint32_t *wg_tri;
+  uint32_t n_wg_vtx;
+  uint32_t n_wg_tri;
   uint32_t n_verts;
   uint32_t n_tris;
   uint32_t clip_cap;

@@ -1127,6 +1127,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
   std::vector<ViewGPU> views;
+  std::vector<uint32_t> wg_vtx, wg_tri;
   uint32_t n_verts = 0, n_tris = 0, tiles = 0, stiles = 0;
   int T = (int)d->shade_tile;
   for (size_t j = 0; j < jobs.size(); j++) {
@@ -1162,6 +1163,10 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         set_err("batch exceeds 2^31 triangles; remaining draws skipped");
         break;
       }
+      g.vtx_wg = (uint32_t)wg_vtx.size();
+      g.tri_wg = (uint32_t)wg_tri.size();
+      wg_vtx.insert(wg_vtx.end(), (g.vertex_count + 255) / 256, (uint32_t)gd.size());
+      wg_tri.insert(wg_tri.end(), (g.tri_count + 255) / 256, (uint32_t)gd.size());
       n_verts += g.vertex_count;
       n_tris += g.tri_count;
       gd.push_back(g);
@@ -1177,7 +1182,9 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   size_t oc = 0, od = al(CNT_N * sizeof(uint32_t));
   size_t bd = gd.size() * sizeof(DrawGPU), bl = lights.size() * sizeof(MopHipLight),
          bv = views.size() * sizeof(ViewGPU);
-  size_t ol = od + al(bd), ov = ol + al(bl), total = ov + bv;
+  size_t bwv = wg_vtx.size() * sizeof(uint32_t), bwt = wg_tri.size() * sizeof(uint32_t);
+  size_t ol = od + al(bd), ov = ol + al(bl), owv = ov + al(bv), owt = owv + al(bwv),
+         total = owt + bwt;
   if (!d->arena.ensure(total))
     return false;
   int slot;
@@ -1189,6 +1196,10 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (bl)
     memcpy(stg + ol, lights.data(), bl);
   memcpy(stg + ov, views.data(), bv);
+  if (bwv)
+    memcpy(stg + owv, wg_vtx.data(), bwv);
+  if (bwt)
+    memcpy(stg + owt, wg_tri.data(), bwt);
   upload(d, d->arena.p, stg, total);
   stage_release(d, slot);
 
@@ -1198,6 +1209,10 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.views = (const ViewGPU *)(d->arena.p + ov);
   fa.ndraws = (uint32_t)gd.size();
   fa.nviews = (uint32_t)views.size();
+  fa.wg_vtx = (const uint32_t *)(d->arena.p + owv);
+  fa.wg_tri = (const uint32_t *)(d->arena.p + owt);
+  fa.n_wg_vtx = (uint32_t)wg_vtx.size();
+  fa.n_wg_tri = (uint32_t)wg_tri.size();
   fa.n_verts = n_verts;
   fa.n_tris = n_tris;
   fa.clip_cap = clip_cap;
@@ -1213,7 +1228,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (fa.fused) {
     uint32_t chk_cap = d->chk_cap_hint > (1u << 20) ? d->chk_cap_hint : (1u << 20);
     if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
-        !d->blk_counts.ensure(n_tris / 256 + 1 + CLIP_RASTER_GRID) ||
+        !d->blk_counts.ensure(wg_tri.size() + 1 + CLIP_RASTER_GRID) ||
         !d->chk.ensure(chk_cap) || !d->big_chk.ensure(subtri_cap) ||
         !d->big_off.ensure((size_t)n_tris + (size_t)clip_cap * 16 + 1))
       return false;
