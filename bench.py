@@ -255,8 +255,22 @@ def batched_form(mod, rounds, tune=True):
     return out
 
 
-def serving_form(mod, rates=(100.0, 1000.0, 5000.0), duration=2.0, n_scenes=16,
-                 plain_rate=1000.0):
+def profiled_traffic(summary, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 counter
+    summary under profiles/ (its `traffic_bytes_per_launch` line), or None."""
+    path = os.path.join(REPO, "profiles", summary)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
+
+
+def serving_form(mod, rates=(100.0, 1000.0, 2500.0, 5000.0), duration=2.0, n_scenes=16,
+                 plain_rate=5000.0):
     """Serving form: 16 resident raster-stress scenes (2048x2048-quad
     displaced terrain, 8.4M triangles, plus 64 icospheres; one shadowed sun,
     4096^2 shadow map rendered once and kept), Poisson request streams at
@@ -302,8 +316,8 @@ def serving_form(mod, rates=(100.0, 1000.0, 5000.0), duration=2.0, n_scenes=16,
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-cores", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -321,11 +335,20 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the frame path shards by viewport/request across GPUs (see DESIGN.md) "
+              "the frame path shards by viewport/request across GPUs, "
               "but no multi-GPU run is part of it.", file=sys.stderr)
         sys.exit(2)
 
     import torch
+    if args.calib_copy_mib:
+        n = args.calib_copy_mib << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {n} bytes done")
+        del a, b
     if args.batch_only:
         mod = load_pkg()
         torch.cuda.init()
@@ -337,15 +360,6 @@ def main():
         torch.cuda.init()
         print(json.dumps({"serving": serving_form(mod)}))
         return
-    if args.calib_copy_mib:
-        n = args.calib_copy_mib << 20
-        a = torch.ones(n, dtype=torch.uint8, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"calibration copy of {n} bytes done")
-        del a, b
     mod = load_pkg()
     from mop_amd import scenes, abi
     lib = mod.lib()
@@ -382,6 +396,24 @@ def main():
         f"(host record + launch {host_ms:.3f} ms/frame)")
     fps = 1e3 / ms_per_step
     mtri = tris / args.steps / (ms_per_step * 1e3)
+
+    import numpy as np
+    # host cost of one frame with the device idle (no waits on earlier
+    # frames) and the device span of one frame (HIP events around the frame's
+    # launches on the backend's stream)
+    idle_host, dev_span = [], []
+    gpu_time = lib.mop_hip_frame_gpu_time_ms
+    gpu_time.restype = C.c_float
+    for _ in range(10):
+        torch.cuda.synchronize()
+        th = time.perf_counter()
+        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
+        idle_host.append(time.perf_counter() - th)
+        torch.cuda.synchronize()
+        dev_span.append(gpu_time(dev))
+    idle_host_ms = float(np.median(idle_host)) * 1e3
+    dev_span_ms = float(np.median(dev_span))
+    log(f"one frame alone: host {idle_host_ms:.3f} ms, device span {dev_span_ms:.3f} ms")
 
     rep, cap = C.c_uint64(), C.c_uint64()
     lib.mop_hip_graph_stats(dev, C.byref(rep), C.byref(cap))
@@ -431,17 +463,21 @@ def main():
     fps_rb = n_rb / (time.perf_counter() - t0)
     log(f"readback: {fps_rb:.2f} frames/s")
 
-    # roofline of the dominant stage: shading, per pixel reads the visibility
-    # word (8 B) and writes HDR RGBA32F + depth + ID (24 B); algorithmic bytes
-    # = 32 B x pixels.  Its ALU side is reported as VALU lane-op throughput.
+    # roofline of the dominant kernel, the shading pass: per pixel it reads the
+    # visibility word (8 B), sets it back to the cleared key (8 B) and writes
+    # HDR RGBA32F + depth + object ID (24 B): algorithmic bytes = 40 B x
+    # pixels.  `traffic` is the HBM bytes per launch rocprofv3 measured for
+    # it (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy in the same run)
+    # in the committed counter summary of this workload.
     shade_ms = stages.get("shade")
     roof = None
     if shade_ms:
         npx = W * H
-        algo_bytes = 32 * npx
+        algo_bytes = 40 * npx
         achieved = algo_bytes / (shade_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": profiled_traffic("frame_1080p_pbr_counters.md", "k_shade"),
                 "kernel": "k_shade", "launch_ms": shade_ms,
                 "algorithmic_bytes_per_launch": algo_bytes}
 
@@ -475,6 +511,8 @@ def main():
         "mtri_per_s": round(mtri, 3),
         "fps_with_readback": round(fps_rb, 3),
         "host_ms_per_frame": round(host_ms, 4),
+        "host_ms_one_frame_idle_device": round(idle_host_ms, 4),
+        "device_ms_one_frame": round(dev_span_ms, 4),
         "graph_replays": rep.value,
         "graph_captures": cap.value,
         "stage_ms": stages,

@@ -267,13 +267,17 @@ int mop_hip_morph_apply(MopHipDevice *dev, MopHipBuffer *vb, const void *bind_po
                         uint32_t vertex_count, const MopHipVertexFormat *fmt,
                         const float *targets, const float *weights, uint32_t target_count);
 
-/* Frame plans replayed from captured hipGraphs (default on): each frame
- * execution is planned on the host; a plan identical in every launch and
- * buffer address to the previous one is captured once (two graph instances
- * with their own pinned upload buffers) and replayed afterwards, with only
- * the uploaded bytes (draws, lights, views) refreshed.  Stage timers force
+/* Frame plans replayed from captured hipGraphs.  mode 1: always, 0: never,
+ * 2 (default): for executions of two or more frames (mop_hip_frames_execute
+ * batches); a lone frame is device-bound with few launches and runs faster
+ * launched directly.  Each execution is planned on the host; a plan
+ * identical in every launch and buffer address to the previous one is
+ * captured once (two instances with their own pinned upload buffers; one
+ * graph per run of same-stream launches, the shadow passes' side stream
+ * forked and joined between them) and replayed afterwards, with only the
+ * uploaded bytes (draws, lights, views) refreshed.  Stage timers force
  * direct launches.  graph_stats reports replays and captures so far. */
-void mop_hip_set_graphs(MopHipDevice *dev, int on);
+void mop_hip_set_graphs(MopHipDevice *dev, int mode);
 void mop_hip_graph_stats(MopHipDevice *dev, uint64_t *replays, uint64_t *captures);
 
 /* Launch-shape tunables (results never depend on them): bbox area up to

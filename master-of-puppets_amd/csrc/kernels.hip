@@ -380,7 +380,7 @@ __global__ __launch_bounds__(CLIP_WG) void k_clip(FrameArgs fa) {
     nq = fa.clip_cap;
   for (uint32_t q = blockIdx.x * CLIP_WG + threadIdx.x; q < nq; q += gridDim.x * CLIP_WG) {
     uint32_t t = fa.clip_queue[q];
-    uint32_t di = find_draw_by_tri(fa.draws, fa.ndraws, t);
+    uint32_t di = fa.fused ? fa.tri_rec[t].w : find_draw_by_tri(fa.draws, fa.ndraws, t);
     const DrawGPU &d = fa.draws[di];
     uint32_t k = t - d.tri_base;
     CV *bufA = pool[threadIdx.x][0], *bufB = pool[threadIdx.x][1];
@@ -1967,8 +1967,17 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint
   __shared__ bool last;
   uint32_t lo = blockIdx.x * chunk, hi = lo + chunk < nblk ? lo + chunk : nblk;
   uint32_t c = 0;
-  for (uint32_t i = lo + threadIdx.x; i < hi; i += WG)
-    c += fa.blk_counts[i];
+  for (uint32_t i0 = lo + threadIdx.x; i0 < hi; i0 += 8 * WG) {
+    uint32_t v[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) { /* eight loads in flight per thread */
+      uint32_t i = i0 + (uint32_t)j * WG;
+      v[j] = i < hi ? fa.blk_counts[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+      c += v[j];
+  }
   part[threadIdx.x] = c;
   __syncthreads();
   for (int s = WG / 2; s > 0; s >>= 1) {
@@ -2565,6 +2574,27 @@ __global__ void k_copy_u8(size_t n, const uint8_t *src, uint8_t *dst) {
     for (; i < n; i++)
       dst[i] = src[i];
   }
+}
+
+/* Host-to-device upload of a frame's small tables (draws, lights, views,
+ * workgroup maps) from pinned host memory: many workgroups reading 16 bytes
+ * per lane over PCIe.  Launched directly (also while a plan is issued or
+ * captured), never recorded; n is rounded up to 16 bytes, which both the
+ * pinned sources and the device tables have room for. */
+__global__ __launch_bounds__(WG) void k_upload(const uint4 *src, uint4 *dst, uint32_t n16) {
+  for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n16; i += gridDim.x * WG)
+    dst[i] = src[i];
+}
+
+void launch_upload(const void *src, void *dst, size_t n, hipStream_t s) {
+  uint32_t n16 = (uint32_t)((n + 15) / 16);
+  if (!n16)
+    return;
+  uint32_t grid = (n16 + WG - 1) / WG;
+  if (grid > 1024)
+    grid = 1024;
+  hipLaunchKernelGGL(k_upload, dim3(grid), dim3(WG), 0, s, (const uint4 *)src, (uint4 *)dst,
+                     n16);
 }
 
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s) {

@@ -44,8 +44,11 @@
  * buffer owned by that graph, refilled before each replay). */
 struct Plan {
   std::vector<uint8_t> sig;
-  std::vector<std::function<void(hipStream_t, const uint8_t *)>> ops; /* arg: upload source */
-  std::vector<uint8_t> blob; /* host bytes of the plan's uploads */
+  /* each op issues on the stream it was recorded for; arg: upload source */
+  std::vector<std::function<void(const uint8_t *)>> ops;
+  std::vector<hipStream_t> ostream; /* the stream of each op */
+  std::vector<uint8_t> marker;      /* 1: a cross-stream event dependency */
+  std::vector<uint8_t> blob;        /* host bytes of the plan's uploads */
 };
 extern thread_local Plan *t_plan;
 
@@ -59,12 +62,15 @@ template <typename... P, typename... A>
 inline void launch(void (*k)(P...), dim3 g, dim3 b, uint32_t shmem, hipStream_t s, A... a) {
   if (Plan *pl = t_plan) {
     sig_put(pl->sig, (uintptr_t)k);
+    sig_put(pl->sig, (uintptr_t)s);
     sig_put(pl->sig, g);
     sig_put(pl->sig, b);
     (sig_put(pl->sig, (P)a), ...);
-    pl->ops.push_back([=](hipStream_t st, const uint8_t *) {
-      hipLaunchKernelGGL(k, g, b, shmem, st, (P)a...);
+    pl->ops.push_back([=](const uint8_t *) {
+      hipLaunchKernelGGL(k, g, b, shmem, s, (P)a...);
     });
+    pl->ostream.push_back(s);
+    pl->marker.push_back(0);
   } else {
     hipLaunchKernelGGL(k, g, b, shmem, s, (P)a...);
   }
@@ -336,6 +342,7 @@ void launch_vis_from_depth(int npx, const float *depth,
 void launch_post_lut(uint8_t *lut, hipStream_t s);
 void launch_tail_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
                        hipStream_t s);
+void launch_upload(const void *src, void *dst, size_t n, hipStream_t s);
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s);

@@ -214,10 +214,18 @@ struct ShadowJob {
 /* A captured frame plan: two graph instances, each with its own pinned
  * upload buffer, used alternately so the host refills one while the device
  * may still read the other. */
+/* A captured plan: its runs of same-stream operations between cross-stream
+ * dependencies are captured as one graph each (graph branches would run in
+ * order on one queue); a replay launches each on its own stream and issues
+ * the dependencies (event record + wait) between them, so the side stream's
+ * shadow passes overlap the main stream's work. */
 struct GraphEntry {
   std::vector<uint8_t> sig;
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  std::vector<hipGraphExec_t> seg[2]; /* per upload buffer, per segment */
+  std::vector<hipStream_t> seg_stream;
+  hipGraphExec_t exec[2] = {nullptr, nullptr}; /* unused */
   uint8_t *blob[2] = {nullptr, nullptr};
+  uint8_t *dblob[2] = {nullptr, nullptr}; /* device aliases of blob */
   hipEvent_t done[2] = {nullptr, nullptr};
   size_t blob_size = 0;
   int next = 0;
@@ -242,6 +250,7 @@ struct StageTimers {
 #define NSTAGE 16
 struct StageSlot {
   void *p = nullptr;
+  uint8_t *dp = nullptr; /* device alias of p (the upload kernel reads it) */
   size_t cap = 0;
   hipEvent_t ev = nullptr;
   bool used = false;
@@ -249,6 +258,11 @@ struct StageSlot {
 
 struct MopHipDevice {
   hipStream_t stream = nullptr;
+  /* queued shadow passes run on `side`, forked from and joined back into
+   * `stream` before the first shading launch that reads the maps */
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool join_pending = false;
   StageTimers tm;
   StageSlot ring[NSTAGE];
   int ring_next = 0;
@@ -289,7 +303,7 @@ struct MopHipDevice {
   std::vector<ShadowDrawHost> shadow_draws;
   std::vector<ShadowJob> shadow_jobs; /* queued shadow passes */
   /* frame plans replayed from hipGraphs (mop_hip_set_graphs) */
-  bool graphs_on = true;
+  int graphs_mode = 2; /* 0 direct, 1 graphs, 2 graphs for multi-frame executions */
   std::vector<GraphEntry> graphs;
   std::vector<uint8_t> last_sig;
   uint64_t plan_clock = 0;
@@ -331,8 +345,16 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
     delete d;
     return nullptr;
   }
+  if (hipStreamCreateWithFlags(&d->side, hipStreamNonBlocking) != hipSuccess) {
+    set_err("hipStreamCreate failed");
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return nullptr;
+  }
   HIPCHK(hipEventCreate(&d->ev0));
   HIPCHK(hipEventCreate(&d->ev1));
+  HIPCHK(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&d->ev_join, hipEventDisableTiming));
   for (int k = 0; k < NSTAGE; k++)
     HIPCHK(hipEventCreateWithFlags(&d->ring[k].ev, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&d->h_counters, CNT_N * sizeof(uint32_t), 0));
@@ -406,6 +428,9 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
     (void)hipHostFree(d->h_counters);
   (void)hipEventDestroy(d->ev0);
   (void)hipEventDestroy(d->ev1);
+  (void)hipEventDestroy(d->ev_fork);
+  (void)hipEventDestroy(d->ev_join);
+  (void)hipStreamDestroy(d->side);
   (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -430,8 +455,12 @@ static uint8_t *stage_acquire(MopHipDevice *d, size_t bytes, int *slot) {
     if (sl.p)
       (void)hipHostFree(sl.p);
     size_t nc = bytes + bytes / 2 + 4096;
-    if (hipHostMalloc(&sl.p, nc, 0) != hipSuccess) {
+    if (hipHostMalloc(&sl.p, nc, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&sl.dp, sl.p, 0) != hipSuccess) {
+      if (sl.p)
+        (void)hipHostFree(sl.p);
       sl.p = nullptr;
+      sl.dp = nullptr;
       sl.cap = 0;
       set_err("hipHostMalloc(%zu) failed", nc);
       return nullptr;
@@ -454,14 +483,23 @@ static void stage_release(MopHipDevice *d, int k) {
 static void upload(MopHipDevice *d, void *dst, const uint8_t *stg, size_t n) {
   if (Plan *pl = t_plan) {
     size_t off = (size_t)(stg - pl->blob.data());
+    hipStream_t s = d->stream; /* the main or the side stream */
     sig_put(pl->sig, (uintptr_t)dst);
     sig_put(pl->sig, n);
-    pl->ops.push_back([=](hipStream_t st, const uint8_t *base) {
-      HIPCHK(hipMemcpyAsync(dst, base + off, n, hipMemcpyHostToDevice, st));
-    });
+    sig_put(pl->sig, (uintptr_t)s);
+    pl->ops.push_back([=](const uint8_t *base) { launch_upload(base + off, dst, n, s); });
+    pl->ostream.push_back(s);
+    pl->marker.push_back(0);
     return;
   }
-  HIPCHK(hipMemcpyAsync(dst, stg, n, hipMemcpyHostToDevice, d->stream));
+  for (int k = 0; k < NSTAGE; k++) {
+    const uint8_t *p = (const uint8_t *)d->ring[k].p;
+    if (p && stg >= p && stg + n <= p + d->ring[k].cap) {
+      launch_upload(d->ring[k].dp + (stg - p), dst, n, d->stream);
+      return;
+    }
+  }
+  set_err("upload source is not a staging slot");
 }
 
 static void tm_collect(MopHipDevice *d);
@@ -1123,6 +1161,8 @@ struct SegJob {
 
 /* One order-free segment of each job's framebuffer, all in one set of
  * launches (views are independent: own visibility buffer, own state). */
+static void join_shadows(MopHipDevice *d);
+
 static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ordered) {
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
@@ -1283,6 +1323,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         TmScope t(d, TM_RASTER);
         launch_raster_big(fa, d->stream);
       }
+      join_shadows(d); /* outside the stage timer: the wait is not shading */
       {
         TmScope t(d, TM_SHADE);
         launch_shade(fa, d->stream);
@@ -1298,6 +1339,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
       launch_emit(fa, d->stream);
     }
     if (ordered) {
+      join_shadows(d);
       TmScope t(d, TM_SHADE);
       launch_ordered(fa, d->tile_counts.p, d->tile_cursor.p, d->bins.p, bin_cap,
                      d->block_sums.p, d->stream);
@@ -1306,12 +1348,14 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         TmScope t(d, TM_RASTER);
         launch_raster(fa, d->stream);
       }
+      join_shadows(d); /* outside the stage timer: the wait is not shading */
       {
         TmScope t(d, TM_SHADE);
         launch_shade(fa, d->stream);
       }
     }
   } else if (!ordered) {
+    join_shadows(d);
     launch_shade(fa, d->stream); /* clears and leaves the visibility buffers clear */
   }
   launch_export_counters(fa, d->stream);
@@ -1378,6 +1422,10 @@ static void free_graph_entry(GraphEntry &e) {
   for (int k = 0; k < 2; k++) {
     if (e.done[k])
       (void)hipEventSynchronize(e.done[k]);
+    for (hipGraphExec_t x : e.seg[k])
+      if (x)
+        (void)hipGraphExecDestroy(x);
+    e.seg[k].clear();
     if (e.exec[k])
       (void)hipGraphExecDestroy(e.exec[k]);
     if (e.blob[k])
@@ -1386,55 +1434,70 @@ static void free_graph_entry(GraphEntry &e) {
       (void)hipEventDestroy(e.done[k]);
     e.exec[k] = nullptr;
     e.blob[k] = nullptr;
+    e.dblob[k] = nullptr;
     e.done[k] = nullptr;
   }
 }
 
 /* Issue a plan's operations on the stream, uploads from a staging slot. */
 static void issue_direct(MopHipDevice *d, const Plan &pl) {
-  const uint8_t *base = nullptr;
+  const uint8_t *base = nullptr; /* device alias of the staged blob */
   int slot = -1;
   if (!pl.blob.empty()) {
     uint8_t *stg = stage_acquire(d, pl.blob.size(), &slot);
     if (!stg)
       return;
     memcpy(stg, pl.blob.data(), pl.blob.size());
-    base = stg;
+    base = d->ring[slot].dp;
   }
   for (const auto &op : pl.ops)
-    op(d->stream, base);
+    op(base);
   stage_release(d, slot);
 }
 
-/* Capture the plan twice (one graph per upload buffer). */
+/* Capture the plan twice (one set of graphs per upload buffer). */
 static bool capture_entry(MopHipDevice *d, const Plan &pl, GraphEntry &e) {
   e.sig = pl.sig;
   e.blob_size = pl.blob.size();
+  e.seg_stream.clear();
+  for (size_t i = 0; i < pl.ops.size(); i++)
+    if (!pl.marker[i] && (i == 0 || pl.marker[i - 1] || pl.ostream[i] != pl.ostream[i - 1]))
+      e.seg_stream.push_back(pl.ostream[i]);
   for (int k = 0; k < 2; k++) {
-    if (hipHostMalloc((void **)&e.blob[k], e.blob_size ? e.blob_size : 16, 0) != hipSuccess ||
+    if (hipHostMalloc((void **)&e.blob[k], e.blob_size + 64, hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&e.dblob[k], e.blob[k], 0) != hipSuccess ||
         hipEventCreateWithFlags(&e.done[k], hipEventDisableTiming) != hipSuccess) {
       free_graph_entry(e);
       return false;
     }
     if (e.blob_size)
       memcpy(e.blob[k], pl.blob.data(), e.blob_size);
-    if (hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      free_graph_entry(e);
-      return false;
+    size_t i = 0;
+    while (i < pl.ops.size()) {
+      if (pl.marker[i]) {
+        i++;
+        continue;
+      }
+      hipStream_t st = pl.ostream[i];
+      if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        free_graph_entry(e);
+        return false;
+      }
+      for (; i < pl.ops.size() && !pl.marker[i] && pl.ostream[i] == st; i++)
+        pl.ops[i](e.dblob[k]);
+      hipGraph_t g = nullptr;
+      hipGraphExec_t x = nullptr;
+      hipError_t er = hipStreamEndCapture(st, &g);
+      if (er != hipSuccess || !g || hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        if (g)
+          (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        free_graph_entry(e);
+        return false;
+      }
+      (void)hipGraphDestroy(g);
+      e.seg[k].push_back(x);
     }
-    for (const auto &op : pl.ops)
-      op(d->stream, e.blob[k]);
-    hipGraph_t g = nullptr;
-    hipError_t er = hipStreamEndCapture(d->stream, &g);
-    if (er != hipSuccess || !g ||
-        hipGraphInstantiate(&e.exec[k], g, nullptr, nullptr, 0) != hipSuccess) {
-      if (g)
-        (void)hipGraphDestroy(g);
-      (void)hipGetLastError();
-      free_graph_entry(e);
-      return false;
-    }
-    (void)hipGraphDestroy(g);
   }
   return true;
 }
@@ -1446,7 +1509,15 @@ static void replay_entry(MopHipDevice *d, GraphEntry &e, const Plan &pl) {
   HIPCHK(hipEventSynchronize(e.done[k])); /* its previous replay has read the buffer */
   if (e.blob_size)
     memcpy(e.blob[k], pl.blob.data(), e.blob_size);
-  HIPCHK(hipGraphLaunch(e.exec[k], d->stream));
+  size_t j = 0;
+  for (size_t i = 0; i < pl.ops.size(); i++) {
+    if (pl.marker[i]) {
+      pl.ops[i](nullptr); /* event record on one stream, wait on the other */
+    } else if (i == 0 || pl.marker[i - 1] || pl.ostream[i] != pl.ostream[i - 1]) {
+      HIPCHK(hipGraphLaunch(e.seg[k][j], e.seg_stream[j]));
+      j++;
+    }
+  }
   HIPCHK(hipEventRecord(e.done[k], d->stream));
   d->graph_replays++;
 }
@@ -1493,7 +1564,11 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
     return;
   HIPCHK(hipEventRecord(d->ev0, d->stream));
   bool planned = false;
-  if (d->graphs_on && !d->tm.on) {
+  /* a lone frame is device-bound with few launches: direct launches run it
+   * faster than graph replays on this runtime; a batch of frames gains
+   * from the replay */
+  bool use_graphs = d->graphs_mode == 1 || (d->graphs_mode == 2 && todo.size() >= 2);
+  if (use_graphs && !d->tm.on) {
     Plan pl;
     t_plan = &pl;
     t_plan_abort = false;
@@ -1518,12 +1593,48 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
   d->timing_pending = true;
 }
 
+/* Fork the side stream from the main one (record, wait) or join it back;
+ * recorded into the active plan like any launch (a captured graph gets the
+ * two branches). */
+static void stream_dep(MopHipDevice *d, hipStream_t from, hipStream_t to, hipEvent_t ev) {
+  auto dep = [from, to, ev]() {
+    HIPCHK(hipEventRecord(ev, from));
+    HIPCHK(hipStreamWaitEvent(to, ev, 0));
+  };
+  if (Plan *pl = t_plan) {
+    sig_put(pl->sig, (uintptr_t)ev);
+    pl->ops.push_back([dep](const uint8_t *) { dep(); });
+    pl->ostream.push_back(to);
+    pl->marker.push_back(1);
+    return;
+  }
+  dep();
+}
+
+/* Before a launch that reads shadow maps: wait for the side stream's passes. */
+static void join_shadows(MopHipDevice *d) {
+  if (!d->join_pending)
+    return;
+  stream_dep(d, d->side, d->stream, d->ev_join);
+  d->join_pending = false;
+}
+
 /* Everything a frame execution launches, in order: queued shadow passes,
  * clears, the segments of every frame (order-free and ordered batches in
  * rounds), the tail.  Runs under a plan or directly. */
 static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &todo) {
-  for (const ShadowJob &j : d->shadow_jobs)
-    run_shadow_job(d, j.map, j.casters);
+  if (!d->shadow_jobs.empty()) {
+    /* the shadow passes overlap the vertex, triangle and raster kernels of
+     * the views: they run on the side stream (every launch, upload and
+     * staging event of run_shadow_job goes to d->stream, swapped here) */
+    stream_dep(d, d->stream, d->side, d->ev_fork);
+    hipStream_t main = d->stream;
+    d->stream = d->side;
+    for (const ShadowJob &j : d->shadow_jobs)
+      run_shadow_job(d, j.map, j.casters);
+    d->stream = main;
+    d->join_pending = true;
+  }
   std::vector<std::vector<std::pair<size_t, size_t>>> segs(todo.size());
   size_t rounds = 0;
   for (size_t i = 0; i < todo.size(); i++) {
@@ -1597,6 +1708,7 @@ static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &
     if (!jobs[1].empty())
       run_batch(d, jobs[1], true);
   }
+  join_shadows(d);
   run_tail(d, todo);
 }
 
@@ -1876,13 +1988,14 @@ extern "C" void mop_hip_shadow_map_destroy(MopHipDevice *d, MopHipShadowMap *m) 
   delete m;
 }
 
-/* Frame plans replayed from captured hipGraphs: on by default.  Off, every
- * frame's launches are issued one by one. */
-extern "C" void mop_hip_set_graphs(MopHipDevice *d, int on) {
+/* Frame plans replayed from captured hipGraphs: 1 always, 0 never (every
+ * launch issued one by one), 2 (default) for executions of two or more
+ * frames. */
+extern "C" void mop_hip_set_graphs(MopHipDevice *d, int mode) {
   if (!d)
     return;
-  d->graphs_on = on != 0;
-  if (!d->graphs_on) {
+  d->graphs_mode = (mode == 0 || mode == 1) ? mode : 2;
+  if (d->graphs_mode == 0) {
     for (GraphEntry &e : d->graphs)
       free_graph_entry(e);
     d->graphs.clear();
