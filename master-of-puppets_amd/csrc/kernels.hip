@@ -1931,8 +1931,10 @@ DEV bool shade_px(const FrameArgs &frame, const ViewGPU &fa, uint32_t s, int px,
  * batch are numbered consecutively.  Every pixel's visibility entry is read
  * and, after the pixel is shaded, set back to the cleared key, so the buffer
  * is clear for the next segment or frame without a clearing pass.  All
- * stores come after all loads. */
-__global__ __launch_bounds__(1024) void k_shade(FrameArgs frame) {
+ * stores come after all loads.  The kernel is bound by the latency of its
+ * chain of dependent gathers; five waves per SIMD (at most 96 VGPRs, a few
+ * spilled) measured faster than four (110 VGPRs) or six. */
+__global__ __launch_bounds__(1024, 5) void k_shade(FrameArgs frame) {
   uint32_t vi = find_view_by_stile(frame.views, frame.nviews, blockIdx.x);
   const ViewGPU &fa = frame.views[vi];
   uint32_t T = frame.shade_tile;
