@@ -293,6 +293,24 @@ int mop_hip_frame_done(MopHipDevice *dev, MopHipFramebuffer *fb);
 /* Wait until everything queued on the device has finished. */
 void mop_hip_device_sync(MopHipDevice *dev);
 
+/* Scratch overflow (no counterpart in the reference, whose scratch grows on
+ * the host).  Every batch ORs its overflow bits (1 clip queue, 2
+ * screen-triangle store, 8 clipped polygon over 12 vertices, 16 tile bins,
+ * 32 big-triangle checkpoints) into a per-device word; this returns it after
+ * waiting for the queued work, and clears it when reset != 0.  A frame that
+ * overflowed is run again with grown stores when it is read back; a caller
+ * that never reads a frame back uses this to know that every executed frame
+ * was complete. */
+uint32_t mop_hip_overflow_flags(MopHipDevice *dev, int reset);
+/* Forced capacities of the clip queue (triangles), the checkpoint store and
+ * the tile bins (entries) for the next executions (0 = none); dropped for a
+ * store once it overflows.  They exercise the overflow recovery. */
+int mop_hip_set_scratch_limits(MopHipDevice *dev, uint32_t clip, uint32_t chk, uint32_t bins);
+/* The same for every device created afterwards in this process. */
+void mop_hip_set_default_scratch_limits(uint32_t clip, uint32_t chk, uint32_t bins);
+/* Frames run again after a scratch overflow in this process. */
+uint64_t mop_hip_overflow_reruns(void);
+
 /* ------------------------------------------------------------------------
  * Device-resident shadow maps and IBL tables (no counterpart in the
  * reference, whose shadow/IBL state is process-global: rasterizer.c:141-183).

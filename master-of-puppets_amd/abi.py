@@ -166,6 +166,10 @@ SIGS = {
     "mop_hip_skin_apply": (C.c_int, [VP, VP, VP, C.c_uint32, VP, VP, C.c_uint32]),
     "mop_hip_morph_apply": (C.c_int, [VP, VP, VP, C.c_uint32, VP, VP, VP, C.c_uint32]),
     "mop_hip_device_sync": (None, [VP]),
+    "mop_hip_overflow_flags": (C.c_uint32, [VP, C.c_int]),
+    "mop_hip_set_scratch_limits": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "mop_hip_set_default_scratch_limits": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "mop_hip_overflow_reruns": (C.c_uint64, []),
 }
 
 

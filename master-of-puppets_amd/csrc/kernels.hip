@@ -2002,6 +2002,12 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint
     for (int k = 0; k < CNT_N; k++)
       o[k] = __hip_atomic_load(&fa.counters[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  if (threadIdx.x == 0 && fa.sticky) {
+    uint32_t ovf =
+        __hip_atomic_load(&fa.counters[CNT_OVERFLOW], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t acc = atomicOr(fa.sticky, ovf) | ovf;
+    *fa.sticky_out = acc;
+  }
 }
 
 void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
@@ -2681,6 +2687,8 @@ __global__ __launch_bounds__(WG) void k_bin_fill(FrameArgs fa, const uint32_t *t
   uint32_t total = fa.counters[CNT_SUBTRI];
   if (total > fa.subtri_cap)
     total = fa.subtri_cap;
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    fa.counters[CNT_BINS] = tile_start[fa.n_tiles]; /* the capacity this batch needs */
   for (uint32_t s = blockIdx.x * WG + threadIdx.x; s < total; s += gridDim.x * WG) {
     SubTri st = fa.subtris[s];
     const ViewGPU &vw = fa.views[fa.draws[st.draw].view];
@@ -2733,7 +2741,6 @@ struct OrderedLds {
   float saved[OT * OT];
   uint32_t ids[OT * OT];
   uint32_t list[OT_CAP];
-  uint32_t next_min;
 };
 
 /* One screen triangle against this lane's pixel, the way one iteration of
@@ -2937,9 +2944,68 @@ DEV void wire_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &
             li, L);
 }
 
+/* One screen triangle of the tile's submission-ordered list against this
+ * lane's pixel: draw changes (depth_write = false saves and restores the
+ * tile's depth), the wave-level bbox skip, then the fragment. */
+struct OrderedState {
+  uint32_t cur_draw;
+  bool saved_live;
+};
+
+DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int px, int py,
+                      bool inside, int li, OrderedLds &L, OrderedState &os) {
+  uint32_t draw = frame.subtris[s].draw;
+  if (draw != os.cur_draw) {
+    /* end of a draw with depth_write = false: restore its depth */
+    if (os.saved_live && inside)
+      L.depth[li] = L.saved[li];
+    os.saved_live = false;
+    os.cur_draw = draw;
+    if (!(frame.draws[draw].flags & MOPH_DRAW_DEPTH_WRITE)) {
+      if (inside)
+        L.saved[li] = L.depth[li];
+      os.saved_live = true;
+    }
+  }
+  SubTri st = frame.subtris[s];
+  if (frame.draws[draw].flags & MOPH_DRAW_WIRE) {
+    int bx0, by0, bx1, by1;
+    bool ok = wire_bbox(st, vw.w, vw.h, bx0, by0, bx1, by1);
+    bool in_bbox = ok && inside && px >= bx0 && px <= bx1 && py >= by0 && py <= by1;
+    if (__ballot(in_bbox) == 0ull)
+      return;
+    if (in_bbox)
+      wire_fragment(frame, vw, st, px, py, li, L);
+    return;
+  }
+  RasterTri r;
+  bool ok = load_raster_tri(vw.w, vw.h, st, r);
+  bool in_bbox = ok && inside && px >= r.es.min_x && px <= r.es.max_x && py >= r.es.min_y &&
+                 py <= r.es.max_y;
+  /* coverage mask of the wave: skip the triangle when no lane is in its bbox */
+  if (__ballot(in_bbox) == 0ull)
+    return;
+  if (in_bbox)
+    ordered_fragment(frame, vw, st, r, px, py, li, L);
+}
+
+/* One workgroup per 16x16 tile.  The tile's bin holds the indices of the
+ * screen triangles whose bbox overlaps it, in arbitrary order; they are
+ * walked in submission order.  A list of at most OT_CAP entries is sorted
+ * in LDS at once.  A longer list is walked in windows of the index space
+ * [lo, hi) holding at most OT_CAP entries each: the window is halved until
+ * its count fits, gathered into LDS, sorted and walked, then the next window
+ * starts at hi (every pass reads the bin once; entries are distinct, so a
+ * window of one index holds at most one).  A batch whose bins overflowed
+ * (flag 16) is run again by the host; its tiles return at once, and a tile
+ * never reads past the bins that were written or an index outside the
+ * screen-triangle store. */
 __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint32_t *tile_start,
-                                                     const uint32_t *bins) {
+                                                     const uint32_t *bins, uint32_t bin_cap) {
   __shared__ OrderedLds L;
+  __shared__ uint32_t count;
+  if (frame.counters[CNT_OVERFLOW] & 16u)
+    return; /* bins incomplete; uniform */
   uint32_t vi = find_view_by_tile(frame.views, frame.nviews, blockIdx.x);
   const ViewGPU &vw = frame.views[vi];
   uint32_t t = blockIdx.x - vw.tile_base;
@@ -2949,85 +3015,83 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
   bool inside = px < vw.w && py < vw.h;
   int li = threadIdx.x;
   size_t idx = (size_t)py * (size_t)vw.w + (size_t)px;
-  uint32_t b0 = tile_start[blockIdx.x], n = tile_start[blockIdx.x + 1] - b0;
+  uint32_t b0 = tile_start[blockIdx.x], b1 = tile_start[blockIdx.x + 1];
+  if (b1 > bin_cap)
+    b1 = bin_cap;
+  uint32_t n = b1 > b0 ? b1 - b0 : 0u;
   if (n == 0)
     return; /* uniform */
+  uint32_t total = frame.counters[CNT_SUBTRI];
+  if (total > frame.subtri_cap)
+    total = frame.subtri_cap;
   if (inside) {
     L.hdr[li] = reinterpret_cast<const float4 *>(vw.hdr)[idx];
     L.depth[li] = vw.depth[idx];
     L.ids[li] = vw.ids[idx];
   }
-  uint32_t cur_draw = 0xffffffffu;
-  bool saved_live = false;
-  uint32_t processed = 0, last = 0;
-  bool first = true;
-  while (processed < n) {
-    int m;
-    if (n <= OT_CAP) {
-      for (uint32_t i = threadIdx.x; i < n; i += WG)
-        L.list[i] = bins[b0 + i];
-      __syncthreads();
-      lds_sort(L.list, (int)n);
-      m = (int)n;
-    } else {
-      /* long list: take entries one at a time in submission order */
-      if (threadIdx.x == 0)
-        L.next_min = 0xffffffffu;
-      __syncthreads();
-      uint32_t mine = 0xffffffffu;
-      for (uint32_t i = threadIdx.x; i < n; i += WG) {
-        uint32_t v = bins[b0 + i];
-        if ((first || v > last) && v < mine)
-          mine = v;
-      }
-      atomicMin(&L.next_min, mine);
-      __syncthreads();
-      L.list[0] = L.next_min;
-      m = 1;
-      __syncthreads();
-    }
-    for (int e = 0; e < m; e++) {
+  OrderedState os = {0xffffffffu, false};
+  if (n <= OT_CAP) {
+    for (uint32_t i = threadIdx.x; i < n; i += WG)
+      L.list[i] = bins[b0 + i];
+    __syncthreads();
+    lds_sort(L.list, (int)n);
+    for (uint32_t e = 0; e < n; e++) {
       uint32_t s = L.list[e];
-      uint32_t draw = frame.subtris[s].draw;
-      if (draw != cur_draw) {
-        /* end of a draw with depth_write = false: restore its depth */
-        if (saved_live && inside)
-          L.depth[li] = L.saved[li];
-        saved_live = false;
-        cur_draw = draw;
-        if (!(frame.draws[draw].flags & MOPH_DRAW_DEPTH_WRITE)) {
-          if (inside)
-            L.saved[li] = L.depth[li];
-          saved_live = true;
+      if (s < total)
+        ordered_step(frame, vw, s, px, py, inside, li, L, os);
+    }
+  } else {
+    uint32_t lo = 0, span = total;
+    while (lo < total) {
+      uint32_t hi, c;
+      for (;;) {
+        hi = (total - lo > span) ? lo + span : total;
+        __syncthreads();
+        if (threadIdx.x == 0)
+          count = 0;
+        __syncthreads();
+        uint32_t mine = 0;
+        for (uint32_t i = threadIdx.x; i < n; i += WG) {
+          uint32_t v = bins[b0 + i];
+          mine += (v >= lo && v < hi) ? 1u : 0u;
+        }
+        if (mine)
+          atomicAdd(&count, mine);
+        __syncthreads();
+        c = count;
+        if (c <= OT_CAP || hi - lo <= 1u)
+          break;
+        span = (hi - lo) >> 1;
+      }
+      if (c) {
+        __syncthreads();
+        if (threadIdx.x == 0)
+          count = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += WG) {
+          uint32_t v = bins[b0 + i];
+          if (v >= lo && v < hi) {
+            uint32_t at = atomicAdd(&count, 1u);
+            if (at < OT_CAP)
+              L.list[at] = v;
+          }
+        }
+        __syncthreads();
+        int m = (int)(c < OT_CAP ? c : OT_CAP);
+        lds_sort(L.list, m);
+        for (int e = 0; e < m; e++) {
+          uint32_t s = L.list[e];
+          if (s < total)
+            ordered_step(frame, vw, s, px, py, inside, li, L, os);
         }
       }
-      SubTri st = frame.subtris[s];
-      if (frame.draws[draw].flags & MOPH_DRAW_WIRE) {
-        int bx0, by0, bx1, by1;
-        bool ok = wire_bbox(st, vw.w, vw.h, bx0, by0, bx1, by1);
-        bool in_bbox = ok && inside && px >= bx0 && px <= bx1 && py >= by0 && py <= by1;
-        if (__ballot(in_bbox) == 0ull)
-          continue;
-        if (in_bbox)
-          wire_fragment(frame, vw, st, px, py, li, L);
-        continue;
-      }
-      RasterTri r;
-      bool ok = load_raster_tri(vw.w, vw.h, st, r);
-      bool in_bbox = ok && inside && px >= r.es.min_x && px <= r.es.max_x &&
-                     py >= r.es.min_y && py <= r.es.max_y;
-      /* coverage mask of the wave: skip the triangle when no lane is in its bbox */
-      if (__ballot(in_bbox) == 0ull)
-        continue;
-      if (in_bbox)
-        ordered_fragment(frame, vw, st, r, px, py, li, L);
+      /* a sparse window lets the next one grow */
+      if (c < OT_CAP / 2 && span < 0x40000000u)
+        span <<= 1;
+      lo = hi;
     }
-    processed += (uint32_t)m;
-    last = L.list[m - 1];
-    first = false;
-    __syncthreads();
   }
-  if (saved_live && inside)
+  if (os.saved_live && inside)
     L.depth[li] = L.saved[li];
   if (inside) {
     reinterpret_cast<float4 *>(vw.hdr)[idx] = L.hdr[li];
@@ -3057,5 +3121,5 @@ void launch_ordered(const FrameArgs &fa, uint32_t *tile_counts, uint32_t *tile_c
                      (uint32_t *)nullptr, 0u);
   launch(k_bin_fill, dim3(grid), dim3(WG), 0, s, fa, tile_counts, tile_cursor, bins,
                      bin_cap);
-  launch(k_tile_ordered, dim3(nt), dim3(WG), 0, s, fa, tile_counts, bins);
+  launch(k_tile_ordered, dim3(nt), dim3(WG), 0, s, fa, tile_counts, bins, bin_cap);
 }

@@ -150,9 +150,14 @@ enum {
                        bit5 big-triangle checkpoint store */
   CNT_SHADOW_BIG = 4,
   CNT_CHK = 5,    /* order-free path: row checkpoint entries of big triangles */
+  CNT_BINS = 6,   /* ordered path: tile-bin entries the batch needed */
   CNT_TICKET = 7, /* k_finish: workgroups done */
   CNT_N = 8
 };
+
+/* Overflow bits that make the runtime grow a scratch store and run the
+ * frame again (clip queue, screen-triangle store, tile bins, checkpoints). */
+#define CNT_OVERFLOW_RERUN 51u
 
 /* One view (framebuffer + the shading state its draws see) of a batch.
  * A batch renders one order-free segment of each of its views in a single
@@ -232,6 +237,12 @@ struct FrameArgs {
   SubTri *subtris;
   uint32_t *big_queue;
   const MopHipLight *lights;
+  /* the device's overflow word: k_finish ORs the batch's overflow bits into
+   * sticky (device memory) and stores the result to sticky_out (a device
+   * alias of host-mapped memory), so the host can check every executed
+   * batch, not only the frames it reads back */
+  uint32_t *sticky;
+  uint32_t *sticky_out;
 };
 
 /* One view of the batched clear. */

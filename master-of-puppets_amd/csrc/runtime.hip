@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -288,6 +289,14 @@ struct MopHipDevice {
   uint32_t small_area = 64, shade_tile = 16, raster_grid = 0;
   DevArr<uint32_t> tile_counts, tile_cursor, bins;
   uint32_t bin_cap_hint = 0;
+  /* forced scratch capacities (mop_hip_set_scratch_limits; 0 = none): used
+   * for the next executions until an overflow makes the runtime grow the
+   * store and run the frame again */
+  uint32_t force_clip = 0, force_chk = 0, force_bins = 0;
+  /* sticky overflow word: device copy (atomic OR by k_finish) and its
+   * host-mapped mirror */
+  DevArr<uint32_t> sticky;
+  uint32_t *h_sticky = nullptr, *d_sticky_out = nullptr;
   uint32_t *h_counters = nullptr;
   DevArr<ViewGPU> views;
   DevArr<PostViewGPU> pviews;
@@ -322,6 +331,12 @@ struct MopHipDevice {
 /* ------------------------------------------------------------------------ */
 /* device / buffers / textures                                               */
 /* ------------------------------------------------------------------------ */
+
+/* Process-wide forced scratch capacities copied into every device created
+ * afterwards (mop_hip_set_default_scratch_limits), and the number of frame
+ * re-runs caused by scratch overflow in this process. */
+static uint32_t g_force_clip = 0, g_force_chk = 0, g_force_bins = 0;
+static std::atomic<uint64_t> g_overflow_reruns{0};
 
 extern "C" MopHipDevice *mop_hip_device_create(void) {
   int n = 0;
@@ -358,6 +373,18 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   for (int k = 0; k < NSTAGE; k++)
     HIPCHK(hipEventCreateWithFlags(&d->ring[k].ev, hipEventDisableTiming));
   HIPCHK(hipHostMalloc((void **)&d->h_counters, CNT_N * sizeof(uint32_t), 0));
+  if (hipHostMalloc((void **)&d->h_sticky, 64, hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&d->d_sticky_out, d->h_sticky, 0) != hipSuccess ||
+      !d->sticky.ensure(1)) {
+    set_err("overflow word allocation failed");
+    mop_hip_device_destroy(d);
+    return nullptr;
+  }
+  *d->h_sticky = 0;
+  launch_fill_u32(1, d->sticky.p, 0u, d->stream);
+  d->force_clip = g_force_clip;
+  d->force_chk = g_force_chk;
+  d->force_bins = g_force_bins;
   if (d->post_lut.ensure(4 * 256))
     launch_post_lut(d->post_lut.p, d->stream);
   return d;
@@ -426,6 +453,9 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->bins.release();
   if (d->h_counters)
     (void)hipHostFree(d->h_counters);
+  if (d->h_sticky)
+    (void)hipHostFree(d->h_sticky);
+  d->sticky.release();
   (void)hipEventDestroy(d->ev0);
   (void)hipEventDestroy(d->ev1);
   (void)hipEventDestroy(d->ev_fork);
@@ -1099,6 +1129,8 @@ static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_t
     clip_cap = 4096;
   if (d->clip_cap_hint > clip_cap)
     clip_cap = d->clip_cap_hint;
+  if (d->force_clip)
+    clip_cap = d->force_clip;
   uint64_t sc = (uint64_t)n_tris + 10ull * clip_cap + 1;
   if (sc > 0xffffff00ull)
     sc = 0xffffff00ull;
@@ -1267,6 +1299,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.fused = (!ordered && n_tris < (1u << 27)) ? 1u : 0u;
   if (fa.fused) {
     uint32_t chk_cap = d->chk_cap_hint > (1u << 20) ? d->chk_cap_hint : (1u << 20);
+    if (d->force_chk)
+      chk_cap = d->force_chk;
     if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
         !d->blk_counts.ensure(wg_tri.size() + 1 + CLIP_RASTER_GRID) ||
         !d->chk.ensure(chk_cap) || !d->big_chk.ensure(subtri_cap) ||
@@ -1292,6 +1326,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.subtris = d->subtris.p;
   fa.big_queue = d->big_queue.p;
   fa.lights = (const MopHipLight *)(d->arena.p + ol);
+  fa.sticky = d->sticky.p;
+  fa.sticky_out = d->d_sticky_out;
 
   if (!ordered)
     for (const SegJob &j : jobs)
@@ -1302,6 +1338,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     uint64_t bc = 2ull * subtri_cap + 65536;
     if (d->bin_cap_hint > bc)
       bc = d->bin_cap_hint;
+    if (d->force_bins)
+      bc = d->force_bins;
     if (bc > 0xffffff00ull)
       bc = 0xffffff00ull;
     bin_cap = (uint32_t)bc;
@@ -1721,7 +1759,7 @@ static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
     if (fb->checked)
       return;
     HIPCHK(hipEventSynchronize(fb->done));
-    uint32_t ovf = 0, need_clip = 0, need_chk = 0;
+    uint32_t ovf = 0, need_clip = 0, need_chk = 0, need_bins = 0;
     MopHipFrameStats st = {};
     int ns = fb->nseg < MOPH_MAX_SEGS ? fb->nseg : MOPH_MAX_SEGS;
     for (int k = 0; k < ns; k++) {
@@ -1731,24 +1769,35 @@ static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
         need_clip = c[CNT_CLIP];
       if (c[CNT_CHK] > need_chk)
         need_chk = c[CNT_CHK];
+      if (c[CNT_BINS] > need_bins)
+        need_bins = c[CNT_BINS];
       st.subtris += c[CNT_SUBTRI];
       st.clipped += c[CNT_CLIP];
       st.big_subtris += c[CNT_BIG];
     }
     if (ovf & 8u)
       set_err("a clipped polygon exceeded %d vertices and was dropped", MOPH_MAX_POLY);
-    if (ovf & 51u) {
+    if (ovf & CNT_OVERFLOW_RERUN) {
       /* clip queue, screen-triangle store, tile bins or checkpoint store too
-       * small: grow, re-run */
-      if (ovf & 3u)
+       * small: grow to what the batch needed (the counters kept counting past
+       * the capacity), drop any forced capacity, re-run */
+      if (ovf & 3u) {
         d->clip_cap_hint = need_clip + need_clip / 4 + 1024;
+        d->force_clip = 0;
+      }
       if (ovf & 32u) {
         uint64_t c = (uint64_t)need_chk + need_chk / 4 + 65536;
         d->chk_cap_hint = c > 0xffffff00ull ? 0xffffff00u : (uint32_t)c;
+        d->force_chk = 0;
       }
-      if (ovf & 16u)
-        d->bin_cap_hint = (uint32_t)(d->bins.cap * 2 < 0xffffff00ull ? d->bins.cap * 2
-                                                                     : 0xffffff00ull);
+      if (ovf & 16u) {
+        uint64_t c = (uint64_t)need_bins + need_bins / 4 + 65536;
+        if (c < (uint64_t)d->bins.cap * 2)
+          c = (uint64_t)d->bins.cap * 2;
+        d->bin_cap_hint = c > 0xffffff00ull ? 0xffffff00u : (uint32_t)c;
+        d->force_bins = 0;
+      }
+      g_overflow_reruns++;
       fb->executed = false;
       continue;
     }
@@ -1819,6 +1868,48 @@ extern "C" void mop_hip_device_sync(MopHipDevice *d) {
   if (d)
     HIPCHK(hipStreamSynchronize(d->stream));
 }
+
+/* OR of the overflow bits of every batch executed on the device since the
+ * last reset (waits for the queued work); reset != 0 clears it. */
+extern "C" uint32_t mop_hip_overflow_flags(MopHipDevice *d, int reset) {
+  if (!d)
+    return 0;
+  HIPCHK(hipStreamSynchronize(d->side));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  uint32_t v = __atomic_load_n(d->h_sticky, __ATOMIC_ACQUIRE);
+  if (reset) {
+    launch_fill_u32(1, d->sticky.p, 0u, d->stream);
+    HIPCHK(hipStreamSynchronize(d->stream));
+    __atomic_store_n(d->h_sticky, 0u, __ATOMIC_RELEASE);
+  }
+  return v;
+}
+
+/* Forced scratch capacities for the next executions on d (0 = none): the
+ * clip queue (triangles), the big-triangle checkpoint store (entries) and
+ * the ordered path's tile bins (entries).  A frame that overflows one is
+ * grown and run again at readback, as any overflow is; the forced value is
+ * then dropped.  Exists to exercise that recovery. */
+extern "C" int mop_hip_set_scratch_limits(MopHipDevice *d, uint32_t clip, uint32_t chk,
+                                          uint32_t bins) {
+  if (!d)
+    return -1;
+  d->force_clip = clip;
+  d->force_chk = chk;
+  d->force_bins = bins;
+  return 0;
+}
+
+/* The same for every device created afterwards in this process (a device
+ * created by a caller that does not expose it, such as the viewport core). */
+extern "C" void mop_hip_set_default_scratch_limits(uint32_t clip, uint32_t chk, uint32_t bins) {
+  g_force_clip = clip;
+  g_force_chk = chk;
+  g_force_bins = bins;
+}
+
+/* Frames run again after a scratch overflow, in this process. */
+extern "C" uint64_t mop_hip_overflow_reruns(void) { return g_overflow_reruns.load(); }
 
 extern "C" void mop_hip_set_exposure(MopHipDevice *d, float e) {
   if (d)

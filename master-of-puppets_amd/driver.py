@@ -29,6 +29,23 @@ class Device:
         arr = (C.c_void_p * len(fbs))(*[f.fb for f in fbs])
         self.lib.mop_hip_frames_execute(self.dev, arr, len(fbs))
 
+    def overflow_flags(self, reset=False):
+        """OR of the scratch-overflow bits of every batch executed since the
+        last reset (waits for the device)."""
+        return int(self.lib.mop_hip_overflow_flags(self.dev, 1 if reset else 0))
+
+    def scratch_limits(self, clip=0, chk=0, bins=0):
+        """Force small scratch capacities for the next executions."""
+        if self.lib.mop_hip_set_scratch_limits(self.dev, clip, chk, bins) != 0:
+            raise RuntimeError("set_scratch_limits failed")
+
+    def frame_stats(self):
+        st = abi.FrameStats()
+        self.lib.mop_hip_get_frame_stats(self.dev, C.byref(st))
+        return {"triangles_in": st.triangles_in, "subtris": st.subtris, "clipped": st.clipped,
+                "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris,
+                "gpu_ms": st.gpu_ms}
+
     def timers(self, on):
         self.lib.mop_hip_timers_enable(self.dev, 1 if on else 0)
 

@@ -586,6 +586,23 @@ def ordered_scene(seed=9, width=160, height=120, with_env=False):
     return sc
 
 
+def dense_ordered_scene(seed=13, width=160, height=120, subdiv=6):
+    """Transparent draws dense enough that single 16x16 tiles overlap more
+    than 4096 screen triangles (the ordered tile kernel then walks its list
+    in windows of the submission order): two finely tessellated alpha
+    icospheres a few tiles across, in front of an opaque scene."""
+    sc = small_scene(seed=seed, width=width, height=height, n=4)
+    sc.name = f"dense_ordered_{seed}_{width}x{height}_s{subdiv}"
+    rng = Xoshiro(seed * 31337 + 5)
+    for k in range(2):
+        col = (rng.uniform(), rng.uniform(), rng.uniform(), 1.0)
+        v, i = icosphere(subdiv, col)
+        m = trs((0.35 * k - 0.2, 0.6 + 0.2 * k, 3.0), random_quat(rng), 0.45)
+        sc.meshes.append(Mesh(v, i, 150 + k, m, True, 0.0, rng.uniform(0.2, 0.9),
+                              blend=1, opacity=0.55))
+    return sc
+
+
 def shading_scene(seed=21, width=160, height=120, lit=True, with_shadow=True):
     """Every shading model the CPU backend dispatches: flat and smooth meshes,
     opaque and blended, with lights (compute_multi_light / smooth_ml) or

@@ -72,9 +72,10 @@ class Server:
 
     def run(self, requests, plain=False, time_limit=None, keep=()):
         """Serve `requests` ((arrival s, scene, yaw, pitch, class) tuples).
-        keep: request indices whose RGBA8 results are kept (in the result's
-        "outputs", index -> (h, w, 4) uint8) to compare with the same request
-        rendered alone."""
+        keep: request indices whose results are kept (in the result's
+        "outputs", index -> dict of RGBA8 colour, depth, object ID and HDR
+        arrays) to compare with the same request rendered alone or by the
+        reference."""
         lat = []
         rejected = 0
         max_queue = 0
@@ -106,11 +107,10 @@ class Server:
             while inflight and all(fb.done() for _, fb in inflight[0][1]):
                 _, items = inflight.popleft()
                 for req, fb in items:
-                    ptr = fb.read_color_raw()
+                    fb.read_color_raw()
                     k = index.get(id(req))
                     if k in keep:
-                        outputs[k] = np.ctypeslib.as_array(ptr, (fb.h * fb.w * 4,)).reshape(
-                            fb.h, fb.w, 4).copy()
+                        outputs[k] = fb.read(color=True, depth=True, ids=True, hdr=True)
                     tdone = time.perf_counter() - t0
                     lat.append(tdone - req[0])
                     self.free[req[4]].append(fb)

@@ -70,6 +70,36 @@ static Routed *routed(void) { return (g_mode >= 2 && g_mode <= 3) ? &g_routed[g_
 static MopViewport *g_cur_vp;
 static int g_shadow_started;
 
+/* Shadow-map size override (0: the core's fixed MOP_SHADOW_MAP_SIZE,
+ * viewport.c:2286).  The core's pass_shadow still computes the light
+ * view-projection and the caster list; the wrappers below render those
+ * casters into a map of this size instead: in modes 0/1 the reference's own
+ * mop_sw_shadow_render_mesh draws into the harness's map (cleared to 1.0
+ * before the first caster, as pass_shadow clears its own, viewport.c:2365-2368)
+ * and mop_sw_shadow_set points the shading at it; in modes 2/3 the routed
+ * hooks get the size.  This is how the benchmarked 2048^2 and 4096^2 maps
+ * are compared with the reference's arithmetic. */
+static int g_shadow_size;
+static MopSwFramebuffer g_shadow_fb;
+
+void mh_set_shadow_size(int size) { g_shadow_size = size > 0 ? size : 0; }
+
+static int override_map_ready(void) {
+  if (g_shadow_fb.width != g_shadow_size || g_shadow_fb.height != g_shadow_size) {
+    mop_sw_framebuffer_free(&g_shadow_fb);
+    memset(&g_shadow_fb, 0, sizeof(g_shadow_fb));
+    if (!mop_sw_framebuffer_alloc(&g_shadow_fb, g_shadow_size, g_shadow_size))
+      return 0;
+  }
+  return 1;
+}
+
+static void override_map_clear(void) {
+  size_t n = (size_t)g_shadow_size * (size_t)g_shadow_size;
+  for (size_t p = 0; p < n; p++)
+    g_shadow_fb.depth[p] = 1.0f;
+}
+
 int mh_set_mode(int mode) {
   if ((mode == 2 || mode == 3) && !g_routed[mode - 2].rhi)
     return -1;
@@ -154,13 +184,20 @@ void __wrap_mop_sw_shadow_render_mesh(const MopVertex *v, uint32_t vc,
                                       MopSwFramebuffer *sfb) {
   Routed *r = routed();
   if (!r) {
+    if (g_shadow_size && override_map_ready()) {
+      if (!g_shadow_started)
+        override_map_clear();
+      g_shadow_started = 1;
+      __real_mop_sw_shadow_render_mesh(v, vc, idx, ic, model, light_vp, &g_shadow_fb);
+      return;
+    }
     __real_mop_sw_shadow_render_mesh(v, vc, idx, ic, model, light_vp, sfb);
     return;
   }
   /* &vp->shadow_fb -> vp (pass_shadow passes the viewport's own member) */
   MopViewport *vp = (MopViewport *)((char *)sfb - offsetof(MopViewport, shadow_fb));
-  r->shadow_mesh(vp->device, v, vc, idx, ic, &model, &light_vp, sfb->width,
-                g_shadow_started ? 0 : 1);
+  r->shadow_mesh(vp->device, v, vc, idx, ic, &model, &light_vp,
+                 g_shadow_size ? g_shadow_size : sfb->width, g_shadow_started ? 0 : 1);
   g_shadow_started = 1;
 }
 
@@ -168,10 +205,19 @@ void __real_mop_sw_shadow_set(const float *, int, int, MopMat4);
 void __wrap_mop_sw_shadow_set(const float *depth, int w, int h, MopMat4 lvp) {
   Routed *r = routed();
   if (!r) {
+    if (g_shadow_size && override_map_ready()) {
+      if (!g_shadow_started)
+        override_map_clear(); /* a pass with no caster */
+      g_shadow_started = 0;
+      __real_mop_sw_shadow_set(g_shadow_fb.depth, g_shadow_size, g_shadow_size, lvp);
+      return;
+    }
     __real_mop_sw_shadow_set(depth, w, h, lvp);
     return;
   }
   g_shadow_started = 0;
+  if (g_shadow_size)
+    w = h = g_shadow_size;
   if (g_cur_vp)
     r->shadow_set(g_cur_vp->device, w, h, &lvp);
 }
@@ -179,11 +225,11 @@ void __wrap_mop_sw_shadow_set(const float *depth, int w, int h, MopMat4 lvp) {
 void __real_mop_sw_shadow_clear(void);
 void __wrap_mop_sw_shadow_clear(void) {
   Routed *r = routed();
+  g_shadow_started = 0;
   if (!r) {
     __real_mop_sw_shadow_clear();
     return;
   }
-  g_shadow_started = 0;
   if (g_cur_vp)
     r->shadow_clear(g_cur_vp->device);
 }
@@ -334,6 +380,10 @@ void mh_set_camera(MopViewport *vp, const float *eye, const float *target,
                           (MopVec3){target[0], target[1], target[2]},
                           (MopVec3){up[0], up[1], up[2]}, fov_deg, znear, zfar);
 }
+
+/* mop_viewport_resize (viewport.h:86): the serving checks render many
+ * requests of one resident scene at different resolutions */
+void mh_resize(MopViewport *vp, int w, int h) { mop_viewport_resize(vp, w, h); }
 
 void mh_set_clear_color(MopViewport *vp, const float *c) {
   mop_viewport_set_clear_color(vp, (MopColor){c[0], c[1], c[2], c[3]});

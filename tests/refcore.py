@@ -47,6 +47,8 @@ def harness():
         h.mh_add_instanced.argtypes = [VP, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                        C.c_uint32, C.c_void_p, C.c_uint32]
         h.mh_set_camera.argtypes = [VP, F, F, F, C.c_float, C.c_float, C.c_float]
+        h.mh_resize.argtypes = [VP, C.c_int, C.c_int]
+        h.mh_set_shadow_size.argtypes = [C.c_int]
         h.mh_set_clear_color.argtypes = [VP, F]
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
         h.mh_set_ambient.argtypes = [VP, C.c_float]
@@ -114,9 +116,14 @@ def _f3(t):
 
 
 class RefViewport:
-    """A reference-core viewport holding one Scene."""
+    """A reference-core viewport holding one Scene.
 
-    def __init__(self, scene, mode, env_path=None):
+    shadow_size: render the core's shadow pass into a map of this size
+    instead of the fixed 1024^2 (the harness override, applied to the
+    reference rasterizer in modes 0/1 and passed to the routed backend in
+    modes 2/3); None keeps the core's size."""
+
+    def __init__(self, scene, mode, env_path=None, shadow_size=None):
         h = harness()
         if mode == 2:
             use_hip()
@@ -125,6 +132,8 @@ class RefViewport:
         if h.mh_set_mode(mode) != 0:
             raise RuntimeError(f"mode {mode} unavailable")
         self.h, self.mode, self.scene = h, mode, scene
+        self.shadow_size = shadow_size
+        self.size = (scene.width, scene.height)
         self.vp = h.mh_viewport_ssaa(scene.width, scene.height, getattr(scene, "ssaa", 1))
         if not self.vp:
             raise RuntimeError("mop_viewport_create failed")
@@ -205,12 +214,26 @@ class RefViewport:
     def render(self):
         if self.h.mh_set_mode(self.mode) != 0:
             raise RuntimeError("mode switch failed")
-        return self.h.mh_render(self.vp)
+        self.h.mh_set_shadow_size(int(self.shadow_size or 0))
+        try:
+            return self.h.mh_render(self.vp)
+        finally:
+            self.h.mh_set_shadow_size(0)
+
+    def set_view(self, width, height, eye):
+        """Resize the viewport and move the camera eye (target, up, fov and
+        planes stay the scene's)."""
+        if (width, height) != self.size:
+            self.h.mh_resize(self.vp, width, height)
+            self.size = (width, height)
+        sc = self.scene
+        self.h.mh_set_camera(self.vp, _f3(eye), _f3(sc.target), _f3(sc.up), sc.fov,
+                             sc.znear, sc.zfar)
 
     def read(self, hdr=True):
         """RGBA8 at the viewport size (the core averages SSAA samples,
         viewport.c:3385-3411); depth, ID and HDR at the internal size."""
-        w, h = self.scene.width, self.scene.height
+        w, h = self.size
         k = getattr(self.scene, "ssaa", 1)
         color = np.zeros((h, w, 4), np.uint8)
         depth = np.zeros((h * k, w * k), np.float32)
@@ -274,8 +297,8 @@ class RefViewport:
             pass
 
 
-def render(scene, mode, frames=1):
-    vp = RefViewport(scene, mode)
+def render(scene, mode, frames=1, shadow_size=None):
+    vp = RefViewport(scene, mode, shadow_size=shadow_size)
     try:
         for _ in range(frames):
             vp.render()
@@ -298,9 +321,25 @@ def compare(a, b):
     res["color_over_1_px"] = int((dc.max(-1) > 1).sum())
     if a.get("hdr") is not None and b.get("hdr") is not None:
         ha, hb = a["hdr"], b["hdr"]
-        rel = np.abs(ha - hb) / np.maximum(np.abs(hb), 1e-6)
-        tol_bad = np.abs(ha - hb) > 1e-4 * np.maximum(np.abs(hb), 1.0)
+        # relative error; |ref| below 1e-6 is compared against 1e-6 (an
+        # absolute floor of 1e-10 at the 1e-4 bar)
+        rel = np.abs(ha - hb) / np.maximum(np.abs(hb), HDR_FLOOR)
         res["hdr_max_rel"] = float(rel.max())
-        res["hdr_over_tol_px"] = int(tol_bad.any(-1).sum())
+        res["hdr_over_tol_px"] = int((rel > HDR_REL_TOL).any(-1).sum())
         res["hdr_bitdiff_px"] = int((ha.view(np.uint32) != hb.view(np.uint32)).any(-1).sum())
     return res
+
+
+# parity bar for the linear HDR buffer: |a - b| <= 1e-4 * max(|ref|, 1e-6)
+HDR_REL_TOL = 1e-4
+HDR_FLOOR = 1e-6
+
+
+def assert_parity(cmp, color_tol=1):
+    """The parity bar: depth and object ID bit-exact, RGBA8 within 1/255,
+    HDR within 1e-4 relative (when compared)."""
+    assert cmp["id_diff_px"] == 0, cmp
+    assert cmp["depth_diff_px"] == 0, cmp
+    assert cmp["color_max_diff"] <= color_tol, cmp
+    if "hdr_max_rel" in cmp:
+        assert cmp["hdr_max_rel"] <= HDR_REL_TOL, cmp

@@ -93,9 +93,11 @@ def test_serving_results_equal_alone():
             (w, h), _ = serve.RES_CLASSES[cls]
             fb = dev.framebuffer(w, h)
             pool[si][0].render(fb, driver.camera_of(pool[si][1], yaw, pitch))
-            alone = fb.read(color=True, depth=False, ids=False)["color"]
+            alone = fb.read(hdr=True)
             fb.close()
-            assert np.array_equal(alone, res["outputs"][k]), k
+            for key in ("color", "depth", "ids", "hdr"):
+                assert np.array_equal(alone[key].view(np.uint8),
+                                      res["outputs"][k][key].view(np.uint8)), (k, key)
         srv.close()
         for d, _ in pool:
             d.close()

@@ -1,0 +1,193 @@
+"""Parity at the stated sizes of every workload, against the reference.
+
+- Configuration 2 (1920x1080, 256 subdivision-4 icospheres, IBL) through the
+  unmodified reference core with the HIP backend, at the core's 1024^2
+  shadow map and at the benchmarked 2048^2 (the harness renders the core's
+  shadow pass at that size in both backends), and through the HIP frame
+  driver the benchmark times (device IBL tables, 2048^2 map).
+- Configuration 3 (3840x2160, 2048x2048-quad terrain + 64 icospheres,
+  8,716,288 triangles, camera inside the terrain, 4096^2 shadow map) through
+  the core and through the frame driver.
+- Configuration 4: the 64-viewport batch executed in one set of launches;
+  eight of its views (the largest by pixels and by triangles among them)
+  against the reference core rendering each alone.
+- Configuration 5: requests served by the continuous-batching loop over
+  resident configuration-3 scenes (4096^2 maps kept on the device), at the
+  mixed resolutions, against the reference core rendering each request.
+
+Bar: depth and object ID bit-exact, RGBA8 within 1/255, HDR within 1e-4
+relative (refcore.assert_parity).  The reference is the CPU backend forced
+single-threaded (submission order)."""
+
+import numpy as np
+import pytest
+
+import refcore
+from mop_amd import scenes
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not refcore.available(), reason="oracle/_ref not built")]
+
+
+def _report(name, cmp):
+    print(f"[parity] {name}: {cmp}", flush=True)
+
+
+def _dropin(sc, shadow_size=None):
+    print(f"[case] drop-in {sc.name} {sc.width}x{sc.height} tris={sc.triangle_count} "
+          f"shadow={shadow_size or 1024}", flush=True)
+    ref = refcore.render(sc, 1, shadow_size=shadow_size)
+    hip = refcore.render(sc, 2, shadow_size=shadow_size)
+    cmp = refcore.compare(hip, ref)
+    _report(f"drop-in {sc.name} shadow {shadow_size or 1024}", cmp)
+    refcore.assert_parity(cmp)
+    assert (ref["ids"] != 0).any()
+    return ref
+
+
+def _driver_frame(dev, sc, shadow_size, cam=None):
+    from mop_amd import driver
+    drv = driver.SceneDriver(dev, sc, shadow_size=shadow_size)
+    fb = dev.framebuffer(sc.width, sc.height)
+    try:
+        drv.render(fb, cam)
+        out = fb.read(hdr=True)
+        out["exposure"] = drv.exposure()
+        return out
+    finally:
+        fb.close()
+        drv.close()
+
+
+@pytest.mark.parametrize("shadow_size", [None, 2048])
+def test_config2_full_size_dropin(shadow_size):
+    _dropin(scenes.config2(), shadow_size)
+
+
+def test_config2_full_size_frame_driver():
+    # the benchmarked path: the frame driver with IBL tables computed on the
+    # device and a 2048^2 shadow map, against the reference core
+    from mop_amd import driver
+    sc = scenes.config2()
+    print(f"[case] driver {sc.name} tris={sc.triangle_count} shadow 2048", flush=True)
+    ref = refcore.render(sc, 1, shadow_size=2048)
+    dev = driver.Device()
+    try:
+        got = _driver_frame(dev, sc, 2048)
+    finally:
+        dev.close()
+    cmp = refcore.compare(got, ref)
+    _report("driver config2 shadow 2048", cmp)
+    assert abs(got["exposure"] - ref["exposure"]) <= 1e-6 * ref["exposure"]
+    refcore.assert_parity(cmp)
+
+
+def test_config3_full_size_dropin():
+    _dropin(scenes.config3(), 4096)
+
+
+def test_config3_full_size_frame_driver():
+    from mop_amd import driver
+    sc = scenes.config3()
+    print(f"[case] driver {sc.name} tris={sc.triangle_count} shadow 4096", flush=True)
+    ref = refcore.render(sc, 1, shadow_size=4096)
+    dev = driver.Device()
+    try:
+        got = _driver_frame(dev, sc, 4096)
+        st = dev.frame_stats()
+        flags = dev.overflow_flags(reset=True)
+    finally:
+        dev.close()
+    cmp = refcore.compare(got, ref)
+    _report("driver config3 shadow 4096", cmp)
+    print(f"[stats] config3: {st} overflow flags {flags:#x}", flush=True)
+    refcore.assert_parity(cmp)
+    assert st["clipped"] > 0  # the camera sits inside the terrain
+
+
+def _pick_views(views, n=8):
+    """Indices of n views: the largest by pixels, the one with the most
+    triangles, and the rest spread over the set."""
+    big_px = max(range(len(views)), key=lambda k: views[k].width * views[k].height)
+    big_tri = max(range(len(views)), key=lambda k: views[k].triangle_count)
+    picks = [big_px, big_tri]
+    for k in np.linspace(0, len(views) - 1, n + 2).astype(int):
+        if len(picks) >= n:
+            break
+        if int(k) not in picks:
+            picks.append(int(k))
+    return picks
+
+
+def test_config4_batched_views_match_reference():
+    from mop_amd import driver
+    views = scenes.config4()
+    picks = _pick_views(views)
+    print(f"[case] config4 batch of {len(views)} views, checking {picks}", flush=True)
+    dev = driver.Device()
+    got = {}
+    try:
+        drv = [driver.SceneDriver(dev, sc, shadow_cache=True) for sc in views]
+        fbs = [dev.framebuffer(sc.width, sc.height) for sc in views]
+        for d, fb in zip(drv, fbs):
+            d.record(fb)
+        dev.execute(fbs)  # one batch: every view in one set of launches
+        flags = dev.overflow_flags(reset=False)
+        for k in picks:
+            got[k] = fbs[k].read(hdr=True)
+        print(f"[stats] config4 batch overflow flags {flags:#x}", flush=True)
+        for fb in fbs:
+            fb.close()
+        for d in drv:
+            d.close()
+    finally:
+        dev.close()
+    for k in picks:
+        sc = views[k]
+        ref = refcore.render(sc, 1)
+        cmp = refcore.compare(got[k], ref)
+        _report(f"config4 view {k} {sc.width}x{sc.height} tris={sc.triangle_count}", cmp)
+        refcore.assert_parity(cmp)
+
+
+def test_config5_served_requests_match_reference():
+    from mop_amd import driver, serve
+    n_scenes = 2
+    pool_scenes = [scenes.config3(1280, 720, seed=0x4D4F5010 + k) for k in range(n_scenes)]
+    dev = driver.Device()
+    try:
+        pool = [(driver.SceneDriver(dev, sc, shadow_cache=True), sc) for sc in pool_scenes]
+        srv = serve.Server(dev, pool, budget_bytes=4 << 30, max_batch=8, inflight=2)
+        # a burst: requests arrive while earlier batches are on the device
+        reqs = serve.make_requests(rate=5000.0, duration=0.004, n_scenes=n_scenes, seed=4242)
+        reqs = reqs[:10]
+        assert len(reqs) >= 8
+        classes = {r[4] for r in reqs}
+        print(f"[case] config5 {len(reqs)} requests, classes {sorted(classes)}", flush=True)
+        keep = list(range(len(reqs)))
+        res = srv.run(reqs, time_limit=120.0, keep=keep)
+        outputs = res.pop("outputs")
+        print(f"[serve] {res}", flush=True)
+        assert res["completed"] == len(reqs)
+        assert res["batches"] < res["completed"]  # requests shared launches
+        srv.close()
+        for d, _ in pool:
+            d.close()
+    finally:
+        dev.close()
+    refs = [refcore.RefViewport(sc, 1, shadow_size=4096) for sc in pool_scenes]
+    try:
+        for k in keep:
+            _, si, yaw, pitch, cls = reqs[k]
+            (w, h), _ = serve.RES_CLASSES[cls]
+            cam = driver.camera_of(pool_scenes[si], yaw, pitch)
+            refs[si].set_view(w, h, (cam.eye.x, cam.eye.y, cam.eye.z))
+            refs[si].render()
+            ref = refs[si].read()
+            cmp = refcore.compare(outputs[k], ref)
+            _report(f"config5 request {k} scene {si} {w}x{h} yaw {yaw:.2f} pitch {pitch:.2f}",
+                    cmp)
+            refcore.assert_parity(cmp)
+    finally:
+        for r in refs:
+            r.close()

@@ -87,7 +87,4 @@ def test_hip_frame_driver_matches_golden(name):
     print(f"[parity] driver {name}: {cmp} exposure={r['exposure']} ref={GOLD[name]['exposure']}",
           flush=True)
     assert abs(r["exposure"] - GOLD[name]["exposure"]) <= 1e-6 * GOLD[name]["exposure"]
-    assert cmp["depth_diff_px"] == 0 and cmp["id_diff_px"] == 0
-    assert cmp["color_max_diff"] <= 1
-    if sc.env is None:
-        assert cmp["hdr_over_tol_px"] == 0
+    refcore.assert_parity(cmp)
