@@ -5,12 +5,17 @@
  *   prefiltered  128x64 and 4 halved levels (roughness l/4), 64 GGX samples
  *   BRDF LUT     256x256 split-sum (scale, bias), 64 GGX samples
  * One lane per output texel.  The tables feed the shading kernel exactly as
- * mop_sw_ibl_set's arrays do.  libm calls (sinf/cosf/atan2f/asinf) are the
- * device library's, so tables agree with the host build to a few ulps.
+ * mop_sw_ibl_set's arrays do, and come out bit-identical to the host
+ * precompute: every sinf/cosf the precompute calls takes one of a fixed set
+ * of arguments (texel-centre angles per row and column of each table, and
+ * 2 pi xi2 of each Hammersley sample), which the host evaluates with its C
+ * library into a small table (ibl_trig_fill); atan2f/asinf of the sample
+ * directions use the host library's algorithms restated in libmf.h.
  */
 #include <math.h>
 
 #include "devmath.h"
+#include "libmf.h"
 #include "mop_hip_internal.h"
 
 #define IRR_W 64
@@ -52,30 +57,36 @@ DEV void env_bilinear(const float *data, int w, int h, float u, float v, float o
 
 /* dir_to_equirect, environment.c:62-68 */
 DEV void dir_to_equirect(float dx, float dy, float dz, float &u, float &v) {
-  float phi = atan2f(dz, dx) + 0.0f;
-  float theta = asinf(dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy));
+  float phi = mop_atan2f(dz, dx) + 0.0f;
+  float theta = mop_asinf(dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy));
   u = phi / (2.0f * PI_F) + 0.5f;
   v = theta / PI_F + 0.5f;
 }
 
-DEV float hammersley_xi2(uint32_t s) {
-  uint32_t bits = s;
-  bits = (bits << 16u) | (bits >> 16u);
-  bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
-  bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
-  bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
-  bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
-  return (float)bits * 2.3283064365386963e-10f;
+/* Layout of the host-evaluated sine/cosine table (floats):
+ *   [0, 128)  cosf(2 pi xi2(s)), s < 128   [128, 256) sinf of the same
+ *   then per table (irradiance 64x32, prefiltered level 0..4): cosf and
+ *   sinf of each column's phi, then cosf and sinf of each row's theta. */
+#define TRIG_SAMPLES 128
+#define TRIG_IRR (2 * TRIG_SAMPLES)
+#define TRIG_PF0 (TRIG_IRR + 2 * IRR_W + 2 * IRR_H)
+
+DEV size_t trig_level(int level) {
+  size_t off = TRIG_PF0;
+  for (int l = 0; l < level; l++) {
+    int w = PF_W >> l, h = PF_H >> l;
+    off += (size_t)(2 * (w < 1 ? 1 : w) + 2 * (h < 1 ? 1 : h));
+  }
+  return off;
 }
 
-/* texel -> unit direction + tangent frame (environment.c:127-161) */
-DEV void texel_frame(int x, int y, int w, int h, float n[3], float t[3], float b[3]) {
-  float u = ((float)x + 0.5f) / (float)w;
-  float v = ((float)y + 0.5f) / (float)h;
-  float phi = (u - 0.5f) * 2.0f * PI_F;
-  float theta = (v - 0.5f) * PI_F;
-  float ct = cosf(theta);
-  float nx = ct * cosf(phi), ny = sinf(theta), nz = ct * sinf(phi);
+/* texel -> unit direction + tangent frame (environment.c:127-161); tab: the
+ * table's block of the sine/cosine table (cos phi[w], sin phi[w], cos
+ * theta[h], sin theta[h]) */
+DEV void texel_frame(const float *tab, int x, int y, int w, int h, float n[3], float t[3],
+                     float b[3]) {
+  float ct = tab[2 * w + y];
+  float nx = ct * tab[x], ny = tab[2 * w + h + y], nz = ct * tab[w + x];
   float len = sqrtf(nx * nx + ny * ny + nz * nz);
   nx /= len;
   ny /= len;
@@ -102,22 +113,20 @@ DEV void texel_frame(int x, int y, int w, int h, float n[3], float t[3], float b
   b[2] = nx * ty - ny * tx;
 }
 
-__global__ void k_irradiance(const float *env, int ew, int eh, float *irr) {
+__global__ void k_irradiance(const float *env, int ew, int eh, float *irr, const float *trig) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= IRR_W * IRR_H)
     return;
   int x = i % IRR_W, y = i / IRR_W;
   float n[3], t[3], b[3];
-  texel_frame(x, y, IRR_W, IRR_H, n, t, b);
+  texel_frame(trig + TRIG_IRR, x, y, IRR_W, IRR_H, n, t, b);
   float acc[3] = {0, 0, 0};
   float total = 0.0f;
   for (int s = 0; s < IRR_SAMPLES; s++) {
     float xi1 = (float)s / (float)IRR_SAMPLES;
-    float xi2 = hammersley_xi2((uint32_t)s);
     float cos_t = sqrtf(xi1);
     float sin_t = sqrtf(1.0f - xi1);
-    float phi_s = 2.0f * PI_F * xi2;
-    float lx = sin_t * cosf(phi_s), ly = cos_t, lz = sin_t * sinf(phi_s);
+    float lx = sin_t * trig[s], ly = cos_t, lz = sin_t * trig[TRIG_SAMPLES + s];
     float wx = t[0] * lx + n[0] * ly + b[0] * lz;
     float wy = t[1] * lx + n[1] * ly + b[1] * lz;
     float wz = t[2] * lx + n[2] * ly + b[2] * lz;
@@ -136,7 +145,7 @@ __global__ void k_irradiance(const float *env, int ew, int eh, float *irr) {
   o[3] = 1.0f;
 }
 
-__global__ void k_prefiltered(const float *env, int ew, int eh, float *pf) {
+__global__ void k_prefiltered(const float *env, int ew, int eh, float *pf, const float *trig) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int level = 0, w = PF_W, h = PF_H;
   size_t offset = 0;
@@ -161,17 +170,15 @@ __global__ void k_prefiltered(const float *env, int ew, int eh, float *pf) {
   if (alpha2 < 1e-7f)
     alpha2 = 1e-7f;
   float n[3], t[3], b[3];
-  texel_frame(x, y, w, h, n, t, b);
+  texel_frame(trig + trig_level(level), x, y, w, h, n, t, b);
   float vx = n[0], vy = n[1], vz = n[2];
   float acc[3] = {0, 0, 0};
   float total = 0.0f;
   for (int s = 0; s < PF_SAMPLES; s++) {
     float xi1 = (float)s / (float)PF_SAMPLES;
-    float xi2 = hammersley_xi2((uint32_t)s);
     float cos_h = sqrtf((1.0f - xi1) / (1.0f + (alpha2 - 1.0f) * xi1));
     float sin_h = sqrtf(1.0f - cos_h * cos_h);
-    float phi_h = 2.0f * PI_F * xi2;
-    float hxl = sin_h * cosf(phi_h), hyl = cos_h, hzl = sin_h * sinf(phi_h);
+    float hxl = sin_h * trig[s], hyl = cos_h, hzl = sin_h * trig[TRIG_SAMPLES + s];
     float hx = t[0] * hxl + n[0] * hyl + b[0] * hzl;
     float hy = t[1] * hxl + n[1] * hyl + b[1] * hzl;
     float hz = t[2] * hxl + n[2] * hyl + b[2] * hzl;
@@ -201,7 +208,7 @@ __global__ void k_prefiltered(const float *env, int ew, int eh, float *pf) {
   o[3] = 1.0f;
 }
 
-__global__ void k_brdf_lut(float *lut) {
+__global__ void k_brdf_lut(float *lut, const float *trig) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= LUT_N * LUT_N)
     return;
@@ -220,11 +227,9 @@ __global__ void k_brdf_lut(float *lut) {
   float scale = 0.0f, bias = 0.0f;
   for (int s = 0; s < LUT_SAMPLES; s++) {
     float xi1 = (float)s / (float)LUT_SAMPLES;
-    float xi2 = hammersley_xi2((uint32_t)s);
     float cos_h = sqrtf((1.0f - xi1) / (1.0f + (alpha2 - 1.0f) * xi1));
     float sin_h = sqrtf(1.0f - cos_h * cos_h);
-    float phi_h = 2.0f * PI_F * xi2;
-    float hx = sin_h * cosf(phi_h), hy = sin_h * sinf(phi_h), hz = cos_h;
+    float hx = sin_h * trig[s], hy = sin_h * trig[TRIG_SAMPLES + s], hz = cos_h;
     float vdh = vx_t * hx + vy_t * hy + vz_t * hz;
     float lz = 2.0f * vdh * hz - vz_t;
     float ndl = lz;
@@ -260,12 +265,68 @@ size_t ibl_pf_texels(void) {
   return n;
 }
 
+/* The sine/cosine table the precompute kernels read, evaluated on the host
+ * with the host C library (called through pointers the compiler cannot see
+ * through, so nothing is folded at build time).  Each argument is formed
+ * with the precompute's own single-precision expressions
+ * (environment.c:126-130, 181, 329, 434). */
+size_t ibl_trig_floats(void) {
+  size_t n = TRIG_PF0;
+  for (int l = 0; l < PF_LEVELS; l++) {
+    int w = PF_W >> l, h = PF_H >> l;
+    n += (size_t)(2 * (w < 1 ? 1 : w) + 2 * (h < 1 ? 1 : h));
+  }
+  return n;
+}
+
+static float (*volatile host_cosf)(float) = cosf;
+static float (*volatile host_sinf)(float) = sinf;
+
+static void trig_block(float *tab, int w, int h) {
+  for (int x = 0; x < w; x++) {
+    float u = ((float)x + 0.5f) / (float)w;
+    float phi = (u - 0.5f) * 2.0f * PI_F;
+    tab[x] = host_cosf(phi);
+    tab[w + x] = host_sinf(phi);
+  }
+  for (int y = 0; y < h; y++) {
+    float v = ((float)y + 0.5f) / (float)h;
+    float theta = (v - 0.5f) * PI_F;
+    tab[2 * w + y] = host_cosf(theta);
+    tab[2 * w + h + y] = host_sinf(theta);
+  }
+}
+
+void ibl_trig_fill(float *tab) {
+  for (int s = 0; s < TRIG_SAMPLES; s++) {
+    uint32_t bits = (uint32_t)s;
+    bits = (bits << 16u) | (bits >> 16u);
+    bits = ((bits & 0x55555555u) << 1u) | ((bits & 0xAAAAAAAAu) >> 1u);
+    bits = ((bits & 0x33333333u) << 2u) | ((bits & 0xCCCCCCCCu) >> 2u);
+    bits = ((bits & 0x0F0F0F0Fu) << 4u) | ((bits & 0xF0F0F0F0u) >> 4u);
+    bits = ((bits & 0x00FF00FFu) << 8u) | ((bits & 0xFF00FF00u) >> 8u);
+    float xi2 = (float)bits * 2.3283064365386963e-10f;
+    float phi = 2.0f * PI_F * xi2;
+    tab[s] = host_cosf(phi);
+    tab[TRIG_SAMPLES + s] = host_sinf(phi);
+  }
+  trig_block(tab + TRIG_IRR, IRR_W, IRR_H);
+  size_t off = TRIG_PF0;
+  for (int l = 0; l < PF_LEVELS; l++) {
+    int w = PF_W >> l, h = PF_H >> l;
+    if (w < 1) w = 1;
+    if (h < 1) h = 1;
+    trig_block(tab + off, w, h);
+    off += (size_t)(2 * w + 2 * h);
+  }
+}
+
 void launch_ibl_precompute(const float *env, int ew, int eh, float *irr, float *pf, float *lut,
-                           hipStream_t s) {
+                           const float *trig, hipStream_t s) {
   hipLaunchKernelGGL(k_irradiance, dim3((IRR_W * IRR_H + 63) / 64), dim3(64), 0, s, env, ew,
-                     eh, irr);
+                     eh, irr, trig);
   size_t npf = ibl_pf_texels();
   hipLaunchKernelGGL(k_prefiltered, dim3((unsigned)((npf + 63) / 64)), dim3(64), 0, s, env, ew,
-                     eh, pf);
-  hipLaunchKernelGGL(k_brdf_lut, dim3((LUT_N * LUT_N + 255) / 256), dim3(256), 0, s, lut);
+                     eh, pf, trig);
+  hipLaunchKernelGGL(k_brdf_lut, dim3((LUT_N * LUT_N + 255) / 256), dim3(256), 0, s, lut, trig);
 }

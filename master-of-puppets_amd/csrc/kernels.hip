@@ -9,6 +9,7 @@
  * ordered per-tile loops, and row-per-lane walks for large triangles.
  */
 #include "devmath.h"
+#include "libmf.h"
 #include "mop_hip_internal.h"
 
 #define WG 256
@@ -1278,10 +1279,10 @@ DEV void ibl_sample(const float *data, int w, int h, float u, float v, float out
 
 /* dir_to_uv, rasterizer.c:210-216 */
 DEV void dir_to_uv(float dx, float dy, float dz, float rot, float &u, float &v) {
-  float phi = atan2f(dz, dx) + rot;
+  float phi = mop_atan2f(dz, dx) + rot;
   float cdy = dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy);
   u = phi / (2.0f * 3.14159265f) + 0.5f;
-  v = asinf(cdy) / 3.14159265f + 0.5f;
+  v = mop_asinf(cdy) / 3.14159265f + 0.5f;
 }
 
 DEV void ibl_irradiance(const ViewGPU &fa, V3 n, float out[3]) {
@@ -1360,33 +1361,11 @@ DEV void ibl_brdf(const ViewGPU &fa, float ndotv, float roughness, float &scale,
 DEV V3 light_pos(const MopHipLight &l) { return V3{l.position.x, l.position.y, l.position.z}; }
 DEV V3 light_dir(const MopHipLight &l) { return V3{l.direction.x, l.direction.y, l.direction.z}; }
 
-/* Well-conditioned arithmetic of the light loop, which only feeds colour
- * (within 1/255 on the 8-bit image, 1e-4 on the HDR buffer): hardware
- * reciprocal and square root (1 ulp) instead of the correctly rounded
- * sequences, and contraction of multiply-adds.  Depth, coverage, the shadow
- * lookup position, every value that selects a table entry and the
- * ill-conditioned GGX terms keep the reference's IEEE operations. */
-DEV float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-DEV float fast_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-DEV V3 fast_norm(V3 v) {
-  float l2 = v.x * v.x + v.y * v.y + v.z * v.z;
-  if (!(l2 >= 1e-16f))
-    return V3{0.0f, 0.0f, 0.0f};
-  float inv = __builtin_amdgcn_rsqf(l2);
-  return V3{v.x * inv, v.y * inv, v.z * inv};
-}
-
-/* 1 - N.H^2 (1 - alpha^2) of the GGX distribution: it cancels to about
- * alpha^2 at the highlight, so it, the half vector and the light direction
- * keep the reference's IEEE operations (no contraction, correctly rounded
- * division and square root). */
-DEV float ggx_denom(float ndoth2, float alpha2) { return ndoth2 * (alpha2 - 1.0f) + 1.0f; }
-
 /* distance attenuation shared by point/spot (rasterizer.c:1625-1635) */
 DEV float atten(const MopHipLight &l, float dist) {
   float a;
   if (l.range > 0.0f) {
-    float r = fast_div(dist, l.range);
+    float r = dist / l.range;
     a = 1.0f - r;
     if (a < 0.0f)
       a = 0.0f;
@@ -1395,7 +1374,7 @@ DEV float atten(const MopHipLight &l, float dist) {
     float d2 = dist * dist;
     if (d2 < 0.01f)
       d2 = 0.01f;
-    a = fast_rcp(d2);
+    a = 1.0f / d2;
   }
   return a;
 }
@@ -1407,11 +1386,12 @@ struct ShadeOut {
 /* compute_multi_light_rgb (rasterizer.c:1596-1688) fused with
  * compute_multi_specular_ggx (rasterizer.c:1701-1814): the two loops touch the
  * same lights in the same order and accumulate into separate sums, so one
- * loop with the PCF lookup shared gives the same values. */
+ * loop with the PCF lookup shared gives the same values.  Every operation is
+ * the reference's, in its order, IEEE-rounded and unfused, so the sums come
+ * out bit-identical. */
 DEV void light_loop(const MopHipLight *L, uint32_t count, const ViewGPU &fa, V3 normal, V3 wp,
                     V3 view_dir, float ndotv, float alpha2, float k, float g1v, float f0_r,
                     float f0_g, float f0_b, float lit[3], float spec[3]) {
-#pragma clang fp contract(fast)
   for (uint32_t i = 0; i < count; i++) {
     const MopHipLight &l = L[i];
     if (!l.active)
@@ -1433,14 +1413,14 @@ DEV void light_loop(const MopHipLight *L, uint32_t count, const ViewGPU &fa, V3 
       V3 dir = v3scale(to_light, 1.0f / dist);
       ndotl_d = v3dot(normal, dir);
       if (l.type == MOP_HIP_LIGHT_SPOT) {
-        V3 spot_dir = fast_norm(light_dir(l));
+        V3 spot_dir = v3norm(light_dir(l));
         float cos_angle = -v3dot(dir, spot_dir);
         if (cos_angle < l.spot_outer_cos) {
           spot = 0.0f;
         } else if (cos_angle < l.spot_inner_cos) {
           float range_s = l.spot_inner_cos - l.spot_outer_cos;
           if (range_s > 1e-6f) {
-            float t = fast_div(cos_angle - l.spot_outer_cos, range_s);
+            float t = (cos_angle - l.spot_outer_cos) / range_s;
             spot = t * t * (3.0f - 2.0f * t);
           }
         }
@@ -1471,9 +1451,9 @@ DEV void light_loop(const MopHipLight *L, uint32_t count, const ViewGPU &fa, V3 
     if (vdoth < 0.0f)
       vdoth = 0.0f;
     float ndoth2 = ndoth * ndoth;
-    float denom_d = ggx_denom(ndoth2, alpha2);
-    float D = fast_div(alpha2, 3.14159265358979323846f * denom_d * denom_d);
-    float g1l = fast_div(ndotl, ndotl * (1.0f - k) + k);
+    float denom_d = ndoth2 * (alpha2 - 1.0f) + 1.0f;
+    float D = alpha2 / (3.14159265358979323846f * denom_d * denom_d);
+    float g1l = ndotl / (ndotl * (1.0f - k) + k);
     float G = g1v * g1l;
     float omv = 1.0f - vdoth;
     float omv2 = omv * omv;
@@ -1484,7 +1464,7 @@ DEV void light_loop(const MopHipLight *L, uint32_t count, const ViewGPU &fa, V3 
     float spec_denom = 4.0f * ndotl * ndotv;
     if (spec_denom < 1e-6f)
       spec_denom = 1e-6f;
-    float spec_term = fast_div(D * G, spec_denom);
+    float spec_term = D * G / spec_denom;
     float weight = spec_term * l.intensity * att_s * ndotl * shadow *
                    3.14159265358979323846f;
     spec[0] += weight * fr_r * l.color.r;
@@ -1496,7 +1476,6 @@ DEV void light_loop(const MopHipLight *L, uint32_t count, const ViewGPU &fa, V3 
 DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d, V3 normal,
                      V3 wp, V3 view_dir,
                      float base_r, float base_g, float base_b, float lit[3], float spec[3]) {
-#pragma clang fp contract(fast)
   lit[0] = lit[1] = lit[2] = d.ambient;
   spec[0] = spec[1] = spec[2] = 0.0f;
   float roughness = d.roughness, metallic = d.metallic;
@@ -1511,7 +1490,7 @@ DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d
   float f0_g = 0.04f * (1.0f - metallic) + base_g * metallic;
   float f0_b = 0.04f * (1.0f - metallic) + base_b * metallic;
   float k = alpha * 0.5f;
-  float g1v = fast_div(ndotv, ndotv * (1.0f - k) + k);
+  float g1v = ndotv / (ndotv * (1.0f - k) + k);
   /* the draws of one view share one light list: when every lane of the wave
    * has the same list, its address is wave-uniform and the light records are
    * read with scalar loads */
@@ -2155,32 +2134,6 @@ DEV float luma_px(uint32_t c) {
 
 DEV float srgb_enc(float c) {
   return c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
-}
-
-/* The colour effects of mop_postprocess_apply on one channel value when the
- * result depends on that value alone (no fog, no vignette): tonemap, gamma,
- * clamp, round (postprocess.c:45-116). */
-DEV uint8_t post_channel(uint8_t o, bool tonemap, bool gamma) {
-  float r = (float)o / 255.0f;
-  if (tonemap)
-    r = r / (r + 1.0f);
-  if (gamma)
-    r = srgb_enc(r > 0.0f ? r : 0.0f);
-  if (r > 1.0f)
-    r = 1.0f;
-  if (r < 0.0f)
-    r = 0.0f;
-  return cvt_u8(r * 255.0f + 0.5f);
-}
-
-/* 4 x 256 table: [tonemap | gamma << 1][value] */
-__global__ void k_post_lut(uint8_t *lut) {
-  lut[blockIdx.x * 256 + threadIdx.x] =
-      post_channel((uint8_t)threadIdx.x, (blockIdx.x & 1u) != 0, (blockIdx.x & 2u) != 0);
-}
-
-void launch_post_lut(uint8_t *lut, hipStream_t s) {
-  launch(k_post_lut, dim3(4), dim3(256), 0, s, lut);
 }
 
 /* The frame tail of one view tile: mop_sw_hdr_resolve, then (when post

@@ -350,7 +350,6 @@ void launch_clear(int npx, const float clear[4], uint8_t *color, float *hdr,
 void launch_vis_from_depth(int npx, const float *depth,
                            unsigned long long *vis, hipStream_t s);
 /* resolve (mode 0) or post (mode 1) over the tiles of n views */
-void launch_post_lut(uint8_t *lut, hipStream_t s);
 void launch_tail_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
                        hipStream_t s);
 void launch_upload(const void *src, void *dst, size_t n, hipStream_t s);

@@ -302,6 +302,7 @@ struct MopHipDevice {
   DevArr<PostViewGPU> pviews;
   DevArr<ClearViewGPU> clear_views;
   DevArr<uint8_t> post_lut; /* 4 x 256 colour-effect tables */
+  DevArr<float> ibl_trig;   /* sine/cosine table of the IBL precompute */
   /* shadow: the map the hook API renders into, and the recorded casters */
   MopHipShadowMap *def_shadow = nullptr;
   MopHipShadowMap *rec_shadow = nullptr; /* map the recorded casters target */
@@ -337,6 +338,31 @@ struct MopHipDevice {
  * re-runs caused by scratch overflow in this process. */
 static uint32_t g_force_clip = 0, g_force_chk = 0, g_force_bins = 0;
 static std::atomic<uint64_t> g_overflow_reruns{0};
+
+/* The per-channel colour-effect tables of mop_postprocess_apply without fog
+ * or vignette (postprocess.c:45-116): [tonemap | gamma << 1][value].  The
+ * sRGB curve calls powf, so the tables are evaluated here with the host C
+ * library, the one the reference calls (through a pointer, so nothing is
+ * folded at build time with a different rounding). */
+static float (*volatile host_powf)(float, float) = powf;
+
+static void post_lut_fill(uint8_t *lut) {
+  for (int variant = 0; variant < 4; variant++)
+    for (int o = 0; o < 256; o++) {
+      float r = (float)o / 255.0f;
+      if (variant & 1)
+        r = r / (r + 1.0f);
+      if (variant & 2) {
+        float c = r > 0.0f ? r : 0.0f;
+        r = c <= 0.0031308f ? c * 12.92f : 1.055f * host_powf(c, 1.0f / 2.4f) - 0.055f;
+      }
+      if (r > 1.0f)
+        r = 1.0f;
+      if (r < 0.0f)
+        r = 0.0f;
+      lut[variant * 256 + o] = (uint8_t)(r * 255.0f + 0.5f);
+    }
+}
 
 extern "C" MopHipDevice *mop_hip_device_create(void) {
   int n = 0;
@@ -385,8 +411,11 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   d->force_clip = g_force_clip;
   d->force_chk = g_force_chk;
   d->force_bins = g_force_bins;
-  if (d->post_lut.ensure(4 * 256))
-    launch_post_lut(d->post_lut.p, d->stream);
+  if (d->post_lut.ensure(4 * 256)) {
+    uint8_t lut[4 * 256];
+    post_lut_fill(lut);
+    HIPCHK(hipMemcpy(d->post_lut.p, lut, sizeof(lut), hipMemcpyHostToDevice));
+  }
   return d;
 }
 
@@ -434,6 +463,7 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->morph_targets.release();
   d->morph_weights.release();
   d->post_lut.release();
+  d->ibl_trig.release();
   mop_hip_shadow_map_destroy(d, d->def_shadow);
   mop_hip_ibl_destroy(d, d->def_ibl);
   for (MopHipBuffer *b : d->shadow_temp) {
@@ -2383,8 +2413,10 @@ extern "C" void mop_hip_ibl_clear(MopHipDevice *d) {
 }
 
 void launch_ibl_precompute(const float *env, int ew, int eh, float *irr, float *pf, float *lut,
-                           hipStream_t s);
+                           const float *trig, hipStream_t s);
 size_t ibl_pf_texels(void);
+size_t ibl_trig_floats(void);
+void ibl_trig_fill(float *tab);
 
 extern "C" MopHipIbl *mop_hip_ibl_create(MopHipDevice *d, const float *env_rgba, int w, int h) {
   if (!d || !env_rgba || w < 2 || h < 2)
@@ -2400,13 +2432,22 @@ extern "C" MopHipIbl *mop_hip_ibl_create(MopHipDevice *d, const float *env_rgba,
   DevArr<float> env;
   bool ok = env.ensure(n_env) && ibl->irr.ensure(64 * 32 * 4) &&
             ibl->pf.ensure(ibl_pf_texels() * 4) && ibl->lut.ensure(256 * 256 * 4);
+  if (ok && !d->ibl_trig.p) {
+    /* the precompute's sine/cosine arguments, evaluated once per device */
+    std::vector<float> tab(ibl_trig_floats());
+    ibl_trig_fill(tab.data());
+    ok = d->ibl_trig.ensure(tab.size()) &&
+         hipMemcpy(d->ibl_trig.p, tab.data(), tab.size() * sizeof(float),
+                   hipMemcpyHostToDevice) == hipSuccess;
+  }
   if (!ok) {
     env.release();
     mop_hip_ibl_destroy(d, ibl);
     return nullptr;
   }
   HIPCHK(hipMemcpy(env.p, env_rgba, n_env * sizeof(float), hipMemcpyHostToDevice));
-  launch_ibl_precompute(env.p, w, h, ibl->irr.p, ibl->pf.p, ibl->lut.p, d->stream);
+  launch_ibl_precompute(env.p, w, h, ibl->irr.p, ibl->pf.p, ibl->lut.p, d->ibl_trig.p,
+                        d->stream);
   HIPCHK(hipStreamSynchronize(d->stream));
   env.release();
   ibl->has_irr = ibl->has_pf = ibl->has_lut = true;
