@@ -5,7 +5,9 @@ cpu_backend.c:588-620).
 
 Bar (north star): depth and object-ID bit-exact; RGBA8 within 1/255 per
 channel; linear HDR within 1e-4 relative (|a-b| <= 1e-4 * max(|ref|, 1e-6)).
-The number of differing pixels is printed for every case.
+This backend meets the stronger bar the tests assert: every buffer
+bit-identical (refcore.assert_parity).  The number of differing pixels is
+printed for every case.
 """
 import pytest
 

@@ -106,6 +106,5 @@ def test_skinned_meshes_drop_in_parity(morph):
     hip = refcore.render(sc, 2)
     cmp = refcore.compare(hip, ref)
     print(f"[parity] {sc.name}: {cmp}", flush=True)
-    assert cmp["id_diff_px"] == 0 and cmp["depth_diff_px"] == 0
-    assert cmp["color_max_diff"] <= 1 and cmp["hdr_max_rel"] <= refcore.HDR_REL_TOL
+    refcore.assert_parity(cmp)
     assert (ref["ids"] >= 400).any()
