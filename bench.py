@@ -3,23 +3,33 @@
 
 Workload (one step = one frame): the 1920x1080 PBR scene -- 256 icospheres of
 subdivision 4 (1,310,720 triangles), 1 shadowed directional light (2048^2
-shadow map), 2 point + 1 spot light, IBL from a 2048x1024 HDR sky -- rendered
-through the HIP backend's frame driver: clear, shadow pass, transform /
-cull / clip / setup, raster + depth/ID, shading, ACES resolve, FXAA + sRGB.
-Inputs (meshes, IBL tables) are resident in HBM before the timed region;
-outputs stay in HBM (the PCIe-inclusive rate with RGBA8+depth+ID readback is
-reported separately as `fps_with_readback`).
+shadow map re-rendered every frame), 2 point + 1 spot light, IBL from a
+2048x1024 HDR sky -- rendered through the HIP backend's frame driver: clear,
+shadow pass, transform / cull / clip / setup, raster + depth/ID, shading,
+ACES resolve, FXAA + sRGB, then RGBA8, depth and object ID copied to host
+memory.  `value` counts frames whose three buffers are on the host: frames
+are rendered into two framebuffers in turn, each one's copy overlapping the
+next frame's kernels.  Inputs (meshes, IBL tables) are resident in HBM before
+the timed region; the rate with outputs left in HBM is `fps_hbm_resident`.
+
+Also measured in the same run: the 3840x2160 raster-stress scene (8.7M
+triangles, camera inside the terrain, 4096^2 shadow map), the 64-viewport
+batched form, the serving form, and the reference CPU backend on this host's
+cores (single-threaded and its thread pool).
 
 Prints ONE JSON line (driver contract) with `roofline` for the dominant
 kernel (timed live with HIP events on the backend's stream) and
 `cpu_baseline`: the reference CPU backend (compiled from the reference
 sources into oracle/_ref) rendering the same scene on this host's cores.
+Every timed region ends with a check of the device's scratch-overflow word:
+a frame that overflowed would have been incomplete, so the run fails.
 """
 import argparse
 import ctypes as C
 import importlib.util
 import json
 import os
+import platform
 import subprocess
 import sys
 import time
@@ -36,8 +46,10 @@ def log(msg):
     sys.stderr.flush()
 
 
-HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # non-packed FP32 lane-ops/s, TOP/s
+HBM_PEAK_GBS = 8000.0                      # MI355X HBM3E, GB/s
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12   # non-packed FP32 lane-ops/s (39.3 T)
+FB_BYTES_PER_PX = 28                       # RGBA8 + RGBA32F + f32 depth + u32 id
+READBACK_MASK = 1 | 2 | 4                  # RGBA8, depth, object ID
 
 
 def load_pkg():
@@ -49,61 +61,309 @@ def load_pkg():
     return mod
 
 
-def make_scene_driver(lib, dev, sc):
-    from mop_amd import abi
-    s = lib.mop_hip_scene_create(dev)
-    if not s:
-        raise RuntimeError("scene create failed: " + lib.mop_hip_last_error().decode())
+def check_overflow(dev, where):
+    """Every batch executed since the last check must have been complete."""
+    flags = dev.overflow_flags(reset=True)
+    if flags:
+        raise RuntimeError(f"{where}: scratch overflow flags {flags:#x} in a timed region")
+    return flags
+
+
+def host_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0)),
+            "cpu_model": model, "smt_active": smt}
+
+
+# ---------------------------------------------------------------------------
+# configuration 2: the headline
+
+
+def frame_bytes(sc):
+    """Algorithmic HBM bytes of one frame (vertex and index buffers read,
+    the 28 B/px framebuffer, the shadow map written and read, the IBL
+    tables), as the measurement plan counts them."""
+    vb = sum(len(m.vertices) * 48 for m in sc.meshes)
+    ib = sum(len(m.indices) * 4 for m in sc.meshes)
+    fb = sc.width * sc.height * FB_BYTES_PER_PX
+    sm = 2 * sc.shadow_size * sc.shadow_size * 4 if any(L.cast_shadows for L in sc.lights) else 0
+    ibl = (64 * 32 + (128 * 64 + 64 * 32 + 32 * 16 + 16 * 8 + 8 * 4) + 256 * 256) * 16 \
+        if sc.env is not None else 0
+    return vb + ib + fb + sm + ibl
+
+
+def profiled(summary, key, kernel):
+    """A per-launch figure (`<key> <kernel>: <value>` line) from a committed
+    rocprofv3 counter summary under profiles/, or None."""
+    path = os.path.join(REPO, "profiles", summary)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"{key} {kernel}:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def pipelined_frames(dev, drv, fbs, cam, steps):
+    """`steps` frames rendered into the framebuffers in turn, each copied to
+    the host (RGBA8, depth, id) behind the next frame's kernels; returns
+    (seconds, triangles submitted, host seconds spent recording and
+    launching).  The clock stops when the last frame's buffers are on the
+    host."""
+    lib = dev.lib
+    iw, ih = C.c_int(), C.c_int()
+
+    def read(fb):
+        lib.mop_hip_framebuffer_read_color(dev.dev, fb.fb, C.byref(iw), C.byref(ih))
+        lib.mop_hip_framebuffer_read_depth(dev.dev, fb.fb, C.byref(iw), C.byref(ih))
+        lib.mop_hip_framebuffer_read_object_id(dev.dev, fb.fb, C.byref(iw), C.byref(ih))
+
+    tris = 0
+    host = 0.0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        fb = fbs[k % len(fbs)]
+        th = time.perf_counter()
+        tris += drv.render(fb, cam)
+        lib.mop_hip_framebuffer_readback_async(dev.dev, fb.fb, READBACK_MASK)
+        host += time.perf_counter() - th
+        if k >= 1:
+            read(fbs[(k - 1) % len(fbs)])
+    read(fbs[(steps - 1) % len(fbs)])
+    return time.perf_counter() - t0, tris, host
+
+
+def config2_form(args):
     import numpy as np
-    for m in sc.meshes:
-        v = np.ascontiguousarray(m.vertices, np.float32)
-        i = np.ascontiguousarray(m.indices, np.uint32)
-        mm = None if m.model is None else np.ascontiguousarray(m.model, np.float32)
-        rc = lib.mop_hip_scene_add_mesh(s, v.ctypes.data, len(v), i.ctypes.data, len(i),
-                                        m.object_id, None if mm is None else mm.ctypes.data,
-                                        int(m.has_material), m.metallic, m.roughness)
-        if rc < 0:
-            raise RuntimeError("add_mesh failed: " + lib.mop_hip_last_error().decode())
-    for L in sc.lights:
-        l = abi.Light()
-        l.type = L.type
-        l.position = abi.Vec3(*L.position)
-        l.direction = abi.Vec3(*L.direction)
-        l.color = abi.Color(*L.color)
-        l.intensity = L.intensity
-        l.range = L.range
-        l.spot_inner_cos = L.inner_cos
-        l.spot_outer_cos = L.outer_cos
-        l.active = True
-        l.cast_shadows = L.cast_shadows
-        lib.mop_hip_scene_add_light(s, C.byref(l))
-    if sc.env is not None:
-        env = np.ascontiguousarray(sc.env, np.float32)
-        rc = lib.mop_hip_scene_set_environment(
-            s, env.ctypes.data_as(C.POINTER(C.c_float)), env.shape[1], env.shape[0],
-            sc.env_rotation, sc.env_intensity)
-        if rc != 0:
-            raise RuntimeError("set_environment failed")
-    lib.mop_hip_scene_set_shadow_size(s, sc.shadow_size)
-    lib.mop_hip_scene_set_post_effects(s, sc.post_effects)
-    cam = abi.Camera(abi.Vec3(*sc.eye), abi.Vec3(*sc.target), abi.Vec3(*sc.up), sc.fov,
-                     sc.znear, sc.zfar)
-    return s, cam
+    from mop_amd import driver, scenes
+    dev = driver.Device()
+    lib = dev.lib
+    sc = scenes.config2()
+    W, H = sc.width, sc.height
+    log("config 2 scene generated")
+    drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
+    fbs = [dev.framebuffer(W, H) for _ in range(2)]
+    cam = drv.cam
+    dev.sync()
+    log("scene resident (meshes, IBL tables)")
+    for k in range(args.warmup):
+        drv.render(fbs[k % 2], cam)
+        dev.sync()
+        log(f"warm-up frame {k} done")
+    check_overflow(dev, "config 2 warm-up")
+
+    # headline: frames with RGBA8 + depth + id on the host
+    secs, tris, host_s = pipelined_frames(dev, drv, fbs, cam, args.steps)
+    check_overflow(dev, "config 2 timed frames")
+    fps = args.steps / secs
+    ms_per_step = secs * 1e3 / args.steps
+    mtri = tris / secs / 1e6
+    log(f"timed: {args.steps} frames with readback, {ms_per_step:.3f} ms/frame "
+        f"({fps:.1f} frames/s, host record + launch {host_s * 1e3 / args.steps:.3f} ms/frame)")
+
+    # the same frames with outputs left in HBM
+    dev.sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        drv.render(fbs[k % 2], cam)
+    dev.sync()
+    hbm_s = time.perf_counter() - t0
+    check_overflow(dev, "config 2 HBM-resident frames")
+    fps_hbm = args.steps / hbm_s
+    log(f"outputs in HBM: {hbm_s * 1e3 / args.steps:.3f} ms/frame ({fps_hbm:.1f} frames/s)")
+
+    # synchronous readback, one frame at a time (no overlap)
+    n_sync = max(5, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(n_sync):
+        drv.render(fbs[0], cam)
+        fbs[0].read(color=True, depth=True, ids=True)
+    fps_sync = n_sync / (time.perf_counter() - t0)
+    check_overflow(dev, "config 2 synchronous readback")
+
+    # host cost of one frame with the device idle, device span of one frame
+    idle_host, dev_span = [], []
+    for _ in range(10):
+        dev.sync()
+        th = time.perf_counter()
+        drv.render(fbs[0], cam)
+        idle_host.append(time.perf_counter() - th)
+        dev.sync()
+        dev_span.append(dev.frame_stats()["gpu_ms"])
+    idle_host_ms = float(np.median(idle_host)) * 1e3
+    dev_span_ms = float(np.median(dev_span))
+    log(f"one frame alone: host {idle_host_ms:.3f} ms, device span {dev_span_ms:.3f} ms")
+    rep, cap = C.c_uint64(), C.c_uint64()
+    lib.mop_hip_graph_stats(dev.dev, C.byref(rep), C.byref(cap))
+
+    # stage times: HIP events around every stage on the backend's stream
+    dev.timers(True)
+    n_st = max(10, args.steps // 2)
+    for k in range(n_st):
+        drv.render(fbs[k % 2], cam)
+    dev.sync()
+    stages = {}
+    for name in ("shadow", "vertex", "setup", "raster", "shade", "resolve", "post"):
+        tot, cnt = dev.timer(name)
+        if cnt:
+            stages[name] = round(tot / cnt, 4)
+    dev.timers(False)
+    check_overflow(dev, "config 2 stage-timed frames")
+    log(f"stage ms over {n_st} frames: {stages}")
+
+    # the timed frames drew the scene: read the last one back
+    out = fbs[0].read(color=False, depth=False, ids=True)
+    st = dev.frame_stats()
+    ids = out["ids"]
+    visible_objects = int(len(np.unique(ids[ids != 0])))
+    covered = float((ids != 0).mean())
+    log(f"check: {visible_objects} objects visible, {covered:.3f} of pixels covered, "
+        f"{st['subtris']} screen triangles")
+    if visible_objects == 0 or st["subtris"] == 0:
+        raise RuntimeError("timed frames drew nothing: " + lib.mop_hip_last_error().decode())
+
+    # roofline of the dominant kernel, the shading pass (HIP events on the
+    # backend's stream).  HBM: the 28 B/px framebuffer of the frame over the
+    # kernel's launch time.  VALU: the kernel's issued VALU instructions per
+    # launch (rocprofv3 SQ_INSTS_VALU, committed summary) x 64 lanes over the
+    # same time, against 39.3 T lane-ops/s.  `bound` is the larger fraction.
+    shade_ms = stages.get("shade")
+    roof = None
+    if shade_ms:
+        npx = W * H
+        algo = FB_BYTES_PER_PX * npx
+        hbm_ach = algo / (shade_ms * 1e-3) / 1e9
+        summ = "frame_1080p_pbr_counters.md"
+        traffic = profiled(summ, "traffic_bytes_per_launch", "k_shade")
+        insts = profiled(summ, "valu_insts_per_launch", "k_shade")
+        valu = None
+        if insts:
+            tops = insts * 64 / (shade_ms * 1e-3) / 1e12
+            valu = {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                    "unit": "TOP/s", "frac": round(tops / VALU_PEAK_TOPS, 5),
+                    "valu_insts_per_launch": insts}
+        hbm = {"bound": "hbm", "achieved": round(hbm_ach, 2), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 5),
+               "traffic": int(traffic) if traffic else None}
+        if valu and valu["frac"] > hbm["frac"]:
+            roof = {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
+                    "unit": "TOP/s", "frac": valu["frac"],
+                    "traffic": hbm["traffic"], "hbm": hbm, "valu": valu}
+        else:
+            roof = dict(hbm)
+            roof["valu"] = valu
+        roof.update({"kernel": "k_shade", "launch_ms": shade_ms,
+                     "algorithmic_bytes_per_launch": algo,
+                     "bytes_per_px": FB_BYTES_PER_PX})
+    fbytes = frame_bytes(sc)
+    frame_roof = {"algorithmic_bytes": fbytes, "device_ms": round(dev_span_ms, 4),
+                  "achieved_gbs": round(fbytes / (dev_span_ms * 1e-3) / 1e9, 2),
+                  "frac": round(fbytes / (dev_span_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+    for fb in fbs:
+        fb.close()
+    drv.close()
+    dev.close()
+    return {
+        "fps": fps, "ms_per_step": ms_per_step, "mtri": mtri, "sc": sc,
+        "extra": {
+            "mtri_per_s": round(mtri, 3),
+            "fps_hbm_resident": round(fps_hbm, 3),
+            "mtri_per_s_hbm_resident": round(tris / hbm_s / 1e6, 3),
+            "fps_synchronous_readback": round(fps_sync, 3),
+            "readback": "RGBA8 + depth + object ID to pinned host memory per frame, "
+                        "copy of frame k overlapping frame k+1 (two framebuffers)",
+            "host_ms_per_frame": round(host_s * 1e3 / args.steps, 4),
+            "host_ms_one_frame_idle_device": round(idle_host_ms, 4),
+            "device_ms_one_frame": round(dev_span_ms, 4),
+            "graph_replays": rep.value,
+            "graph_captures": cap.value,
+            "stage_ms": stages,
+            "overflow_flags": 0,
+            "frame_stats": {"subtris": st["subtris"], "clipped": st["clipped"],
+                            "big_subtris": st["big_subtris"], "shadow_tris": st["shadow_tris"],
+                            "visible_objects": visible_objects,
+                            "covered_pixel_fraction": round(covered, 4)},
+            "frame_roofline": frame_roof,
+        },
+        "roofline": roof,
+    }
 
 
-def cpu_baseline(seconds, cores=1):
-    """Reference CPU backend on the same scene, in a child process pinned to
-    `cores` host CPUs, through its single-threaded draw path (the path
-    cpu_draw takes without a thread pool, cpu_backend.c:588-620).  The
-    shipped pool (cores - 1 workers, rasterizer_mt.c:347-394) was measured
-    on the 16-core GPU host at 70 s for the 800x600 one-sphere frame (its
-    idle workers keep re-taking the finished work item under the pool mutex
-    while the caller waits to clear it), so it is not a usable baseline
-    there; on an 8-core development host it rendered this scene 12% faster
-    than the single-threaded path.  Returns a dict or None."""
-    harness = os.path.join(REPO, "oracle", "_ref", "ref_harness.so")
-    if not os.path.exists(harness):
-        return None
+# ---------------------------------------------------------------------------
+# configuration 3: 4K raster stress
+
+
+def config3_form(frames):
+    """The 3840x2160 raster-stress scene: 2048x2048-quad displaced terrain +
+    64 icospheres (8,716,288 triangles), camera inside the terrain, one
+    shadowed sun with a 4096^2 map re-rendered every frame."""
+    from mop_amd import driver, scenes
+    dev = driver.Device()
+    sc = scenes.config3()
+    log(f"config 3: scene generated ({sc.triangle_count} triangles)")
+    drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
+    fbs = [dev.framebuffer(sc.width, sc.height) for _ in range(2)]
+    for k in range(2):
+        drv.render(fbs[k], drv.cam)
+    dev.sync()
+    check_overflow(dev, "config 3 warm-up")
+    secs, tris, _ = pipelined_frames(dev, drv, fbs, drv.cam, frames)
+    check_overflow(dev, "config 3 timed frames")
+    dev.sync()
+    t0 = time.perf_counter()
+    for k in range(frames):
+        drv.render(fbs[k % 2], drv.cam)
+    dev.sync()
+    hbm_s = time.perf_counter() - t0
+    check_overflow(dev, "config 3 HBM-resident frames")
+    drv.render(fbs[0], drv.cam)
+    dev.sync()
+    dev_ms = dev.frame_stats()["gpu_ms"]
+    fbs[0].read(color=False, depth=False, ids=True)
+    st = dev.frame_stats()
+    out = {"width": sc.width, "height": sc.height, "triangles": sc.triangle_count,
+           "shadow_map": sc.shadow_size, "frames": frames,
+           "frames_per_s": round(frames / secs, 3),
+           "mtri_per_s": round(tris / secs / 1e6, 2),
+           "frames_per_s_hbm_resident": round(frames / hbm_s, 3),
+           "mtri_per_s_hbm_resident": round(tris / hbm_s / 1e6, 2),
+           "device_ms_one_frame": round(dev_ms, 4),
+           "clipped": st["clipped"], "screen_triangles": st["subtris"],
+           "big_screen_triangles": st["big_subtris"], "overflow_flags": 0}
+    log(f"config 3: {out}")
+    for fb in fbs:
+        fb.close()
+    drv.close()
+    dev.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# reference CPU baseline
+
+
+def _cpu_child(seconds, cores, mode, timeout):
+    """Reference CPU backend (oracle/_ref) rendering configuration 2 with its
+    fixed 1024^2 shadow map in a child process pinned to `cores` host CPUs:
+    mode 1 = its single-threaded draw path, mode 0 = its thread pool (cores -
+    1 workers plus the calling thread, cpu_backend.c:60-71).  Returns a dict
+    or None (failed or timed out)."""
     code = f"""
 import os, sys, json, time
 os.sched_setaffinity(0, set(sorted(os.sched_getaffinity(0))[:{cores}]))
@@ -111,15 +371,15 @@ sys.path.insert(0, {os.path.join(REPO, 'tests')!r})
 import conftest, refcore
 from mop_amd import scenes
 sc = scenes.config2(shadow_size=1024)
-vp = refcore.RefViewport(sc, 1)
+vp = refcore.RefViewport(sc, {mode})
 sys.stderr.write("[cpu baseline] scene loaded\\n"); sys.stderr.flush()
 vp.render()                      # warm-up frame
 times = []
 t_end = time.time() + {seconds}
 while True:
     times.append(vp.render())
-    sys.stderr.write(f"[cpu baseline] frame {{len(times)}}: {{times[-1]:.1f}} ms\\n"); sys.stderr.flush()
-    if time.time() > t_end and len(times) >= 1:
+    sys.stderr.write(f"[cpu baseline mode {mode}, {cores} cores] frame {{len(times)}}: {{times[-1]:.1f}} ms\\n"); sys.stderr.flush()
+    if time.time() > t_end and len(times) >= 3:
         break
 tris = vp.triangle_count()
 times.sort()
@@ -127,23 +387,58 @@ print(json.dumps({{"frames": len(times), "median_ms": times[len(times)//2], "tri
 """
     try:
         out = subprocess.run([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True,
-                             timeout=seconds * 6 + 240, cwd=REPO)
+                             timeout=timeout, cwd=REPO)
         line = [l for l in out.stdout.splitlines() if l.startswith("{")]
-        if not line:
-            return None
-        r = json.loads(line[-1])
-    except Exception as e:  # baseline is reported, never required
-        sys.stderr.write(f"cpu_baseline failed: {e}\n")
+        return json.loads(line[-1]) if line else None
+    except subprocess.TimeoutExpired:
+        log(f"cpu baseline: mode {mode} on {cores} cores did not finish in {timeout} s")
         return None
-    fps = 1000.0 / r["median_ms"]
-    return {"value": round(fps, 4), "unit": "frames/s", "cores": cores,
-            "kind": "reference",
-            "mtri_per_s": round(r["tris"] / (r["median_ms"] * 1e3), 3),
-            "sample": f"reference CPU backend (oracle/_ref, gcc -O2) through its "
-                      f"single-threaded draw path on {cores} pinned core(s), rendering the "
-                      f"same 1920x1080 PBR scene with a 1024^2 shadow map (the reference's "
-                      f"fixed size): median of {r['frames']} frames after 1 warm-up, "
-                      f"mop_viewport_render + read_color wall time"}
+    except Exception as e:  # the baseline is reported, never required
+        log(f"cpu baseline failed: {e}")
+        return None
+
+
+def cpu_baseline(seconds, pool_cores):
+    harness = os.path.join(REPO, "oracle", "_ref", "ref_harness.so")
+    if not os.path.exists(harness):
+        return None
+    info = host_info()
+    usable = info["usable_cpus"]
+    single = _cpu_child(seconds, 1, 1, seconds * 6 + 240)
+    runs = []
+    for c in pool_cores:
+        if c > usable:
+            continue
+        r = _cpu_child(seconds, c, 0, seconds * 3 + 120)
+        runs.append({"cores": c, "completed": r is not None,
+                     **({"frames_per_s": round(1000.0 / r["median_ms"], 4),
+                         "median_ms": round(r["median_ms"], 2), "frames": r["frames"]}
+                        if r else {})})
+        log(f"cpu baseline: thread pool on {c} cores: {runs[-1]}")
+    best = max((r for r in runs if r["completed"]), key=lambda r: r["frames_per_s"],
+               default=None)
+    tris = single["tris"] if single else None
+    one = None
+    if single:
+        one = {"value": round(1000.0 / single["median_ms"], 4), "unit": "frames/s",
+               "cores": 1, "median_ms": round(single["median_ms"], 2),
+               "frames": single["frames"]}
+    head = best or (dict(one, frames_per_s=one["value"]) if one else None)
+    if head is None:
+        return None
+    value = head["frames_per_s"]
+    return {"value": value, "unit": "frames/s", "cores": head["cores"], "kind": "reference",
+            "mtri_per_s": round(tris * value / 1e6, 3) if tris else None,
+            "sample": f"reference CPU backend (oracle/_ref, gcc -O2) rendering the same "
+                      f"1920x1080 PBR scene with its fixed 1024^2 shadow map, "
+                      f"{'its thread pool' if best else 'its single-threaded draw path'} on "
+                      f"{head['cores']} pinned core(s): median frame of a ~{seconds:.0f} s run "
+                      f"after 1 warm-up, mop_viewport_render + read_color wall time",
+            "single_thread": one, "thread_pool": runs, "host": info}
+
+
+# ---------------------------------------------------------------------------
+# batched and serving forms
 
 
 def batched_form(mod, rounds, tune=True):
@@ -152,23 +447,25 @@ def batched_form(mod, rounds, tune=True):
     batched = all 64 recorded, then executed in one set of launches.
     Throughput: a stream of batches, each recorded and launched while the
     previous one runs.  Latency: isolated batches (each finished before the
-    next is recorded), from submission of the batch to each frame's finish."""
+    next is recorded), from submission of the batch to each frame's finish.
+    The shadow maps are rendered once and kept (they do not depend on the
+    camera), in both the plain and batched modes."""
     import numpy as np
     from mop_amd import driver, scenes
     dev = driver.Device()
+    lib = dev.lib
     views = scenes.config4()
     log(f"batched: {len(views)} views generated")
     drv = [driver.SceneDriver(dev, sc, shadow_cache=True) for sc in views]
     log("batched: scenes resident")
     fbs = [dev.framebuffer(sc.width, sc.height) for sc in views]
     tris = sum(sc.triangle_count for sc in views)
-    # warm-up (also renders the cached shadow maps)
-    for d, fb in zip(drv, fbs):
+    for d, fb in zip(drv, fbs):  # warm-up (also renders the kept shadow maps)
         d.render(fb)
     dev.sync()
+    check_overflow(dev, "batched warm-up")
     log("batched: warm-up done")
 
-    # autotune the launch shapes on this batch (results never depend on them)
     from mop_amd import autotune
 
     def one_batch():
@@ -180,10 +477,11 @@ def batched_form(mod, rounds, tune=True):
     if tune:
         best, table = autotune.autotune(dev, one_batch)
         log(f"batched: autotuned (small_area, shade_tile, raster_grid) = {best}: "
-            f"{table[best]:.3f} ms (default {table.get((64, 16, 8192), float('nan')):.3f} ms)")
+            f"{table[best]:.3f} ms")
     else:
         best, table = (64, 16, 8192), {}
         autotune.set_tuning(dev, *best)
+    check_overflow(dev, "batched autotune")
     t0 = time.perf_counter()
     for r in range(rounds):
         for d, fb in zip(drv, fbs):
@@ -191,10 +489,11 @@ def batched_form(mod, rounds, tune=True):
             dev.sync()
         log(f"batched: plain round {r} done")
     plain_s = (time.perf_counter() - t0) / rounds
+    check_overflow(dev, "plain rounds")
+    rep0, cap0 = C.c_uint64(), C.c_uint64()
+    lib.mop_hip_graph_stats(dev.dev, C.byref(rep0), C.byref(cap0))
     lat = []
     rec_s, exe_s, gpu_ms = [], [], []
-    gpu_time = dev.lib.mop_hip_frame_gpu_time_ms
-    gpu_time.restype = C.c_float
     t0 = time.perf_counter()
     for r in range(rounds):
         tb = time.perf_counter()
@@ -214,13 +513,12 @@ def batched_form(mod, rounds, tune=True):
                 else:
                     still.append(k)
             pending = still
-        gpu_ms.append(gpu_time(dev.dev))
+        gpu_ms.append(dev.frame_stats()["gpu_ms"])
         log(f"batched: batched round {r} done: record {rec_s[-1]*1e3:.2f} ms, launch "
             f"{exe_s[-1]*1e3:.2f} ms, device span {gpu_ms[-1]:.2f} ms")
     batch_s = (time.perf_counter() - t0) / rounds
+    check_overflow(dev, "isolated batches")
     # streamed batches: batch r+1 is recorded and launched while batch r runs
-    # (the backend double-buffers its upload staging), as a renderer serving
-    # a continuous stream would; one wait at the end
     n_stream = max(rounds, 10)
     dev.sync()
     t0 = time.perf_counter()
@@ -230,10 +528,14 @@ def batched_form(mod, rounds, tune=True):
         dev.execute(fbs)
     dev.sync()
     stream_s = (time.perf_counter() - t0) / n_stream
-    log(f"batched: {n_stream} streamed batches, {stream_s * 1e3:.3f} ms per batch")
+    check_overflow(dev, "streamed batches")
+    rep1, cap1 = C.c_uint64(), C.c_uint64()
+    lib.mop_hip_graph_stats(dev.dev, C.byref(rep1), C.byref(cap1))
+    log(f"batched: {n_stream} streamed batches, {stream_s * 1e3:.3f} ms per batch; graph "
+        f"replays {rep1.value - rep0.value}, captures {cap1.value - cap0.value}")
     lat = np.array(lat) * 1e3
-    out = {"views": len(views), "triangles_per_batch": tris,
-           "pixels_per_batch": sum(sc.width * sc.height for sc in views),
+    px = sum(sc.width * sc.height for sc in views)
+    out = {"views": len(views), "triangles_per_batch": tris, "pixels_per_batch": px,
            "plain_frames_per_s": round(len(views) / plain_s, 2),
            "batched_frames_per_s": round(len(views) / stream_s, 2),
            "batched_over_plain": round(plain_s / stream_s, 3),
@@ -245,6 +547,17 @@ def batched_form(mod, rounds, tune=True):
            "host_record_ms": round(float(np.median(rec_s)) * 1e3, 3),
            "host_launch_ms": round(float(np.median(exe_s)) * 1e3, 3),
            "device_span_ms": round(float(np.median(gpu_ms)), 3),
+           "batch_roofline": {
+               "algorithmic_bytes": px * FB_BYTES_PER_PX + sum(
+                   len(m.vertices) * 48 + len(m.indices) * 4 for sc in views for m in sc.meshes),
+               "frac_of_hbm_peak_at_device_span": round(
+                   (px * FB_BYTES_PER_PX + sum(len(m.vertices) * 48 + len(m.indices) * 4
+                                               for sc in views for m in sc.meshes))
+                   / (float(np.median(gpu_ms)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "graph_replays": rep1.value - rep0.value,
+           "graph_captures": cap1.value - cap0.value,
+           "shadow_cache": True,
+           "overflow_flags": 0,
            "tuning": {"small_area": best[0], "shade_tile": best[1], "raster_grid": best[2],
                       "candidates_ms": {str(k): round(v, 3) for k, v in table.items()}}}
     for fb in fbs:
@@ -255,29 +568,15 @@ def batched_form(mod, rounds, tune=True):
     return out
 
 
-def profiled_traffic(summary, kernel):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 counter
-    summary under profiles/ (its `traffic_bytes_per_launch` line), or None."""
-    path = os.path.join(REPO, "profiles", summary)
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-                    return int(float(line.split(":")[1]))
-    except OSError:
-        pass
-    return None
-
-
-def serving_form(mod, rates=(100.0, 1000.0, 2500.0, 5000.0), duration=2.0, n_scenes=16,
+def serving_form(mod, rates=(100.0, 300.0, 1000.0, 3000.0, 5000.0), duration=8.0, n_scenes=16,
                  plain_rate=5000.0):
     """Serving form: 16 resident raster-stress scenes (2048x2048-quad
     displaced terrain, 8.4M triangles, plus 64 icospheres; one shadowed sun,
-    4096^2 shadow map rendered once and kept), Poisson request streams at
-    several rates, resolution mix 25% 640x480 / 50% 1280x720 / 25% 1920x1080,
-    continuous batching admitted by an HBM budget; p50/p99 from arrival to
-    RGBA8 on the host.  The plain mode serves the same stream one request per
-    launch."""
+    4096^2 shadow map rendered once and kept on the device), Poisson request
+    streams at several rates, resolution mix 25% 640x480 / 50% 1280x720 /
+    25% 1920x1080, continuous batching admitted by an HBM budget; p50/p99
+    from arrival to RGBA8 on the host.  The plain mode serves the same
+    stream one request per launch."""
     from mop_amd import driver, scenes, serve
     dev = driver.Device()
     pool = []
@@ -286,9 +585,9 @@ def serving_form(mod, rates=(100.0, 1000.0, 2500.0, 5000.0), duration=2.0, n_sce
         pool.append((driver.SceneDriver(dev, sc, shadow_cache=True), sc))
         log(f"serving: scene {k} resident ({sc.triangle_count} triangles)")
     srv = serve.Server(dev, pool, budget_bytes=16 << 30, max_batch=32, inflight=2)
-    # warm-up: every scene once per class (shadow maps, scratch, graphs)
     warm = [(0.0, si, 0.0, 0.0, c) for si in range(n_scenes) for c in range(3)]
     srv.run(warm, time_limit=60.0)
+    check_overflow(dev, "serving warm-up")
     log("serving: warm-up done")
 
     def clean(r):
@@ -296,15 +595,19 @@ def serving_form(mod, rates=(100.0, 1000.0, 2500.0, 5000.0), duration=2.0, n_sce
         return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
 
     out = {"scenes": n_scenes, "triangles_per_scene": pool[0][1].triangle_count,
-           "duration_s": duration, "batched": {}}
+           "duration_s": duration, "shadow_cache": True, "admission": srv.admission(),
+           "batched": {}}
     for rate in rates:
         reqs = serve.make_requests(rate, duration, n_scenes, seed=0x4D4F5005 + int(rate))
         out["batched"][str(int(rate))] = r = clean(srv.run(reqs, time_limit=duration * 3 + 5.0))
+        check_overflow(dev, f"serving at {rate:.0f} req/s")
         log(f"serving batched {rate:.0f} req/s: {r}")
     reqs = serve.make_requests(plain_rate, duration, n_scenes,
                                seed=0x4D4F5005 + int(plain_rate))
     out["plain"] = {str(int(plain_rate)): clean(srv.run(reqs, plain=True,
                                                         time_limit=duration * 3 + 5.0))}
+    check_overflow(dev, "serving plain")
+    out["overflow_flags"] = 0
     log(f"serving plain {plain_rate:.0f} req/s: {out['plain']}")
     srv.close()
     for d, _ in pool:
@@ -313,21 +616,28 @@ def serving_form(mod, rates=(100.0, 1000.0, 2500.0, 5000.0), duration=2.0, n_sce
     return out
 
 
+# ---------------------------------------------------------------------------
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-cores", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-pool-cores", type=str, default="4,8,16",
+                    help="core counts at which the reference thread pool is timed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--config3-frames", type=int, default=10)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 64-viewport batched form (for profiling)")
     ap.add_argument("--no-tune", action="store_true",
                     help="batched form with the default launch shapes")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--batch-rounds", type=int, default=8)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the "
@@ -349,186 +659,58 @@ def main():
         torch.cuda.synchronize()
         log(f"calibration copy of {n} bytes done")
         del a, b
+    mod = load_pkg()
+    torch.cuda.init()
     if args.batch_only:
-        mod = load_pkg()
-        torch.cuda.init()
         print(json.dumps({"batched_64_viewports": batched_form(mod, args.batch_rounds,
                                                               not args.no_tune)}))
         return
     if args.serve_only:
-        mod = load_pkg()
-        torch.cuda.init()
-        print(json.dumps({"serving": serving_form(mod)}))
+        print(json.dumps({"serving": serving_form(mod, duration=args.serve_seconds)}))
         return
-    mod = load_pkg()
-    from mop_amd import scenes, abi
-    lib = mod.lib()
-    torch.cuda.init()
-    dev = lib.mop_hip_device_create()
-    if not dev:
-        raise RuntimeError("HIP device unavailable: " + lib.mop_hip_last_error().decode())
 
-    sc = scenes.config2()
-    W, H = sc.width, sc.height
-    log("config 2 scene generated")
-    drv, cam = make_scene_driver(lib, dev, sc)
-    fb = lib.mop_hip_framebuffer_create(dev, C.byref(abi.FramebufferDesc(W, H)))
-    torch.cuda.synchronize()
-    log("scene resident (meshes, IBL tables)")
-
-    for k in range(args.warmup):
-        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
-        torch.cuda.synchronize()
-        log(f"warm-up frame {k} done")
-    tris = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    host_s = 0.0
-    for _ in range(args.steps):
-        th = time.perf_counter()
-        tris += lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
-        host_s += time.perf_counter() - th
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    host_ms = host_s * 1e3 / args.steps
-    log(f"timed: {args.steps} frames, {ms_per_step:.3f} ms/frame "
-        f"(host record + launch {host_ms:.3f} ms/frame)")
-    fps = 1e3 / ms_per_step
-    mtri = tris / args.steps / (ms_per_step * 1e3)
-
-    import numpy as np
-    # host cost of one frame with the device idle (no waits on earlier
-    # frames) and the device span of one frame (HIP events around the frame's
-    # launches on the backend's stream)
-    idle_host, dev_span = [], []
-    gpu_time = lib.mop_hip_frame_gpu_time_ms
-    gpu_time.restype = C.c_float
-    for _ in range(10):
-        torch.cuda.synchronize()
-        th = time.perf_counter()
-        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
-        idle_host.append(time.perf_counter() - th)
-        torch.cuda.synchronize()
-        dev_span.append(gpu_time(dev))
-    idle_host_ms = float(np.median(idle_host)) * 1e3
-    dev_span_ms = float(np.median(dev_span))
-    log(f"one frame alone: host {idle_host_ms:.3f} ms, device span {dev_span_ms:.3f} ms")
-
-    rep, cap = C.c_uint64(), C.c_uint64()
-    lib.mop_hip_graph_stats(dev, C.byref(rep), C.byref(cap))
-    log(f"graph replays {rep.value}, captures {cap.value}")
-
-    # stage times: the same frames with HIP events around every stage on the
-    # backend's stream (events force direct launches, so this is a separate
-    # timed region of the same workload)
-    lib.mop_hip_timers_enable(dev, 1)
-    n_st = max(10, args.steps // 2)
-    for _ in range(n_st):
-        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
-    torch.cuda.synchronize()
-    stages = {}
-    for name in ("shadow", "vertex", "setup", "raster", "shade", "resolve", "post"):
-        tot, cnt = C.c_double(), C.c_uint64()
-        if lib.mop_hip_timers_read(dev, name.encode(), C.byref(tot), C.byref(cnt)) == 0 and cnt.value:
-            stages[name] = round(tot.value / cnt.value, 4)
-    lib.mop_hip_timers_enable(dev, 0)
-    log(f"stage ms over {n_st} frames: {stages}")
-
-    # the timed frames must have drawn the scene: read the last one back
-    # (this also runs the backend's scratch-overflow check and frame stats)
-    import numpy as np
-    iw, ih = C.c_int(), C.c_int()
-    ids = np.ctypeslib.as_array(
-        lib.mop_hip_framebuffer_read_object_id(dev, fb, C.byref(iw), C.byref(ih)),
-        (W * H,)).copy()
-    st = abi.FrameStats()
-    lib.mop_hip_get_frame_stats(dev, C.byref(st))
-    visible_objects = int(len(np.unique(ids[ids != 0])))
-    covered = float((ids != 0).mean())
-    log(f"check: {visible_objects} objects visible, {covered:.3f} of pixels covered, "
-        f"{st.subtris} screen triangles")
-    if visible_objects == 0 or st.subtris == 0:
-        raise RuntimeError("timed frames drew nothing: " + lib.mop_hip_last_error().decode())
-
-    # PCIe-inclusive: frame + RGBA8 + depth + ID on the host
-    n_rb = max(5, args.steps // 5)
-    iw, ih = C.c_int(), C.c_int()
-    t0 = time.perf_counter()
-    for _ in range(n_rb):
-        lib.mop_hip_scene_render(drv, fb, C.byref(cam), W, H)
-        lib.mop_hip_framebuffer_read_color(dev, fb, C.byref(iw), C.byref(ih))
-        lib.mop_hip_framebuffer_read_depth(dev, fb, C.byref(iw), C.byref(ih))
-        lib.mop_hip_framebuffer_read_object_id(dev, fb, C.byref(iw), C.byref(ih))
-    fps_rb = n_rb / (time.perf_counter() - t0)
-    log(f"readback: {fps_rb:.2f} frames/s")
-
-    # roofline of the dominant kernel, the shading pass: per pixel it reads the
-    # visibility word (8 B), sets it back to the cleared key (8 B) and writes
-    # HDR RGBA32F + depth + object ID (24 B): algorithmic bytes = 40 B x
-    # pixels.  `traffic` is the HBM bytes per launch rocprofv3 measured for
-    # it (FETCH_SIZE and WRITE_SIZE, scaled by a plain copy in the same run)
-    # in the committed counter summary of this workload.
-    shade_ms = stages.get("shade")
-    roof = None
-    if shade_ms:
-        npx = W * H
-        algo_bytes = 40 * npx
-        achieved = algo_bytes / (shade_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": profiled_traffic("frame_1080p_pbr_counters.md", "k_shade"),
-                "kernel": "k_shade", "launch_ms": shade_ms,
-                "algorithmic_bytes_per_launch": algo_bytes}
-
+    c2 = config2_form(args)
+    sc = c2["sc"]
+    c3 = None if args.no_config3 else config3_form(args.config3_frames)
     batch = None if args.no_batch else batched_form(mod, args.batch_rounds, not args.no_tune)
     if batch:
         log(f"batched: {batch}")
-    serving = None if args.no_serve else serving_form(mod)
+    serving = None if args.no_serve else serving_form(mod, duration=args.serve_seconds)
+    cpu = None
     if not args.no_cpu:
         log("cpu baseline: starting")
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, args.cpu_cores)
-    if cpu:
-        log(f"cpu baseline: {cpu['value']} frames/s on {cpu['cores']} cores")
+        cores = [int(c) for c in args.cpu_pool_cores.split(",") if c.strip()]
+        cpu = cpu_baseline(args.cpu_seconds, cores)
+        if cpu:
+            log(f"cpu baseline: {cpu['value']} frames/s on {cpu['cores']} cores")
 
     out = {
-        "metric": "frames/s (Mtri/s alongside), 1080p PBR scene",
-        "value": round(fps, 3),
+        "metric": "frames/s with RGBA8 + depth + object ID on the host (Mtri/s alongside), "
+                  "1080p PBR scene",
+        "value": round(c2["fps"], 3),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(c2["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded scene generator; config 2 of the workload set)",
+        "data": "synthetic (seeded scene generator; the 1080p PBR scene of the workload set)",
         "config": {"workload": "1920x1080 PBR: 256 icospheres (1,310,720 tris), "
-                               "sun + 2048^2 shadow map, 2 point, 1 spot, 2048x1024 HDR IBL",
-                   "width": W, "height": H, "triangles": sc.triangle_count,
+                               "sun + 2048^2 shadow map, 2 point, 1 spot, 2048x1024 HDR IBL; "
+                               "RGBA8 + depth + id read back every frame",
+                   "width": sc.width, "height": sc.height, "triangles": sc.triangle_count,
                    "parallelism": "single GPU, single viewport per frame"},
-        "mtri_per_s": round(mtri, 3),
-        "fps_with_readback": round(fps_rb, 3),
-        "host_ms_per_frame": round(host_ms, 4),
-        "host_ms_one_frame_idle_device": round(idle_host_ms, 4),
-        "device_ms_one_frame": round(dev_span_ms, 4),
-        "graph_replays": rep.value,
-        "graph_captures": cap.value,
-        "stage_ms": stages,
-        "frame_stats": {"subtris": st.subtris, "clipped": st.clipped,
-                        "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris,
-                        "visible_objects": visible_objects,
-                        "covered_pixel_fraction": round(covered, 4)},
-        "roofline": roof,
+        **c2["extra"],
+        "roofline": c2["roofline"],
+        "raster_stress_4k": c3,
         "batched_64_viewports": batch,
         "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    lib.mop_hip_framebuffer_destroy(dev, fb)
-    lib.mop_hip_scene_destroy(drv)
-    lib.mop_hip_device_destroy(dev)
 
 
 if __name__ == "__main__":

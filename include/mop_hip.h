@@ -232,6 +232,15 @@ void mop_hip_postprocess_apply(MopHipDevice *dev, MopHipFramebuffer *fb, uint32_
  * read back to host memory owned by the framebuffer. */
 const float *mop_hip_framebuffer_read_hdr(MopHipDevice *dev, MopHipFramebuffer *fb, int *out_w,
                                           int *out_h);
+/* Asynchronous readback (no counterpart in the reference, whose buffers are
+ * host memory; it serves the deferred readback that frame_submit /
+ * frame_wait_readback imply, rhi.h:163-171).  Starts copying the current
+ * frame's buffers -- mask: 1 RGBA8, 2 depth, 4 object ID, 8 HDR -- to the
+ * framebuffer's host buffers on a copy engine once the frame finishes, so
+ * the copy overlaps the next frames' kernels; the read_* calls above then
+ * wait for it and return the same pointers.  A frame re-run after a scratch
+ * overflow, or new contents, makes the read_* calls copy again. */
+void mop_hip_framebuffer_readback_async(MopHipDevice *dev, MopHipFramebuffer *fb, uint32_t mask);
 
 /* ------------------------------------------------------------------------
  * Batched execution (no counterpart in the reference, whose frames are

@@ -170,6 +170,7 @@ SIGS = {
     "mop_hip_set_scratch_limits": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32]),
     "mop_hip_set_default_scratch_limits": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "mop_hip_overflow_reruns": (C.c_uint64, []),
+    "mop_hip_framebuffer_readback_async": (None, [VP, VP, C.c_uint32]),
 }
 
 

@@ -152,6 +152,14 @@ struct MopHipFramebuffer {
   int nseg = 0;
   hipEvent_t done = nullptr;
   size_t skipped = 0;
+  /* asynchronous readback (mop_hip_framebuffer_readback_async): copies of
+   * the execution numbered rb_seq into the host buffers, on the device's copy
+   * stream, finished when rb_done completes; exec_seq numbers the contents
+   * of the device buffers (every execution or re-run of the tail) */
+  hipEvent_t rb_done = nullptr;
+  bool rb_pending = false;
+  uint32_t rb_mask = 0;
+  uint64_t rb_seq = 0, exec_seq = 0;
 };
 
 #define MOPH_MAX_SEGS 32
@@ -259,6 +267,9 @@ struct StageSlot {
 
 struct MopHipDevice {
   hipStream_t stream = nullptr;
+  /* device-to-host copies of asynchronous readbacks: they overlap the next
+   * frames' kernels on `stream` */
+  hipStream_t copy = nullptr;
   /* queued shadow passes run on `side`, forked from and joined back into
    * `stream` before the first shading launch that reads the maps */
   hipStream_t side = nullptr;
@@ -392,6 +403,13 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
     delete d;
     return nullptr;
   }
+  if (hipStreamCreateWithFlags(&d->copy, hipStreamNonBlocking) != hipSuccess) {
+    set_err("hipStreamCreate failed");
+    (void)hipStreamDestroy(d->side);
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return nullptr;
+  }
   HIPCHK(hipEventCreate(&d->ev0));
   HIPCHK(hipEventCreate(&d->ev1));
   HIPCHK(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
@@ -491,6 +509,10 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   (void)hipEventDestroy(d->ev_fork);
   (void)hipEventDestroy(d->ev_join);
   (void)hipStreamDestroy(d->side);
+  if (d->copy) {
+    (void)hipStreamSynchronize(d->copy);
+    (void)hipStreamDestroy(d->copy);
+  }
   (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -892,6 +914,9 @@ static void fb_free(MopHipFramebuffer *fb) {
   if (fb->h_hdr) (void)hipHostFree(fb->h_hdr);
   if (fb->h_cnt) (void)hipHostFree(fb->h_cnt);
   if (fb->done) (void)hipEventDestroy(fb->done);
+  if (fb->rb_done) (void)hipEventDestroy(fb->rb_done);
+  fb->rb_done = nullptr;
+  fb->rb_pending = false;
   fb->h_cnt = fb->d_cnt = nullptr;
   fb->done = nullptr;
   fb->color[0] = fb->color[1] = nullptr;
@@ -926,7 +951,8 @@ static bool fb_alloc(MopHipDevice *d, MopHipFramebuffer *fb, int w, int h) {
             hipHostMalloc((void **)&fb->h_cnt, MOPH_MAX_SEGS * CNT_N * 4,
                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
             hipHostGetDevicePointer((void **)&fb->d_cnt, fb->h_cnt, 0) == hipSuccess &&
-            hipEventCreateWithFlags(&fb->done, hipEventDisableTiming) == hipSuccess;
+            hipEventCreateWithFlags(&fb->done, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&fb->rb_done, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     set_err("framebuffer allocation %dx%d failed", w, h);
     fb_free(fb);
@@ -956,8 +982,10 @@ extern "C" MopHipFramebuffer *mop_hip_framebuffer_create(MopHipDevice *d,
 extern "C" void mop_hip_framebuffer_destroy(MopHipDevice *d, MopHipFramebuffer *fb) {
   if (!fb)
     return;
-  if (d)
+  if (d) {
     (void)hipStreamSynchronize(d->stream);
+    (void)hipStreamSynchronize(d->copy);
+  }
   fb_free(fb);
   delete fb;
 }
@@ -967,6 +995,7 @@ extern "C" void mop_hip_framebuffer_resize(MopHipDevice *d, MopHipFramebuffer *f
   if (!d || !fb)
     return;
   (void)hipStreamSynchronize(d->stream);
+  (void)hipStreamSynchronize(d->copy);
   fb_free(fb);
   fb->draws.clear();
   fb->light_pool.clear();
@@ -982,6 +1011,16 @@ extern "C" void mop_hip_framebuffer_resize(MopHipDevice *d, MopHipFramebuffer *f
 
 static void invalidate_host(MopHipFramebuffer *fb) {
   fb->v_color = fb->v_depth = fb->v_ids = fb->v_hdr = false;
+}
+
+/* Before new contents go into fb's device buffers: the kernels wait (on the
+ * device) for an asynchronous readback still copying the old ones, and that
+ * readback no longer counts as the buffers' contents. */
+static void fb_new_contents(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (fb->rb_pending)
+    HIPCHK(hipStreamWaitEvent(d->stream, fb->rb_done, 0));
+  fb->exec_seq++;
+  invalidate_host(fb);
 }
 
 extern "C" void mop_hip_frame_begin(MopHipDevice *d, MopHipFramebuffer *fb, MopHipColor c) {
@@ -1153,7 +1192,7 @@ extern "C" void mop_hip_draw_instanced(MopHipDevice *d, MopHipFramebuffer *fb,
 
 static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_tris,
                                  uint32_t ndraws, uint32_t nlights, uint32_t nviews,
-                                 uint32_t &clip_cap, uint32_t &subtri_cap) {
+                                 bool fused, uint32_t &clip_cap, uint32_t &subtri_cap) {
   clip_cap = n_tris / 16;
   if (clip_cap < 4096)
     clip_cap = 4096;
@@ -1168,14 +1207,19 @@ static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_t
   (void)ndraws;
   (void)nlights;
   (void)nviews;
-  return d->vpos.ensure(n_verts ? n_verts : 1) &&
-         d->vattr.ensure((size_t)(n_verts ? n_verts : 1) * 3) &&
-         d->cpos.ensure((size_t)clip_cap * MOPH_MAX_POLY) &&
-         d->cattr.ensure((size_t)clip_cap * MOPH_MAX_POLY * 3) && d->cnt_poly.ensure(clip_cap) &&
-         d->tri_counts.ensure((size_t)n_tris + 1) && d->clip_queue.ensure(clip_cap) &&
-         d->clip_slot.ensure(n_tris ? n_tris : 1) && d->counters.ensure(CNT_N) &&
-         d->block_sums.ensure(n_tris / (256 * 16) + 2) && d->subtris.ensure(subtri_cap) &&
-         d->big_queue.ensure(subtri_cap);
+  bool ok = d->vpos.ensure(n_verts ? n_verts : 1) &&
+            d->vattr.ensure((size_t)(n_verts ? n_verts : 1) * 3) &&
+            d->cpos.ensure((size_t)clip_cap * MOPH_MAX_POLY) &&
+            d->cattr.ensure((size_t)clip_cap * MOPH_MAX_POLY * 3) &&
+            d->cnt_poly.ensure(clip_cap) && d->clip_queue.ensure(clip_cap) &&
+            d->counters.ensure(CNT_N) && d->block_sums.ensure(n_tris / (256 * 16) + 2) &&
+            d->big_queue.ensure(subtri_cap);
+  /* the screen-triangle store and the per-triangle scan belong to the
+   * ordered path (and batches too large for the fused keys) only */
+  if (ok && !fused)
+    ok = d->tri_counts.ensure((size_t)n_tris + 1) && d->clip_slot.ensure(n_tris ? n_tris : 1) &&
+         d->subtris.ensure(subtri_cap);
+  return ok;
 }
 
 static uint32_t view_tiles(int w, int h) {
@@ -1275,8 +1319,11 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     }
   }
   uint32_t clip_cap, subtri_cap;
+  /* order-free batches use the fused path while (triangle << 4 | fan index)
+   * fits the 32-bit sequence half of the visibility key */
+  bool fused = !ordered && n_tris < (1u << 27);
   if (!ensure_frame_scratch(d, n_verts, n_tris, (uint32_t)gd.size(), (uint32_t)lights.size(),
-                            (uint32_t)views.size(), clip_cap, subtri_cap))
+                            (uint32_t)views.size(), fused, clip_cap, subtri_cap))
     return false;
   /* one upload per batch: [counters (zeroed) | draws | lights | views], each
    * 256-byte aligned */
@@ -1324,9 +1371,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.small_area = d->small_area;
   fa.shade_tile = d->shade_tile;
   fa.raster_grid = d->raster_grid;
-  /* order-free batches use the fused path while (triangle << 4 | fan index)
-   * fits the 32-bit sequence half of the visibility key */
-  fa.fused = (!ordered && n_tris < (1u << 27)) ? 1u : 0u;
+  fa.fused = fused ? 1u : 0u;
   if (fa.fused) {
     uint32_t chk_cap = d->chk_cap_hint > (1u << 20) ? d->chk_cap_hint : (1u << 20);
     if (d->force_chk)
@@ -1630,6 +1675,8 @@ static void execute_frames(MopHipDevice *d, MopHipFramebuffer *const *fbs, size_
       todo.push_back(fbs[i]);
   if (todo.empty())
     return;
+  for (MopHipFramebuffer *fb : todo)
+    fb_new_contents(d, fb);
   HIPCHK(hipEventRecord(d->ev0, d->stream));
   bool planned = false;
   /* a lone frame is device-bound with few launches: direct launches run it
@@ -1972,6 +2019,42 @@ extern "C" void mop_hip_get_frame_stats(MopHipDevice *d, MopHipFrameStats *out) 
 
 static void ensure_ready(MopHipDevice *d, MopHipFramebuffer *fb) { check_frame(d, fb); }
 
+/* True when an asynchronous readback of the current contents copied `bit`:
+ * waits for that copy. */
+static bool take_async(MopHipFramebuffer *fb, uint32_t bit) {
+  if (!fb->rb_pending || !(fb->rb_mask & bit) || fb->rb_seq != fb->exec_seq)
+    return false;
+  HIPCHK(hipEventSynchronize(fb->rb_done));
+  return true;
+}
+
+/* Start copying the current frame's buffers (mask: 1 RGBA8, 2 depth, 4
+ * object ID, 8 HDR) into the framebuffer's host buffers on the copy stream,
+ * after the frame finishes; the read_* calls then wait for that copy
+ * instead of issuing their own.  Executes a recorded frame first. */
+extern "C" void mop_hip_framebuffer_readback_async(MopHipDevice *d, MopHipFramebuffer *fb,
+                                                   uint32_t mask) {
+  if (!d || !fb || !(mask & 15u))
+    return;
+  if (!fb->executed)
+    execute_frames(d, &fb, 1);
+  size_t n = (size_t)fb->w * fb->h;
+  HIPCHK(hipStreamWaitEvent(d->copy, fb->done, 0));
+  if (mask & 1u)
+    HIPCHK(hipMemcpyAsync(fb->h_color, fb->color[fb->cur], n * 4, hipMemcpyDeviceToHost,
+                          d->copy));
+  if (mask & 2u)
+    HIPCHK(hipMemcpyAsync(fb->h_depth, fb->depth, n * 4, hipMemcpyDeviceToHost, d->copy));
+  if (mask & 4u)
+    HIPCHK(hipMemcpyAsync(fb->h_ids, fb->ids, n * 4, hipMemcpyDeviceToHost, d->copy));
+  if (mask & 8u)
+    HIPCHK(hipMemcpyAsync(fb->h_hdr, fb->hdr, n * 16, hipMemcpyDeviceToHost, d->copy));
+  HIPCHK(hipEventRecord(fb->rb_done, d->copy));
+  fb->rb_pending = true;
+  fb->rb_mask = mask & 15u;
+  fb->rb_seq = fb->exec_seq;
+}
+
 extern "C" const uint8_t *mop_hip_framebuffer_read_color(MopHipDevice *d, MopHipFramebuffer *fb,
                                                          int *ow, int *oh) {
   if (!d || !fb)
@@ -1980,9 +2063,11 @@ extern "C" const uint8_t *mop_hip_framebuffer_read_color(MopHipDevice *d, MopHip
   if (ow) *ow = fb->w;
   if (oh) *oh = fb->h;
   if (!fb->v_color) {
-    HIPCHK(hipMemcpyAsync(fb->h_color, fb->color[fb->cur], (size_t)fb->w * fb->h * 4,
-                          hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
+    if (!take_async(fb, 1u)) {
+      HIPCHK(hipMemcpyAsync(fb->h_color, fb->color[fb->cur], (size_t)fb->w * fb->h * 4, hipMemcpyDeviceToHost,
+                            d->stream));
+      HIPCHK(hipStreamSynchronize(d->stream));
+    }
     fb->v_color = true;
   }
   return fb->h_color;
@@ -1997,9 +2082,11 @@ extern "C" const uint32_t *mop_hip_framebuffer_read_object_id(MopHipDevice *d,
   if (ow) *ow = fb->w;
   if (oh) *oh = fb->h;
   if (!fb->v_ids) {
-    HIPCHK(hipMemcpyAsync(fb->h_ids, fb->ids, (size_t)fb->w * fb->h * 4, hipMemcpyDeviceToHost,
-                          d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
+    if (!take_async(fb, 4u)) {
+      HIPCHK(hipMemcpyAsync(fb->h_ids, fb->ids, (size_t)fb->w * fb->h * 4, hipMemcpyDeviceToHost,
+                            d->stream));
+      HIPCHK(hipStreamSynchronize(d->stream));
+    }
     fb->v_ids = true;
   }
   return fb->h_ids;
@@ -2013,9 +2100,11 @@ extern "C" const float *mop_hip_framebuffer_read_depth(MopHipDevice *d, MopHipFr
   if (ow) *ow = fb->w;
   if (oh) *oh = fb->h;
   if (!fb->v_depth) {
-    HIPCHK(hipMemcpyAsync(fb->h_depth, fb->depth, (size_t)fb->w * fb->h * 4,
-                          hipMemcpyDeviceToHost, d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
+    if (!take_async(fb, 2u)) {
+      HIPCHK(hipMemcpyAsync(fb->h_depth, fb->depth, (size_t)fb->w * fb->h * 4, hipMemcpyDeviceToHost,
+                            d->stream));
+      HIPCHK(hipStreamSynchronize(d->stream));
+    }
     fb->v_depth = true;
   }
   return fb->h_depth;
@@ -2029,9 +2118,11 @@ extern "C" const float *mop_hip_framebuffer_read_hdr(MopHipDevice *d, MopHipFram
   if (ow) *ow = fb->w;
   if (oh) *oh = fb->h;
   if (!fb->v_hdr) {
-    HIPCHK(hipMemcpyAsync(fb->h_hdr, fb->hdr, (size_t)fb->w * fb->h * 16, hipMemcpyDeviceToHost,
-                          d->stream));
-    HIPCHK(hipStreamSynchronize(d->stream));
+    if (!take_async(fb, 8u)) {
+      HIPCHK(hipMemcpyAsync(fb->h_hdr, fb->hdr, (size_t)fb->w * fb->h * 16, hipMemcpyDeviceToHost,
+                            d->stream));
+      HIPCHK(hipStreamSynchronize(d->stream));
+    }
     fb->v_hdr = true;
   }
   return fb->h_hdr;
@@ -2498,6 +2589,7 @@ extern "C" void mop_hip_hdr_resolve(MopHipDevice *d, MopHipFramebuffer *fb, floa
     return;
   }
   /* the tail recomputes from the HDR buffer, so re-running it is exact */
+  fb_new_contents(d, fb);
   std::vector<MopHipFramebuffer *> one(1, fb);
   run_tail(d, one);
   HIPCHK(hipEventRecord(fb->done, d->stream));
@@ -2531,6 +2623,7 @@ extern "C" void mop_hip_postprocess_apply(MopHipDevice *d, MopHipFramebuffer *fb
     return;
   }
   /* resolve + post from the HDR buffer: the same as post on the resolved image */
+  fb_new_contents(d, fb);
   std::vector<MopHipFramebuffer *> one(1, fb);
   run_tail(d, one);
   HIPCHK(hipEventRecord(fb->done, d->stream));

@@ -87,6 +87,12 @@ class Framebuffer:
             out["hdr"] = np.ctypeslib.as_array(p, (n * 4,)).reshape(h, w, 4).copy()
         return out
 
+    def readback_async(self, color=True, depth=True, ids=True, hdr=False):
+        """Start copying this frame's buffers to host memory behind the
+        device's later work; read() then waits for the copy."""
+        mask = (1 if color else 0) | (2 if depth else 0) | (4 if ids else 0) | (8 if hdr else 0)
+        self.d.lib.mop_hip_framebuffer_readback_async(self.d.dev, self.fb, mask)
+
     def read_color_raw(self):
         """Bring RGBA8 to host (pinned buffer owned by the framebuffer)."""
         iw, ih = C.c_int(), C.c_int()

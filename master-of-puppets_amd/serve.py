@@ -5,10 +5,13 @@ Resident state: every scene's meshes, IBL tables and shadow map stay in HBM
 lights, not on the camera).  Requests arrive as a Poisson stream; each picks a
 scene, an orbit delta of the scene camera and a resolution class.
 
-Admission is by HBM budget: the budget is cut into framebuffer slots per
-resolution class (48 B/px on the device); a request is admitted when a slot of
-its class is free, waits in a bounded queue otherwise, and is rejected when the
-queue is full.  Continuous batching: while up to `inflight` batches run on the
+Admission is by HBM budget.  A share of the budget is cut into framebuffer
+slots per resolution class (56 B/px on the device); the rest bounds the
+scratch a batch allocates (80 B per transformed vertex, 80 B per triangle for
+the triangle records, big-triangle queues and the clip store, plus a fixed
+checkpoint store).  A request is admitted into the next batch when a slot of
+its class is free and the batch's scratch stays within that bound; it waits
+in a bounded queue otherwise, and is rejected when the queue is full.  Continuous batching: while up to `inflight` batches run on the
 device, newly arrived requests are recorded into free slots and launched as
 the next batch; finished batches are read back (RGBA8 to host) and their
 slots freed.  Latency = arrival to host readback.  The plain mode serves the
@@ -23,7 +26,18 @@ import numpy as np
 from . import driver
 
 RES_CLASSES = [((640, 480), 0.25), ((1280, 720), 0.50), ((1920, 1080), 0.25)]
-BYTES_PER_PX = 48  # colour x2, HDR, depth, id, visibility, edge values
+# device bytes of one framebuffer pixel: colour x2 (8), HDR (16), depth (4),
+# id (4), visibility (8), edge values (16)
+BYTES_PER_PX = 56
+# device scratch of a batch, grown to the largest batch and kept: per
+# transformed vertex (clip and screen positions, three attribute float4) and
+# per triangle (record 16, big-triangle queue, offsets and checkpoint index
+# 12, clip store 768 B per clip slot at one slot per 16 triangles), plus the
+# checkpoint store's floor
+SCRATCH_PER_VERTEX = 80
+SCRATCH_PER_TRIANGLE = 80
+SCRATCH_FIXED = 64 << 20
+SLOT_SHARE = 0.25
 
 
 def make_requests(rate, duration, n_scenes, seed=0x4D4F5005):
@@ -50,16 +64,34 @@ class Server:
         # fragments by (triangle << 4 | fan index) in 32 bits
         self.max_batch_tris = max_batch_tris
         self.tris = [sc.triangle_count for _, sc in scenes]
+        self.scratch = [SCRATCH_PER_VERTEX * sum(len(m.vertices) for m in sc.meshes) +
+                        SCRATCH_PER_TRIANGLE * sc.triangle_count for _, sc in scenes]
         self.inflight = inflight
+        self.budget = budget_bytes
         self.free = []
         self.slots = []
+        slot_bytes = 0
         for (w, h), share in RES_CLASSES:
-            n = int(budget_bytes * share // (w * h * BYTES_PER_PX))
+            n = int(budget_bytes * SLOT_SHARE * share // (w * h * BYTES_PER_PX))
             n = max(1, min(n, max_slots_per_class))
             fbs = [device.framebuffer(w, h) for _ in range(n)]
+            slot_bytes += n * w * h * BYTES_PER_PX
             self.slots.append(fbs)
             self.free.append(list(fbs))
+        self.slot_bytes = slot_bytes
+        self.scratch_budget = budget_bytes - slot_bytes - SCRATCH_FIXED
+        if self.scratch_budget < max(self.scratch):
+            raise ValueError("HBM budget below one request's scratch")
         self.queue_cap = 4 * sum(len(s) for s in self.slots)
+
+    def admission(self):
+        """How the HBM budget is divided."""
+        return {"budget_bytes": self.budget, "slot_bytes": self.slot_bytes,
+                "slots_per_class": {f"{w}x{h}": len(f) for ((w, h), _), f in
+                                    zip(RES_CLASSES, self.slots)},
+                "scratch_budget_bytes": self.scratch_budget,
+                "scratch_bytes_per_request_max": max(self.scratch),
+                "max_batch_triangles": self.max_batch_tris}
 
     def close(self):
         for fbs in self.slots:
@@ -123,13 +155,16 @@ class Server:
                 skipped = collections.deque()
                 limit = 1 if plain else self.max_batch
                 tris = 0
+                scratch = 0
                 while queue and len(batch) < limit:
                     req = queue.popleft()
-                    if batch and tris + self.tris[req[1]] > self.max_batch_tris:
+                    if batch and (tris + self.tris[req[1]] > self.max_batch_tris or
+                                  scratch + self.scratch[req[1]] > self.scratch_budget):
                         queue.appendleft(req)
                         break
                     if self.free[req[4]]:
                         tris += self.tris[req[1]]
+                        scratch += self.scratch[req[1]]
                         fb = self.free[req[4]].pop()
                         batch.append((req, fb))
                     else:
