@@ -54,7 +54,7 @@ def test_serving_loop_completes_requests():
         for k in range(3):
             sc = scenes.small_scene(seed=60 + k, width=64, height=48, n=5)
             pool.append((driver.SceneDriver(dev, sc, shadow_cache=True), sc))
-        srv = serve.Server(dev, pool, budget_bytes=256 << 20, max_batch=8)
+        srv = serve.Server(dev, pool, budget_bytes=4 << 30, max_batch=8)
         reqs = serve.make_requests(rate=400.0, duration=0.5, n_scenes=len(pool))
         res = srv.run(reqs, time_limit=20.0)
         print("serving:", res)
@@ -81,7 +81,7 @@ def test_serving_results_equal_alone():
         for k in range(3):
             sc = scenes.small_scene(seed=70 + k, width=64, height=48, n=5, with_env=(k == 1))
             pool.append((driver.SceneDriver(dev, sc, shadow_cache=True), sc))
-        srv = serve.Server(dev, pool, budget_bytes=256 << 20, max_batch=8, inflight=2)
+        srv = serve.Server(dev, pool, budget_bytes=4 << 30, max_batch=8, inflight=2)
         reqs = serve.make_requests(rate=600.0, duration=0.25, n_scenes=len(pool), seed=1234)
         keep = list(range(0, min(len(reqs), 24), 3))
         res = srv.run(reqs, time_limit=20.0, keep=keep)
