@@ -460,12 +460,6 @@ def batched_form(mod, rounds, tune=True):
     log("batched: scenes resident")
     fbs = [dev.framebuffer(sc.width, sc.height) for sc in views]
     tris = sum(sc.triangle_count for sc in views)
-    for d, fb in zip(drv, fbs):  # warm-up (also renders the kept shadow maps)
-        d.render(fb)
-    dev.sync()
-    check_overflow(dev, "batched warm-up")
-    log("batched: warm-up done")
-
     from mop_amd import autotune
 
     def one_batch():
@@ -473,6 +467,18 @@ def batched_form(mod, rounds, tune=True):
             d.record(fb)
         dev.execute(fbs)
         dev.sync()
+
+    # warm-up: every view alone (renders the kept shadow maps), then one batch
+    # under every launch shape, untimed: a batch reports the scratch it
+    # needed and the device sizes later batches from it, so the timed
+    # batches below run complete (checked after each timed region)
+    for d, fb in zip(drv, fbs):
+        d.render(fb)
+    for cand in (autotune.candidates() if tune else [(64, 16, 8192)]):
+        autotune.set_tuning(dev, *cand)
+        one_batch()
+    warm_flags = dev.overflow_flags(reset=True)
+    log(f"batched: warm-up done (scratch grown after overflow flags {warm_flags:#x})")
 
     if tune:
         best, table = autotune.autotune(dev, one_batch)

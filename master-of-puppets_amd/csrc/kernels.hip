@@ -1982,10 +1982,24 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint
       o[k] = __hip_atomic_load(&fa.counters[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   if (threadIdx.x == 0 && fa.sticky) {
+    /* word 0: overflow bits; words 1-3: the largest clip-queue,
+     * checkpoint-store and tile-bin needs seen (the host sizes the next
+     * batches from them) */
+    const int need[3] = {CNT_CLIP, CNT_CHK, CNT_BINS};
     uint32_t ovf =
         __hip_atomic_load(&fa.counters[CNT_OVERFLOW], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t acc = atomicOr(fa.sticky, ovf) | ovf;
-    *fa.sticky_out = acc;
+    uint32_t mx[3];
+    for (int k = 0; k < 3; k++) {
+      uint32_t v =
+          __hip_atomic_load(&fa.counters[need[k]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t old = atomicMax(fa.sticky + 1 + k, v);
+      mx[k] = old > v ? old : v;
+    }
+    fa.sticky_out[1] = mx[0];
+    fa.sticky_out[2] = mx[1];
+    fa.sticky_out[3] = mx[2];
+    fa.sticky_out[0] = acc;
   }
 }
 

@@ -237,10 +237,12 @@ struct FrameArgs {
   SubTri *subtris;
   uint32_t *big_queue;
   const MopHipLight *lights;
-  /* the device's overflow word: k_finish ORs the batch's overflow bits into
-   * sticky (device memory) and stores the result to sticky_out (a device
-   * alias of host-mapped memory), so the host can check every executed
-   * batch, not only the frames it reads back */
+  /* the device's overflow words: k_finish ORs the batch's overflow bits into
+   * sticky[0] (device memory) and keeps the largest clip-queue, checkpoint
+   * and tile-bin needs in sticky[1..3], and stores all four to sticky_out (a
+   * device alias of host-mapped memory), so the host can check every
+   * executed batch, not only the frames it reads back, and size the next
+   * batches */
   uint32_t *sticky;
   uint32_t *sticky_out;
 };

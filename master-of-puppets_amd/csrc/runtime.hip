@@ -419,13 +419,13 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   HIPCHK(hipHostMalloc((void **)&d->h_counters, CNT_N * sizeof(uint32_t), 0));
   if (hipHostMalloc((void **)&d->h_sticky, 64, hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&d->d_sticky_out, d->h_sticky, 0) != hipSuccess ||
-      !d->sticky.ensure(1)) {
+      !d->sticky.ensure(4)) {
     set_err("overflow word allocation failed");
     mop_hip_device_destroy(d);
     return nullptr;
   }
-  *d->h_sticky = 0;
-  launch_fill_u32(1, d->sticky.p, 0u, d->stream);
+  memset(d->h_sticky, 0, 4 * sizeof(uint32_t));
+  launch_fill_u32(4, d->sticky.p, 0u, d->stream);
   d->force_clip = g_force_clip;
   d->force_chk = g_force_chk;
   d->force_bins = g_force_bins;
@@ -1190,6 +1190,18 @@ extern "C" void mop_hip_draw_instanced(MopHipDevice *d, MopHipFramebuffer *fb,
 /* execution                                                                 */
 /* ------------------------------------------------------------------------ */
 
+/* The largest need of scratch store k (1 clip queue, 2 checkpoints, 3 tile
+ * bins) any finished batch reported, with a quarter of headroom: a batch
+ * that overflowed a store sizes the following ones even when its own frames
+ * are never read back (and so never run again).  0 before any report. */
+static uint32_t grown_need(const MopHipDevice *d, int k) {
+  uint32_t v = __atomic_load_n(d->h_sticky + k, __ATOMIC_ACQUIRE);
+  if (!v)
+    return 0;
+  uint64_t g = (uint64_t)v + v / 4 + 1024;
+  return g > 0xffffff00ull ? 0xffffff00u : (uint32_t)g;
+}
+
 static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_tris,
                                  uint32_t ndraws, uint32_t nlights, uint32_t nviews,
                                  bool fused, uint32_t &clip_cap, uint32_t &subtri_cap) {
@@ -1200,6 +1212,8 @@ static bool ensure_frame_scratch(MopHipDevice *d, uint32_t n_verts, uint32_t n_t
     clip_cap = d->clip_cap_hint;
   if (d->force_clip)
     clip_cap = d->force_clip;
+  else if (grown_need(d, 1) > clip_cap)
+    clip_cap = grown_need(d, 1);
   uint64_t sc = (uint64_t)n_tris + 10ull * clip_cap + 1;
   if (sc > 0xffffff00ull)
     sc = 0xffffff00ull;
@@ -1374,6 +1388,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.fused = fused ? 1u : 0u;
   if (fa.fused) {
     uint32_t chk_cap = d->chk_cap_hint > (1u << 20) ? d->chk_cap_hint : (1u << 20);
+    if (grown_need(d, 2) > chk_cap)
+      chk_cap = grown_need(d, 2);
     if (d->force_chk)
       chk_cap = d->force_chk;
     if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
@@ -1413,6 +1429,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     uint64_t bc = 2ull * subtri_cap + 65536;
     if (d->bin_cap_hint > bc)
       bc = d->bin_cap_hint;
+    if (grown_need(d, 3) > bc)
+      bc = grown_need(d, 3);
     if (d->force_bins)
       bc = d->force_bins;
     if (bc > 0xffffff00ull)
@@ -1954,7 +1972,7 @@ extern "C" uint32_t mop_hip_overflow_flags(MopHipDevice *d, int reset) {
   HIPCHK(hipStreamSynchronize(d->side));
   HIPCHK(hipStreamSynchronize(d->stream));
   uint32_t v = __atomic_load_n(d->h_sticky, __ATOMIC_ACQUIRE);
-  if (reset) {
+  if (reset) { /* the flags only; the needs keep sizing later batches */
     launch_fill_u32(1, d->sticky.p, 0u, d->stream);
     HIPCHK(hipStreamSynchronize(d->stream));
     __atomic_store_n(d->h_sticky, 0u, __ATOMIC_RELEASE);
