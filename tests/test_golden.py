@@ -2,7 +2,6 @@
 CPU backend): the CPU restatement must reproduce them bit for bit; the HIP
 frame driver (standalone path used by the benchmark, batching and serving)
 must match depth/ID bit-exactly and colour within 1/255."""
-import ctypes as C
 import hashlib
 import json
 import os
@@ -46,34 +45,20 @@ def render_driver(sc):
     """Render a Scene through the HIP frame driver; returns the buffers.  The
     shadow map is the reference's fixed 1024^2 (viewport.c:2286), which the
     goldens were rendered with whatever size the scene asks for."""
-    import copy
-    import mop_amd
-    sc = copy.copy(sc)
-    sc.shadow_size = 1024
-    from mop_amd import abi
-    import bench
-    lib = mop_amd.lib()
-    dev = lib.mop_hip_device_create()
-    assert dev, lib.mop_hip_last_error()
+    from mop_amd import driver
+    dev = driver.Device()
     try:
-        drv, cam = bench.make_scene_driver(lib, dev, sc)
-        fb = lib.mop_hip_framebuffer_create(dev, C.byref(abi.FramebufferDesc(sc.width, sc.height)))
-        tris = lib.mop_hip_scene_render(drv, fb, C.byref(cam), sc.width, sc.height)
-        w, h = C.c_int(), C.c_int()
-        n = sc.width * sc.height
-        out = {
-            "color": np.ctypeslib.as_array(lib.mop_hip_framebuffer_read_color(dev, fb, C.byref(w), C.byref(h)), (n * 4,)).reshape(sc.height, sc.width, 4).copy(),
-            "depth": np.ctypeslib.as_array(lib.mop_hip_framebuffer_read_depth(dev, fb, C.byref(w), C.byref(h)), (n,)).reshape(sc.height, sc.width).copy(),
-            "ids": np.ctypeslib.as_array(lib.mop_hip_framebuffer_read_object_id(dev, fb, C.byref(w), C.byref(h)), (n,)).reshape(sc.height, sc.width).copy(),
-            "hdr": np.ctypeslib.as_array(lib.mop_hip_framebuffer_read_hdr(dev, fb, C.byref(w), C.byref(h)), (n * 4,)).reshape(sc.height, sc.width, 4).copy(),
-            "triangles": tris,
-            "exposure": lib.mop_hip_scene_exposure(drv),
-        }
-        lib.mop_hip_framebuffer_destroy(dev, fb)
-        lib.mop_hip_scene_destroy(drv)
+        drv = driver.SceneDriver(dev, sc, shadow_size=1024)
+        fb = dev.framebuffer(sc.width, sc.height)
+        tris = drv.render(fb)
+        out = fb.read(hdr=True)
+        out["triangles"] = tris
+        out["exposure"] = drv.exposure()
+        fb.close()
+        drv.close()
         return out
     finally:
-        lib.mop_hip_device_destroy(dev)
+        dev.close()
 
 
 @pytest.mark.gpu
