@@ -344,6 +344,25 @@ void mop_hip_shadow_end(MopHipDevice *dev, const MopHipMat4 *light_vp);
 /* Make later draws test an already rendered map. */
 void mop_hip_shadow_bind(MopHipDevice *dev, const MopHipShadowMap *map,
                          const MopHipMat4 *light_vp);
+/* The map's depths (size * size floats) on the host, after the queued work. */
+int mop_hip_shadow_map_read(MopHipDevice *dev, const MopHipShadowMap *map, float *out);
+
+/* Allocation-free shadow pass.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Renders one caster
+ * mesh (resident buffers vb/ib, model and light view-projection as
+ * mop_hip_shadow_draw takes them) into map on the device's stream, clearing
+ * the map to depth 1.0 first when clear != 0; call it once per caster, with
+ * clear on the first only.  The result equals mop_hip_shadow_begin / draw /
+ * end with the same casters.  It allocates nothing: its working memory is a
+ * fixed 20 MiB per GPU, held until the process ends, for at most 524,288
+ * vertices and 1,048,576 triangles per call (larger meshes are refused:
+ * -1 and mop_hip_last_error()).  Every call on one GPU shares that memory,
+ * so calls that overlap on one GPU -- from two devices or two streams --
+ * race.  Returns 0 when the work is queued. */
+int mop_hip_shadow_render_noalloc(MopHipDevice *dev, MopHipShadowMap *map, MopHipBuffer *vb,
+                                  uint32_t vertex_count, MopHipBuffer *ib, uint32_t index_count,
+                                  const MopHipMat4 *model, const MopHipMat4 *light_vp, int clear);
 
 /* IBL tables computed on the device from an equirect RGBA32F environment
  * (environment.c:115-484: irradiance 64x32, prefiltered 128x64 x 5 levels,

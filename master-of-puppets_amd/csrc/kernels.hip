@@ -2521,6 +2521,16 @@ __global__ __launch_bounds__(WG) void k_shadow_big(ShadowArgsGPU sa) {
   }
 }
 
+/* One caster record written from kernel arguments (no host staging). */
+__global__ void k_put_shadow_draw(ShadowDrawGPU g, ShadowDrawGPU *dst) {
+  if (threadIdx.x == 0)
+    *dst = g;
+}
+
+void launch_put_shadow_draw(const ShadowDrawGPU &g, ShadowDrawGPU *dst, hipStream_t s) {
+  launch(k_put_shadow_draw, dim3(1), dim3(64), 0, s, g, dst);
+}
+
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s) {
   if (!sa.n_tris)
     return;

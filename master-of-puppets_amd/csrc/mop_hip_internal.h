@@ -358,6 +358,7 @@ void launch_upload(const void *src, void *dst, size_t n, hipStream_t s);
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s);
+void launch_put_shadow_draw(const ShadowDrawGPU &g, ShadowDrawGPU *dst, hipStream_t s);
 void launch_shadow_finalize(int n, const uint32_t *keys, float *depth,
                             hipStream_t s);
 void launch_fill_u32(size_t n, uint32_t *p, uint32_t v, hipStream_t s);

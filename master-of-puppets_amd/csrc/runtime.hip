@@ -308,6 +308,9 @@ struct MopHipDevice {
    * host-mapped mirror */
   DevArr<uint32_t> sticky;
   uint32_t *h_sticky = nullptr, *d_sticky_out = nullptr;
+  /* device address of `scratch` (mop_hip_shadow_render_noalloc); null when
+   * the symbol lookup failed, and the entry point then reports an error */
+  uint8_t *noalloc_scratch = nullptr;
   uint32_t *h_counters = nullptr;
   DevArr<ViewGPU> views;
   DevArr<PostViewGPU> pviews;
@@ -349,6 +352,22 @@ struct MopHipDevice {
  * re-runs caused by scratch overflow in this process. */
 static uint32_t g_force_clip = 0, g_force_chk = 0, g_force_bins = 0;
 static std::atomic<uint64_t> g_overflow_reruns{0};
+
+/* Scratch of mop_hip_shadow_render_noalloc, held by every GPU the library
+ * runs on until the process ends: [0, 256) counters, [256, 512) the caster
+ * record, then the light-space clip and shadow-map positions of up to
+ * NOALLOC_MAX_VERTS vertices (16 B each, twice) and the big-triangle queue
+ * of up to NOALLOC_MAX_TRIS triangles (4 B each): 20 MiB + 512 B.  In host
+ * code this name is a host-side stand-in, not a device address: the device
+ * address comes from hipGetSymbolAddress at device creation. */
+#define NOALLOC_MAX_VERTS (1u << 19)
+#define NOALLOC_MAX_TRIS (1u << 20)
+#define NOALLOC_OFF_DRAW 256u
+#define NOALLOC_OFF_LCLIP 512u
+#define NOALLOC_OFF_LSCR (NOALLOC_OFF_LCLIP + (size_t)NOALLOC_MAX_VERTS * 16u)
+#define NOALLOC_OFF_BIG (NOALLOC_OFF_LSCR + (size_t)NOALLOC_MAX_VERTS * 16u)
+#define NOALLOC_BYTES (NOALLOC_OFF_BIG + (size_t)NOALLOC_MAX_TRIS * 4u)
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
 
 /* The per-channel colour-effect tables of mop_postprocess_apply without fog
  * or vignette (postprocess.c:45-116): [tonemap | gamma << 1][value].  The
@@ -429,6 +448,12 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   d->force_clip = g_force_clip;
   d->force_chk = g_force_chk;
   d->force_bins = g_force_bins;
+  /* this device's copy of the allocation-free entry point's scratch */
+  if (hipGetSymbolAddress((void **)&d->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
+      !d->noalloc_scratch) {
+    (void)hipGetLastError();
+    d->noalloc_scratch = nullptr;
+  }
   if (d->post_lut.ensure(4 * 256)) {
     uint8_t lut[4 * 256];
     post_lut_fill(lut);
@@ -2419,6 +2444,96 @@ extern "C" void mop_hip_shadow_end(MopHipDevice *d, const MopHipMat4 *light_vp) 
     return;
   queue_shadow(d);
   shadow_enable(d, d->rec_shadow, *light_vp);
+}
+
+/* Render one caster mesh into map on the device's stream with no
+ * allocation: the light-space vertex cache, the big-triangle queue, the
+ * counters and the caster record live in the file-scope `scratch` array
+ * (through its device address noalloc_scratch), the record is written by a
+ * kernel from its arguments and the counters and the map are cleared by fill
+ * kernels.  The same kernels as mop_hip_shadow_begin/draw/end, so the map
+ * is identical; the depth minimum does not depend on caster order. */
+extern "C" int mop_hip_shadow_render_noalloc(MopHipDevice *d, MopHipShadowMap *map,
+                                             MopHipBuffer *vb, uint32_t vertex_count,
+                                             MopHipBuffer *ib, uint32_t index_count,
+                                             const MopHipMat4 *model,
+                                             const MopHipMat4 *light_vp, int clear) {
+  if (!d || !map || !vb || !ib || !model || !light_vp) {
+    set_err("shadow_render_noalloc: null argument");
+    return -1;
+  }
+  if (!d->noalloc_scratch) {
+    set_err("shadow_render_noalloc: scratch symbol unavailable on this device");
+    return -1;
+  }
+  if ((size_t)vertex_count * sizeof(MopHipVertex) > vb->size ||
+      (size_t)index_count * sizeof(uint32_t) > ib->size) {
+    set_err("shadow_render_noalloc: counts exceed the buffers");
+    return -1;
+  }
+  uint32_t n_tris = index_count / 3;
+  if (vertex_count > NOALLOC_MAX_VERTS || n_tris > NOALLOC_MAX_TRIS) {
+    set_err("shadow_render_noalloc: %u vertices / %u triangles exceed the scratch "
+            "maximum of %u / %u", vertex_count, n_tris, NOALLOC_MAX_VERTS, NOALLOC_MAX_TRIS);
+    return -1;
+  }
+  uint8_t *base = d->noalloc_scratch;
+  int size = map->size;
+  if (clear)
+    launch_shadow_clear(size * size, map->keys.p, d->stream);
+  if (n_tris == 0 || vertex_count == 0)
+    return 0;
+  ShadowDrawGPU g;
+  memset(&g, 0, sizeof(g));
+  MopHipMat4 mvp = mat4_mul(*light_vp, *model); /* rasterizer.c:361 */
+  memcpy(g.mvp, mvp.d, sizeof(g.mvp));
+  g.vbuf = (const uint8_t *)vb->dev;
+  g.ibuf = (const uint32_t *)ib->dev;
+  g.vertex_count = vertex_count;
+  g.tri_count = n_tris;
+  g.tri_base = 0;
+  g.vtx_base = 0;
+  ShadowDrawGPU *draw = (ShadowDrawGPU *)(base + NOALLOC_OFF_DRAW);
+  uint32_t *counters = (uint32_t *)base;
+  launch_fill_u32(CNT_N, counters, 0u, d->stream);
+  launch_put_shadow_draw(g, draw, d->stream);
+  ShadowArgsGPU sa;
+  memset(&sa, 0, sizeof(sa));
+  sa.draws = draw;
+  sa.ndraws = 1;
+  sa.n_tris = n_tris;
+  sa.size = size;
+  sa.big_cap = NOALLOC_MAX_TRIS;
+  sa.n_verts = vertex_count;
+  sa.lclip = (float4 *)(base + NOALLOC_OFF_LCLIP);
+  sa.lscr = (float4 *)(base + NOALLOC_OFF_LSCR);
+  sa.keys = map->keys.p;
+  sa.big_queue = (uint32_t *)(base + NOALLOC_OFF_BIG);
+  sa.counters = counters;
+  launch_shadow_raster(sa, d->stream);
+  return 0;
+}
+
+/* The map's depths (the ordered keys decoded) into a host array of size^2
+ * floats, after the queued work (queued shadow passes included). */
+extern "C" int mop_hip_shadow_map_read(MopHipDevice *d, const MopHipShadowMap *map, float *out) {
+  if (!d || !map || !out)
+    return -1;
+  flush_shadow_jobs(d);
+  size_t n = (size_t)map->size * map->size;
+  std::vector<uint32_t> keys(n);
+  if (hipMemcpyAsync(keys.data(), map->keys.p, n * 4, hipMemcpyDeviceToHost, d->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(d->stream) != hipSuccess) {
+    set_err("shadow map read failed");
+    return -1;
+  }
+  for (size_t i = 0; i < n; i++) { /* inverse of the ordered key map */
+    uint32_t k = keys[i];
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    memcpy(&out[i], &u, 4);
+  }
+  return 0;
 }
 
 extern "C" void mop_hip_shadow_bind(MopHipDevice *d, const MopHipShadowMap *map,
