@@ -122,10 +122,11 @@ def profiled(summary, key, kernel):
 
 def pipelined_frames(dev, drv, fbs, cam, steps):
     """`steps` frames rendered into the framebuffers in turn, each copied to
-    the host (RGBA8, depth, id) behind the next frame's kernels; returns
-    (seconds, triangles submitted, host seconds spent recording and
-    launching).  The clock stops when the last frame's buffers are on the
-    host."""
+    the host (RGBA8, depth, id) behind the next frames' kernels; frame k is
+    read on the host after frame k + len(fbs) - 1 is launched, so the copy
+    engine always has the next frame queued.  Returns (seconds, triangles
+    submitted, host seconds spent recording and launching).  The clock stops
+    when the last frame's buffers are on the host."""
     lib = dev.lib
     iw, ih = C.c_int(), C.c_int()
 
@@ -136,6 +137,7 @@ def pipelined_frames(dev, drv, fbs, cam, steps):
 
     tris = 0
     host = 0.0
+    lag = len(fbs) - 1
     t0 = time.perf_counter()
     for k in range(steps):
         fb = fbs[k % len(fbs)]
@@ -143,9 +145,10 @@ def pipelined_frames(dev, drv, fbs, cam, steps):
         tris += drv.render(fb, cam)
         lib.mop_hip_framebuffer_readback_async(dev.dev, fb.fb, READBACK_MASK)
         host += time.perf_counter() - th
-        if k >= 1:
-            read(fbs[(k - 1) % len(fbs)])
-    read(fbs[(steps - 1) % len(fbs)])
+        if k >= lag:
+            read(fbs[(k - lag) % len(fbs)])
+    for k in range(max(steps - lag, 0), steps):
+        read(fbs[k % len(fbs)])
     return time.perf_counter() - t0, tris, host
 
 
@@ -158,7 +161,7 @@ def config2_form(args):
     W, H = sc.width, sc.height
     log("config 2 scene generated")
     drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
-    fbs = [dev.framebuffer(W, H) for _ in range(2)]
+    fbs = [dev.framebuffer(W, H) for _ in range(3)]
     cam = drv.cam
     dev.sync()
     log("scene resident (meshes, IBL tables)")
@@ -291,7 +294,7 @@ def config2_form(args):
             "mtri_per_s_hbm_resident": round(tris / hbm_s / 1e6, 3),
             "fps_synchronous_readback": round(fps_sync, 3),
             "readback": "RGBA8 + depth + object ID to pinned host memory per frame, "
-                        "copy of frame k overlapping frame k+1 (two framebuffers)",
+                        "copies of frame k overlapping frames k+1, k+2 (three framebuffers)",
             "host_ms_per_frame": round(host_s * 1e3 / args.steps, 4),
             "host_ms_one_frame_idle_device": round(idle_host_ms, 4),
             "device_ms_one_frame": round(dev_span_ms, 4),
@@ -309,6 +312,32 @@ def config2_form(args):
     }
 
 
+def profile_frames(args):
+    """Configuration-2 frames with outputs left in HBM, nothing else (the
+    workload of the committed kernel-trace and counter summaries)."""
+    from mop_amd import driver, scenes
+    dev = driver.Device()
+    sc = scenes.config2()
+    drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
+    fb = dev.framebuffer(sc.width, sc.height)
+    for _ in range(args.warmup):
+        drv.render(fb)
+    dev.sync()
+    log("profile: warm-up done")
+    t0 = time.perf_counter()
+    for _ in range(args.profile_frames):
+        drv.render(fb)
+    dev.sync()
+    secs = time.perf_counter() - t0
+    flags = check_overflow(dev, "profile frames")
+    log(f"profile: {args.profile_frames} frames, {secs * 1e3 / args.profile_frames:.3f} ms/frame")
+    fb.close()
+    drv.close()
+    dev.close()
+    return {"profile_frames": args.profile_frames, "ms_per_frame": secs * 1e3 / args.profile_frames,
+            "overflow_flags": flags}
+
+
 # ---------------------------------------------------------------------------
 # configuration 3: 4K raster stress
 
@@ -322,7 +351,7 @@ def config3_form(frames):
     sc = scenes.config3()
     log(f"config 3: scene generated ({sc.triangle_count} triangles)")
     drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
-    fbs = [dev.framebuffer(sc.width, sc.height) for _ in range(2)]
+    fbs = [dev.framebuffer(sc.width, sc.height) for _ in range(3)]
     for k in range(2):
         drv.render(fbs[k], drv.cam)
     dev.sync()
@@ -650,6 +679,9 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--batch-rounds", type=int, default=8)
+    ap.add_argument("--profile-frames", type=int, default=0,
+                    help="only render this many configuration-2 frames with outputs left in "
+                         "HBM, after the warm-up, and exit (for rocprofv3)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the "
                          "reference dispatch that scales rocprofv3 FETCH/WRITE_SIZE)")
@@ -678,6 +710,9 @@ def main():
         return
     if args.serve_only:
         print(json.dumps({"serving": serving_form(mod, duration=args.serve_seconds)}))
+        return
+    if args.profile_frames:
+        print(json.dumps(profile_frames(args)))
         return
 
     c2 = config2_form(args)
