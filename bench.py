@@ -161,7 +161,7 @@ def config2_form(args):
     W, H = sc.width, sc.height
     log("config 2 scene generated")
     drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
-    fbs = [dev.framebuffer(W, H) for _ in range(3)]
+    fbs = [dev.framebuffer(W, H) for _ in range(args.readback_depth)]
     cam = drv.cam
     dev.sync()
     log("scene resident (meshes, IBL tables)")
@@ -293,8 +293,11 @@ def config2_form(args):
             "fps_hbm_resident": round(fps_hbm, 3),
             "mtri_per_s_hbm_resident": round(tris / hbm_s / 1e6, 3),
             "fps_synchronous_readback": round(fps_sync, 3),
+            "readback_gbs": round(fps * W * H * 12 / 1e9, 2),
+            "readback_frac_of_pcie_gen5_x16": round(fps * W * H * 12 / 64e9, 4),
             "readback": "RGBA8 + depth + object ID to pinned host memory per frame, "
-                        "copies of frame k overlapping frames k+1, k+2 (three framebuffers)",
+                        f"copies of frame k overlapping the next frames ({args.readback_depth} "
+                        f"framebuffers)",
             "host_ms_per_frame": round(host_s * 1e3 / args.steps, 4),
             "host_ms_one_frame_idle_device": round(idle_host_ms, 4),
             "device_ms_one_frame": round(dev_span_ms, 4),
@@ -679,6 +682,8 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
     ap.add_argument("--batch-rounds", type=int, default=8)
+    ap.add_argument("--readback-depth", type=int, default=2,
+                    help="framebuffers the headline loop renders into in turn")
     ap.add_argument("--profile-frames", type=int, default=0,
                     help="only render this many configuration-2 frames with outputs left in "
                          "HBM, after the warm-up, and exit (for rocprofv3)")

@@ -925,17 +925,13 @@ extern "C" bool mop_hip_texture_read_rgba8(MopHipDevice *d, MopHipTexture *t, ui
 /* ------------------------------------------------------------------------ */
 
 static void fb_free(MopHipFramebuffer *fb) {
-  for (int i = 0; i < 2; i++)
-    if (fb->color[i])
-      (void)hipFree(fb->color[i]);
+  /* RGBA8, depth and id share one block on each side (color[0] / h_color
+   * head it) so a readback of the three is one copy */
+  if (fb->color[0]) (void)hipFree(fb->color[0]);
   if (fb->hdr) (void)hipFree(fb->hdr);
-  if (fb->depth) (void)hipFree(fb->depth);
-  if (fb->ids) (void)hipFree(fb->ids);
   if (fb->vis) (void)hipFree(fb->vis);
   if (fb->wbuf) (void)hipFree(fb->wbuf);
   if (fb->h_color) (void)hipHostFree(fb->h_color);
-  if (fb->h_depth) (void)hipHostFree(fb->h_depth);
-  if (fb->h_ids) (void)hipHostFree(fb->h_ids);
   if (fb->h_hdr) (void)hipHostFree(fb->h_hdr);
   if (fb->h_cnt) (void)hipHostFree(fb->h_cnt);
   if (fb->done) (void)hipEventDestroy(fb->done);
@@ -962,22 +958,24 @@ static bool fb_alloc(MopHipDevice *d, MopHipFramebuffer *fb, int w, int h) {
   size_t n = (size_t)w * h;
   fb->w = w;
   fb->h = h;
-  bool ok = hipMalloc(&fb->color[0], n * 4) == hipSuccess &&
-            hipMalloc(&fb->color[1], n * 4) == hipSuccess &&
+  /* [RGBA8 | depth | id], 12 B/px, on the device and in pinned host memory */
+  bool ok = hipMalloc(&fb->color[0], n * 12) == hipSuccess &&
             hipMalloc(&fb->hdr, n * 16) == hipSuccess &&
-            hipMalloc(&fb->depth, n * 4) == hipSuccess &&
-            hipMalloc(&fb->ids, n * 4) == hipSuccess &&
             hipMalloc(&fb->vis, n * 8) == hipSuccess &&
             hipMalloc(&fb->wbuf, n * 16) == hipSuccess &&
-            hipHostMalloc((void **)&fb->h_color, n * 4, 0) == hipSuccess &&
-            hipHostMalloc((void **)&fb->h_depth, n * 4, 0) == hipSuccess &&
-            hipHostMalloc((void **)&fb->h_ids, n * 4, 0) == hipSuccess &&
+            hipHostMalloc((void **)&fb->h_color, n * 12, 0) == hipSuccess &&
             hipHostMalloc((void **)&fb->h_hdr, n * 16, 0) == hipSuccess &&
             hipHostMalloc((void **)&fb->h_cnt, MOPH_MAX_SEGS * CNT_N * 4,
                           hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
             hipHostGetDevicePointer((void **)&fb->d_cnt, fb->h_cnt, 0) == hipSuccess &&
             hipEventCreateWithFlags(&fb->done, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&fb->rb_done, hipEventDisableTiming) == hipSuccess;
+  if (ok) {
+    fb->depth = (float *)(fb->color[0] + n * 4);
+    fb->ids = (uint32_t *)(fb->color[0] + n * 8);
+    fb->h_depth = (float *)(fb->h_color + n * 4);
+    fb->h_ids = (uint32_t *)(fb->h_color + n * 8);
+  }
   if (!ok) {
     set_err("framebuffer allocation %dx%d failed", w, h);
     fb_free(fb);
@@ -2081,8 +2079,14 @@ extern "C" void mop_hip_framebuffer_readback_async(MopHipDevice *d, MopHipFrameb
     return;
   if (!fb->executed)
     execute_frames(d, &fb, 1);
+  uint32_t requested = mask & 15u;
   size_t n = (size_t)fb->w * fb->h;
   HIPCHK(hipStreamWaitEvent(d->copy, fb->done, 0));
+  if ((mask & 7u) == 7u && fb->cur == 0) {
+    /* RGBA8, depth and id are one block: one copy */
+    HIPCHK(hipMemcpyAsync(fb->h_color, fb->color[0], n * 12, hipMemcpyDeviceToHost, d->copy));
+    mask &= ~7u;
+  }
   if (mask & 1u)
     HIPCHK(hipMemcpyAsync(fb->h_color, fb->color[fb->cur], n * 4, hipMemcpyDeviceToHost,
                           d->copy));
@@ -2094,7 +2098,7 @@ extern "C" void mop_hip_framebuffer_readback_async(MopHipDevice *d, MopHipFrameb
     HIPCHK(hipMemcpyAsync(fb->h_hdr, fb->hdr, n * 16, hipMemcpyDeviceToHost, d->copy));
   HIPCHK(hipEventRecord(fb->rb_done, d->copy));
   fb->rb_pending = true;
-  fb->rb_mask = mask & 15u;
+  fb->rb_mask = requested;
   fb->rb_seq = fb->exec_seq;
 }
 
