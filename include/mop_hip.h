@@ -193,6 +193,12 @@ void mop_hip_texture_destroy(MopHipDevice *dev, MopHipTexture *tex);       /* rh
 bool mop_hip_texture_read_rgba8(MopHipDevice *dev, MopHipTexture *tex, uint8_t *out,
                                 size_t size);                              /* rhi.h:319-320 */
 float mop_hip_frame_gpu_time_ms(MopHipDevice *dev);                        /* rhi.h:233 */
+/* draw_skybox rhi.h:291-293: the environment map as the frame's background,
+ * from the inverse view-projection and camera position the core passes
+ * (viewport.c:2209-2219); recorded to land before the frame's draws. */
+void mop_hip_draw_skybox(MopHipDevice *dev, MopHipFramebuffer *fb, MopHipTexture *env_map,
+                         const float *inv_vp, const float *cam_pos, float rotation,
+                         float intensity);
 void mop_hip_set_exposure(MopHipDevice *dev, float exposure);              /* rhi.h:238 */
 
 /* The filled MopRhiBackend-layout table (name "hip"). */
@@ -223,6 +229,14 @@ void mop_hip_ibl_set(MopHipDevice *dev, const float *irradiance, int irr_w, int 
                      const float *brdf_lut, int brdf_size, float rotation, float intensity);
 /* mop_sw_ibl_clear (rasterizer.c:183). */
 void mop_hip_ibl_clear(MopHipDevice *dev);
+/* pass_background_hdri_cpu (viewport.c:2128-2180): the core's CPU skybox,
+ * with its inputs (vp->env_texture, vp->cam_fov_radians, cam_eye,
+ * cam_target, cam_up, env_rotation, env_intensity); the background equals
+ * the CPU backend's bit for bit.  A core integrating this backend calls it
+ * where it now calls the CPU skybox for the CPU backend. */
+void mop_hip_draw_background_hdri(MopHipDevice *dev, MopHipFramebuffer *fb, MopHipTexture *env,
+                                  float fov_radians, MopHipVec3 eye, MopHipVec3 target,
+                                  MopHipVec3 up, float rotation, float intensity);
 /* mop_sw_hdr_resolve (rasterizer.c:2776-2803). */
 void mop_hip_hdr_resolve(MopHipDevice *dev, MopHipFramebuffer *fb, float exposure);
 /* mop_postprocess_apply (postprocess.c:24-116): FXAA then fog/tonemap/gamma/vignette. */
@@ -232,6 +246,15 @@ void mop_hip_postprocess_apply(MopHipDevice *dev, MopHipFramebuffer *fb, uint32_
  * read back to host memory owned by the framebuffer. */
 const float *mop_hip_framebuffer_read_hdr(MopHipDevice *dev, MopHipFramebuffer *fb, int *out_w,
                                           int *out_h);
+/* mop_viewport_read_color's SSAA box downsample (viewport.c:3385-3411) run on
+ * the device: the internal image averaged over factor x factor blocks
+ * (truncating integer mean per channel), returned at (w / factor) x
+ * (h / factor) in host memory owned by the framebuffer.  A core integrating
+ * this backend calls it where it now reads the internal image and filters on
+ * the host. */
+const uint8_t *mop_hip_framebuffer_read_color_downsampled(MopHipDevice *dev,
+                                                          MopHipFramebuffer *fb, int factor,
+                                                          int *out_w, int *out_h);
 /* Asynchronous readback (no counterpart in the reference, whose buffers are
  * host memory; it serves the deferred readback that frame_submit /
  * frame_wait_readback imply, rhi.h:163-171).  Starts copying the current

@@ -172,6 +172,7 @@ SIGS = {
     "mop_hip_overflow_reruns": (C.c_uint64, []),
     "mop_hip_framebuffer_readback_async": (None, [VP, VP, C.c_uint32]),
     "mop_hip_shadow_map_read": (C.c_int, [VP, VP, c_float_p]),
+    "mop_hip_framebuffer_read_color_downsampled": (c_u8_p, [VP, VP, C.c_int, c_int_p, c_int_p]),
     "mop_hip_shadow_render_noalloc": (C.c_int, [VP, VP, VP, C.c_uint32, VP, C.c_uint32,
                                                 C.POINTER(Mat4), C.POINTER(Mat4), C.c_int]),
 }

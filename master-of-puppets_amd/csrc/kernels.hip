@@ -1815,8 +1815,9 @@ DEV uint32_t xcd_tile(uint32_t b, uint32_t n) {
 DEV void shade_clear_px(const ViewGPU &fa, size_t idx) {
   if (!fa.shade_clear)
     return;
-  reinterpret_cast<float4 *>(fa.hdr)[idx] = make_float4(fa.clear[0], fa.clear[1], fa.clear[2],
-                                                        fa.clear[3]);
+  if (fa.shade_clear == 1)
+    reinterpret_cast<float4 *>(fa.hdr)[idx] =
+        make_float4(fa.clear[0], fa.clear[1], fa.clear[2], fa.clear[3]);
   fa.depth[idx] = 1.0f;
   fa.ids[idx] = 0u;
 }
@@ -2519,6 +2520,88 @@ __global__ __launch_bounds__(WG) void k_shadow_big(ShadowArgsGPU sa) {
     }
     __syncthreads();
   }
+}
+
+/* SSAA box downsample of mop_viewport_read_color (viewport.c:3385-3411): each
+ * output pixel is the integer mean (truncating division) of its sf x sf
+ * block of the internal RGBA8 image, channel by channel. */
+__global__ __launch_bounds__(WG) void k_downsample(const uint8_t *src, int src_w, uint8_t *dst,
+                                                   int pw, int ph, int sf) {
+  int i = (int)(blockIdx.x * WG + threadIdx.x);
+  if (i >= pw * ph)
+    return;
+  int x = i % pw, y = i / pw;
+  int r = 0, g = 0, b = 0, a = 0;
+  for (int dy = 0; dy < sf; dy++) {
+    const uchar4 *row = reinterpret_cast<const uchar4 *>(src) + (size_t)(y * sf + dy) * src_w;
+    for (int dx = 0; dx < sf; dx++) {
+      uchar4 p = row[x * sf + dx];
+      r += p.x;
+      g += p.y;
+      b += p.z;
+      a += p.w;
+    }
+  }
+  int n = sf * sf;
+  reinterpret_cast<uchar4 *>(dst)[i] =
+      make_uchar4((uint8_t)(r / n), (uint8_t)(g / n), (uint8_t)(b / n), (uint8_t)(a / n));
+}
+
+void launch_downsample(const uint8_t *src, int src_w, uint8_t *dst, int pw, int ph, int sf,
+                       hipStream_t s) {
+  int n = pw * ph;
+  if (n > 0)
+    launch(k_downsample, dim3((n + WG - 1) / WG), dim3(WG), 0, s, src, src_w, dst, pw, ph, sf);
+}
+
+/* The environment background (pass_background_hdri_cpu, viewport.c:2128-2180):
+ * per pixel the camera ray through its centre, normalized, mapped to the
+ * equirect map with the host library's atan2f/asinf, sampled bilinearly and
+ * scaled by the intensity; alpha 1.  Mode 1 forms the ray from the camera
+ * basis and tan(fov/2) the host computed in the CPU path's order (bit-exact
+ * with it); mode 2 from the inverse view-projection and the camera position
+ * (the RHI draw_skybox arguments). */
+__global__ __launch_bounds__(WG) void k_background(BackgroundGPU bg) {
+  int i = (int)(blockIdx.x * WG + threadIdx.x);
+  if (i >= bg.w * bg.h)
+    return;
+  int x = i % bg.w, y = i / bg.w;
+  float dx, dy, dz;
+  if (bg.mode == 1) {
+    float ndc_y = 1.0f - 2.0f * ((float)y + 0.5f) / (float)bg.h;
+    float ndc_x = 2.0f * ((float)x + 0.5f) / (float)bg.w - 1.0f;
+    dx = bg.fwd[0] + bg.right[0] * ndc_x * bg.half_w + bg.up[0] * ndc_y * bg.half_h;
+    dy = bg.fwd[1] + bg.right[1] * ndc_x * bg.half_w + bg.up[1] * ndc_y * bg.half_h;
+    dz = bg.fwd[2] + bg.right[2] * ndc_x * bg.half_w + bg.up[2] * ndc_y * bg.half_h;
+  } else {
+    float ndc_x = 2.0f * ((float)x + 0.5f) / (float)bg.w - 1.0f;
+    float ndc_y = 1.0f - 2.0f * ((float)y + 0.5f) / (float)bg.h;
+    float4 p = mat4_mul_vec4(bg.inv_vp, ndc_x, ndc_y, 1.0f, 1.0f);
+    float iw = 1.0f / p.w;
+    dx = p.x * iw - bg.cam[0];
+    dy = p.y * iw - bg.cam[1];
+    dz = p.z * iw - bg.cam[2];
+  }
+  float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  dx /= len;
+  dy /= len;
+  dz /= len;
+  const float pi = 3.14159265358979f;
+  float phi_val = mop_atan2f(dz, dx) - bg.rotation;
+  float clamped_dy = dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy);
+  float theta_val = mop_asinf(clamped_dy);
+  float eu = phi_val / (2.0f * pi) + 0.5f;
+  float ev = theta_val / pi + 0.5f;
+  float smp[3];
+  ibl_sample(bg.env, bg.env_w, bg.env_h, eu, ev, smp);
+  reinterpret_cast<float4 *>(bg.hdr)[i] =
+      make_float4(smp[0] * bg.intensity, smp[1] * bg.intensity, smp[2] * bg.intensity, 1.0f);
+}
+
+void launch_background(const BackgroundGPU &bg, hipStream_t s) {
+  int n = bg.w * bg.h;
+  if (n > 0 && bg.env)
+    launch(k_background, dim3((n + WG - 1) / WG), dim3(WG), 0, s, bg);
 }
 
 /* One caster record written from kernel arguments (no host staging). */

@@ -185,7 +185,9 @@ struct ViewGPU {
   int32_t irr_w, irr_h, pf_w, pf_h, pf_levels, lut_size;
   float ibl_rot, ibl_int;
   /* 1: this segment opens a cleared frame, and the shading pass writes the
-   * clear values (colour, depth 1, id 0) to the pixels no triangle covers */
+   * clear values (colour, depth 1, id 0) to the pixels no triangle covers;
+   * 2: the same but depth and id only (an environment background already
+   * wrote the colour) */
   int32_t shade_clear;
   float clear[4];
 };
@@ -359,6 +361,25 @@ void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);
 void launch_shadow_raster(const ShadowArgsGPU &sa, hipStream_t s);
 void launch_put_shadow_draw(const ShadowDrawGPU &g, ShadowDrawGPU *dst, hipStream_t s);
+void launch_downsample(const uint8_t *src, int src_w, uint8_t *dst, int pw, int ph, int sf,
+                       hipStream_t s);
+
+/* Environment background of one view (the CPU skybox of viewport.c:2128-2180
+ * or the RHI draw_skybox form): written to every pixel's HDR before the
+ * frame's draws. */
+struct BackgroundGPU {
+  const float *env; /* RGBA32F equirect */
+  int32_t env_w, env_h;
+  int32_t w, h;
+  int32_t mode; /* 1: camera basis (CPU path), 2: inverse view-projection */
+  float fwd[3], right[3], up[3];
+  float half_w, half_h;
+  float inv_vp[16];
+  float cam[3];
+  float rotation, intensity;
+  float *hdr;
+};
+void launch_background(const BackgroundGPU &bg, hipStream_t s);
 void launch_shadow_finalize(int n, const uint32_t *keys, float *depth,
                             hipStream_t s);
 void launch_fill_u32(size_t n, uint32_t *p, uint32_t v, hipStream_t s);

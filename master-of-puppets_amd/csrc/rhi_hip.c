@@ -38,6 +38,7 @@ static const MopHipRhiBackend s_hip_backend = {
     .frame_gpu_time_ms = mop_hip_frame_gpu_time_ms,
     .set_exposure = mop_hip_set_exposure,
     .texture_read_rgba8 = mop_hip_texture_read_rgba8,
+    .draw_skybox = mop_hip_draw_skybox,
 };
 
 const void *mop_rhi_backend_hip(void) { return &s_hip_backend; }

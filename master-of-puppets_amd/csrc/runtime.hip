@@ -96,7 +96,8 @@ struct MopHipTexture {
   int w = 0, h = 0;
   bool hdr = false;
   void *host = nullptr;
-  void *dev = nullptr; /* RGBA8 only (draw textures) */
+  void *dev = nullptr; /* RGBA8 draw textures; RGBA32F environment maps (uploaded on
+                          first use as a background) */
 };
 
 struct PendingDraw {
@@ -160,6 +161,15 @@ struct MopHipFramebuffer {
   bool rb_pending = false;
   uint32_t rb_mask = 0;
   uint64_t rb_seq = 0, exec_seq = 0;
+  /* environment background of the recorded frame (mop_hip_draw_background_hdri
+   * or the draw_skybox slot) */
+  bool has_bg = false;
+  BackgroundGPU bg = {};
+  /* SSAA-downsampled RGBA8 (mop_hip_framebuffer_read_color_downsampled) */
+  uint8_t *ds_dev = nullptr, *ds_host = nullptr;
+  size_t ds_cap = 0;
+  uint64_t ds_seq = ~0ull;
+  int ds_factor = 0;
 };
 
 #define MOPH_MAX_SEGS 32
@@ -900,6 +910,99 @@ extern "C" MopHipTexture *mop_hip_texture_create_hdr(MopHipDevice *d, int w, int
   return t;
 }
 
+/* Device copy of an RGBA32F texture (environment map), made on first use. */
+static const float *texture_device_f32(MopHipDevice *d, MopHipTexture *t) {
+  if (!t || !t->hdr || !t->host)
+    return nullptr;
+  if (!t->dev) {
+    size_t n = (size_t)t->w * t->h * 4 * sizeof(float);
+    if (hipMalloc(&t->dev, n) != hipSuccess) {
+      t->dev = nullptr;
+      set_err("environment map upload failed");
+      return nullptr;
+    }
+    HIPCHK(hipMemcpy(t->dev, t->host, n, hipMemcpyHostToDevice));
+  }
+  return (const float *)t->dev;
+}
+
+/* pass_background_hdri_cpu (viewport.c:2128-2180) for this backend: the
+ * environment as the frame's background, from the camera the CPU path uses
+ * (vertical fov in radians, eye, target, up).  The camera basis and
+ * tan(fov/2) are formed here in the CPU path's order (math.c:30-50); the
+ * device maps each pixel's ray as it does, so the background is the CPU
+ * backend's bit for bit.  Recorded like a draw: it lands after the frame's
+ * clear and before its draws. */
+extern "C" void mop_hip_draw_background_hdri(MopHipDevice *d, MopHipFramebuffer *fb,
+                                             MopHipTexture *env, float fov_radians,
+                                             MopHipVec3 eye, MopHipVec3 target, MopHipVec3 up,
+                                             float rotation, float intensity) {
+  if (!d || !fb)
+    return;
+  const float *ed = texture_device_f32(d, env);
+  if (!ed)
+    return;
+  auto sub = [](MopHipVec3 a, MopHipVec3 b) { return MopHipVec3{a.x - b.x, a.y - b.y, a.z - b.z}; };
+  auto cross = [](MopHipVec3 a, MopHipVec3 b) {
+    return MopHipVec3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+  };
+  auto norm = [](MopHipVec3 v) {
+    float len = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    if (len < 1e-8f)
+      return MopHipVec3{0.0f, 0.0f, 0.0f};
+    float inv = 1.0f / len;
+    return MopHipVec3{v.x * inv, v.y * inv, v.z * inv};
+  };
+  MopHipVec3 fwd = norm(sub(target, eye));
+  MopHipVec3 right = norm(cross(fwd, up));
+  MopHipVec3 upv = cross(right, fwd);
+  float aspect = (float)fb->w / (float)fb->h;
+  float half_h = tanf(fov_radians * 0.5f);
+  float half_w = half_h * aspect;
+  BackgroundGPU &bg = fb->bg;
+  memset(&bg, 0, sizeof(bg));
+  bg.env = ed;
+  bg.env_w = env->w;
+  bg.env_h = env->h;
+  bg.mode = 1;
+  bg.fwd[0] = fwd.x; bg.fwd[1] = fwd.y; bg.fwd[2] = fwd.z;
+  bg.right[0] = right.x; bg.right[1] = right.y; bg.right[2] = right.z;
+  bg.up[0] = upv.x; bg.up[1] = upv.y; bg.up[2] = upv.z;
+  bg.half_w = half_w;
+  bg.half_h = half_h;
+  bg.rotation = rotation;
+  bg.intensity = intensity;
+  fb->has_bg = true;
+  fb->executed = false;
+}
+
+/* The RHI's draw_skybox slot (rhi.h:291-293), which the core calls for
+ * backends other than the CPU one (viewport.c:2209-2219): the background
+ * from the inverse view-projection and camera position. */
+extern "C" void mop_hip_draw_skybox(MopHipDevice *d, MopHipFramebuffer *fb, MopHipTexture *env,
+                                    const float *inv_vp, const float *cam_pos, float rotation,
+                                    float intensity) {
+  if (!d || !fb || !inv_vp || !cam_pos)
+    return;
+  const float *ed = texture_device_f32(d, env);
+  if (!ed)
+    return;
+  BackgroundGPU &bg = fb->bg;
+  memset(&bg, 0, sizeof(bg));
+  bg.env = ed;
+  bg.env_w = env->w;
+  bg.env_h = env->h;
+  bg.mode = 2;
+  memcpy(bg.inv_vp, inv_vp, sizeof(bg.inv_vp));
+  bg.cam[0] = cam_pos[0];
+  bg.cam[1] = cam_pos[1];
+  bg.cam[2] = cam_pos[2];
+  bg.rotation = rotation;
+  bg.intensity = intensity;
+  fb->has_bg = true;
+  fb->executed = false;
+}
+
 extern "C" void mop_hip_texture_destroy(MopHipDevice *d, MopHipTexture *t) {
   if (!t)
     return;
@@ -933,6 +1036,11 @@ static void fb_free(MopHipFramebuffer *fb) {
   if (fb->wbuf) (void)hipFree(fb->wbuf);
   if (fb->h_color) (void)hipHostFree(fb->h_color);
   if (fb->h_hdr) (void)hipHostFree(fb->h_hdr);
+  if (fb->ds_dev) (void)hipFree(fb->ds_dev);
+  if (fb->ds_host) (void)hipHostFree(fb->ds_host);
+  fb->ds_dev = fb->ds_host = nullptr;
+  fb->ds_cap = 0;
+  fb->ds_seq = ~0ull;
   if (fb->h_cnt) (void)hipHostFree(fb->h_cnt);
   if (fb->done) (void)hipEventDestroy(fb->done);
   if (fb->rb_done) (void)hipEventDestroy(fb->rb_done);
@@ -1062,6 +1170,7 @@ extern "C" void mop_hip_frame_begin(MopHipDevice *d, MopHipFramebuffer *fb, MopH
   fb->resolve_set = false;
   fb->post_effects = 0;
   fb->has_fog = false;
+  fb->has_bg = false;
   d->state_dirty = true;
   invalidate_host(fb);
 }
@@ -1291,7 +1400,7 @@ static void fill_view(MopHipFramebuffer *fb, const ShadeState &st, int seg, View
   v.ibl_rot = st.ibl_rot;
   v.ibl_int = st.ibl_int;
   if (seg == 0 && fb->shade_clear) {
-    v.shade_clear = 1;
+    v.shade_clear = fb->has_bg ? 2 : 1;
     memcpy(v.clear, fb->clear, sizeof(v.clear));
   }
 }
@@ -1852,6 +1961,16 @@ static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &
       launch_clear_views(d->clear_views.p, (uint32_t)cv.size(), cpx, d->stream);
     }
   }
+  /* environment backgrounds, after the clears and before any draw (the
+   * core's background pass follows frame_begin, viewport.c:3211-3219) */
+  for (MopHipFramebuffer *fb : todo)
+    if (fb->has_bg) {
+      BackgroundGPU bg = fb->bg;
+      bg.hdr = fb->hdr;
+      bg.w = fb->w;
+      bg.h = fb->h;
+      launch_background(bg, d->stream);
+    }
   for (size_t r = 0; r < rounds; r++) {
     std::vector<SegJob> jobs[2];
     for (size_t i = 0; i < todo.size(); i++)
@@ -2173,6 +2292,46 @@ extern "C" const float *mop_hip_framebuffer_read_hdr(MopHipDevice *d, MopHipFram
     fb->v_hdr = true;
   }
   return fb->h_hdr;
+}
+
+/* The RGBA8 image box-filtered by an integer factor on the device
+ * (viewport.c:3385-3411 does it on the host after framebuffer_read_color):
+ * the internal w x h image to (w / factor) x (h / factor), each channel the
+ * truncated integer mean of its factor x factor block.  The pointer is host
+ * memory owned by the framebuffer, valid until the next frame. */
+extern "C" const uint8_t *mop_hip_framebuffer_read_color_downsampled(MopHipDevice *d,
+                                                                     MopHipFramebuffer *fb,
+                                                                     int factor, int *ow,
+                                                                     int *oh) {
+  if (!d || !fb || factor < 1 || factor > 16)
+    return nullptr;
+  ensure_ready(d, fb);
+  int pw = fb->w / factor, ph = fb->h / factor;
+  if (ow) *ow = pw;
+  if (oh) *oh = ph;
+  if (pw < 1 || ph < 1)
+    return nullptr;
+  if (fb->ds_seq == fb->exec_seq && fb->ds_factor == factor)
+    return fb->ds_host;
+  size_t bytes = (size_t)pw * ph * 4;
+  if (bytes > fb->ds_cap) {
+    if (fb->ds_dev) (void)hipFree(fb->ds_dev);
+    if (fb->ds_host) (void)hipHostFree(fb->ds_host);
+    fb->ds_dev = fb->ds_host = nullptr;
+    fb->ds_cap = 0;
+    if (hipMalloc(&fb->ds_dev, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&fb->ds_host, bytes, 0) != hipSuccess) {
+      set_err("downsample buffer allocation failed");
+      return nullptr;
+    }
+    fb->ds_cap = bytes;
+  }
+  launch_downsample(fb->color[fb->cur], fb->w, fb->ds_dev, pw, ph, factor, d->stream);
+  HIPCHK(hipMemcpyAsync(fb->ds_host, fb->ds_dev, bytes, hipMemcpyDeviceToHost, d->stream));
+  HIPCHK(hipStreamSynchronize(d->stream));
+  fb->ds_seq = fb->exec_seq;
+  fb->ds_factor = factor;
+  return fb->ds_host;
 }
 
 extern "C" uint32_t mop_hip_pick_read_id(MopHipDevice *d, MopHipFramebuffer *fb, int x, int y) {

@@ -138,6 +138,7 @@ class Scene:
     ssaa: int = 1                 # MopViewportDesc.ssaa_factor (internal buffers ssaa x size)
     light_dir: tuple = (0.3, 1.0, 0.5)  # viewport light_dir (viewport.c:643)
     ambient: float = None         # None keeps the viewport default (0.25)
+    env_background: bool = False  # environment map drawn as the skybox behind the scene
 
     @property
     def triangle_count(self):

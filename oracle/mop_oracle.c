@@ -121,6 +121,7 @@ typedef struct ShadeState {
 struct MopHipFramebuffer {
   int w, h;
   uint8_t *color;
+  uint8_t *ds; /* SSAA-downsampled colour */
   float *hdr, *depth;
   uint32_t *ids;
   uint64_t *vis;
@@ -301,6 +302,8 @@ static void fb_free(MopHipFramebuffer *fb) {
   free(fb->vis);
   free(fb->tris);
   free(fb->draws);
+  free(fb->ds);
+  fb->ds = NULL;
   fb->tris = NULL;
   fb->draws = NULL;
   fb->ntris = fb->cap = fb->ndraws = fb->dcap = 0;
@@ -1733,6 +1736,96 @@ const uint8_t *mop_oracle_framebuffer_read_color(MopHipDevice *d, MopHipFramebuf
   if (h) *h = fb->h;
   return fb->color;
 }
+/* viewport.c:3385-3411: box filter of sf x sf blocks, truncating integer
+ * mean per channel (the restatement's counterpart of the device kernel) */
+const uint8_t *mop_oracle_framebuffer_read_color_downsampled(MopHipDevice *d,
+                                                             MopHipFramebuffer *fb, int sf,
+                                                             int *ow, int *oh) {
+  (void)d;
+  int pw = fb->w / sf, ph = fb->h / sf;
+  if (ow) *ow = pw;
+  if (oh) *oh = ph;
+  if (sf < 1 || pw < 1 || ph < 1)
+    return NULL;
+  free(fb->ds);
+  fb->ds = malloc((size_t)pw * ph * 4);
+  if (!fb->ds)
+    return NULL;
+  for (int y = 0; y < ph; y++)
+    for (int x = 0; x < pw; x++) {
+      int acc[4] = {0, 0, 0, 0};
+      for (int dy = 0; dy < sf; dy++)
+        for (int dx = 0; dx < sf; dx++) {
+          const uint8_t *p = &fb->color[((size_t)(y * sf + dy) * fb->w + x * sf + dx) * 4];
+          for (int c = 0; c < 4; c++)
+            acc[c] += p[c];
+        }
+      for (int c = 0; c < 4; c++)
+        fb->ds[((size_t)y * pw + x) * 4 + c] = (uint8_t)(acc[c] / (sf * sf));
+    }
+  return fb->ds;
+}
+
+/* pass_background_hdri_cpu (viewport.c:2128-2180) restated: the camera ray
+ * of every pixel mapped to the equirect environment and sampled bilinearly
+ * into the HDR buffer (the restatement draws it at once: frame_begin has
+ * cleared and no draw has run yet). */
+void mop_oracle_draw_background_hdri(MopHipDevice *d, MopHipFramebuffer *fb, MopHipTexture *env,
+                                     float fov, V3 eye, V3 target, V3 up, float rotation,
+                                     float intensity) {
+  (void)d;
+  if (!fb || !env || !env->hdr)
+    return;
+  const float *data = (const float *)env->data;
+  int sw = fb->w, sh = fb->h;
+  float aspect = (float)sw / (float)sh;
+  float half_h = tanf(fov * 0.5f);
+  float half_w = half_h * aspect;
+  V3 fwd = vnorm(vsub(target, eye));
+  V3 c1 = v3(fwd.y * up.z - fwd.z * up.y, fwd.z * up.x - fwd.x * up.z,
+             fwd.x * up.y - fwd.y * up.x);
+  V3 right = vnorm(c1);
+  V3 u = v3(right.y * fwd.z - right.z * fwd.y, right.z * fwd.x - right.x * fwd.z,
+            right.x * fwd.y - right.y * fwd.x);
+  const float pi = 3.14159265358979f;
+  for (int y = 0; y < sh; y++) {
+    float ndc_y = 1.0f - 2.0f * ((float)y + 0.5f) / (float)sh;
+    for (int x = 0; x < sw; x++) {
+      float ndc_x = 2.0f * ((float)x + 0.5f) / (float)sw - 1.0f;
+      float dx = fwd.x + right.x * ndc_x * half_w + u.x * ndc_y * half_h;
+      float dy = fwd.y + right.y * ndc_x * half_w + u.y * ndc_y * half_h;
+      float dz = fwd.z + right.z * ndc_x * half_w + u.z * ndc_y * half_h;
+      float len = sqrtf(dx * dx + dy * dy + dz * dz);
+      dx /= len;
+      dy /= len;
+      dz /= len;
+      float phi_val = atan2f(dz, dx) - rotation;
+      float cdy = dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy);
+      float theta_val = asinf(cdy);
+      float eu = phi_val / (2.0f * pi) + 0.5f;
+      float ev = theta_val / pi + 0.5f;
+      eu = eu - floorf(eu);
+      ev = ev - floorf(ev);
+      float fx = eu * (float)(env->w - 1), fy = ev * (float)(env->h - 1);
+      int x0 = (int)fx, y0 = (int)fy;
+      int x1 = (x0 + 1) % env->w;
+      int y1 = y0 + 1 < env->h ? y0 + 1 : env->h - 1;
+      float sx = fx - (float)x0, sy = fy - (float)y0;
+      const float *p00 = &data[((size_t)y0 * env->w + x0) * 4];
+      const float *p10 = &data[((size_t)y0 * env->w + x1) * 4];
+      const float *p01 = &data[((size_t)y1 * env->w + x0) * 4];
+      const float *p11 = &data[((size_t)y1 * env->w + x1) * 4];
+      float *o = &fb->hdr[((size_t)y * sw + x) * 4];
+      for (int c = 0; c < 3; c++) {
+        float top = p00[c] * (1.0f - sx) + p10[c] * sx;
+        float bot = p01[c] * (1.0f - sx) + p11[c] * sx;
+        o[c] = (top * (1.0f - sy) + bot * sy) * intensity;
+      }
+      o[3] = 1.0f;
+    }
+  }
+}
+
 const uint32_t *mop_oracle_framebuffer_read_object_id(MopHipDevice *d, MopHipFramebuffer *fb,
                                                       int *w, int *h) {
   if (w) *w = fb->w;

@@ -60,6 +60,14 @@ typedef struct Routed {
   void (*hdr_resolve)(MopRhiDevice *, MopRhiFramebuffer *, float);
   void (*postprocess)(MopRhiDevice *, MopRhiFramebuffer *, uint32_t, const MopFogParams *);
   const float *(*read_hdr)(MopRhiDevice *, MopRhiFramebuffer *, int *, int *);
+  /* optional: the SSAA downsample of mop_viewport_read_color on the backend */
+  const uint8_t *(*read_color_ds)(MopRhiDevice *, MopRhiFramebuffer *, int, int *, int *);
+  /* optional: the CPU skybox (viewport.c:2128-2180) on the backend, and the
+   * table with frame_begin wrapped to issue it */
+  void (*background)(MopRhiDevice *, MopRhiFramebuffer *, MopRhiTexture *, float, MopVec3,
+                     MopVec3, MopVec3, float, float);
+  void (*frame_begin)(MopRhiDevice *, MopRhiFramebuffer *, MopColor);
+  MopRhiBackend table;
 } Routed;
 
 static Routed g_routed[2];
@@ -113,6 +121,22 @@ static void *sym(void *h, const char *prefix, const char *name) {
   return dlsym(h, buf);
 }
 
+/* The viewport whose environment background the routed backend draws in
+ * this frame (mh_render sets it; the core's own background pass is skipped
+ * for it, since it would write the CPU backend's framebuffer layout). */
+static MopViewport *g_bg_vp;
+
+static void routed_frame_begin(MopRhiDevice *dev, MopRhiFramebuffer *fb, MopColor clear) {
+  Routed *r = routed();
+  if (!r)
+    return;
+  r->frame_begin(dev, fb, clear);
+  MopViewport *vp = g_bg_vp;
+  if (vp && vp->device == dev && vp->framebuffer == fb && vp->env_texture)
+    r->background(dev, fb, vp->env_texture, vp->cam_fov_radians, vp->cam_eye, vp->cam_target,
+                  vp->cam_up, vp->env_rotation, vp->env_intensity);
+}
+
 /* Loads a backend library for mode 2 (prefix "mop_hip_") or mode 3
  * (prefix "mop_oracle_"); its table comes from mop_rhi_backend_<table>(). */
 int mh_load_backend(int mode, const char *so_path, const char *prefix, const char *table) {
@@ -135,13 +159,25 @@ int mh_load_backend(int mode, const char *so_path, const char *prefix, const cha
   r->hdr_resolve = sym(h, prefix, "hdr_resolve");
   r->postprocess = sym(h, prefix, "postprocess_apply");
   r->read_hdr = sym(h, prefix, "framebuffer_read_hdr");
+  r->read_color_ds = sym(h, prefix, "framebuffer_read_color_downsampled");
   if (!fb || !r->shadow_mesh || !r->shadow_set || !r->shadow_clear || !r->ibl_set ||
       !r->ibl_clear || !r->hdr_resolve || !r->postprocess || !r->read_hdr) {
     fprintf(stderr, "mh_load_backend: missing symbol in %s\n", so_path);
     return -2;
   }
   r->rhi = fb();
-  return r->rhi ? 0 : -3;
+  if (!r->rhi)
+    return -3;
+  r->background = sym(h, prefix, "draw_background_hdri");
+  if (r->background) {
+    /* frame_begin followed by the environment background, which the core
+     * would draw next (viewport.c:3211-3219) */
+    r->table = *r->rhi;
+    r->frame_begin = r->rhi->frame_begin;
+    r->table.frame_begin = routed_frame_begin;
+    r->rhi = &r->table;
+  }
+  return 0;
 }
 
 int mh_load_hip(const char *so_path) { return mh_load_backend(2, so_path, "mop_hip_", "hip"); }
@@ -283,6 +319,17 @@ void __wrap_mop_postprocess_apply(MopSwFramebuffer *fb, uint32_t effects,
   r->postprocess(g_cur_vp->device, (MopRhiFramebuffer *)fb, effects, fog);
 }
 
+/* mop_viewport_read_color (viewport.c:3364-3418): with a routed backend that
+ * has its own SSAA downsample, the box filter runs there (as the core would
+ * call it for that backend) instead of on the host. */
+const uint8_t *__real_mop_viewport_read_color(MopViewport *, int *, int *);
+const uint8_t *__wrap_mop_viewport_read_color(MopViewport *vp, int *w, int *h) {
+  Routed *r = routed();
+  if (!r || !r->read_color_ds || !vp || vp->ssaa_factor <= 1)
+    return __real_mop_viewport_read_color(vp, w, h);
+  return r->read_color_ds(vp->device, vp->framebuffer, vp->ssaa_factor, w, h);
+}
+
 /* ---- scene construction through the public API ------------------------- */
 
 MopViewport *mh_viewport_ssaa(int w, int h, int ssaa) {
@@ -379,6 +426,12 @@ void mh_set_camera(MopViewport *vp, const float *eye, const float *target,
   mop_viewport_set_camera(vp, (MopVec3){eye[0], eye[1], eye[2]},
                           (MopVec3){target[0], target[1], target[2]},
                           (MopVec3){up[0], up[1], up[2]}, fov_deg, znear, zfar);
+}
+
+/* mop_viewport_set_environment_background (environment.c:868-874): the
+ * environment map as the skybox behind the scene */
+void mh_set_env_background(MopViewport *vp, int on) {
+  mop_viewport_set_environment_background(vp, on != 0);
 }
 
 /* mop_viewport_resize (viewport.h:86): the serving checks render many
@@ -514,8 +567,19 @@ static double now_ms(void) {
  * CPU baseline times).  Returns wall milliseconds. */
 double mh_render(MopViewport *vp) {
   g_cur_vp = vp;
+  Routed *r = routed();
+  bool bg = r && r->background && vp->show_env_background && vp->env_hdr_data &&
+            (vp->env_type == MOP_ENV_HDRI || vp->env_type == MOP_ENV_PROCEDURAL_SKY);
+  if (bg) {
+    vp->show_env_background = false;
+    g_bg_vp = vp;
+  }
   double t0 = now_ms();
   mop_viewport_render(vp);
+  if (bg) {
+    vp->show_env_background = true;
+    g_bg_vp = NULL;
+  }
   int w, h;
   (void)mop_viewport_read_color(vp, &w, &h);
   return now_ms() - t0;

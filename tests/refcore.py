@@ -48,6 +48,7 @@ def harness():
                                        C.c_uint32, C.c_void_p, C.c_uint32]
         h.mh_set_camera.argtypes = [VP, F, F, F, C.c_float, C.c_float, C.c_float]
         h.mh_resize.argtypes = [VP, C.c_int, C.c_int]
+        h.mh_set_env_background.argtypes = [VP, C.c_int]
         h.mh_set_shadow_size.argtypes = [C.c_int]
         h.mh_set_clear_color.argtypes = [VP, F]
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
@@ -210,6 +211,8 @@ class RefViewport:
                 pass
             if rc != 0:
                 raise RuntimeError(f"set_environment failed ({rc})")
+            if getattr(scene, "env_background", False):
+                h.mh_set_env_background(self.vp, 1)
 
     def render(self):
         if self.h.mh_set_mode(self.mode) != 0:

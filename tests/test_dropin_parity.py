@@ -79,6 +79,24 @@ def test_wireframe_render_mode(lit):
     _check(scenes.wireframe_scene(32, 320, 200, lit=lit))
 
 
+@pytest.mark.parametrize("rotation", [0.0, 0.7])
+def test_environment_background_skybox(rotation):
+    # the core's CPU skybox, drawn by the backend's background hook before the
+    # frame's draws (opaque, blended and instanced over it)
+    sc = scenes.ordered_scene(10, 200, 150, with_env=True)
+    sc.env_background = True
+    sc.env_rotation = rotation
+    sc.name += f"_sky{rotation}"
+    _check(sc)
+
+
+def test_ssaa_3_device_downsample():
+    # mop_viewport_read_color's box filter on the device (factor 3)
+    sc = scenes.small_scene(seed=26, width=150, height=100, with_env=True)
+    sc.ssaa = 3
+    _check(sc)
+
+
 def test_ssaa_2_downsample_and_pick():
     sc = scenes.small_scene(seed=25, width=128, height=96, with_env=True)
     sc.ssaa = 2

@@ -68,6 +68,16 @@ def test_shadow_size_override_changes_the_map():
     assert (a["hdr"] != b["hdr"]).any()
 
 
+def test_environment_background_skybox():
+    # the CPU skybox (viewport.c:2128-2180) under opaque, transparent and
+    # instanced draws, with a rotated environment
+    sc = scenes.ordered_scene(10, 200, 150, with_env=True)
+    sc.env_background = True
+    sc.env_rotation = 0.7
+    a, _ = _same(sc)
+    assert (a["ids"] == 0).any()
+
+
 def test_dense_ordered_long_tile_lists():
     # tiles overlapped by more than 4096 transparent screen triangles
     _same(scenes.dense_ordered_scene())
