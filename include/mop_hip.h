@@ -237,6 +237,16 @@ void mop_hip_ibl_clear(MopHipDevice *dev);
 void mop_hip_draw_background_hdri(MopHipDevice *dev, MopHipFramebuffer *fb, MopHipTexture *env,
                                   float fov_radians, MopHipVec3 eye, MopHipVec3 target,
                                   MopHipVec3 up, float rotation, float intensity);
+/* mop_overlay_builtin_outline (overlay_builtin.c:1309-1400; called
+ * viewport.c:2738-2739 and :2906-2907): the object outline painted on the
+ * resolved RGBA8 image before the post effects, with the viewport's theme
+ * accent and outline opacities and its selection list.  Applied on the
+ * device by the frame's tail, so the image read back already carries it;
+ * the RGBA8 result equals the overlay's in-place host loop bit for bit.  A
+ * core integrating this backend calls it instead of the host overlay. */
+void mop_hip_draw_outline(MopHipDevice *dev, MopHipFramebuffer *fb, MopHipColor accent,
+                          float alpha_selected, float alpha_unselected,
+                          const uint32_t *selected_ids, uint32_t selected_count);
 /* mop_sw_hdr_resolve (rasterizer.c:2776-2803). */
 void mop_hip_hdr_resolve(MopHipDevice *dev, MopHipFramebuffer *fb, float exposure);
 /* mop_postprocess_apply (postprocess.c:24-116): FXAA then fog/tonemap/gamma/vignette. */

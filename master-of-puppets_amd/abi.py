@@ -173,6 +173,10 @@ SIGS = {
     "mop_hip_framebuffer_readback_async": (None, [VP, VP, C.c_uint32]),
     "mop_hip_shadow_map_read": (C.c_int, [VP, VP, c_float_p]),
     "mop_hip_framebuffer_read_color_downsampled": (c_u8_p, [VP, VP, C.c_int, c_int_p, c_int_p]),
+    "mop_hip_draw_skybox": (None, [VP, VP, VP, c_float_p, c_float_p, C.c_float, C.c_float]),
+    "mop_hip_draw_background_hdri": (None, [VP, VP, VP, C.c_float, Vec3, Vec3, Vec3, C.c_float,
+                                            C.c_float]),
+    "mop_hip_draw_outline": (None, [VP, VP, Color, C.c_float, C.c_float, c_u32_p, C.c_uint32]),
     "mop_hip_shadow_render_noalloc": (C.c_int, [VP, VP, VP, C.c_uint32, VP, C.c_uint32,
                                                 C.POINTER(Mat4), C.POINTER(Mat4), C.c_int]),
 }

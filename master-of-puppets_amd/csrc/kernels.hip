@@ -2151,16 +2151,81 @@ DEV float srgb_enc(float c) {
   return c <= 0.0031308f ? c * 12.92f : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
 }
 
-/* The frame tail of one view tile: mop_sw_hdr_resolve, then (when post
- * effects are requested) mop_sw_fxaa (rasterizer.c:2625-2761) and the
- * per-pixel effects of mop_postprocess_apply (postprocess.c:36-116).  FXAA
- * reads only resolved colours of the pixel's 3x3 neighbourhood, so each
- * workgroup resolves its 16x16 tile plus a one-pixel halo into LDS and never
- * writes the intermediate image.  Without fog or vignette the colour effects
- * are a per-channel table (v.lut). */
+#define OUTLINE_CHROME 0xFFFD0000u
+
+DEV bool sel_has(const uint32_t *sel, uint32_t n, uint32_t id) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (sel[mid] < id)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && sel[lo] == id;
+}
+
+/* one paint of the outline overlay on a channel value (overlay_builtin.c:
+ * 1383-1395): (u8)(c (1 - alpha) + accent alpha) */
+DEV uint32_t outline_paint(uint32_t c, uint32_t acc, float alpha) {
+  return cvt_u8((float)c * (1.0f - alpha) + (float)acc * alpha);
+}
+
+/* The object outline of mop_overlay_builtin_outline (overlay_builtin.c:
+ * 1309-1400) at one destination pixel.  The overlay scans the image in row
+ * order; an edge pixel of a selected object paints the 5x5 block around it,
+ * an edge pixel of another object paints itself, and every paint blends
+ * into the value the earlier paints left.  Paints with one alpha commute, so
+ * a destination's result depends only on how many selected edges of its
+ * 5x5 block come before it in scan order, whether it is an edge itself, and
+ * how many selected edges come after it.  flag: bit 0 edge, bit 1 selected;
+ * (fx, fy) is the destination in the flag array, whose rows are `pitch`
+ * bytes. */
+DEV uint32_t outline_px(uint32_t c, const uint8_t *flag, int pitch, int fx, int fy,
+                        const PostViewGPU &v) {
+  int before = 0, after = 0;
+  for (int dy = -2; dy <= 2; dy++)
+    for (int dx = -2; dx <= 2; dx++) {
+      if (flag[(fy + dy) * pitch + fx + dx] != 3)
+        continue;
+      if (dy < 0 || (dy == 0 && dx < 0))
+        before++;
+      else if (dy > 0 || dx > 0)
+        after++;
+    }
+  uint32_t self = flag[fy * pitch + fx];
+  if (!before && !after && self != 1 && self != 3)
+    return c;
+  uint32_t out = c & 0xff000000u; /* alpha is left alone */
+  for (int k = 0; k < 3; k++) {
+    uint32_t x = ch(c, k), acc = (v.accent >> (8 * k)) & 255u;
+    for (int j = 0; j < before; j++)
+      x = outline_paint(x, acc, v.alpha_sel);
+    if (self == 3)
+      x = outline_paint(x, acc, v.alpha_sel);
+    else if (self == 1)
+      x = outline_paint(x, acc, v.alpha_unsel);
+    for (int j = 0; j < after; j++)
+      x = outline_paint(x, acc, v.alpha_sel);
+    out |= x << (8 * k);
+  }
+  return out;
+}
+
+/* The frame tail of one view tile: mop_sw_hdr_resolve, the object outline
+ * when requested, then (when post effects are requested) mop_sw_fxaa
+ * (rasterizer.c:2625-2761) and the per-pixel effects of
+ * mop_postprocess_apply (postprocess.c:36-116).  FXAA reads only resolved
+ * colours of the pixel's 3x3 neighbourhood, so each workgroup resolves its
+ * 16x16 tile plus a one-pixel halo into LDS and never writes the
+ * intermediate image; the outline of that 18x18 block needs the edge flags
+ * of a 22x22 block, which need the ids of a 24x24 block.  Without fog or
+ * vignette the colour effects are a per-channel table (v.lut). */
 __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t nviews) {
   __shared__ uint32_t col[18][18];
   __shared__ float lum[18][18];
+  __shared__ uint32_t oid[24][24];
+  __shared__ uint8_t oflag[22 * 22];
   uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const PostViewGPU &v = views[find_post_view(views, nviews, tile)];
   uint32_t t = tile - v.tile_base;
@@ -2170,18 +2235,68 @@ __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t 
   int y = y0 + (int)(threadIdx.x >> 4);
   const int w = v.w, h = v.h;
   const uint32_t effects = v.effects;
-  if (effects == 0) {
+  const bool outline = v.outline != 0;
+  if (effects == 0 && !outline) {
     if (x < w && y < h) {
       size_t i = (size_t)y * (size_t)w + (size_t)x;
       reinterpret_cast<uint32_t *>(v.dst)[i] = resolve_px(v, i);
     }
-    return; /* uniform: effects is per view */
+    return; /* uniform: effects and outline are per view */
+  }
+  if (outline) {
+    for (int i = (int)threadIdx.x; i < 24 * 24; i += WG) {
+      int gx = x0 + i % 24 - 4, gy = y0 + i / 24 - 4;
+      uint32_t id = 0;
+      if (gx >= 0 && gx < w && gy >= 0 && gy < h)
+        id = v.ids[(size_t)gy * (size_t)w + (size_t)gx];
+      oid[i / 24][i % 24] = id;
+    }
+    __syncthreads();
+    /* edge pixels: a non-background, non-chrome id with a 4-neighbour in the
+     * image whose id differs and is not chrome (overlay_builtin.c:1342-1370) */
+    for (int i = (int)threadIdx.x; i < 22 * 22; i += WG) {
+      int fx = i % 22, fy = i / 22;
+      int gx = x0 + fx - 3, gy = y0 + fy - 3;
+      uint8_t f = 0;
+      if (gx >= 0 && gx < w && gy >= 0 && gy < h) {
+        int ox = fx + 1, oy = fy + 1;
+        uint32_t id = oid[oy][ox];
+        if (id != 0 && id < OUTLINE_CHROME) {
+          bool edge = false;
+          uint32_t nid;
+          if (gx > 0) {
+            nid = oid[oy][ox - 1];
+            edge |= nid != id && nid < OUTLINE_CHROME;
+          }
+          if (gx < w - 1) {
+            nid = oid[oy][ox + 1];
+            edge |= nid != id && nid < OUTLINE_CHROME;
+          }
+          if (gy > 0) {
+            nid = oid[oy - 1][ox];
+            edge |= nid != id && nid < OUTLINE_CHROME;
+          }
+          if (gy < h - 1) {
+            nid = oid[oy + 1][ox];
+            edge |= nid != id && nid < OUTLINE_CHROME;
+          }
+          if (edge)
+            f = sel_has(v.sel, v.n_sel, id) ? 3 : 1;
+        }
+      }
+      oflag[i] = f;
+    }
+    __syncthreads();
   }
   for (int i = (int)threadIdx.x; i < 18 * 18; i += WG) {
     int lx = x0 + i % 18 - 1, ly = y0 + i / 18 - 1;
     uint32_t c = 0;
-    if (lx >= 0 && lx < w && ly >= 0 && ly < h)
+    if (lx >= 0 && lx < w && ly >= 0 && ly < h) {
       c = resolve_px(v, (size_t)ly * (size_t)w + (size_t)lx);
+      /* chrome destinations are never painted (overlay_builtin.c:1380-1381) */
+      if (outline && oid[i / 18 + 3][i % 18 + 3] < OUTLINE_CHROME)
+        c = outline_px(c, oflag, 22, i % 18 + 2, i / 18 + 2, v);
+    }
     col[i / 18][i % 18] = c;
     lum[i / 18][i % 18] = luma_px(c);
   }
@@ -2190,6 +2305,10 @@ __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t 
     return;
   const int tx = (int)(threadIdx.x & 15) + 1, ty = (int)(threadIdx.x >> 4) + 1;
   uint32_t cM = col[ty][tx];
+  if (effects == 0) {
+    reinterpret_cast<uint32_t *>(v.dst)[(size_t)y * (size_t)w + (size_t)x] = cM;
+    return;
+  }
   uint32_t o0 = ch(cM, 0), o1 = ch(cM, 1), o2 = ch(cM, 2), o3 = ch(cM, 3);
   if ((effects & MOP_HIP_POST_FXAA) && w >= 3 && h >= 3 && x >= 1 && x < w - 1 && y >= 1 &&
       y < h - 1) {

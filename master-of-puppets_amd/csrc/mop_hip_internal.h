@@ -276,6 +276,14 @@ struct PostViewGPU {
   int32_t has_fog;
   int32_t pad_;
   const uint8_t *lut; /* per-channel table of the colour effects, or null */
+  /* object outline on the resolved image before the effects
+   * (overlay_builtin.c:1309-1400); outline == 0 leaves it off */
+  const uint32_t *sel; /* selected object ids, sorted ascending, no repeats */
+  uint32_t n_sel;
+  uint32_t accent; /* (u8)(accent * 255) per channel, packed r | g << 8 | b << 16 */
+  float alpha_sel, alpha_unsel;
+  int32_t outline;
+  int32_t pad2_;
 };
 
 /* One shadow caster mesh (mop_sw_shadow_render_mesh arguments). */

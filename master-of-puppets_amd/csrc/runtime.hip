@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
 #include <mutex>
 #include <unordered_map>
@@ -165,6 +166,11 @@ struct MopHipFramebuffer {
    * or the draw_skybox slot) */
   bool has_bg = false;
   BackgroundGPU bg = {};
+  /* object outline of the recorded frame (mop_hip_draw_outline) */
+  bool outline = false;
+  uint32_t outline_accent = 0;
+  float outline_alpha_sel = 0.0f, outline_alpha_unsel = 0.0f;
+  std::vector<uint32_t> outline_sel; /* sorted, no repeats */
   /* SSAA-downsampled RGBA8 (mop_hip_framebuffer_read_color_downsampled) */
   uint8_t *ds_dev = nullptr, *ds_host = nullptr;
   size_t ds_cap = 0;
@@ -324,6 +330,7 @@ struct MopHipDevice {
   uint32_t *h_counters = nullptr;
   DevArr<ViewGPU> views;
   DevArr<PostViewGPU> pviews;
+  DevArr<uint32_t> outline_ids; /* selection lists of the views' outlines */
   DevArr<ClearViewGPU> clear_views;
   DevArr<uint8_t> post_lut; /* 4 x 256 colour-effect tables */
   DevArr<float> ibl_trig;   /* sine/cosine table of the IBL precompute */
@@ -530,6 +537,7 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   }
   d->views.release();
   d->pviews.release();
+  d->outline_ids.release();
   d->clear_views.release();
   d->tile_counts.release();
   d->tile_cursor.release();
@@ -1171,6 +1179,7 @@ extern "C" void mop_hip_frame_begin(MopHipDevice *d, MopHipFramebuffer *fb, MopH
   fb->post_effects = 0;
   fb->has_fog = false;
   fb->has_bg = false;
+  fb->outline = false;
   d->state_dirty = true;
   invalidate_host(fb);
 }
@@ -1633,6 +1642,13 @@ static void run_tail(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fb
   if (fbs.empty())
     return;
   std::vector<PostViewGPU> pv(fbs.size());
+  size_t n_sel = 0;
+  for (MopHipFramebuffer *fb : fbs)
+    if (fb->outline)
+      n_sel += fb->outline_sel.size();
+  if (n_sel && !d->outline_ids.ensure(n_sel))
+    return;
+  size_t sel_off = 0;
   uint32_t tiles = 0;
   for (size_t i = 0; i < fbs.size(); i++) {
     MopHipFramebuffer *fb = fbs[i];
@@ -1665,17 +1681,37 @@ static void run_tail(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fb
                     ((fb->post_effects & MOP_HIP_POST_GAMMA) ? 2 : 0);
       v.lut = d->post_lut.p + variant * 256;
     }
+    if (fb->outline) {
+      v.outline = 1;
+      v.sel = d->outline_ids.p + sel_off;
+      v.n_sel = (uint32_t)fb->outline_sel.size();
+      sel_off += fb->outline_sel.size();
+      v.accent = fb->outline_accent;
+      v.alpha_sel = fb->outline_alpha_sel;
+      v.alpha_unsel = fb->outline_alpha_unsel;
+    }
     fb->cur = 0;
     fb->v_color = false;
   }
   if (!d->pviews.ensure(pv.size()))
     return;
+  /* views and selection lists staged together (the second copy's source
+   * 16-byte aligned, as uploads read 16 bytes per lane) */
+  size_t pv_bytes = pv.size() * sizeof(PostViewGPU);
+  size_t pv_pad = (pv_bytes + 15) & ~(size_t)15;
   int slot;
-  uint8_t *stg = stage_acquire(d, pv.size() * sizeof(PostViewGPU), &slot);
+  uint8_t *stg = stage_acquire(d, pv_pad + n_sel * 4, &slot);
   if (!stg)
     return;
-  memcpy(stg, pv.data(), pv.size() * sizeof(PostViewGPU));
-  upload(d, d->pviews.p, stg, pv.size() * sizeof(PostViewGPU));
+  memcpy(stg, pv.data(), pv_bytes);
+  for (MopHipFramebuffer *fb : fbs)
+    if (fb->outline && !fb->outline_sel.empty()) {
+      memcpy(stg + pv_pad, fb->outline_sel.data(), fb->outline_sel.size() * 4);
+      pv_pad += fb->outline_sel.size() * 4;
+    }
+  upload(d, d->pviews.p, stg, pv_bytes);
+  if (n_sel)
+    upload(d, d->outline_ids.p, stg + ((pv_bytes + 15) & ~(size_t)15), n_sel * 4);
   stage_release(d, slot);
   TmScope t(d, TM_POST);
   launch_tail_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
@@ -2904,6 +2940,31 @@ extern "C" void mop_hip_frame_request_post(MopHipDevice *d, MopHipFramebuffer *f
   fb->has_fog = fog != nullptr;
   if (fog)
     fb->fog = *fog;
+}
+
+extern "C" void mop_hip_draw_outline(MopHipDevice *d, MopHipFramebuffer *fb, MopHipColor accent,
+                                     float alpha_selected, float alpha_unselected,
+                                     const uint32_t *selected_ids, uint32_t selected_count) {
+  if (!d || !fb || (selected_count && !selected_ids))
+    return;
+  fb->outline = true;
+  /* the accent as the overlay converts it (overlay_builtin.c:1330-1333) */
+  uint32_t ar = (uint8_t)(accent.r * 255.0f), ag = (uint8_t)(accent.g * 255.0f),
+           ab = (uint8_t)(accent.b * 255.0f);
+  fb->outline_accent = ar | (ag << 8) | (ab << 16);
+  fb->outline_alpha_sel = alpha_selected;
+  fb->outline_alpha_unsel = alpha_unselected;
+  fb->outline_sel.assign(selected_ids, selected_ids + selected_count);
+  std::sort(fb->outline_sel.begin(), fb->outline_sel.end());
+  fb->outline_sel.erase(std::unique(fb->outline_sel.begin(), fb->outline_sel.end()),
+                        fb->outline_sel.end());
+  if (!fb->executed)
+    return; /* applied by the frame's tail */
+  /* the tail recomputes from the HDR buffer: resolve, outline, effects */
+  fb_new_contents(d, fb);
+  std::vector<MopHipFramebuffer *> one(1, fb);
+  run_tail(d, one);
+  HIPCHK(hipEventRecord(fb->done, d->stream));
 }
 
 extern "C" void mop_hip_postprocess_apply(MopHipDevice *d, MopHipFramebuffer *fb,

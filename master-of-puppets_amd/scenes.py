@@ -11,6 +11,7 @@ Randomness: splitmix64-seeded xoshiro256** (one stream per scene seed).
 Albedo goes into the vertex colour because the CPU path shades with vertex
 colour and ignores MopMaterial.base_color (cpu_backend.c:265-267).
 """
+import dataclasses
 from dataclasses import dataclass, field
 import math
 
@@ -139,6 +140,9 @@ class Scene:
     light_dir: tuple = (0.3, 1.0, 0.5)  # viewport light_dir (viewport.c:643)
     ambient: float = None         # None keeps the viewport default (0.25)
     env_background: bool = False  # environment map drawn as the skybox behind the scene
+    # object outline overlay: {"accent": rgb, "alpha_selected": a,
+    # "alpha_unselected": b, "selected": [object ids]} or None (off)
+    outline: dict = None
 
     @property
     def triangle_count(self):
@@ -494,6 +498,24 @@ def small_scene(seed=7, width=160, height=120, n=12, with_shadow=True, with_env=
                                outer_cos=math.cos(math.radians(30))))
     if with_env:
         sc.env = sky_env(256, 128)
+    return sc
+
+
+CHROME_ID = 0xFFFD0003  # an id in the range the outline treats as editor chrome
+
+
+def outline_scene(seed=21, width=200, height=150, selected=(1, 3, 6), alpha_selected=0.9,
+                  alpha_unselected=0.35, accent=(1.0, 0.25, 0.6), with_chrome=True):
+    """small_scene with the object outline overlay on: the selected objects
+    (the ground plane among them) get the thick outline, every other object
+    the thin one; one sphere carries an id in the chrome range, which is
+    never outlined nor painted over."""
+    sc = small_scene(seed=seed, width=width, height=height, n=10)
+    sc.name = f"outline_{seed}_{width}x{height}"
+    if with_chrome:
+        sc.meshes[4] = dataclasses.replace(sc.meshes[4], object_id=CHROME_ID)
+    sc.outline = {"accent": accent, "alpha_selected": alpha_selected,
+                  "alpha_unselected": alpha_unselected, "selected": list(selected)}
     return sc
 
 

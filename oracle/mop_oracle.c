@@ -20,14 +20,20 @@
  *                               194-348 (IBL, PCF)
  *   shadow pass                 mop_sw_shadow_render_mesh rasterizer.c:357-493
  *   HDR resolve                 mop_sw_hdr_resolve     rasterizer.c:2770-2803
+ *   object outline              mop_overlay_builtin_outline
+ *                                                      overlay_builtin.c:1309-1400
  *   FXAA + post                 mop_sw_fxaa 2625-2761, postprocess.c:24-116
+ *   environment background      pass_background_hdri_cpu viewport.c:2128-2180
+ *   SSAA box downsample         mop_viewport_read_color viewport.c:3385-3411
  *
  * Checked against the reference build (oracle/_ref) by tests/test_oracle.py:
  * depth, object-ID, HDR and RGBA8 bit-identical on every scene tested.
  *
- * Draw classes: like the HIP backend it renders opaque, opacity >= 1,
- * depth-tested/written, smooth, lit, non-wireframe draws; alpha/additive
- * draws at opacity 0 are no-ops; other draws are skipped and counted.
+ * Draw classes: opaque smooth-lit draws (opacity >= 1, depth-tested and
+ * written) go through the order-free visibility restatement above; every
+ * other draw (blended, flat, unlit, wireframe, depth-test or depth-write off)
+ * is rasterized in submission order by draw_ordered, as the reference's
+ * single-threaded path does (cpu_draw, cpu_backend.c:588-620).
  */
 #include <math.h>
 #include <stdio.h>
@@ -1736,6 +1742,62 @@ const uint8_t *mop_oracle_framebuffer_read_color(MopHipDevice *d, MopHipFramebuf
   if (h) *h = fb->h;
   return fb->color;
 }
+/* mop_overlay_builtin_outline (overlay_builtin.c:1309-1400) on the resolved
+ * image: classify every pixel first (the id buffer does not change while
+ * the overlay paints), then paint in row order -- 5x5 blocks around edges of
+ * selected objects, single pixels at edges of other objects -- each paint
+ * blending into the current colour. */
+#define ORACLE_CHROME 0xFFFD0000u
+void mop_oracle_draw_outline(MopHipDevice *d, MopHipFramebuffer *fb, MopHipColor accent,
+                             float alpha_sel, float alpha_unsel, const uint32_t *sel,
+                             uint32_t n_sel) {
+  (void)d;
+  int w = fb->w, h = fb->h;
+  const uint32_t *ids = fb->ids;
+  uint8_t acc[3] = {(uint8_t)(accent.r * 255.0f), (uint8_t)(accent.g * 255.0f),
+                    (uint8_t)(accent.b * 255.0f)};
+  uint8_t *kind = calloc((size_t)w * h, 1); /* 1 edge of an unselected object, 2 selected */
+  if (!kind)
+    return;
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      uint32_t id = ids[(size_t)y * w + x];
+      if (id == 0 || id >= ORACLE_CHROME)
+        continue;
+      const int nx[4] = {x - 1, x + 1, x, x}, ny[4] = {y, y, y - 1, y + 1};
+      int edge = 0;
+      for (int k = 0; k < 4 && !edge; k++) {
+        if (nx[k] < 0 || nx[k] >= w || ny[k] < 0 || ny[k] >= h)
+          continue;
+        uint32_t o = ids[(size_t)ny[k] * w + nx[k]];
+        edge = o != id && o < ORACLE_CHROME;
+      }
+      if (!edge)
+        continue;
+      int selected = 0;
+      for (uint32_t i = 0; i < n_sel && !selected; i++)
+        selected = sel[i] == id;
+      kind[(size_t)y * w + x] = selected ? 2 : 1;
+    }
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++) {
+      int k = kind[(size_t)y * w + x];
+      if (!k)
+        continue;
+      int r = k == 2 ? 2 : 0;
+      float a = k == 2 ? alpha_sel : alpha_unsel;
+      for (int py = y - r; py <= y + r; py++)
+        for (int px = x - r; px <= x + r; px++) {
+          if (px < 0 || px >= w || py < 0 || py >= h || ids[(size_t)py * w + px] >= ORACLE_CHROME)
+            continue;
+          uint8_t *c = fb->color + ((size_t)py * w + px) * 4;
+          for (int j = 0; j < 3; j++)
+            c[j] = (uint8_t)(c[j] * (1.0f - a) + acc[j] * a);
+        }
+    }
+  free(kind);
+}
+
 /* viewport.c:3385-3411: box filter of sf x sf blocks, truncating integer
  * mean per channel (the restatement's counterpart of the device kernel) */
 const uint8_t *mop_oracle_framebuffer_read_color_downsampled(MopHipDevice *d,

@@ -67,6 +67,9 @@ typedef struct Routed {
   void (*background)(MopRhiDevice *, MopRhiFramebuffer *, MopRhiTexture *, float, MopVec3,
                      MopVec3, MopVec3, float, float);
   void (*frame_begin)(MopRhiDevice *, MopRhiFramebuffer *, MopColor);
+  /* optional: the object outline (overlay_builtin.c:1309-1400) on the backend */
+  void (*outline)(MopRhiDevice *, MopRhiFramebuffer *, MopColor, float, float, const uint32_t *,
+                  uint32_t);
   MopRhiBackend table;
 } Routed;
 
@@ -160,6 +163,7 @@ int mh_load_backend(int mode, const char *so_path, const char *prefix, const cha
   r->postprocess = sym(h, prefix, "postprocess_apply");
   r->read_hdr = sym(h, prefix, "framebuffer_read_hdr");
   r->read_color_ds = sym(h, prefix, "framebuffer_read_color_downsampled");
+  r->outline = sym(h, prefix, "draw_outline");
   if (!fb || !r->shadow_mesh || !r->shadow_set || !r->shadow_clear || !r->ibl_set ||
       !r->ibl_clear || !r->hdr_resolve || !r->postprocess || !r->read_hdr) {
     fprintf(stderr, "mh_load_backend: missing symbol in %s\n", so_path);
@@ -330,6 +334,21 @@ const uint8_t *__wrap_mop_viewport_read_color(MopViewport *vp, int *w, int *h) {
   return r->read_color_ds(vp->device, vp->framebuffer, vp->ssaa_factor, w, h);
 }
 
+/* mop_overlay_builtin_outline (overlay_builtin.c:1309-1400), called after the
+ * HDR resolve and before the post effects (viewport.c:2733-2739): with a
+ * routed backend that has its own outline, the viewport's theme and
+ * selection go to it instead of the host loop over the read-back image. */
+void __real_mop_overlay_builtin_outline(MopViewport *vp, void *user_data);
+void __wrap_mop_overlay_builtin_outline(MopViewport *vp, void *user_data) {
+  Routed *r = routed();
+  if (!r || !r->outline || !vp) {
+    __real_mop_overlay_builtin_outline(vp, user_data);
+    return;
+  }
+  r->outline(vp->device, vp->framebuffer, vp->theme.accent, vp->theme.outline_opacity_selected,
+             vp->theme.outline_opacity_unselected, vp->selected_ids, vp->selected_count);
+}
+
 /* ---- scene construction through the public API ------------------------- */
 
 MopViewport *mh_viewport_ssaa(int w, int h, int ssaa) {
@@ -432,6 +451,23 @@ void mh_set_camera(MopViewport *vp, const float *eye, const float *target,
  * environment map as the skybox behind the scene */
 void mh_set_env_background(MopViewport *vp, int on) {
   mop_viewport_set_environment_background(vp, on != 0);
+}
+
+/* The object outline overlay (overlay.h:68) with the theme's accent and
+ * outline opacities (theme.h:26-69), and the selection it outlines
+ * (selection.h:48; additive toggles). */
+void mh_set_outline(MopViewport *vp, int on, const float *accent, float alpha_selected,
+                    float alpha_unselected) {
+  MopTheme t = *mop_viewport_get_theme(vp);
+  t.accent = (MopColor){accent[0], accent[1], accent[2], 1.0f};
+  t.outline_opacity_selected = alpha_selected;
+  t.outline_opacity_unselected = alpha_unselected;
+  mop_viewport_set_theme(vp, &t);
+  mop_viewport_set_overlay_enabled(vp, MOP_OVERLAY_OUTLINE, on != 0);
+}
+
+void mh_select(MopViewport *vp, uint32_t id, int additive) {
+  mop_viewport_select_object(vp, id, additive != 0);
 }
 
 /* mop_viewport_resize (viewport.h:86): the serving checks render many

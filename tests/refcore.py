@@ -49,6 +49,8 @@ def harness():
         h.mh_set_camera.argtypes = [VP, F, F, F, C.c_float, C.c_float, C.c_float]
         h.mh_resize.argtypes = [VP, C.c_int, C.c_int]
         h.mh_set_env_background.argtypes = [VP, C.c_int]
+        h.mh_set_outline.argtypes = [VP, C.c_int, F, C.c_float, C.c_float]
+        h.mh_select.argtypes = [VP, C.c_uint32, C.c_int]
         h.mh_set_shadow_size.argtypes = [C.c_int]
         h.mh_set_clear_color.argtypes = [VP, F]
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
@@ -213,6 +215,14 @@ class RefViewport:
                 raise RuntimeError(f"set_environment failed ({rc})")
             if getattr(scene, "env_background", False):
                 h.mh_set_env_background(self.vp, 1)
+        ol = getattr(scene, "outline", None)
+        if ol is not None:
+            # {"accent": rgb, "alpha_selected": a, "alpha_unselected": b,
+            #  "selected": [ids]} -- the outline overlay with that theme
+            h.mh_set_outline(self.vp, 1, _f3(ol["accent"]), ol["alpha_selected"],
+                             ol["alpha_unselected"])
+            for oid in ol.get("selected", ()):
+                h.mh_select(self.vp, int(oid), 1)
 
     def render(self):
         if self.h.mh_set_mode(self.mode) != 0:

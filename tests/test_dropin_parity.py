@@ -90,6 +90,26 @@ def test_environment_background_skybox(rotation):
     _check(sc)
 
 
+@pytest.mark.parametrize("alpha_unselected", [0.0, 0.35])
+def test_object_outline(alpha_unselected):
+    # mop_overlay_builtin_outline on the device, fused into the frame tail
+    # between the resolve and FXAA + gamma: thick (selected) and thin
+    # (unselected) outlines with order-dependent overlaps, a chrome-range id
+    sc = scenes.outline_scene(alpha_unselected=alpha_unselected)
+    sc.name += f"_un{alpha_unselected}"
+    _check(sc)
+
+
+def test_object_outline_no_post_two_frames_ssaa():
+    # the outline-only tail (no post effects), re-recorded every frame, on the
+    # internal SSAA image before the downsample
+    sc = scenes.outline_scene(seed=22, width=120, height=90, selected=(2, 5))
+    sc.post_effects = 0
+    sc.ssaa = 2
+    sc.name += "_nopost_ssaa2"
+    _check(sc, frames=2)
+
+
 def test_ssaa_3_device_downsample():
     # mop_viewport_read_color's box filter on the device (factor 3)
     sc = scenes.small_scene(seed=26, width=150, height=100, with_env=True)

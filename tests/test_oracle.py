@@ -4,6 +4,8 @@ backend (single-threaded, submission order) and the restatement, on the same
 scenes; every buffer must be bit-identical.  A planted mistake in the
 restatement's arithmetic (a dropped term, a swapped operand) changes depth
 or colour bits somewhere in these scenes, so each stage is pinned here."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -76,6 +78,23 @@ def test_environment_background_skybox():
     sc.env_rotation = 0.7
     a, _ = _same(sc)
     assert (a["ids"] == 0).any()
+
+
+@pytest.mark.parametrize("alpha_unselected", [0.0, 0.35])
+def test_object_outline(alpha_unselected):
+    # mop_overlay_builtin_outline (overlay_builtin.c:1309-1400) before FXAA and
+    # gamma: thick outlines of selected objects overlapping thin ones of
+    # others (order-dependent blends), a chrome-range id left alone; the
+    # theme default leaves unselected objects unpainted (theme.c:72-73)
+    sc = scenes.outline_scene(alpha_unselected=alpha_unselected)
+    a, b = _same(sc)
+    plain = dataclasses.replace(sc, outline=None)
+    c = refcore.render(plain, 1)
+    changed = (c["color"] != b["color"]).any(axis=2)
+    assert changed.sum() > 200
+    assert (a["ids"] == scenes.CHROME_ID).any()
+    # the overlay paints RGBA8 only
+    assert np.array_equal(c["depth"], b["depth"]) and np.array_equal(c["ids"], b["ids"])
 
 
 def test_dense_ordered_long_tile_lists():
