@@ -79,6 +79,12 @@ class Camera(C.Structure):
                 ("znear", C.c_float), ("zfar", C.c_float)]
 
 
+class LoadedMesh(C.Structure):
+    _fields_ = [("vertex_buffer", C.c_void_p), ("index_buffer", C.c_void_p),
+                ("vertex_count", C.c_uint32), ("index_count", C.c_uint32),
+                ("bbox_min", Vec3), ("bbox_max", Vec3), ("aabb_min", Vec3), ("aabb_max", Vec3)]
+
+
 class FrameStats(C.Structure):
     _fields_ = [("triangles_in", C.c_uint32), ("subtris", C.c_uint32), ("clipped", C.c_uint32),
                 ("big_subtris", C.c_uint32), ("shadow_tris", C.c_uint32), ("gpu_ms", C.c_float)]
@@ -177,6 +183,18 @@ SIGS = {
     "mop_hip_draw_background_hdri": (None, [VP, VP, VP, C.c_float, Vec3, Vec3, Vec3, C.c_float,
                                             C.c_float]),
     "mop_hip_draw_outline": (None, [VP, VP, Color, C.c_float, C.c_float, c_u32_p, C.c_uint32]),
+    "mop_hip_scene_file_load": (VP, [C.c_char_p]),
+    "mop_hip_scene_file_free": (None, [VP]),
+    "mop_hip_scene_file_mesh_count": (C.c_uint32, [VP]),
+    "mop_hip_scene_file_light_count": (C.c_uint32, [VP]),
+    "mop_hip_scene_file_get_camera": (C.c_bool, [VP, C.POINTER(Vec3), C.POINTER(Vec3),
+                                                 C.POINTER(Vec3), c_float_p, c_float_p,
+                                                 c_float_p]),
+    "mop_hip_scene_file_get_light": (C.c_bool, [VP, C.c_uint32, C.POINTER(Light)]),
+    "mop_hip_scene_file_get_mesh": (C.c_bool, [VP, VP, C.c_uint32, C.POINTER(LoadedMesh)]),
+    "mop_hip_loaded_mesh_free": (None, [VP, C.POINTER(LoadedMesh)]),
+    "mop_hip_scene_add_loaded_mesh": (C.c_int, [VP, C.POINTER(LoadedMesh), C.c_uint32, VP,
+                                                C.c_int, C.c_float, C.c_float]),
     "mop_hip_shadow_render_noalloc": (C.c_int, [VP, VP, VP, C.c_uint32, VP, C.c_uint32,
                                                 C.POINTER(Mat4), C.POINTER(Mat4), C.c_int]),
 }

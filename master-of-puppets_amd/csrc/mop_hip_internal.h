@@ -392,4 +392,10 @@ void launch_shadow_finalize(int n, const uint32_t *keys, float *depth,
                             hipStream_t s);
 void launch_fill_u32(size_t n, uint32_t *p, uint32_t v, hipStream_t s);
 
+/* runtime services for the other translation units */
+MopHipBuffer *buffer_adopt_device(MopHipDevice *d, void *dev, size_t size);
+void *buffer_device_ptr(MopHipBuffer *b);
+hipStream_t device_stream(MopHipDevice *d);
+void runtime_set_err(const char *fmt, ...);
+
 #endif

@@ -759,6 +759,35 @@ extern "C" void mop_hip_buffer_update(MopHipDevice *d, MopHipBuffer *b, const vo
                    hipMemcpyHostToDevice));
 }
 
+/* A buffer whose contents the device wrote (a mesh decoded in HBM): the
+ * host copy is fetched on the first buffer_read. */
+MopHipBuffer *buffer_adopt_device(MopHipDevice *d, void *dev, size_t size) {
+  MopHipBuffer *b = new MopHipBuffer();
+  b->size = size;
+  b->host = malloc(size ? size : 1);
+  if (!b->host) {
+    delete b;
+    return nullptr;
+  }
+  b->dev = dev;
+  b->owner = d;
+  b->host_stale = true;
+  std::lock_guard<std::mutex> lk(d->mu);
+  d->host2buf[b->host] = b;
+  return b;
+}
+
+void *buffer_device_ptr(MopHipBuffer *b) { return b ? b->dev : nullptr; }
+hipStream_t device_stream(MopHipDevice *d) { return d->stream; }
+
+void runtime_set_err(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(t_err, sizeof(t_err), fmt, ap);
+  va_end(ap);
+  fprintf(stderr, "[mop-hip] %s\n", t_err);
+}
+
 /* Host-visible bytes (the shadow pass and skinning read them,
  * viewport.c:2426-2429, 1539); refreshed from the device after skinning. */
 extern "C" const void *mop_hip_buffer_read(MopHipBuffer *b) {

@@ -18,6 +18,7 @@
 #include <string.h>
 
 #include "mop_hip_scene.h"
+#include "mop_hip_scene_file.h"
 
 /* ---- host math in the reference's operation order (math.c) -------------- */
 
@@ -214,6 +215,42 @@ int mop_hip_scene_add_mesh(MopHipScene *s, const MopHipVertex *v, uint32_t nv,
     if (p.y > m->aabb_max.y) m->aabb_max.y = p.y;
     if (p.z > m->aabb_max.z) m->aabb_max.z = p.z;
   }
+  s->mesh_count++;
+  s->shadow_valid = 0;
+  return (int)s->mesh_count - 1;
+}
+
+int mop_hip_scene_add_loaded_mesh(MopHipScene *s, MopHipLoadedMesh *lm, uint32_t object_id,
+                                  const float *model16, int has_material, float metallic,
+                                  float roughness) {
+  if (!s || !lm || !lm->vertex_buffer || !lm->index_buffer || lm->vertex_count == 0 ||
+      lm->index_count < 3)
+    return -1;
+  if (s->mesh_count == s->mesh_cap) {
+    uint32_t nc = s->mesh_cap ? s->mesh_cap * 2 : 64;
+    SceneMesh *nm = realloc(s->meshes, nc * sizeof(SceneMesh));
+    if (!nm)
+      return -1;
+    s->meshes = nm;
+    s->mesh_cap = nc;
+  }
+  SceneMesh *m = &s->meshes[s->mesh_count];
+  memset(m, 0, sizeof(*m));
+  m->vb = lm->vertex_buffer;
+  m->ib = lm->index_buffer;
+  m->vertex_count = lm->vertex_count;
+  m->index_count = lm->index_count;
+  m->object_id = object_id;
+  if (model16)
+    memcpy(m->model.d, model16, sizeof(m->model.d));
+  else
+    m->model = m4_identity();
+  m->has_material = has_material;
+  m->metallic = metallic;
+  m->roughness = roughness;
+  m->aabb_min = lm->aabb_min;
+  m->aabb_max = lm->aabb_max;
+  memset(lm, 0, sizeof(*lm));
   s->mesh_count++;
   s->shadow_valid = 0;
   return (int)s->mesh_count - 1;

@@ -129,7 +129,10 @@ class SceneDriver:
         self.s = lib.mop_hip_scene_create(device.dev)
         if not self.s:
             raise RuntimeError("scene_create failed")
-        for m in sc.meshes:
+        mop_file = getattr(sc, "mop_file", None)
+        if mop_file is not None:
+            self._add_file_meshes(lib, sc, mop_file)
+        for m in (sc.meshes if mop_file is None else ()):
             v = np.ascontiguousarray(m.vertices, np.float32)
             i = np.ascontiguousarray(m.indices, np.uint32)
             mm = None if m.model is None else np.ascontiguousarray(m.model, np.float32)
@@ -161,6 +164,32 @@ class SceneDriver:
         lib.mop_hip_scene_set_post_effects(self.s, sc.post_effects)
         lib.mop_hip_scene_set_shadow_cache(self.s, 1 if shadow_cache else 0)
         self.cam = camera_of(sc)
+
+    def _add_file_meshes(self, lib, sc, path):
+        """Mesh k of the .mop v2 file decoded on the device (quantized
+        sections never leave their 18-byte form on the host) and added with
+        the object id, transform and material of sc.meshes[k]."""
+        f = lib.mop_hip_scene_file_load(str(path).encode())
+        if not f:
+            raise RuntimeError("scene_file_load failed: " + lib.mop_hip_last_error().decode())
+        try:
+            n = lib.mop_hip_scene_file_mesh_count(f)
+            if n != len(sc.meshes):
+                raise RuntimeError(f"{path} holds {n} meshes, the scene {len(sc.meshes)}")
+            for k, m in enumerate(sc.meshes):
+                lm = abi.LoadedMesh()
+                if not lib.mop_hip_scene_file_get_mesh(self.d.dev, f, k, C.byref(lm)):
+                    raise RuntimeError("scene_file_get_mesh failed: " +
+                                       lib.mop_hip_last_error().decode())
+                mm = None if m.model is None else np.ascontiguousarray(m.model, np.float32)
+                rc = lib.mop_hip_scene_add_loaded_mesh(
+                    self.s, C.byref(lm), m.object_id, None if mm is None else mm.ctypes.data,
+                    int(m.has_material), m.metallic, m.roughness)
+                if rc < 0:
+                    lib.mop_hip_loaded_mesh_free(self.d.dev, C.byref(lm))
+                    raise RuntimeError("add_loaded_mesh failed")
+        finally:
+            lib.mop_hip_scene_file_free(f)
 
     def render(self, fb, cam=None):
         return self.d.lib.mop_hip_scene_render(self.s, fb.fb, C.byref(cam or self.cam), fb.w, fb.h)

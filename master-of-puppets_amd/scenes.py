@@ -143,6 +143,9 @@ class Scene:
     # object outline overlay: {"accent": rgb, "alpha_selected": a,
     # "alpha_unselected": b, "selected": [object ids]} or None (off)
     outline: dict = None
+    # .mop v2 file holding the meshes' geometry (mesh k of the file is
+    # meshes[k]); the frame driver then decodes it on the device
+    mop_file: str = None
 
     @property
     def triangle_count(self):
@@ -516,6 +519,28 @@ def outline_scene(seed=21, width=200, height=150, selected=(1, 3, 6), alpha_sele
         sc.meshes[4] = dataclasses.replace(sc.meshes[4], object_id=CHROME_ID)
     sc.outline = {"accent": accent, "alpha_selected": alpha_selected,
                   "alpha_unselected": alpha_unselected, "selected": list(selected)}
+    return sc
+
+
+def file_scene(seed=31, width=200, height=150, big=True):
+    """Scene for the .mop v2 loader: small_scene's ground and icospheres with
+    per-vertex colours and uvs in [0, 1] (every quantized field varies) and,
+    with big, a 256 x 256-quad terrain (66,049 vertices: the saver switches
+    to 32-bit indices from 65,536 vertices, mop_scene.c:684)."""
+    sc = small_scene(seed=seed, width=width, height=height, n=7)
+    sc.name = f"file_{seed}_{width}x{height}"
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for m in sc.meshes:
+        v = m.vertices.copy()
+        v[:, 6:9] = rng.random((len(v), 3), dtype=np.float32)
+        v[:, 10:12] = rng.random((len(v), 2), dtype=np.float32)
+        meshes.append(dataclasses.replace(m, vertices=v))
+    if big:
+        v, i, _ = terrain(256, 12.0, 0.6, seed, (0.5, 0.6, 0.4, 1.0))
+        v[:, 10:12] = rng.random((len(v), 2), dtype=np.float32)
+        meshes.append(Mesh(v, i, 90, colmajor(mat_translate((0.0, -2.5, -3.0))), True, 0.0, 0.7))
+    sc.meshes = meshes
     return sc
 
 

@@ -470,6 +470,60 @@ void mh_select(MopViewport *vp, uint32_t id, int additive) {
   mop_viewport_select_object(vp, id, additive != 0);
 }
 
+/* mop_scene_save (mop_scene.c:544-768) of the viewport's meshes, lights and
+ * camera; flags MOP_SAVE_QUANTIZE for 18-byte quantized vertices */
+int mh_save_scene(MopViewport *vp, const char *path, uint32_t flags) {
+  return mop_scene_save(vp, path, flags);
+}
+
+/* mop_scene_load and its counts and camera (mop_scene.c:225-519); cam gets
+ * eye, target, up, fov (radians), near, far.  0 on success. */
+int mh_scene_file_info(const char *path, uint32_t *nmesh, uint32_t *nlight, float *cam) {
+  MopSceneFile *f = mop_scene_load(path);
+  if (!f)
+    return -1;
+  *nmesh = mop_scene_mesh_count(f);
+  *nlight = mop_scene_light_count(f);
+  MopVec3 e, t, u;
+  float fov = 0, n = 0, fa = 0;
+  if (cam && mop_scene_get_camera(f, &e, &t, &u, &fov, &n, &fa)) {
+    float c[12] = {e.x, e.y, e.z, t.x, t.y, t.z, u.x, u.y, u.z, fov, n, fa};
+    memcpy(cam, c, sizeof(c));
+  }
+  mop_scene_free(f);
+  return 0;
+}
+
+/* mop_scene_get_mesh (mop_scene.c:397-487): the reference's host decode of
+ * mesh idx.  Sizes go to *vc / *ic; the arrays are copied when the
+ * capacities suffice (vertices as 48-byte MopVertex).  bbox6 gets the
+ * header's box.  0 on success. */
+int mh_scene_file_mesh(const char *path, uint32_t idx, void *verts, uint32_t vcap,
+                       uint32_t *indices, uint32_t icap, uint32_t *vc, uint32_t *ic,
+                       float *bbox6) {
+  MopSceneFile *f = mop_scene_load(path);
+  if (!f)
+    return -1;
+  MopLoadedMesh m;
+  bool ok = mop_scene_get_mesh(f, idx, &m);
+  mop_scene_free(f);
+  if (!ok)
+    return -2;
+  *vc = m.vertex_count;
+  *ic = m.index_count;
+  if (verts && vcap >= m.vertex_count)
+    memcpy(verts, m.vertices, (size_t)m.vertex_count * sizeof(MopVertex));
+  if (indices && icap >= m.index_count)
+    memcpy(indices, m.indices, (size_t)m.index_count * sizeof(uint32_t));
+  if (bbox6) {
+    float b[6] = {m.bbox_min.x, m.bbox_min.y, m.bbox_min.z,
+                  m.bbox_max.x, m.bbox_max.y, m.bbox_max.z};
+    memcpy(bbox6, b, sizeof(b));
+  }
+  mop_load_free(&m);
+  return 0;
+}
+
 /* mop_viewport_resize (viewport.h:86): the serving checks render many
  * requests of one resident scene at different resolutions */
 void mh_resize(MopViewport *vp, int w, int h) { mop_viewport_resize(vp, w, h); }

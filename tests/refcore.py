@@ -51,6 +51,12 @@ def harness():
         h.mh_set_env_background.argtypes = [VP, C.c_int]
         h.mh_set_outline.argtypes = [VP, C.c_int, F, C.c_float, C.c_float]
         h.mh_select.argtypes = [VP, C.c_uint32, C.c_int]
+        h.mh_save_scene.argtypes = [VP, C.c_char_p, C.c_uint32]
+        h.mh_scene_file_info.argtypes = [C.c_char_p, C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32), C.c_void_p]
+        h.mh_scene_file_mesh.argtypes = [C.c_char_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                         C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32), C.c_void_p]
         h.mh_set_shadow_size.argtypes = [C.c_int]
         h.mh_set_clear_color.argtypes = [VP, F]
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
@@ -308,6 +314,46 @@ class RefViewport:
             self.close()
         except Exception:
             pass
+
+
+def save_scene_file(scene, path, quantize=True):
+    """The scene's meshes, lights and camera written by the reference's
+    mop_scene_save (.mop v2); quantize selects 18-byte quantized vertices."""
+    v = RefViewport(scene, 1)
+    try:
+        rc = v.h.mh_save_scene(v.vp, str(path).encode(), 1 if quantize else 0)
+    finally:
+        v.close()
+    if rc != 0:
+        raise RuntimeError(f"mop_scene_save failed ({rc})")
+
+
+def scene_file_info(path):
+    """(mesh count, light count, camera floats) by the reference's loader, or
+    None when mop_scene_load rejects the file."""
+    h = harness()
+    nm, nl = C.c_uint32(), C.c_uint32()
+    cam = (C.c_float * 12)()
+    if h.mh_scene_file_info(str(path).encode(), C.byref(nm), C.byref(nl), cam) != 0:
+        return None
+    return nm.value, nl.value, np.array(cam[:], np.float32)
+
+
+def scene_file_mesh(path, idx):
+    """The reference's host decode of mesh idx (mop_scene_get_mesh):
+    vertices (n, 12) float32, indices uint32, header box (6,) float32."""
+    h = harness()
+    vc, ic = C.c_uint32(), C.c_uint32()
+    if h.mh_scene_file_mesh(str(path).encode(), idx, None, 0, None, 0, C.byref(vc),
+                            C.byref(ic), None) != 0:
+        raise RuntimeError(f"mop_scene_get_mesh({idx}) failed")
+    v = np.zeros((vc.value, 12), np.float32)
+    i = np.zeros(ic.value, np.uint32)
+    box = np.zeros(6, np.float32)
+    if h.mh_scene_file_mesh(str(path).encode(), idx, v.ctypes.data, vc.value, i.ctypes.data,
+                            ic.value, C.byref(vc), C.byref(ic), box.ctypes.data) != 0:
+        raise RuntimeError(f"mop_scene_get_mesh({idx}) failed")
+    return v, i, box
 
 
 def render(scene, mode, frames=1, shadow_size=None):

@@ -26,7 +26,7 @@ def test_layout_matches_reference_headers():
 
 def declared_symbols():
     names = set()
-    for h in ("mop_hip.h", "mop_hip_scene.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(REPO, "include")) if f.endswith(".h")):
         src = open(os.path.join(REPO, "include", h)).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         names |= set(re.findall(r"\b(mop_(?:hip|rhi)_\w+)\s*\(", src))
