@@ -619,24 +619,58 @@ def serving_form(mod, rates=(100.0, 300.0, 1000.0, 3000.0, 5000.0), duration=8.0
     25% 1920x1080, continuous batching admitted by an HBM budget; p50/p99
     from arrival to RGBA8 on the host.  The plain mode serves the same
     stream one request per launch."""
-    from mop_amd import driver, scenes, serve
+    import dataclasses
+    import tempfile
+    from mop_amd import driver, mopfile, scenes, serve
     dev = driver.Device()
     pool = []
+    tmp = tempfile.mkdtemp(prefix="mop_pool_")
+    load_s, file_bytes = [], 0
+    host_s = None
     for k in range(n_scenes):
         sc = scenes.config3(1280, 720, seed=0x4D4F5010 + k)
-        pool.append((driver.SceneDriver(dev, sc, shadow_cache=True), sc))
-        log(f"serving: scene {k} resident ({sc.triangle_count} triangles)")
+        if k == 0:
+            # the same geometry made resident from host arrays (48 B per
+            # vertex, 4 B per index over PCIe), for comparison
+            t0 = time.perf_counter()
+            d = driver.SceneDriver(dev, sc, shadow_cache=True)
+            dev.sync()
+            host_s = time.perf_counter() - t0
+            d.close()
+        path = os.path.join(tmp, f"scene{k}.mop")
+        mopfile.save(sc, path, quantized=True)
+        file_bytes += os.path.getsize(path)
+        t0 = time.perf_counter()
+        d = driver.SceneDriver(dev, dataclasses.replace(sc, mop_file=path), shadow_cache=True)
+        dev.sync()
+        load_s.append(time.perf_counter() - t0)
+        os.remove(path)
+        pool.append((d, sc))
+        log(f"serving: scene {k} resident from {path} ({sc.triangle_count} triangles, "
+            f"{load_s[-1] * 1e3:.1f} ms)")
+    os.rmdir(tmp)
     srv = serve.Server(dev, pool, budget_bytes=16 << 30, max_batch=32, inflight=2)
+    # warm-up: every scene at every resolution class, then a saturating burst
+    # of random orbit requests; a scratch store that overflowed grows to the
+    # largest need seen (plus headroom) before the timed streams
     warm = [(0.0, si, 0.0, 0.0, c) for si in range(n_scenes) for c in range(3)]
     srv.run(warm, time_limit=60.0)
-    check_overflow(dev, "serving warm-up")
-    log("serving: warm-up done")
+    srv.run(serve.make_requests(5000.0, 0.2, n_scenes, seed=0x4D4F50FF), time_limit=60.0)
+    flags = dev.overflow_flags(reset=True)
+    log("serving: warm-up done" + (f" (scratch grown after overflow flags {flags:#x})"
+                                   if flags else ""))
 
     def clean(r):
         r.pop("outputs", None)
         return {k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()}
 
     out = {"scenes": n_scenes, "triangles_per_scene": pool[0][1].triangle_count,
+           "scene_files": {
+               "format": ".mop v2, quantized vertices (18 B) and indices, decoded on the device",
+               "bytes_per_scene": file_bytes // n_scenes,
+               "load_ms_per_scene": round(1e3 * sum(load_s) / len(load_s), 2),
+               "load_gb_per_s": round(file_bytes / sum(load_s) / 1e9, 2),
+               "host_arrays_ms_per_scene": round(1e3 * host_s, 2)},
            "duration_s": duration, "shadow_cache": True, "admission": srv.admission(),
            "batched": {}}
     for rate in rates:

@@ -12,8 +12,10 @@
   eight of its views (the largest by pixels and by triangles among them)
   against the reference core rendering each alone.
 - Configuration 5: requests served by the continuous-batching loop over
-  resident configuration-3 scenes (4096^2 maps kept on the device), at the
-  mixed resolutions, against the reference core rendering each request.
+  resident configuration-3 scenes streamed in from quantized .mop v2 files
+  and decoded on the device (4096^2 maps kept on the device), at the mixed
+  resolutions, against the reference core rendering each request from its
+  own decode of the same files.
 
 Bar: depth and object ID bit-exact, RGBA8 within 1/255, HDR within 1e-4
 relative (refcore.assert_parity).  The reference is the CPU backend forced
@@ -150,10 +152,23 @@ def test_config4_batched_views_match_reference():
         refcore.assert_parity(cmp)
 
 
-def test_config5_served_requests_match_reference():
-    from mop_amd import driver, serve
+def test_config5_served_requests_match_reference(tmp_path):
+    # the resident pool is streamed in from quantized .mop v2 files and
+    # decoded on the device; the reference core renders the same files as its
+    # own loader decodes them (mop_scene_get_mesh)
+    import dataclasses
+    from mop_amd import driver, mopfile, serve
     n_scenes = 2
-    pool_scenes = [scenes.config3(1280, 720, seed=0x4D4F5010 + k) for k in range(n_scenes)]
+    pool_scenes = []
+    for k in range(n_scenes):
+        sc = scenes.config3(1280, 720, seed=0x4D4F5010 + k)
+        path = str(tmp_path / f"pool{k}.mop")
+        mopfile.save(sc, path, quantized=True)
+        meshes = []
+        for j, m in enumerate(sc.meshes):
+            v, i, _ = refcore.scene_file_mesh(path, j)
+            meshes.append(dataclasses.replace(m, vertices=v, indices=i))
+        pool_scenes.append(dataclasses.replace(sc, meshes=meshes, mop_file=path))
     dev = driver.Device()
     try:
         pool = [(driver.SceneDriver(dev, sc, shadow_cache=True), sc) for sc in pool_scenes]

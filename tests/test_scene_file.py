@@ -124,6 +124,20 @@ def test_truncated_mesh_section_is_refused(tmp_path):
         refcore.scene_file_mesh(cut, last)
 
 
+@pytest.mark.parametrize("quantize", [True, False])
+def test_writer_bytes_equal_reference_saver(tmp_path, quantize):
+    # mop_amd.mopfile.save against mop_scene_save (mop_scene.c:544-768) for the
+    # same meshes, lights and camera; the big terrain takes 32-bit indices
+    from mop_amd import mopfile
+    sc, path = _saved(tmp_path, quantize=quantize, big=True)
+    ours = str(tmp_path / "ours.mop")
+    mopfile.save(sc, ours, quantized=quantize)
+    a, b = open(ours, "rb").read(), open(path, "rb").read()
+    assert len(a) == len(b)
+    diff = np.flatnonzero(np.frombuffer(a, np.uint8) != np.frombuffer(b, np.uint8))
+    assert diff.size == 0, f"{diff.size} bytes differ, first at {diff[:8]}"
+
+
 def test_golden_fixture_decodes_to_committed_arrays():
     # tests/golden/scene_quantized.mop was written by the reference's saver;
     # its decode by the reference loader is committed beside it
