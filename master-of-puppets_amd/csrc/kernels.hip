@@ -2221,11 +2221,27 @@ DEV uint32_t outline_px(uint32_t c, const uint8_t *flag, int pitch, int fx, int 
  * intermediate image; the outline of that 18x18 block needs the edge flags
  * of a 22x22 block, which need the ids of a 24x24 block.  Without fog or
  * vignette the colour effects are a per-channel table (v.lut). */
+/* LDS of the outline: the ids of a 24x24 block and the edge flags of a
+ * 22x22 block; a stand-in of one row when no view asks for the outline
+ * (the code that indexes it is then never compiled in) */
+template <bool OUTLINE> struct OutlineLds {
+  uint32_t oid[24][24];
+  uint8_t oflag[22 * 22];
+};
+template <> struct OutlineLds<false> {
+  uint32_t oid[1][24];
+  uint8_t oflag[4];
+};
+
+/* OUTLINE: some view of the launch asks for the outline (the LDS for its
+ * ids and edge flags is only reserved then) */
+template <bool OUTLINE>
 __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t nviews) {
   __shared__ uint32_t col[18][18];
   __shared__ float lum[18][18];
-  __shared__ uint32_t oid[24][24];
-  __shared__ uint8_t oflag[22 * 22];
+  __shared__ OutlineLds<OUTLINE> ol_lds;
+  auto &oid = ol_lds.oid;
+  uint8_t *oflag = ol_lds.oflag;
   uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const PostViewGPU &v = views[find_post_view(views, nviews, tile)];
   uint32_t t = tile - v.tile_base;
@@ -2235,7 +2251,7 @@ __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t 
   int y = y0 + (int)(threadIdx.x >> 4);
   const int w = v.w, h = v.h;
   const uint32_t effects = v.effects;
-  const bool outline = v.outline != 0;
+  const bool outline = OUTLINE && v.outline != 0;
   if (effects == 0 && !outline) {
     if (x < w && y < h) {
       size_t i = (size_t)y * (size_t)w + (size_t)x;
@@ -2430,10 +2446,13 @@ __global__ __launch_bounds__(WG) void k_tail(const PostViewGPU *views, uint32_t 
 }
 
 void launch_tail_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
-                       hipStream_t s) {
+                       bool outline, hipStream_t s) {
   if (!n_tiles)
     return;
-  launch(k_tail, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
+  if (outline)
+    launch(k_tail<true>, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
+  else
+    launch(k_tail<false>, dim3(n_tiles), dim3(WG), 0, s, views, nviews);
 }
 
 __global__ void k_fill_u32(size_t n, uint32_t *p, uint32_t v) {

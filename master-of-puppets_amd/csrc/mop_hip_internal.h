@@ -363,7 +363,7 @@ void launch_vis_from_depth(int npx, const float *depth,
                            unsigned long long *vis, hipStream_t s);
 /* resolve (mode 0) or post (mode 1) over the tiles of n views */
 void launch_tail_views(const PostViewGPU *views, uint32_t nviews, uint32_t n_tiles,
-                       hipStream_t s);
+                       bool outline, hipStream_t s);
 void launch_upload(const void *src, void *dst, size_t n, hipStream_t s);
 void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s);
 void launch_shadow_clear(int n, uint32_t *keys, hipStream_t s);

@@ -1743,7 +1743,10 @@ static void run_tail(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &fb
     upload(d, d->outline_ids.p, stg + ((pv_bytes + 15) & ~(size_t)15), n_sel * 4);
   stage_release(d, slot);
   TmScope t(d, TM_POST);
-  launch_tail_views(d->pviews.p, (uint32_t)pv.size(), tiles, d->stream);
+  bool any_outline = false;
+  for (MopHipFramebuffer *fb : fbs)
+    any_outline |= fb->outline;
+  launch_tail_views(d->pviews.p, (uint32_t)pv.size(), tiles, any_outline, d->stream);
 }
 
 static void free_graph_entry(GraphEntry &e) {
