@@ -655,7 +655,9 @@ def serving_form(mod, rates=(100.0, 300.0, 1000.0, 3000.0, 5000.0), duration=8.0
     # largest need seen (plus headroom) before the timed streams
     warm = [(0.0, si, 0.0, 0.0, c) for si in range(n_scenes) for c in range(3)]
     srv.run(warm, time_limit=60.0)
-    srv.run(serve.make_requests(5000.0, 0.2, n_scenes, seed=0x4D4F50FF), time_limit=60.0)
+    for k in range(3):
+        srv.run(serve.make_requests(5000.0, 0.25, n_scenes, seed=0x4D4F50F0 + k),
+                time_limit=60.0)
     flags = dev.overflow_flags(reset=True)
     log("serving: warm-up done" + (f" (scratch grown after overflow flags {flags:#x})"
                                    if flags else ""))

@@ -26,6 +26,7 @@
 #ifndef MOP_HIP_INTERNAL_H
 #define MOP_HIP_INTERNAL_H
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -52,6 +53,11 @@ struct Plan {
 };
 extern thread_local Plan *t_plan;
 
+/* When set (stage timers on, no plan active), the next direct launch is made
+ * with hipExtLaunchKernelGGL and these events, which the runtime timestamps
+ * at the dispatch's own start and end, then both are cleared. */
+extern thread_local hipEvent_t t_kernel_ev[2];
+
 template <typename T> inline void sig_put(std::vector<uint8_t> &s, const T &v) {
   const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
   s.insert(s.end(), p, p + sizeof(T));
@@ -71,6 +77,10 @@ inline void launch(void (*k)(P...), dim3 g, dim3 b, uint32_t shmem, hipStream_t 
     });
     pl->ostream.push_back(s);
     pl->marker.push_back(0);
+  } else if (t_kernel_ev[0]) {
+    hipEvent_t e0 = t_kernel_ev[0], e1 = t_kernel_ev[1];
+    t_kernel_ev[0] = t_kernel_ev[1] = nullptr;
+    hipExtLaunchKernelGGL(k, g, b, shmem, s, e0, e1, 0u, (P)a...);
   } else {
     hipLaunchKernelGGL(k, g, b, shmem, s, (P)a...);
   }

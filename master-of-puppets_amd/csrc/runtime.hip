@@ -184,6 +184,7 @@ struct MopHipFramebuffer {
  * without a plan (growing frees memory that recorded operations point at) */
 static thread_local bool t_plan_abort = false;
 thread_local Plan *t_plan = nullptr;
+thread_local hipEvent_t t_kernel_ev[2] = {nullptr, nullptr};
 
 template <class T> struct DevArr {
   T *p = nullptr;
@@ -640,21 +641,41 @@ static hipEvent_t tm_event(MopHipDevice *d) {
   return t.pool[t.next++];
 }
 
+/* Stage timer: events recorded on the stream around the stage's launches,
+ * or -- for a stage of one kernel launched directly (one_kernel) -- the
+ * timestamps of that kernel's own dispatch (hipExtLaunchKernelGGL), which
+ * leave out the command processor's time between an event and a dispatch. */
 struct TmScope {
   MopHipDevice *d;
   int id;
-  hipEvent_t a = nullptr;
-  TmScope(MopHipDevice *dev, int which) : d(dev), id(which) {
-    if (d->tm.on && (a = tm_event(d)))
+  hipEvent_t a = nullptr, b = nullptr;
+  bool exact = false;
+  TmScope(MopHipDevice *dev, int which, bool one_kernel = false) : d(dev), id(which) {
+    if (!d->tm.on || !(a = tm_event(d)))
+      return;
+    if (one_kernel && !t_plan && (b = tm_event(d))) {
+      exact = true;
+      t_kernel_ev[0] = a;
+      t_kernel_ev[1] = b;
+    } else {
       (void)hipEventRecord(a, d->stream);
+    }
   }
   ~TmScope() {
     if (!a)
       return;
-    hipEvent_t b = tm_event(d);
-    if (!b)
-      return;
-    (void)hipEventRecord(b, d->stream);
+    if (exact) {
+      if (t_kernel_ev[0]) { /* nothing was launched: an empty interval */
+        t_kernel_ev[0] = t_kernel_ev[1] = nullptr;
+        (void)hipEventRecord(a, d->stream);
+        (void)hipEventRecord(b, d->stream);
+      }
+    } else {
+      b = tm_event(d);
+      if (!b)
+        return;
+      (void)hipEventRecord(b, d->stream);
+    }
     d->tm.pending.push_back({id, {a, b}});
     if (d->tm.pending.size() >= 1024)
       tm_collect(d);
@@ -1361,14 +1382,17 @@ extern "C" void mop_hip_draw_instanced(MopHipDevice *d, MopHipFramebuffer *fb,
 /* ------------------------------------------------------------------------ */
 
 /* The largest need of scratch store k (1 clip queue, 2 checkpoints, 3 tile
- * bins) any finished batch reported, with a quarter of headroom: a batch
- * that overflowed a store sizes the following ones even when its own frames
- * are never read back (and so never run again).  0 before any report. */
+ * bins) any finished batch reported, with headroom: a batch that overflowed
+ * a store sizes the following ones even when its own frames are never read
+ * back (and so never run again).  The checkpoint store gets half again (its
+ * need follows how close big triangles come to the camera, which varies
+ * most between batches of one stream), the others a quarter.  0 before any
+ * report. */
 static uint32_t grown_need(const MopHipDevice *d, int k) {
   uint32_t v = __atomic_load_n(d->h_sticky + k, __ATOMIC_ACQUIRE);
   if (!v)
     return 0;
-  uint64_t g = (uint64_t)v + v / 4 + 1024;
+  uint64_t g = (uint64_t)v + (k == 2 ? v / 2 : v / 4) + 1024;
   return g > 0xffffff00ull ? 0xffffff00u : (uint32_t)g;
 }
 
@@ -1626,7 +1650,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
       }
       join_shadows(d); /* outside the stage timer: the wait is not shading */
       {
-        TmScope t(d, TM_SHADE);
+        TmScope t(d, TM_SHADE, true);
         launch_shade(fa, d->stream);
       }
       launch_export_counters(fa, d->stream);
