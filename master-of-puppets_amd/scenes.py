@@ -146,6 +146,7 @@ class Scene:
     # .mop v2 file holding the meshes' geometry (mesh k of the file is
     # meshes[k]); the frame driver then decodes it on the device
     mop_file: str = None
+    fog: tuple = None             # ((r, g, b, a), near, far) for MOP_POST_FOG, or None
 
     @property
     def triangle_count(self):

@@ -536,6 +536,13 @@ void mh_set_post_effects(MopViewport *vp, uint32_t effects) {
   mop_viewport_set_post_effects(vp, effects);
 }
 
+/* mop_viewport_set_fog (postprocess.h:42): colour rgba, near, far */
+void mh_set_fog(MopViewport *vp, const float *rgba, float near_dist, float far_dist) {
+  MopFogParams f = {.color = {rgba[0], rgba[1], rgba[2], rgba[3]},
+                    .near_dist = near_dist, .far_dist = far_dist};
+  mop_viewport_set_fog(vp, &f);
+}
+
 void mh_set_ambient(MopViewport *vp, float a) { mop_viewport_set_ambient(vp, a); }
 
 void mh_set_shading(MopViewport *vp, int mode) {

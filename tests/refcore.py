@@ -60,6 +60,7 @@ def harness():
         h.mh_set_shadow_size.argtypes = [C.c_int]
         h.mh_set_clear_color.argtypes = [VP, F]
         h.mh_set_post_effects.argtypes = [VP, C.c_uint32]
+        h.mh_set_fog.argtypes = [VP, F, C.c_float, C.c_float]
         h.mh_set_ambient.argtypes = [VP, C.c_float]
         h.mh_set_exposure.argtypes = [VP, C.c_float]
         h.mh_set_shading.argtypes = [VP, C.c_int]
@@ -205,6 +206,9 @@ class RefViewport:
         h.mh_set_camera(self.vp, _f3(scene.eye), _f3(scene.target), _f3(scene.up), scene.fov,
                         scene.znear, scene.zfar)
         h.mh_set_post_effects(self.vp, scene.post_effects)
+        fog = getattr(scene, "fog", None)
+        if fog is not None:  # (rgba, near, far)
+            h.mh_set_fog(self.vp, (C.c_float * 4)(*fog[0]), fog[1], fog[2])
         h.mh_set_shading(self.vp, getattr(scene, "shading", 1))
         h.mh_set_render_mode(self.vp, getattr(scene, "render_mode", 0))
         if scene.env is not None:

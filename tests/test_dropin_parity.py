@@ -110,6 +110,33 @@ def test_object_outline_no_post_two_frames_ssaa():
     _check(sc, frames=2)
 
 
+POST_CASES = {
+    "fog_gamma_fxaa": (1 | 8 | 16, ((0.55, 0.6, 0.7, 1.0), 6.0, 20.0)),
+    "tonemap_gamma_vignette": (1 | 2 | 4, None),
+    "tonemap_fxaa": (2 | 16, None),
+    "all": (1 | 2 | 4 | 8 | 16, ((0.2, 0.25, 0.3, 1.0), 2.0, 30.0)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(POST_CASES))
+def test_post_effects(case):
+    # fog, tonemap, gamma, vignette: without fog and vignette the colour
+    # effects are per-channel tables the host fills with its own powf; with
+    # them the device evaluates the sRGB curve per pixel with its own powf,
+    # which the required bar (1/255) covers -- the differing pixels are
+    # reported
+    effects, fog = POST_CASES[case]
+    sc = scenes.small_scene(seed=41, width=180, height=130, with_env=True)
+    sc.post_effects, sc.fog = effects, fog
+    sc.name += "_" + case
+    print(f"[case] {sc.name}", flush=True)
+    ref = refcore.render(sc, 1)
+    hip = refcore.render(sc, 2)
+    cmp = refcore.compare(hip, ref)
+    print(f"[parity] {sc.name}: {cmp}", flush=True)
+    refcore.assert_parity(cmp, exact=not (effects & (4 | 8)))
+
+
 def test_ssaa_3_device_downsample():
     # mop_viewport_read_color's box filter on the device (factor 3)
     sc = scenes.small_scene(seed=26, width=150, height=100, with_env=True)

@@ -97,6 +97,25 @@ def test_object_outline(alpha_unselected):
     assert np.array_equal(c["depth"], b["depth"]) and np.array_equal(c["ids"], b["ids"])
 
 
+POST_CASES = {
+    "fog_gamma_fxaa": (1 | 8 | 16, ((0.55, 0.6, 0.7, 1.0), 6.0, 20.0)),
+    "tonemap_gamma_vignette": (1 | 2 | 4, None),
+    "tonemap_fxaa": (2 | 16, None),
+    "all": (1 | 2 | 4 | 8 | 16, ((0.2, 0.25, 0.3, 1.0), 2.0, 30.0)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(POST_CASES))
+def test_post_effects(case):
+    # mop_postprocess_apply (postprocess.c:24-116): fog by depth, Reinhard
+    # tonemap, sRGB gamma, vignette, after FXAA
+    effects, fog = POST_CASES[case]
+    sc = scenes.small_scene(seed=41, width=180, height=130, with_env=True)
+    sc.post_effects, sc.fog = effects, fog
+    sc.name += "_" + case
+    _same(sc)
+
+
 def test_dense_ordered_long_tile_lists():
     # tiles overlapped by more than 4096 transparent screen triangles
     _same(scenes.dense_ordered_scene())
