@@ -734,7 +734,9 @@ DEV bool frag_depth(const RasterTri &r, float w0, float w1, float w2, float &z) 
 
 /* Visibility update: a 64-bit atomic min with no return value, so a lane
  * walking a triangle issues its pixels back to back without waiting on the
- * memory system (the min is resolved in L2). */
+ * memory system.  The atomic is performed beyond the XCD's L2 (the eight
+ * XCDs' L2s are not coherent with each other), which makes it several times
+ * dearer than a store. */
 DEV void vis_update(unsigned long long *vis, size_t idx, unsigned long long key) {
   atomicMin(vis + idx, key);
 }
@@ -1227,7 +1229,7 @@ DEV float shadow_pcf(const ViewGPU &fa, V3 wp) {
         lit += (ndcz <= key_depth(row[dx]) + bias) ? 1 : 0;
       row += fa.shadow_w;
     }
-    return (float)lit / 9.0f;
+    return mop_div_recip((float)lit, 9.0f, 1.0f / 9.0f);
   }
   int lit = 0, total = 0;
   for (int dy = -1; dy <= 1; dy++) {
@@ -1279,10 +1281,11 @@ DEV void ibl_sample(const float *data, int w, int h, float u, float v, float out
 
 /* dir_to_uv, rasterizer.c:210-216 */
 DEV void dir_to_uv(float dx, float dy, float dz, float rot, float &u, float &v) {
+  const float two_pi = 2.0f * 3.14159265f, pi = 3.14159265f;
   float phi = mop_atan2f(dz, dx) + rot;
   float cdy = dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy);
-  u = phi / (2.0f * 3.14159265f) + 0.5f;
-  v = mop_asinf(cdy) / 3.14159265f + 0.5f;
+  u = mop_div_recip(phi, two_pi, 1.0f / two_pi) + 0.5f;
+  v = mop_div_recip(mop_asinf(cdy), pi, 1.0f / pi) + 0.5f;
 }
 
 DEV void ibl_irradiance(const ViewGPU &fa, V3 n, float out[3]) {
@@ -1291,7 +1294,9 @@ DEV void ibl_irradiance(const ViewGPU &fa, V3 n, float out[3]) {
   if (len < 1e-6f)
     return;
   float u, v;
-  dir_to_uv(n.x / len, n.y / len, n.z / len, fa.ibl_rot, u, v);
+  float il = 1.0f / len; /* the three quotients n / len share it */
+  dir_to_uv(mop_div_recip(n.x, len, il), mop_div_recip(n.y, len, il), mop_div_recip(n.z, len, il),
+            fa.ibl_rot, u, v);
   ibl_sample(fa.irr, fa.irr_w, fa.irr_h, u, v, out);
   out[0] *= fa.ibl_int;
   out[1] *= fa.ibl_int;
@@ -1325,7 +1330,9 @@ DEV void ibl_prefiltered(const ViewGPU &fa, V3 refl, float roughness, float out[
   if (lh < 1)
     lh = 1;
   float u, v;
-  dir_to_uv(refl.x / len, refl.y / len, refl.z / len, fa.ibl_rot, u, v);
+  float il = 1.0f / len;
+  dir_to_uv(mop_div_recip(refl.x, len, il), mop_div_recip(refl.y, len, il),
+            mop_div_recip(refl.z, len, il), fa.ibl_rot, u, v);
   ibl_sample(fa.pf + offset * 4, lw, lh, u, v, out);
   out[0] *= fa.ibl_int;
   out[1] *= fa.ibl_int;
@@ -1911,9 +1918,9 @@ DEV bool shade_px(const FrameArgs &frame, const ViewGPU &fa, uint32_t s, int px,
  * batch are numbered consecutively.  Every pixel's visibility entry is read
  * and, after the pixel is shaded, set back to the cleared key, so the buffer
  * is clear for the next segment or frame without a clearing pass.  All
- * stores come after all loads.  The kernel is bound by the latency of its
- * chain of dependent gathers; five waves per SIMD (at most 96 VGPRs, a few
- * spilled) measured faster than four (110 VGPRs) or six. */
+ * stores come after all loads.  The kernel is bound by VALU issue and the
+ * latency of its chain of dependent gathers; five waves per SIMD (96 VGPRs,
+ * a few spilled) measured faster than four (110 VGPRs) or six. */
 __global__ __launch_bounds__(1024, 5) void k_shade(FrameArgs frame) {
   uint32_t vi = find_view_by_stile(frame.views, frame.nviews, blockIdx.x);
   const ViewGPU &fa = frame.views[vi];
@@ -2721,15 +2728,16 @@ __global__ __launch_bounds__(WG) void k_background(BackgroundGPU bg) {
     dz = p.z * iw - bg.cam[2];
   }
   float len = sqrtf(dx * dx + dy * dy + dz * dz);
-  dx /= len;
-  dy /= len;
-  dz /= len;
+  float il = 1.0f / len;
+  dx = mop_div_recip(dx, len, il);
+  dy = mop_div_recip(dy, len, il);
+  dz = mop_div_recip(dz, len, il);
   const float pi = 3.14159265358979f;
   float phi_val = mop_atan2f(dz, dx) - bg.rotation;
   float clamped_dy = dy < -1.0f ? -1.0f : (dy > 1.0f ? 1.0f : dy);
   float theta_val = mop_asinf(clamped_dy);
-  float eu = phi_val / (2.0f * pi) + 0.5f;
-  float ev = theta_val / pi + 0.5f;
+  float eu = mop_div_recip(phi_val, 2.0f * pi, 1.0f / (2.0f * pi)) + 0.5f;
+  float ev = mop_div_recip(theta_val, pi, 1.0f / pi) + 0.5f;
   float smp[3];
   ibl_sample(bg.env, bg.env_w, bg.env_h, eu, ev, smp);
   reinterpret_cast<float4 *>(bg.hdr)[i] =

@@ -1,6 +1,9 @@
 /*
  * libmf.h -- single-precision atan2f and asinf that return the same bits as
- * the host C library the reference is built against (glibc 2.35 on x86-64,
+ * the host C library the reference is built against, and a division by a
+ * precomputed reciprocal that returns IEEE division's bits (mop_div_recip).
+ *
+ * atan2f, asinf: the host library is glibc 2.35 on x86-64,
  * whose float versions are not correctly rounded: a correctly rounded
  * atan2f differs from it on about 16% of arguments, asinf on about 7%).
  *
@@ -40,6 +43,7 @@ static __device__ __forceinline__ uint32_t lmf_bits(float f) { return __float_as
 static __device__ __forceinline__ float lmf_float(uint32_t u) { return __uint_as_float(u); }
 #define LMF_SQRTF sqrtf
 #define LMF_FABSF fabsf
+#define LMF_FMAF __builtin_fmaf
 #else
 #include <math.h>
 #include <string.h>
@@ -57,7 +61,28 @@ static inline float lmf_float(uint32_t u) {
 }
 #define LMF_SQRTF sqrtf
 #define LMF_FABSF fabsf
+#define LMF_FMAF fmaf
 #endif
+
+/* a / b from y = 1 / b (an IEEE division, computed once for a constant or
+ * shared divisor): q = a y, then q + (a - q b) y with the residual exact by
+ * fused multiply-add (Markstein's correction), which is the correctly
+ * rounded quotient when y is the correctly rounded reciprocal and no
+ * intermediate leaves the normal range.  Quotients of magnitude outside
+ * [2^-99, 2^97] (zeros, tiny, huge, NaN), numerators outside [2^-100, 2^100]
+ * and reciprocals outside [2^-120, 2^120] take the division (there the
+ * residual or the reciprocal would lose bits to underflow).  Checked against
+ * IEEE division on every float numerator for the constant divisors the
+ * shading uses and on random pairs (tests/test_divide.py). */
+LMF_FN float mop_div_recip(float a, float b, float y) {
+  float q = a * y;
+  float aq = LMF_FABSF(q), ay = LMF_FABSF(y), aa = LMF_FABSF(a);
+  if (!(aq >= 0x1p-99f && aq <= 0x1p97f && ay >= 0x1p-120f && ay <= 0x1p120f &&
+        aa >= 0x1p-100f && aa <= 0x1p100f))
+    return a / b;
+  float r = LMF_FMAF(-q, b, a);
+  return LMF_FMAF(r, y, q);
+}
 
 /* atan at the reduction breakpoints, split hi + lo */
 LMF_CONST float lmf_atan_hi[4] = {4.6364760399e-01f, 7.8539812565e-01f, 9.8279368877e-01f,
@@ -70,45 +95,45 @@ LMF_CONST float lmf_at[11] = {3.3333334327e-01f,  -2.0000000298e-01f, 1.42857149
                               6.6610731184e-02f,  -5.8335702866e-02f, 4.9768779427e-02f,
                               -3.6531571299e-02f, 1.6285819933e-02f};
 
+/* The three functions below evaluate every path and select the result, so
+ * that the lanes of a wave never diverge: one division and one polynomial
+ * per atanf (the argument reduction of each interval is a quotient, |x|/1
+ * for the smallest), one division, one square root and one polynomial per
+ * asinf.  Each selected value is the same expression as the branch it
+ * stands for, so the results are the branchy forms' bit for bit. */
+
 LMF_FN float mop_atanf(float x) {
-  int32_t hx = (int32_t)lmf_bits(x), ix = hx & 0x7fffffff, id;
-  if (ix >= 0x4c000000) { /* |x| >= 2^25 */
-    if (ix > 0x7f800000)
-      return x + x; /* NaN */
-    return hx > 0 ? lmf_atan_hi[3] + lmf_atan_lo[3] : -lmf_atan_hi[3] - lmf_atan_lo[3];
-  }
-  if (ix < 0x3ee00000) { /* |x| < 0.4375 */
-    if (ix < 0x31000000) /* |x| < 2^-29 */
-      return x;
-    id = -1;
-  } else {
-    x = LMF_FABSF(x);
-    if (ix < 0x3f980000) {   /* |x| < 1.1875 */
-      if (ix < 0x3f300000) { /* 7/16 <= |x| < 11/16 */
-        id = 0;
-        x = (2.0f * x - 1.0f) / (2.0f + x);
-      } else { /* 11/16 <= |x| < 19/16 */
-        id = 1;
-        x = (x - 1.0f) / (x + 1.0f);
-      }
-    } else if (ix < 0x401c0000) { /* |x| < 2.4375 */
-      id = 2;
-      x = (x - 1.5f) / (1.0f + 1.5f * x);
-    } else { /* 2.4375 <= |x| < 2^25 */
-      id = 3;
-      x = -1.0f / x;
-    }
-  }
-  float z = x * x;
+  int32_t hx = (int32_t)lmf_bits(x), ix = hx & 0x7fffffff;
+  float ax = LMF_FABSF(x);
+  /* reduction interval: -1 below 7/16 (no reduction), else 0..3 */
+  int id = ix < 0x3ee00000 ? -1
+           : ix < 0x3f300000 ? 0    /* 7/16 <= |x| < 11/16 */
+           : ix < 0x3f980000 ? 1    /* 11/16 <= |x| < 19/16 */
+           : ix < 0x401c0000 ? 2    /* 19/16 <= |x| < 2.4375 */
+                             : 3;   /* 2.4375 <= |x| */
+  float num = id < 0 ? x : id == 0 ? 2.0f * ax - 1.0f : id == 1 ? ax - 1.0f
+            : id == 2 ? ax - 1.5f : -1.0f;
+  float den = id < 0 ? 1.0f : id == 0 ? 2.0f + ax : id == 1 ? ax + 1.0f
+            : id == 2 ? 1.0f + 1.5f * ax : ax;
+  float t = num / den; /* x / 1 is x */
+  float z = t * t;
   float w = z * z;
   float s1 = z * (lmf_at[0] +
                   w * (lmf_at[2] + w * (lmf_at[4] + w * (lmf_at[6] + w * (lmf_at[8] +
                                                                           w * lmf_at[10])))));
   float s2 = w * (lmf_at[1] + w * (lmf_at[3] + w * (lmf_at[5] + w * (lmf_at[7] + w * lmf_at[9]))));
-  if (id < 0)
-    return x - x * (s1 + s2);
-  z = lmf_atan_hi[id] - ((x * (s1 + s2) - lmf_atan_lo[id]) - x);
-  return hx < 0 ? -z : z;
+  float hi = id <= 0 ? lmf_atan_hi[0] : id == 1 ? lmf_atan_hi[1] : id == 2 ? lmf_atan_hi[2]
+                                                                          : lmf_atan_hi[3];
+  float lo = id <= 0 ? lmf_atan_lo[0] : id == 1 ? lmf_atan_lo[1] : id == 2 ? lmf_atan_lo[2]
+                                                                          : lmf_atan_lo[3];
+  float big = hi - ((t * (s1 + s2) - lo) - t);
+  float r = id < 0 ? t - t * (s1 + s2) : (hx < 0 ? -big : big);
+  if (ix < 0x31000000) /* |x| < 2^-29 */
+    r = x;
+  if (ix >= 0x4c000000) /* |x| >= 2^25, or NaN */
+    r = ix > 0x7f800000 ? x + x
+                        : (hx > 0 ? lmf_atan_hi[3] + lmf_atan_lo[3] : -lmf_atan_hi[3] - lmf_atan_lo[3]);
+  return r;
 }
 
 LMF_FN float mop_atan2f(float y, float x) {
@@ -116,50 +141,33 @@ LMF_FN float mop_atan2f(float y, float x) {
               pi = 3.1415927410e+00f, pi_lo = -8.7422776573e-08f, tiny = 1.0e-30f;
   int32_t hx = (int32_t)lmf_bits(x), ix = hx & 0x7fffffff;
   int32_t hy = (int32_t)lmf_bits(y), iy = hy & 0x7fffffff;
-  if (ix > 0x7f800000 || iy > 0x7f800000)
-    return x + y; /* NaN */
-  if (hx == 0x3f800000)
-    return mop_atanf(y); /* x = 1 */
   int m = ((hy >> 31) & 1) | ((hx >> 30) & 2); /* 2 sign(x) + sign(y) */
-  if (iy == 0) {
-    if (m <= 1)
-      return y;
-    return m == 2 ? pi + tiny : -pi - tiny;
+  int x_one = hx == 0x3f800000;
+  /* atanf(y) when x = 1, else atanf(|y/x|) */
+  float za = mop_atanf(x_one ? y : LMF_FABSF(y / x));
+  int32_t k = (iy - ix) >> 23;
+  float z = k > 60 ? pi_o_2 + 0.5f * pi_lo : ((hx < 0 && k < -60) ? 0.0f : za);
+  float r = m == 0 ? z : m == 1 ? lmf_float(lmf_bits(z) ^ 0x80000000u)
+          : m == 2 ? pi - (z - pi_lo) : (z - pi_lo) - pi;
+  /* the special cases, lowest precedence first */
+  if (iy == 0x7f800000)
+    r = hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+  if (ix == 0x7f800000) {
+    if (iy == 0x7f800000)
+      r = m == 0 ? pi_o_4 + tiny : m == 1 ? -pi_o_4 - tiny
+        : m == 2 ? 3.0f * pi_o_4 + tiny : -3.0f * pi_o_4 - tiny;
+    else
+      r = m == 0 ? 0.0f : m == 1 ? -0.0f : m == 2 ? pi + tiny : -pi - tiny;
   }
   if (ix == 0)
-    return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
-  if (ix == 0x7f800000) {
-    if (iy == 0x7f800000) {
-      switch (m) {
-      case 0: return pi_o_4 + tiny;
-      case 1: return -pi_o_4 - tiny;
-      case 2: return 3.0f * pi_o_4 + tiny;
-      default: return -3.0f * pi_o_4 - tiny;
-      }
-    }
-    switch (m) {
-    case 0: return 0.0f;
-    case 1: return -0.0f;
-    case 2: return pi + tiny;
-    default: return -pi - tiny;
-    }
-  }
-  if (iy == 0x7f800000)
-    return hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
-  int32_t k = (iy - ix) >> 23;
-  float z;
-  if (k > 60)
-    z = pi_o_2 + 0.5f * pi_lo; /* |y/x| > 2^60 */
-  else if (hx < 0 && k < -60)
-    z = 0.0f;
-  else
-    z = mop_atanf(LMF_FABSF(y / x));
-  switch (m) {
-  case 0: return z;
-  case 1: return lmf_float(lmf_bits(z) ^ 0x80000000u);
-  case 2: return pi - (z - pi_lo);
-  default: return (z - pi_lo) - pi;
-  }
+    r = hy < 0 ? -pi_o_2 - tiny : pi_o_2 + tiny;
+  if (iy == 0)
+    r = m <= 1 ? y : (m == 2 ? pi + tiny : -pi - tiny);
+  if (x_one)
+    r = za;
+  if (ix > 0x7f800000 || iy > 0x7f800000)
+    r = x + y; /* NaN */
+  return r;
 }
 
 LMF_FN float mop_asinf(float x) {
@@ -169,32 +177,26 @@ LMF_FN float mop_asinf(float x) {
   const float p0 = 1.666675248e-1f, p1 = 7.495297643e-2f, p2 = 4.547037598e-2f,
               p3 = 2.417951451e-2f, p4 = 4.216630880e-2f;
   int32_t hx = (int32_t)lmf_bits(x), ix = hx & 0x7fffffff;
-  if (ix == 0x3f800000)
-    return x * pio2_hi + x * pio2_lo; /* +-1 */
-  if (ix > 0x3f800000)
-    return (x - x) / (x - x); /* |x| > 1: NaN */
-  if (ix < 0x3f000000) {      /* |x| < 0.5 */
-    if (ix < 0x32000000)      /* |x| < 2^-27 */
-      return x;
-    float t = x * x;
-    float w = t * (p0 + t * (p1 + t * (p2 + t * (p3 + t * p4))));
-    return x + x * w;
-  }
-  float w = 1.0f - LMF_FABSF(x);
-  float t = w * 0.5f;
+  int small = ix < 0x3f000000; /* |x| < 0.5 */
+  /* the polynomial's argument: x^2 below 0.5, (1 - |x|) / 2 above */
+  float t = small ? x * x : (1.0f - LMF_FABSF(x)) * 0.5f;
   float p = t * (p0 + t * (p1 + t * (p2 + t * (p3 + t * p4))));
   float s = LMF_SQRTF(t);
-  if (ix >= 0x3F79999A) { /* |x| > 0.975 */
-    t = pio2_hi - (2.0f * (s + s * p) - pio2_lo);
-  } else {
-    float wh = lmf_float(lmf_bits(s) & 0xfffff000u); /* s with its low 12 bits cleared */
-    float c = (t - wh * wh) / (s + wh);
-    float r = p;
-    p = 2.0f * s * r - (pio2_lo - 2.0f * c);
-    float q = pio4_hi - 2.0f * wh;
-    t = pio4_hi - (p - q);
-  }
-  return hx > 0 ? t : -t;
+  /* |x| > 0.975 */
+  float ra = pio2_hi - (2.0f * (s + s * p) - pio2_lo);
+  /* 0.5 <= |x| <= 0.975: s with its low 12 bits cleared, and the correction */
+  float wh = lmf_float(lmf_bits(s) & 0xfffff000u);
+  float c = (t - wh * wh) / (s + wh);
+  float pp = 2.0f * s * p - (pio2_lo - 2.0f * c);
+  float q = pio4_hi - 2.0f * wh;
+  float rb = pio4_hi - (pp - q);
+  float large = ix >= 0x3F79999A ? ra : rb;
+  float r = small ? (ix < 0x32000000 ? x : x + x * p) : (hx > 0 ? large : -large);
+  if (ix == 0x3f800000)
+    r = x * pio2_hi + x * pio2_lo; /* +-1 */
+  if (ix > 0x3f800000)
+    r = (x - x) / (x - x); /* |x| > 1: NaN */
+  return r;
 }
 
 #endif
