@@ -12,6 +12,8 @@
 #include <stdint.h>
 
 #define DEV __device__ __forceinline__
+/* also compiled for the host (the shading pass's key checks, shade_key.h) */
+#define HD __host__ __device__ __forceinline__
 
 struct V3 {
   float x, y, z;
@@ -44,7 +46,7 @@ DEV float4 mat4_mul_vec4(const float *m, float x, float y, float z, float w) {
 
 /* C's (int) cast of a float as x86-64 executes it (cvttss2si): truncation,
  * and 0x80000000 for NaN or anything outside int32 range. */
-DEV int cvt_i32(float f) {
+HD int cvt_i32(float f) {
   if (!(f >= -2147483648.0f && f < 2147483648.0f))
     return (int)0x80000000;
   return (int)f;
@@ -64,7 +66,7 @@ DEV float key_depth(uint32_t k) {
 }
 
 /* Perspective divide + viewport, rasterizer.c:2520-2532 (also 396-407). */
-DEV void screen_xform(float4 p, float half_w, float half_h, float &sx,
+HD void screen_xform(float4 p, float half_w, float half_h, float &sx,
                       float &sy, float &sz, float &iw) {
   iw = 1.0f / p.w;
   sx = (p.x * iw + 1.0f) * half_w;
@@ -82,7 +84,7 @@ struct EdgeSetup {
   float w_row[3];
 };
 
-DEV bool edge_setup(const float *sx, const float *sy, int W, int H,
+HD bool edge_setup(const float *sx, const float *sy, int W, int H,
                     EdgeSetup &s) {
   float fmin_x = sx[0];
   if (sx[1] < fmin_x)
@@ -164,19 +166,19 @@ DEV bool edge_setup(const float *sx, const float *sy, int W, int H,
 }
 
 /* AA edge scale, rasterizer.c:1898-1903 */
-DEV float inv_edge_len(float dx, float dy) {
+HD float inv_edge_len(float dx, float dy) {
   float len = sqrtf(dx * dx + dy * dy);
   return (len > 1e-6f) ? 1.0f / len : 0.0f;
 }
 
-DEV long long bbox_area(const EdgeSetup &s) {
+HD long long bbox_area(const EdgeSetup &s) {
   return (long long)(s.max_x - s.min_x + 1) * (long long)(s.max_y - s.min_y + 1);
 }
 
 /* checkpoints per bbox row of a big screen triangle: one per CHK_SPAN columns */
 #define CHK_SHIFT 4
 #define CHK_SPAN (1 << CHK_SHIFT)
-DEV uint32_t chk_per_row(const EdgeSetup &es) {
+HD uint32_t chk_per_row(const EdgeSetup &es) {
   return (uint32_t)(es.max_x - es.min_x + CHK_SPAN) >> CHK_SHIFT;
 }
 
