@@ -517,12 +517,16 @@ def batched_form(mod, rounds, tune=True):
     log(f"batched: warm-up done (scratch grown after overflow flags {warm_flags:#x})")
 
     if tune:
-        best, table = autotune.autotune(dev, one_batch)
+        for w in autotune.SHADE_WAVES:  # untimed: every shading variant once
+            autotune.set_shade_waves(dev, w)
+            one_batch()
+        best, table, best_w, wtable = autotune.autotune(dev, one_batch)
         log(f"batched: autotuned (small_area, shade_tile, raster_grid) = {best}: "
-            f"{table[best]:.3f} ms")
+            f"{table[best]:.3f} ms; shading waves per SIMD {best_w}: {wtable[best_w]:.3f} ms")
     else:
-        best, table = (64, 16, 8192), {}
+        best, table, best_w, wtable = (64, 16, 8192), {}, 5, {}
         autotune.set_tuning(dev, *best)
+        autotune.set_shade_waves(dev, best_w)
     check_overflow(dev, "batched autotune")
     t0 = time.perf_counter()
     for r in range(rounds):
@@ -601,7 +605,9 @@ def batched_form(mod, rounds, tune=True):
            "shadow_cache": True,
            "overflow_flags": 0,
            "tuning": {"small_area": best[0], "shade_tile": best[1], "raster_grid": best[2],
-                      "candidates_ms": {str(k): round(v, 3) for k, v in table.items()}}}
+                      "shade_waves_per_simd": best_w,
+                      "candidates_ms": {str(k): round(v, 3) for k, v in table.items()},
+                      "shade_waves_ms": {str(k): round(v, 3) for k, v in wtable.items()}}}
     for fb in fbs:
         fb.close()
     for d in drv:

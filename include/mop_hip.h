@@ -330,6 +330,10 @@ void mop_hip_graph_stats(MopHipDevice *dev, uint64_t *replays, uint64_t *capture
  * Returns 0, or -1 for an invalid value. */
 int mop_hip_set_tuning(MopHipDevice *dev, uint32_t small_area, uint32_t shade_tile,
                        uint32_t raster_grid);
+/* Waves per SIMD the shading pass runs at: 4, 5 (default), 6 or 8 (the
+ * kernel is compiled for each; fewer waves leave more registers per lane).
+ * Results never depend on it.  Returns 0, or -1 for another value. */
+int mop_hip_set_shade_waves(MopHipDevice *dev, uint32_t waves);
 /* 1 when fb's frame has finished on the device (non-blocking). */
 int mop_hip_frame_done(MopHipDevice *dev, MopHipFramebuffer *fb);
 /* Wait until everything queued on the device has finished. */

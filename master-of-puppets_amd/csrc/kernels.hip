@@ -1716,9 +1716,12 @@ DEV bool shade_px(const FrameArgs &frame, const ViewGPU &fa, uint32_t s, int px,
  * and, after the pixel is shaded, set back to the cleared key, so the buffer
  * is clear for the next segment or frame without a clearing pass.  All
  * stores come after all loads.  The kernel is bound by VALU issue and the
- * latency of its chain of dependent gathers; five waves per SIMD (96 VGPRs,
- * a few spilled) measured faster than four (110 VGPRs) or six. */
-__global__ __launch_bounds__(1024, 5) void k_shade(FrameArgs frame) {
+ * latency of its chain of dependent gathers.  It is compiled for 4, 5, 6
+ * and 8 waves per SIMD (at most 128, 96, 80 and 64 VGPRs, the tighter ones
+ * spilling); the autotuner picks one per batch (mop_hip_set_shade_waves),
+ * five by default. */
+template <int WAVES>
+__global__ __launch_bounds__(1024, WAVES) void k_shade(FrameArgs frame) {
   uint32_t vi = find_view_by_stile(frame.views, frame.nviews, blockIdx.x);
   const ViewGPU &fa = frame.views[vi];
   uint32_t T = frame.shade_tile;
@@ -1824,7 +1827,13 @@ void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
 void launch_shade(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_stiles)
     return;
-  launch(k_shade, dim3(fa.n_stiles), dim3(fa.shade_tile * fa.shade_tile), 0, s, fa);
+  dim3 g(fa.n_stiles), b(fa.shade_tile * fa.shade_tile);
+  switch (fa.shade_waves) {
+  case 4: launch(k_shade<4>, g, b, 0, s, fa); break;
+  case 6: launch(k_shade<6>, g, b, 0, s, fa); break;
+  case 8: launch(k_shade<8>, g, b, 0, s, fa); break;
+  default: launch(k_shade<5>, g, b, 0, s, fa); break;
+  }
 }
 
 /* --------------------------------------------------------------------- */

@@ -223,6 +223,7 @@ struct FrameArgs {
   uint32_t small_area;  /* bbox pixels up to which one lane walks a triangle */
   uint32_t shade_tile;  /* shading tile edge: 8, 16 or 32 */
   uint32_t raster_grid; /* workgroups of the small-triangle raster (0 = auto) */
+  uint32_t shade_waves; /* waves per SIMD the shading pass is compiled for: 4, 5, 6 or 8 */
   uint32_t fused;       /* 1: order-free path (no screen-triangle store) */
   float4 *vpos;   /* n_verts clip positions */
   float4 *vscr;   /* n_verts screen positions (sx, sy, sz, 1/w) in the draw's view */

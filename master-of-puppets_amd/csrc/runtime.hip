@@ -314,7 +314,7 @@ struct MopHipDevice {
   DevArr<SubTri> subtris;
   uint32_t clip_cap_hint = 0;
   /* tunables (mop_hip_set_tuning) */
-  uint32_t small_area = 64, shade_tile = 16, raster_grid = 0;
+  uint32_t small_area = 64, shade_tile = 16, raster_grid = 0, shade_waves = 5;
   DevArr<uint32_t> tile_counts, tile_cursor, bins;
   uint32_t bin_cap_hint = 0;
   /* forced scratch capacities (mop_hip_set_scratch_limits; 0 = none): used
@@ -1578,6 +1578,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.n_stiles = stiles;
   fa.small_area = d->small_area;
   fa.shade_tile = d->shade_tile;
+  fa.shade_waves = d->shade_waves;
   fa.raster_grid = d->raster_grid;
   fa.fused = fused ? 1u : 0u;
   if (fa.fused) {
@@ -2175,6 +2176,13 @@ extern "C" void mop_hip_frames_execute(MopHipDevice *d, MopHipFramebuffer *const
  * walks a screen triangle, shading tile edge (8, 16 or 32 pixels) and the
  * number of waves of the large-triangle raster (0 = 8192).  Results do not
  * depend on them; speed does. */
+extern "C" int mop_hip_set_shade_waves(MopHipDevice *d, uint32_t waves) {
+  if (!d || (waves != 4 && waves != 5 && waves != 6 && waves != 8))
+    return -1;
+  d->shade_waves = waves;
+  return 0;
+}
+
 extern "C" int mop_hip_set_tuning(MopHipDevice *d, uint32_t small_area, uint32_t shade_tile,
                                   uint32_t raster_grid) {
   if (!d || (shade_tile != 8 && shade_tile != 16 && shade_tile != 32) || small_area < 1)

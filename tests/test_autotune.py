@@ -1,5 +1,5 @@
 """The launch shapes the autotuner picks must never change a result: every
-candidate renders a mixed batch (large ground-plane triangles, near-plane
+candidate (and every waves-per-SIMD variant of the shading pass) renders a mixed batch (large ground-plane triangles, near-plane
 clipping, shadows, IBL) bit-identical to the default shapes, and the tuner
 leaves a valid candidate set."""
 import numpy as np
@@ -37,9 +37,19 @@ def test_every_tuning_renders_the_same_frames():
                 for key in ("color", "depth", "ids", "hdr"):
                     assert np.array_equal(got[key].view(np.uint8), ref[k][key].view(np.uint8)), \
                         (cand, scs[k].name, key)
-        best, table = autotune.autotune(dev, one_batch, reps=1)
-        print("autotune:", best, table, flush=True)
+        for w in autotune.SHADE_WAVES:
+            autotune.set_tuning(dev, 64, 16, 8192)
+            autotune.set_shade_waves(dev, w)
+            one_batch()
+            for k, fb in enumerate(fbs):
+                got = fb.read(hdr=True)
+                for key in ("color", "depth", "ids", "hdr"):
+                    assert np.array_equal(got[key].view(np.uint8), ref[k][key].view(np.uint8)), \
+                        ("shade waves", w, scs[k].name, key)
+        best, table, best_w, wtable = autotune.autotune(dev, one_batch, reps=1)
+        print("autotune:", best, table, best_w, wtable, flush=True)
         assert best in table and len(table) == len(autotune.candidates())
+        assert best_w in autotune.SHADE_WAVES and len(wtable) == len(autotune.SHADE_WAVES)
         for fb in fbs:
             fb.close()
         for d in drv:
