@@ -1368,20 +1368,13 @@ DEV void lights_eval(const FrameArgs &frame, const ViewGPU &fa, const DrawGPU &d
                f0_g, f0_b, lit, spec);
 }
 
-DEV void load_attr(const FrameArgs &fa, uint32_t slot, float4 &a0, float4 &a1, float4 &a2) {
-  const float4 *p = (slot < fa.n_verts) ? fa.vattr + (size_t)slot * 3
-                                        : fa.cattr + (size_t)(slot - fa.n_verts) * 3;
-  a0 = p[0];
-  a1 = p[1];
-  a2 = p[2];
-}
-
 /* Colour of one fragment of screen triangle st with edge values w0..w2:
  * perspective-correct interpolation, multi-light Lambert + GGX + PCF + IBL,
  * composition and clamp (rasterizer.c:1950-2092).  Also returns z. */
 DEV void shade_fragment(const FrameArgs &frame, const ViewGPU &fa, const SubTri &st,
-                        const RasterTri &r, float w0, float w1, float w2, float &z,
-                        float &pr, float &pg, float &pb) {
+                        const RasterTri &r, const float4 *A0, const float4 *A1,
+                        const float4 *A2, float w0, float w1, float w2, float &z, float &pr,
+                        float &pg, float &pb) {
   float b0 = w0 * r.es.inv_area;
   float b1 = w1 * r.es.inv_area;
   float b2 = w2 * r.es.inv_area;
@@ -1395,10 +1388,6 @@ DEV void shade_fragment(const FrameArgs &frame, const ViewGPU &fa, const SubTri 
   pc0 *= inv_pc;
   pc1 *= inv_pc;
   pc2 *= inv_pc;
-  float4 A0[3], A1[3], A2[3];
-  load_attr(frame, st.a[0], A0[0], A1[0], A2[0]);
-  load_attr(frame, st.a[1], A0[1], A1[1], A2[1]);
-  load_attr(frame, st.a[2], A0[2], A1[2], A2[2]);
   /* attribute k of vertex v: n=(A0.x,A0.y,A0.z) wp=(A0.w,A1.x,A1.y) c=(A1.z,A1.w,A2.x,A2.y) */
   V3 n = V3{pc0 * A0[0].x + pc1 * A0[1].x + pc2 * A0[2].x,
             pc0 * A0[0].y + pc1 * A0[1].y + pc2 * A0[2].y,
@@ -1550,11 +1539,8 @@ DEV V3 draw_light_dir(const DrawGPU &d) { return V3{d.light_dir[0], d.light_dir[
  * (rasterizer.c:2556-2596), unlit draws use light_dir Lambert
  * (rasterizer.c:1199-1220). */
 DEV void flat_color(const FrameArgs &frame, const ViewGPU &fa, const SubTri &st,
-                    const DrawGPU &d, int sm, float &cr, float &cg, float &cb) {
-  float4 A0[3], A1[3], A2[3];
-  load_attr(frame, st.a[0], A0[0], A1[0], A2[0]);
-  load_attr(frame, st.a[1], A0[1], A1[1], A2[1]);
-  load_attr(frame, st.a[2], A0[2], A1[2], A2[2]);
+                    const DrawGPU &d, int sm, const float4 *A0, const float4 *A1,
+                    const float4 *A2, float &cr, float &cg, float &cb) {
   V3 fn = v3norm(V3{(A0[0].x + A0[1].x + A0[2].x), (A0[0].y + A0[1].y + A0[2].y),
                     (A0[0].z + A0[1].z + A0[2].z)});
   float avg_r = (A1[0].z + A1[1].z + A1[2].z) * (1.0f / 3.0f);
@@ -1603,9 +1589,7 @@ DEV void unlit_smooth_color(const FrameArgs &frame, const SubTri &st, const Rast
   pc1 *= inv_pc;
   pc2 *= inv_pc;
   float4 A0[3], A1[3], A2[3];
-  load_attr(frame, st.a[0], A0[0], A1[0], A2[0]);
-  load_attr(frame, st.a[1], A0[1], A1[1], A2[1]);
-  load_attr(frame, st.a[2], A0[2], A1[2], A2[2]);
+  load_tri_attr(frame, st, A0, A1, A2);
   V3 n = V3{pc0 * A0[0].x + pc1 * A0[1].x + pc2 * A0[2].x,
             pc0 * A0[0].y + pc1 * A0[1].y + pc2 * A0[2].y,
             pc0 * A0[0].z + pc1 * A0[1].z + pc2 * A0[2].z};
@@ -1702,9 +1686,9 @@ DEV bool shade_px(const FrameArgs &frame, const ViewGPU &fa, uint32_t s, int px,
   float pr, pg, pb;
   if (flat) {
     z = zf;
-    flat_color(frame, fa, st, d, sm, pr, pg, pb);
+    flat_color(frame, fa, st, d, sm, k.A0, k.A1, k.A2, pr, pg, pb);
   } else {
-    shade_fragment(frame, fa, st, r, w0, w1, w2, z, pr, pg, pb);
+    shade_fragment(frame, fa, st, r, k.A0, k.A1, k.A2, w0, w1, w2, z, pr, pg, pb);
   }
   out = make_float4(pr, pg, pb, 1.0f);
   id = d.object_id;
@@ -2777,7 +2761,9 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
     if (depth_test && !(zf < L.depth[li]))
       return;
     float cr, cg, cb;
-    flat_color(frame, vw, st, d, sm, cr, cg, cb);
+    float4 A0[3], A1[3], A2[3];
+    load_tri_attr(frame, st, A0, A1, A2);
+    flat_color(frame, vw, st, d, sm, A0, A1, A2, cr, cg, cb);
     float ca = clamp01f(d.opacity);
     if (sm == MOPH_SM_UNLIT_FLAT)
       ca = (float)cvt_u8(ca * 255.0f) / 255.0f;
@@ -2820,7 +2806,9 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
     bool silhouette = (L.depth[li] - z) > 0.005f;
     cov = (min_d < 1.0f && silhouette) ? min_d : 1.0f;
   } else {
-    shade_fragment(frame, vw, st, r, w0, w1, w2, z, pr, pg, pb);
+    float4 A0[3], A1[3], A2[3];
+    load_tri_attr(frame, st, A0, A1, A2);
+    shade_fragment(frame, vw, st, r, A0, A1, A2, w0, w1, w2, z, pr, pg, pb);
     cov = (min_d < 1.0f && blend != MOP_HIP_BLEND_OPAQUE) ? min_d : 1.0f;
   }
   float fa_ = op * cov;
@@ -2944,7 +2932,9 @@ DEV void wire_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &
                        int py, int li, OrderedLds &L) {
   const DrawGPU &d = frame.draws[st.draw];
   float cr, cg, cb;
-  flat_color(frame, vw, st, d, draw_sm(d), cr, cg, cb);
+  float4 A0[3], A1[3], A2[3];
+  load_tri_attr(frame, st, A0, A1, A2);
+  flat_color(frame, vw, st, d, draw_sm(d), A0, A1, A2, cr, cg, cb);
   float fr = (float)cvt_u8((cr > 1.0f ? 1.0f : cr) * 255.0f) / 255.0f;
   float fg = (float)cvt_u8((cg > 1.0f ? 1.0f : cg) * 255.0f) / 255.0f;
   float fb_ = (float)cvt_u8((cb > 1.0f ? 1.0f : cb) * 255.0f) / 255.0f;

@@ -163,10 +163,27 @@ HD void load_subtri(const FrameArgs &fa, uint32_t s, SubTri &st) {
 }
 
 
+/* The three attribute float4 of attribute slot `slot` (vertex cache or
+ * clip store). */
+HD void load_attr(const FrameArgs &fa, uint32_t slot, float4 &a0, float4 &a1, float4 &a2) {
+  const float4 *p = (slot < fa.n_verts) ? fa.vattr + (size_t)slot * 3
+                                        : fa.cattr + (size_t)(slot - fa.n_verts) * 3;
+  a0 = p[0];
+  a1 = p[1];
+  a2 = p[2];
+}
+
+/* The attributes of a sub-triangle's three vertices. */
+HD void load_tri_attr(const FrameArgs &fa, const SubTri &st, float4 *A0, float4 *A1, float4 *A2) {
+  for (int i = 0; i < 3; i++)
+    load_attr(fa, st.a[i], A0[i], A1[i], A2[i]);
+}
+
 /* What the shading of one pixel needs from its key. */
 struct KeyFetch {
   SubTri st;
   RasterTri r; /* inv_e is not set: shading does not use it */
+  float4 A0[3], A1[3], A2[3]; /* the vertices' attributes */
   float w0, w1, w2, zf;
   int sm;
   bool flat;
@@ -198,6 +215,9 @@ HD bool shade_key_fetch(const FrameArgs &frame, const ViewGPU &fa, uint32_t s, i
   uint32_t na = frame.n_verts + frame.clip_cap * MOPH_MAX_POLY;
   if (k.st.draw >= frame.ndraws || k.st.a[0] >= na || k.st.a[1] >= na || k.st.a[2] >= na)
     return false;
+  /* the attributes depend on the record alone: issued now, they are in
+   * flight while the edge setup and the walk replay run */
+  load_tri_attr(frame, k.st, k.A0, k.A1, k.A2);
   RasterTri &r = k.r;
   if (!load_raster_tri_fast(fa.w, fa.h, k.st, r) || px < r.es.min_x || px > r.es.max_x ||
       py < r.es.min_y || py > r.es.max_y)

@@ -90,6 +90,12 @@ int main(int argc, char **argv) {
   for (uint32_t i = 0; i < clip_cap * MOPH_MAX_POLY; i++)
     cpos[i] = make_float4(junk(-2.0f, 2.0f), junk(-2.0f, 2.0f), junk(-2.0f, 2.0f),
                           junk(0.05f, 3.0f));
+  float4 *vattr = table<float4>((size_t)n_verts * 3);
+  for (uint32_t i = 0; i < n_verts * 3; i++)
+    vattr[i] = make_float4(junk(-1.0f, 1.0f), junk(-1.0f, 1.0f), junk(-1.0f, 1.0f), junk(0.0f, 1.0f));
+  float4 *cattr = table<float4>((size_t)clip_cap * MOPH_MAX_POLY * 3);
+  for (uint32_t i = 0; i < clip_cap * MOPH_MAX_POLY * 3; i++)
+    cattr[i] = make_float4(junk(-1.0f, 1.0f), junk(-1.0f, 1.0f), junk(-1.0f, 1.0f), junk(0.0f, 1.0f));
   uint32_t nbo = n_tris + clip_cap * 16 + 1;
   uint32_t *big_off = table<uint32_t>(nbo);
   for (uint32_t i = 0; i < nbo; i++)
@@ -129,6 +135,8 @@ int main(int argc, char **argv) {
     fa.chk_cap = chk_cap;
     fa.big_off = big_off;
     fa.cpos = cpos;
+    fa.vattr = vattr;
+    fa.cattr = cattr;
     fa.subtris = subtris;
     fa.subtri_cap = subtri_cap;
     for (long i = 0; i < keys; i++) {
@@ -157,6 +165,8 @@ int main(int argc, char **argv) {
         accepted[fused]++;
         sum += (double)k.w0 + (double)k.w1 + (double)k.w2 + (double)k.zf + (double)k.sm +
                (double)k.st.sx[0] + (double)k.st.sy[2] + (double)k.st.iw[1] + (double)k.st.sz[2];
+        for (int j = 0; j < 3; j++)
+          sum += (double)k.A0[j].x + (double)k.A1[j].w + (double)k.A2[j].y;
         big[fused] += bbox_area(k.r.es) > (long long)fa.small_area;
         uint32_t na = n_verts + clip_cap * MOPH_MAX_POLY;
         if (k.st.draw >= ndraws || k.st.a[0] >= na || k.st.a[1] >= na || k.st.a[2] >= na ||
@@ -176,6 +186,8 @@ int main(int argc, char **argv) {
   free(tri_rec);
   free(vscr);
   free(cpos);
+  free(vattr);
+  free(cattr);
   free(big_off);
   free(chk);
   free(subtris);
