@@ -247,9 +247,11 @@ def config2_form(args):
 
     # roofline of the dominant kernel, the shading pass (HIP events on the
     # backend's stream).  HBM: the 28 B/px framebuffer of the frame over the
-    # kernel's launch time.  VALU: the kernel's issued VALU instructions per
-    # launch (rocprofv3 SQ_INSTS_VALU, committed summary) x 64 lanes over the
-    # same time, against 39.3 T lane-ops/s.  `bound` is the larger fraction.
+    # kernel's launch time.  VALU: the algorithmic floating-point operations
+    # of the frame's shading (the op-counting build of the CPU restatement,
+    # committed summary) over the same time, against 39.3 T lane-ops/s; the
+    # issued VALU instructions (rocprofv3 SQ_INSTS_VALU x 64 lanes) are
+    # reported beside them.  `bound` is the larger of the two fractions.
     shade_ms = stages.get("shade")
     roof = None
     if shade_ms:
@@ -259,12 +261,19 @@ def config2_form(args):
         summ = "frame_1080p_pbr_counters.md"
         traffic = profiled(summ, "traffic_bytes_per_launch", "k_shade")
         insts = profiled(summ, "valu_insts_per_launch", "k_shade")
+        ops = profiled(summ, "algorithmic_ops_per_launch", "k_shade")
         valu = None
-        if insts:
-            tops = insts * 64 / (shade_ms * 1e-3) / 1e12
+        if ops:
+            tops = ops / (shade_ms * 1e-3) / 1e12
             valu = {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
                     "unit": "TOP/s", "frac": round(tops / VALU_PEAK_TOPS, 5),
-                    "valu_insts_per_launch": insts}
+                    "algorithmic_ops_per_launch": int(ops)}
+            if insts:
+                issued = insts * 64 / (shade_ms * 1e-3) / 1e12
+                valu.update({"valu_insts_per_launch": insts,
+                             "issued_lane_ops_per_s": round(issued * 1e12, 0),
+                             "issue_frac": round(issued / VALU_PEAK_TOPS, 5),
+                             "issued_over_algorithmic": round(insts * 64 / ops, 3)})
         hbm = {"bound": "hbm", "achieved": round(hbm_ach, 2), "peak": HBM_PEAK_GBS,
                "unit": "GB/s", "frac": round(hbm_ach / HBM_PEAK_GBS, 5),
                "traffic": int(traffic) if traffic else None}
