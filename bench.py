@@ -120,6 +120,26 @@ def profiled(summary, key, kernel):
     return None
 
 
+def d2h_link_gbs(nbytes, reps=40):
+    """Device-to-host bandwidth of back-to-back copies of one frame's bytes
+    into pinned host memory, nothing else running: the ceiling of the
+    read-back rate."""
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    src.fill_(1)
+    for _ in range(5):
+        dst.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src, non_blocking=True)
+    torch.cuda.synchronize()
+    gbs = nbytes * reps / (time.perf_counter() - t0) / 1e9
+    del src, dst
+    return gbs
+
+
 def pipelined_frames(dev, drv, fbs, cam, steps):
     """`steps` frames rendered into the framebuffers in turn, each copied to
     the host (RGBA8, depth, id) behind the next frames' kernels; frame k is
@@ -203,6 +223,8 @@ def config2_form(args):
     fps_sync = n_sync / (time.perf_counter() - t0)
     log(f"synchronous readback: {fps_sync:.1f} frames/s")
     check_overflow(dev, "config 2 synchronous readback")
+    link_gbs = d2h_link_gbs(W * H * 12)
+    log(f"device-to-host copy of one frame's 12 B/px alone: {link_gbs:.2f} GB/s")
 
     # host cost of one frame with the device idle, device span of one frame
     idle_host, dev_span = [], []
@@ -304,6 +326,8 @@ def config2_form(args):
             "fps_synchronous_readback": round(fps_sync, 3),
             "readback_gbs": round(fps * W * H * 12 / 1e9, 2),
             "readback_frac_of_pcie_gen5_x16": round(fps * W * H * 12 / 64e9, 4),
+            "d2h_copy_gbs": round(link_gbs, 2),
+            "readback_frac_of_d2h_copy": round(fps * W * H * 12 / 1e9 / link_gbs, 4),
             "readback": "RGBA8 + depth + object ID to pinned host memory per frame, "
                         f"copies of frame k overlapping the next frames ({args.readback_depth} "
                         f"framebuffers)",
