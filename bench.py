@@ -423,6 +423,58 @@ def config3_form(frames):
     return out
 
 
+def transparent_form(frames, only=False):
+    """The 1080p PBR scene with a quarter of its icospheres blended (alpha,
+    additive, multiply): the opaque draws take the order-free path, the 64
+    blended ones the submission-ordered tile path, after them, farthest
+    first.  With only=True: just the HBM-resident frames (for rocprofv3)."""
+    from mop_amd import driver, scenes
+    dev = driver.Device()
+    sc = scenes.config2_transparent()
+    drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
+    fbs = [dev.framebuffer(sc.width, sc.height) for _ in range(2)]
+    for k in range(3):
+        drv.render(fbs[k % 2], drv.cam)
+    dev.sync()
+    check_overflow(dev, "transparent warm-up")
+    out = {"width": sc.width, "height": sc.height, "triangles": sc.triangle_count,
+           "blended_meshes": sum(1 for m in sc.meshes if m.blend), "frames": frames}
+    if not only:
+        secs, tris, _ = pipelined_frames(dev, drv, fbs, drv.cam, frames)
+        check_overflow(dev, "transparent timed frames")
+        out["frames_per_s"] = round(frames / secs, 3)
+    dev.sync()
+    t0 = time.perf_counter()
+    for k in range(frames):
+        drv.render(fbs[k % 2], drv.cam)
+    dev.sync()
+    hbm_s = time.perf_counter() - t0
+    check_overflow(dev, "transparent HBM-resident frames")
+    out["frames_per_s_hbm_resident"] = round(frames / hbm_s, 3)
+    if not only:
+        drv.render(fbs[0], drv.cam)
+        dev.sync()
+        out["device_ms_one_frame"] = round(dev.frame_stats()["gpu_ms"], 4)
+        dev.timers(True)
+        for k in range(20):
+            drv.render(fbs[k % 2], drv.cam)
+        dev.sync()
+        st = {}
+        for name in ("shadow", "vertex", "setup", "raster", "shade", "resolve", "post"):
+            tot, cnt = dev.timer(name)
+            if cnt:
+                st[name] = round(tot / 20, 4)
+        dev.timers(False)
+        check_overflow(dev, "transparent stage-timed frames")
+        out["stage_ms_per_frame"] = st
+    log(f"transparent 1080p: {out}")
+    for fb in fbs:
+        fb.close()
+    drv.close()
+    dev.close()
+    return out
+
+
 # ---------------------------------------------------------------------------
 # reference CPU baseline
 
@@ -748,6 +800,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-config3", action="store_true")
+    ap.add_argument("--no-transparent", action="store_true")
+    ap.add_argument("--transparent-frames", type=int, default=100)
+    ap.add_argument("--transparent-only", action="store_true",
+                    help="only the HBM-resident frames of the transparent 1080p scene "
+                         "(for profiling)")
     ap.add_argument("--config3-frames", type=int, default=10)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 64-viewport batched form (for profiling)")
@@ -794,10 +851,15 @@ def main():
     if args.profile_frames:
         print(json.dumps(profile_frames(args)))
         return
+    if args.transparent_only:
+        print(json.dumps({"transparent_1080p": transparent_form(args.transparent_frames,
+                                                                only=True)}))
+        return
 
     c2 = config2_form(args)
     sc = c2["sc"]
     c3 = None if args.no_config3 else config3_form(args.config3_frames)
+    tr = None if args.no_transparent else transparent_form(args.transparent_frames)
     batch = None if args.no_batch else batched_form(mod, args.batch_rounds, not args.no_tune)
     if batch:
         log(f"batched: {batch}")
@@ -832,6 +894,7 @@ def main():
         **c2["extra"],
         "roofline": c2["roofline"],
         "raster_stress_4k": c3,
+        "transparent_1080p": tr,
         "batched_64_viewports": batch,
         "serving": serving,
         "cpu_baseline": cpu,

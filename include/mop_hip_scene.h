@@ -43,6 +43,14 @@ void mop_hip_scene_set_exposure(MopHipScene *s, float exposure);
 float mop_hip_scene_exposure(const MopHipScene *s);
 void mop_hip_scene_set_clear_color(MopHipScene *s, MopHipColor c);
 void mop_hip_scene_set_post_effects(MopHipScene *s, uint32_t effects);
+/* Blend mode (MOP_HIP_BLEND_*) and opacity (clamped to [0, 1]) of mesh
+ * `mesh`: mop_mesh_set_blend_mode and mop_mesh_set_opacity
+ * (scene.h:111-125, viewport.c:1815-1837) in one call, the mode given being
+ * the mode drawn (no promotion of an opaque mesh to alpha).  Blended meshes
+ * cast no shadow and are drawn after the opaque ones, farthest first
+ * (pass_scene_transparent, viewport.c:2470-2546).  Returns 0 or -1. */
+int mop_hip_scene_set_mesh_blend(MopHipScene *s, uint32_t mesh, uint32_t blend_mode,
+                                 float opacity);
 /* Record one frame of s into fb without executing it (execute a set of
  * recorded frames with mop_hip_frames_execute).  Returns triangles submitted. */
 uint32_t mop_hip_scene_record(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,

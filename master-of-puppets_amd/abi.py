@@ -160,6 +160,7 @@ SIGS = {
     "mop_hip_scene_exposure": (C.c_float, [VP]),
     "mop_hip_scene_set_clear_color": (None, [VP, Color]),
     "mop_hip_scene_set_post_effects": (None, [VP, C.c_uint32]),
+    "mop_hip_scene_set_mesh_blend": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_float]),
     "mop_hip_scene_render": (C.c_uint32, [VP, VP, C.POINTER(Camera), C.c_int, C.c_int]),
     "mop_hip_scene_triangles": (C.c_uint32, [VP]),
     "mop_hip_scene_record": (C.c_uint32, [VP, VP, C.POINTER(Camera), C.c_int, C.c_int]),

@@ -2991,6 +2991,44 @@ DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int
     ordered_fragment(frame, vw, st, r, px, py, li, L);
 }
 
+/* Tags each sorted list entry with the waves of the tile it concerns: wave w
+ * holds rows 4w..4w+3 of the 16x16 tile, and an entry whose primitive's bbox
+ * (the bbox ordered_step tests, prim_bbox) misses those rows cannot touch the
+ * wave's pixels, so the wave passes it over without loading it.  The bboxes
+ * of the list are computed by all threads at once.  Bits 28-31 of an entry
+ * hold the mask (indices stay below 2^28: the caller checks); an entry
+ * outside the store gets an empty mask. */
+DEV void tag_waves(const FrameArgs &frame, const ViewGPU &vw, uint32_t *list, int m,
+                   uint32_t total, int tile_y0) {
+  for (int i = (int)threadIdx.x; i < m; i += WG) {
+    uint32_t s = list[i];
+    uint32_t mask = 0;
+    if (s < total) {
+      int bx0, by0, bx1, by1;
+      if (prim_bbox(frame, frame.subtris[s], vw.w, vw.h, bx0, by0, bx1, by1)) {
+        int r0 = by0 - tile_y0, r1 = by1 - tile_y0;
+#pragma unroll
+        for (int w = 0; w < OT * OT / 64; w++)
+          if (r0 <= 4 * w + 3 && r1 >= 4 * w)
+            mask |= 1u << w;
+      }
+    }
+    list[i] = (s & 0x0fffffffu) | (mask << 28);
+  }
+  __syncthreads();
+}
+
+/* The entries of a tagged list this wave walks, in order. */
+DEV void walk_tagged(const FrameArgs &frame, const ViewGPU &vw, const uint32_t *list, int m,
+                     int px, int py, bool inside, int li, OrderedLds &L, OrderedState &os) {
+  uint32_t wbit = 1u << (28 + (threadIdx.x >> 6));
+  for (int e = 0; e < m; e++) {
+    uint32_t v = list[e];
+    if (v & wbit)
+      ordered_step(frame, vw, v & 0x0fffffffu, px, py, inside, li, L, os);
+  }
+}
+
 /* One workgroup per 16x16 tile.  The tile's bin holds the indices of the
  * screen triangles whose bbox overlaps it, in arbitrary order; they are
  * walked in submission order.  A list of at most OT_CAP entries is sorted
@@ -3026,6 +3064,8 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
   uint32_t total = frame.counters[CNT_SUBTRI];
   if (total > frame.subtri_cap)
     total = frame.subtri_cap;
+  bool tag = total <= 0x10000000u; /* indices fit the 28 bits of a tagged entry */
+  int tile_y0 = (int)(t / tiles_x) * OT;
   if (inside) {
     L.hdr[li] = reinterpret_cast<const float4 *>(vw.hdr)[idx];
     L.depth[li] = vw.depth[idx];
@@ -3037,10 +3077,15 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
       L.list[i] = bins[b0 + i];
     __syncthreads();
     lds_sort(L.list, (int)n);
-    for (uint32_t e = 0; e < n; e++) {
-      uint32_t s = L.list[e];
-      if (s < total)
-        ordered_step(frame, vw, s, px, py, inside, li, L, os);
+    if (tag) {
+      tag_waves(frame, vw, L.list, (int)n, total, tile_y0);
+      walk_tagged(frame, vw, L.list, (int)n, px, py, inside, li, L, os);
+    } else {
+      for (uint32_t e = 0; e < n; e++) {
+        uint32_t s = L.list[e];
+        if (s < total)
+          ordered_step(frame, vw, s, px, py, inside, li, L, os);
+      }
     }
   } else {
     uint32_t lo = 0, span = total;
@@ -3081,10 +3126,15 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
         __syncthreads();
         int m = (int)(c < OT_CAP ? c : OT_CAP);
         lds_sort(L.list, m);
-        for (int e = 0; e < m; e++) {
-          uint32_t s = L.list[e];
-          if (s < total)
-            ordered_step(frame, vw, s, px, py, inside, li, L, os);
+        if (tag) {
+          tag_waves(frame, vw, L.list, m, total, tile_y0);
+          walk_tagged(frame, vw, L.list, m, px, py, inside, li, L, os);
+        } else {
+          for (int e = 0; e < m; e++) {
+            uint32_t s = L.list[e];
+            if (s < total)
+              ordered_step(frame, vw, s, px, py, inside, li, L, os);
+          }
         }
       }
       /* a sparse window lets the next one grow */

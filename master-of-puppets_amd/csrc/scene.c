@@ -4,7 +4,8 @@
  * with the pass order and per-draw arguments of mop_viewport_render
  * (viewport.c:2993-3317): clear, exposure, shadow pass (pass_shadow,
  * viewport.c:2288-2442), IBL state (rg_ibl_setup, :2692-2706), opaque draws
- * (pass_scene_opaque + EMIT_DRAW, :2015-2088, :2449-2467), HDR resolve
+ * (pass_scene_opaque + EMIT_DRAW, :2015-2088, :2449-2467), transparent draws
+ * back to front (pass_scene_transparent, :2470-2546), HDR resolve
  * (:2727-2730) and post-processing (:2950-2961).
  *
  * It is the caller the benchmark, the batched form and the serving loop use
@@ -118,6 +119,8 @@ typedef struct SceneMesh {
   int has_material;
   float metallic, roughness;
   MopHipVec3 aabb_min, aabb_max; /* local */
+  uint32_t blend;                 /* MopHipBlendMode */
+  float opacity;
 } SceneMesh;
 
 struct MopHipScene {
@@ -205,6 +208,8 @@ int mop_hip_scene_add_mesh(MopHipScene *s, const MopHipVertex *v, uint32_t nv,
   m->has_material = has_material;
   m->metallic = metallic;
   m->roughness = roughness;
+  m->blend = MOP_HIP_BLEND_OPAQUE; /* viewport.c mesh defaults */
+  m->opacity = 1.0f;
   m->aabb_min = m->aabb_max = v[0].position;
   for (uint32_t i = 1; i < nv; i++) {
     MopHipVec3 p = v[i].position;
@@ -248,6 +253,8 @@ int mop_hip_scene_add_loaded_mesh(MopHipScene *s, MopHipLoadedMesh *lm, uint32_t
   m->has_material = has_material;
   m->metallic = metallic;
   m->roughness = roughness;
+  m->blend = MOP_HIP_BLEND_OPAQUE; /* viewport.c mesh defaults */
+  m->opacity = 1.0f;
   m->aabb_min = lm->aabb_min;
   m->aabb_max = lm->aabb_max;
   memset(lm, 0, sizeof(*lm));
@@ -337,6 +344,16 @@ void mop_hip_scene_set_clear_color(MopHipScene *s, MopHipColor c) {
   if (s)
     s->clear = c;
 }
+int mop_hip_scene_set_mesh_blend(MopHipScene *s, uint32_t mesh, uint32_t blend_mode,
+                                 float opacity) {
+  if (!s || mesh >= s->mesh_count || blend_mode > MOP_HIP_BLEND_MULTIPLY)
+    return -1;
+  s->meshes[mesh].blend = blend_mode;
+  s->meshes[mesh].opacity = opacity < 0.0f ? 0.0f : (opacity > 1.0f ? 1.0f : opacity);
+  s->shadow_valid = 0; /* the shadow pass draws opaque meshes only */
+  return 0;
+}
+
 void mop_hip_scene_set_post_effects(MopHipScene *s, uint32_t effects) {
   if (s)
     s->post_effects = effects;
@@ -353,10 +370,14 @@ static int shadow_light_vp(const MopHipScene *s, MopHipMat4 *out) {
       li = (int)i;
       break;
     }
-  if (li < 0 || s->mesh_count == 0)
+  if (li < 0)
     return 0;
   float mn[3] = {1e9f, 1e9f, 1e9f}, mx[3] = {-1e9f, -1e9f, -1e9f};
+  int any = 0;
   for (uint32_t i = 0; i < s->mesh_count; i++) {
+    if (s->meshes[i].blend != MOP_HIP_BLEND_OPAQUE)
+      continue;
+    any = 1;
     const float *d = s->meshes[i].model.d;
     float cx = d[12], cy = d[13], cz = d[14];
     float sx = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
@@ -373,6 +394,8 @@ static int shadow_light_vp(const MopHipScene *s, MopHipMat4 *out) {
     if (cy + r > mx[1]) mx[1] = cy + r;
     if (cz + r > mx[2]) mx[2] = cz + r;
   }
+  if (!any)
+    return 0;
   MopHipVec3 nd = v3_norm(s->lights[li].direction);
   float ccx = (mn[0] + mx[0]) * 0.5f, ccy = (mn[1] + mx[1]) * 0.5f, ccz = (mn[2] + mx[2]) * 0.5f;
   float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
@@ -453,6 +476,40 @@ static int mesh_outside(const SceneMesh *m, const MopHipMat4 *vpm) {
   return 0;
 }
 
+/* EMIT_DRAW (viewport.c:2015-2088) of one scene mesh; returns its triangles. */
+static uint32_t emit_draw(MopHipScene *s, MopHipFramebuffer *fb, const SceneMesh *m,
+                          const MopHipMat4 *view, const MopHipMat4 *proj, MopHipVec3 eye,
+                          bool cast) {
+  MopHipDrawCall c;
+  memset(&c, 0, sizeof(c));
+  c.vertex_buffer = m->vb;
+  c.index_buffer = m->ib;
+  c.vertex_count = m->vertex_count;
+  c.index_count = m->index_count;
+  c.object_id = m->object_id;
+  c.model = m->model;
+  c.view = *view;
+  c.projection = *proj;
+  c.mvp = m4_mul(*proj, m4_mul(*view, m->model));
+  c.base_color = (MopHipColor){1, 1, 1, 1};
+  c.opacity = m->opacity;
+  c.light_dir = v3(0.3f, 1.0f, 0.5f);
+  c.ambient = s->ambient;
+  c.shading_mode = MOP_HIP_SHADING_SMOOTH;
+  c.depth_test = true;
+  c.depth_write = true;
+  c.backface_cull = true;
+  c.blend_mode = (int32_t)m->blend;
+  c.metallic = m->has_material ? m->metallic : 0.0f;
+  c.roughness = m->has_material ? m->roughness : 0.5f;
+  c.lights = s->lights;
+  c.light_count = s->light_count;
+  c.cam_eye = eye;
+  c.cast_shadows = cast;
+  mop_hip_draw(s->dev, fb, &c);
+  return m->index_count / 3;
+}
+
 uint32_t mop_hip_scene_record(MopHipScene *s, MopHipFramebuffer *fb, const MopHipCamera *cam,
                               int width, int height) {
   if (!s || !fb || !cam)
@@ -483,6 +540,8 @@ uint32_t mop_hip_scene_record(MopHipScene *s, MopHipFramebuffer *fb, const MopHi
         mop_hip_shadow_begin(dev, s->shadow);
         for (uint32_t i = 0; i < s->mesh_count; i++) {
           const SceneMesh *m = &s->meshes[i];
+          if (m->blend != MOP_HIP_BLEND_OPAQUE)
+            continue;
           mop_hip_shadow_draw(dev, m->vb, m->vertex_count, m->ib, m->index_count, &m->model,
                               &lvp);
         }
@@ -505,36 +564,50 @@ uint32_t mop_hip_scene_record(MopHipScene *s, MopHipFramebuffer *fb, const MopHi
   uint32_t tris = 0;
   for (uint32_t i = 0; i < s->mesh_count; i++) {
     const SceneMesh *m = &s->meshes[i];
-    if (mesh_outside(m, &vpm))
+    if (m->blend != MOP_HIP_BLEND_OPAQUE || mesh_outside(m, &vpm))
       continue;
-    MopHipDrawCall c;
-    memset(&c, 0, sizeof(c));
-    c.vertex_buffer = m->vb;
-    c.index_buffer = m->ib;
-    c.vertex_count = m->vertex_count;
-    c.index_count = m->index_count;
-    c.object_id = m->object_id;
-    c.model = m->model;
-    c.view = view;
-    c.projection = proj;
-    c.mvp = m4_mul(proj, m4_mul(view, m->model));
-    c.base_color = (MopHipColor){1, 1, 1, 1};
-    c.opacity = 1.0f;
-    c.light_dir = v3(0.3f, 1.0f, 0.5f);
-    c.ambient = s->ambient;
-    c.shading_mode = MOP_HIP_SHADING_SMOOTH;
-    c.depth_test = true;
-    c.depth_write = true;
-    c.backface_cull = true;
-    c.blend_mode = MOP_HIP_BLEND_OPAQUE;
-    c.metallic = m->has_material ? m->metallic : 0.0f;
-    c.roughness = m->has_material ? m->roughness : 0.5f;
-    c.lights = s->lights;
-    c.light_count = s->light_count;
-    c.cam_eye = eye;
-    c.cast_shadows = cast;
-    mop_hip_draw(dev, fb, &c);
-    tris += m->index_count / 3;
+    tris += emit_draw(s, fb, m, &view, &proj, eye, cast);
+  }
+  /* transparent pass (pass_scene_transparent, viewport.c:2470-2546): the
+   * visible blended meshes, farthest first by the squared distance of their
+   * model translation to the eye; the insertion sort keeps submission order
+   * among equal distances */
+  uint32_t nt = 0;
+  for (uint32_t i = 0; i < s->mesh_count; i++)
+    if (s->meshes[i].blend != MOP_HIP_BLEND_OPAQUE)
+      nt++;
+  if (nt) {
+    uint32_t *ord = malloc(nt * sizeof(uint32_t));
+    float *dist = malloc(nt * sizeof(float));
+    if (ord && dist) {
+      uint32_t k = 0;
+      for (uint32_t i = 0; i < s->mesh_count; i++) {
+        const SceneMesh *m = &s->meshes[i];
+        if (m->blend == MOP_HIP_BLEND_OPAQUE || mesh_outside(m, &vpm))
+          continue;
+        float dx = m->model.d[12] - eye.x, dy = m->model.d[13] - eye.y,
+              dz = m->model.d[14] - eye.z;
+        ord[k] = i;
+        dist[k] = dx * dx + dy * dy + dz * dz;
+        k++;
+      }
+      for (uint32_t j = 1; j < k; j++) {
+        uint32_t oi = ord[j];
+        float od = dist[j];
+        int q = (int)j - 1;
+        while (q >= 0 && dist[q] < od) {
+          ord[q + 1] = ord[q];
+          dist[q + 1] = dist[q];
+          q--;
+        }
+        ord[q + 1] = oi;
+        dist[q + 1] = od;
+      }
+      for (uint32_t j = 0; j < k; j++)
+        tris += emit_draw(s, fb, &s->meshes[ord[j]], &view, &proj, eye, cast);
+    }
+    free(ord);
+    free(dist);
   }
   /* rg_frame_end + rg_cleanup: resolve with the exposure, then post */
   mop_hip_frame_request_post(dev, fb, s->exposure, s->post_effects, NULL);

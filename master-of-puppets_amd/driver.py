@@ -141,6 +141,12 @@ class SceneDriver:
                                             int(m.has_material), m.metallic, m.roughness)
             if rc < 0:
                 raise RuntimeError("add_mesh failed: " + lib.mop_hip_last_error().decode())
+        # blend mode and opacity as the core's mesh setters leave them
+        for k, m in enumerate(sc.meshes):
+            blend, op = getattr(m, "blend", 0), getattr(m, "opacity", 1.0)
+            if blend != 0 or op < 1.0:
+                if lib.mop_hip_scene_set_mesh_blend(self.s, k, blend, min(op, 1.0)) != 0:
+                    raise RuntimeError("set_mesh_blend failed")
         for L in sc.lights:
             l = abi.Light()
             l.type = L.type

@@ -442,6 +442,20 @@ def config2(width=1920, height=1080, n_instances=256, subdiv=4, seed=0x4D4F5002,
     return sc
 
 
+def config2_transparent(width=1920, height=1080, every=4, **kw):
+    """The 1080p PBR scene with every `every`-th icosphere blended (alpha,
+    additive and multiply in turn, opacity 0.3-0.8): those go through the
+    submission-ordered path, drawn after the opaque ones, farthest first."""
+    sc = config2(width=width, height=height, **kw)
+    sc.name = sc.name.replace("cfg2_pbr", "cfg2_transparent")
+    rng = Xoshiro(0x4D4F5012)
+    for k in range(0, len(sc.meshes), every):
+        m = sc.meshes[k]
+        m.blend = 1 + (k // every) % 3
+        m.opacity = rng.uniform(0.3, 0.8)
+    return sc
+
+
 def config3(width=3840, height=2160, n_quads=2048, n_spheres=64, seed=0x4D4F5003,
             shadow_size=4096):
     """4K raster stress: noise-displaced grid, camera just under the surface,

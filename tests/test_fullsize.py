@@ -84,6 +84,24 @@ def test_config2_full_size_frame_driver():
     refcore.assert_parity(cmp)
 
 
+def test_config2_transparent_full_size():
+    # the ordered path at 1080p: a quarter of the 256 icospheres blended
+    # (alpha, additive, multiply), through the core and through the frame
+    # driver's own back-to-front transparent pass
+    from mop_amd import driver
+    sc = scenes.config2_transparent()
+    ref = _dropin(sc, 2048)
+    print(f"[case] driver {sc.name} shadow 2048", flush=True)
+    dev = driver.Device()
+    try:
+        got = _driver_frame(dev, sc, 2048)
+    finally:
+        dev.close()
+    cmp = refcore.compare(got, ref)
+    _report("driver config2 transparent shadow 2048", cmp)
+    refcore.assert_parity(cmp)
+
+
 def test_config3_full_size_dropin():
     _dropin(scenes.config3(), 4096)
 
