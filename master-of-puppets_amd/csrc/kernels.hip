@@ -634,11 +634,16 @@ DEV bool frag_depth(const RasterTri &r, float w0, float w1, float w2, float &z) 
 
 /* Visibility update: a 64-bit atomic min with no return value, so a lane
  * walking a triangle issues its pixels back to back without waiting on the
- * memory system.  The atomic is performed beyond the XCD's L2 (the eight
- * XCDs' L2s are not coherent with each other), which makes it several times
- * dearer than a store. */
+ * memory system.  It is issued as a global (not flat) atomic: a flat one
+ * also counts against the LDS/scalar-load counter, so every later wait for
+ * an LDS or scalar load would wait for the atomics in flight too.  Callers
+ * hold the buffer pointer and pitch in registers: read through the view
+ * record inside a walk, they would be reloaded after every atomic (which
+ * might alias the record) and the load's wait would drain the atomics
+ * issued before it, one memory round trip per pixel. */
+typedef __attribute__((address_space(1))) unsigned long long gvis_t;
 DEV void vis_update(unsigned long long *vis, size_t idx, unsigned long long key) {
-  atomicMin(vis + idx, key);
+  __hip_atomic_fetch_min((gvis_t *)(vis + idx), key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* The whole bbox walk of one small screen triangle by one lane, updating the
@@ -646,6 +651,8 @@ DEV void vis_update(unsigned long long *vis, size_t idx, unsigned long long key)
  * the flat path's edge test and incremental z (rasterizer.c:1000-1026). */
 DEV void walk_small_vis(const ViewGPU &vw, const RasterTri &r, bool flat,
                         unsigned long long seq) {
+  unsigned long long *const vis = vw.vis; /* registers, not the record (vis_update) */
+  const size_t pitch = (size_t)vw.w;
   float w0r = r.es.w_row[0], w1r = r.es.w_row[1], w2r = r.es.w_row[2];
   if (flat) {
     FlatZ fz = flat_z_setup(r);
@@ -654,8 +661,8 @@ DEV void walk_small_vis(const ViewGPU &vw, const RasterTri &r, bool flat,
       float w0 = w0r, w1 = w1r, w2 = w2r, z = zr;
       for (int x = r.es.min_x; x <= r.es.max_x; x++) {
         if (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z) {
-          size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
-          vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+          size_t idx = (size_t)y * pitch + (size_t)x;
+          vis_update(vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
         }
         w0 += r.es.e_dx[0];
         w1 += r.es.e_dx[1];
@@ -674,8 +681,8 @@ DEV void walk_small_vis(const ViewGPU &vw, const RasterTri &r, bool flat,
     for (int x = r.es.min_x; x <= r.es.max_x; x++) {
       float z;
       if (frag_depth(r, w0, w1, w2, z)) {
-        size_t idx = (size_t)y * (size_t)vw.w + (size_t)x;
-        vis_update(vw.vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
+        size_t idx = (size_t)y * pitch + (size_t)x;
+        vis_update(vis, idx, ((unsigned long long)depth_key(z) << 32) | seq);
       }
       w0 += r.es.e_dx[0];
       w1 += r.es.e_dx[1];
@@ -762,6 +769,10 @@ __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
     FlatZ fz = {0.0f, 0.0f, 0.0f};
     if (flat)
       fz = flat_z_setup(r);
+    /* registers, not the record (vis_update) */
+    unsigned long long *const vis = vw.vis;
+    float4 *const wbuf = vw.wbuf;
+    const size_t pitch = (size_t)vw.w;
     unsigned long long seq = (unsigned long long)(s + 1);
     int nrows = r.es.max_y - r.es.min_y + 1;
     /* order-free path: checkpoint store of this triangle (or none) */
@@ -801,7 +812,7 @@ __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
         int y = r.es.min_y + row;
         float w0 = rows[0][threadIdx.x], w1 = rows[1][threadIdx.x], w2 = rows[2][threadIdx.x];
         float zf = rows[3][threadIdx.x];
-        size_t rowbase = (size_t)y * (size_t)vw.w;
+        size_t rowbase = (size_t)y * pitch;
         float4 *crow = chk ? chk + (size_t)row * nchk : nullptr;
         int xa, xb;
         {
@@ -833,9 +844,9 @@ __global__ __launch_bounds__(BIG_WG) void k_raster_big(FrameArgs fa) {
           if (hit) {
             unsigned long long key = ((unsigned long long)depth_key(z) << 32) | seq;
             if (MODE == 0) {
-              vis_update(vw.vis, rowbase + x, key);
-            } else if (vw.vis[rowbase + x] == key) {
-              vw.wbuf[rowbase + x] = make_float4(w0, w1, w2, z);
+              vis_update(vis, rowbase + x, key);
+            } else if (vis[rowbase + x] == key) {
+              wbuf[rowbase + x] = make_float4(w0, w1, w2, z);
             }
           }
           w0 += r.es.e_dx[0];
