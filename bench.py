@@ -562,6 +562,28 @@ def cpu_baseline(seconds, pool_cores):
 # batched and serving forms
 
 
+def raster_atomic_roofline(lib):
+    """Roofline of the batch's dominant raster kernel, k_tri: its visibility
+    atomics (64-bit atomic min, no return value, one per covered fragment)
+    over its launch time, both from the committed counter summary of the
+    batch, against the rate at which this GPU performs such atomics on
+    random addresses, measured now (mop_hip_probe_atomic_rate)."""
+    n = profiled("batch64_counters.md", "atomics_per_launch", "k_tri")
+    us = profiled("batch64_counters.md", "launch_us", "k_tri")
+    rand = lib.mop_hip_probe_atomic_rate(0)
+    coal = lib.mop_hip_probe_atomic_rate(1)
+    log(f"atomic probe: {rand:.2f} G/s random, {coal:.2f} G/s with a wave on 512 consecutive bytes")
+    out = {"kernel": "k_tri", "bound": "memory-side atomics", "unit": "G atomics/s",
+           "peak": round(rand, 2), "peak_coalesced": round(coal, 2),
+           "source": "atomics and launch time: profiles/batch64_counters.md; peak: measured now"}
+    if n and us and rand > 0:
+        out["achieved"] = round(n / (us * 1e-6) / 1e9, 2)
+        out["frac"] = round(n / (us * 1e-6) / 1e9 / rand, 3)
+        out["atomics_per_launch"] = int(n)
+        out["launch_us"] = us
+    return out
+
+
 def batched_form(mod, rounds, tune=True):
     """The 64-viewport workload (uneven resolutions, 1e3..2e6 triangles each):
     plain = one viewport per launch set, each finished before the next;
@@ -666,6 +688,7 @@ def batched_form(mod, rounds, tune=True):
         f"replays {rep1.value - rep0.value}, captures {cap1.value - cap0.value}")
     lat = np.array(lat) * 1e3
     px = sum(sc.width * sc.height for sc in views)
+    raster = raster_atomic_roofline(lib)
     out = {"views": len(views), "triangles_per_batch": tris, "pixels_per_batch": px,
            "plain_frames_per_s": round(len(views) / plain_s, 2),
            "batched_frames_per_s": round(len(views) / stream_s, 2),
@@ -685,6 +708,7 @@ def batched_form(mod, rounds, tune=True):
                    (px * FB_BYTES_PER_PX + sum(len(m.vertices) * 48 + len(m.indices) * 4
                                                for sc in views for m in sc.meshes))
                    / (float(np.median(gpu_ms)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+           "raster_roofline": raster,
            "graph_replays": rep1.value - rep0.value,
            "graph_captures": cap1.value - cap0.value,
            "shadow_cache": True,

@@ -437,6 +437,13 @@ void mop_hip_timers_enable(MopHipDevice *dev, int on);
 int mop_hip_timers_read(MopHipDevice *dev, const char *stage, double *total_ms,
                         uint64_t *count);
 
+/* The rate, in G atomics/s, at which the current GPU performs 64-bit atomic
+ * mins with no return value -- the order-free raster's visibility update --
+ * when every lane of a wave targets its own random address (pattern 0) or
+ * the 64 lanes target 512 consecutive bytes (pattern 1).  Times 33.5 M
+ * atomics on a 512 MiB buffer it allocates and frees; negative on error. */
+double mop_hip_probe_atomic_rate(int pattern);
+
 #ifdef __cplusplus
 }
 #endif

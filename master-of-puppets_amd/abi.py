@@ -135,6 +135,7 @@ SIGS = {
     "mop_hip_timers_enable": (None, [VP, C.c_int]),
     "mop_hip_timers_read": (C.c_int, [VP, C.c_char_p, C.POINTER(C.c_double),
                                       C.POINTER(C.c_uint64)]),
+    "mop_hip_probe_atomic_rate": (C.c_double, [C.c_int]),
     "mop_hip_shadow_map_create": (VP, [VP, C.c_int]),
     "mop_hip_shadow_map_destroy": (None, [VP, VP]),
     "mop_hip_shadow_begin": (None, [VP, VP]),
