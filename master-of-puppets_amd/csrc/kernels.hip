@@ -2742,18 +2742,45 @@ DEV void lds_sort(uint32_t *k, int n) {
   }
 }
 
+/* Fragments whose colour is still to be computed, per wave of the tile (one
+ * entry per fragment, in the order the fragments were met). */
+#define OQ 64
+#define OQ_WAVES (OT * OT / 64)
+
 struct OrderedLds {
   float4 hdr[OT * OT];
   float depth[OT * OT];
   float saved[OT * OT];
   uint32_t ids[OT * OT];
   uint32_t list[OT_CAP];
+  uint32_t q_s[OQ_WAVES][OQ];    /* screen triangle */
+  uint32_t q_meta[OQ_WAVES][OQ]; /* lane | kind << 8 | blend << 12 */
+  float q_fa[OQ_WAVES][OQ];      /* the blend's alpha */
+  float4 q_v[OQ_WAVES][OQ];      /* edge values in, colour out */
+  uint32_t q_n[OQ_WAVES];
+};
+
+/* What the in-order pass decides for one fragment of this lane's pixel:
+ * whether a colour is to be blended, how, and how to get it.  kind 0: the
+ * lit smooth model's colour from the edge values in v; kind 1: the flat
+ * model's colour; kind 2: the colour is v already (unlit smooth). */
+struct OrdFrag {
+  bool q;
+  uint32_t kind, blend;
+  float fa;
+  float4 v;
 };
 
 /* One screen triangle against this lane's pixel, the way one iteration of
- * the smooth multi-light loop treats it (rasterizer.c:1929-2140). */
+ * the smooth multi-light loop treats it (rasterizer.c:1929-2140) or the
+ * flat loop (rasterizer.c:1000-1085): coverage, the depth test, and the
+ * depth and id writes, all against the tile as the earlier fragments left
+ * it.  None of these depends on a fragment's colour, so the colour and its
+ * blend are deferred (f): ordered_flush computes the colours of a wave's
+ * pending fragments with one lane each and applies the blends in order. */
 DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &st,
-                          const RasterTri &r, int px, int py, int li, OrderedLds &L) {
+                          const RasterTri &r, int px, int py, int li, OrderedLds &L,
+                          OrdFrag &f) {
   const DrawGPU &d = frame.draws[st.draw];
   int sm = draw_sm(d);
   bool flat = sm_flat(sm);
@@ -2765,34 +2792,23 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
   if (!(w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f))
     return;
   bool depth_test = (d.flags & MOPH_DRAW_DEPTH_TEST) != 0;
-  int blend = (int)((d.flags >> MOPH_DRAW_BLEND_SHIFT) & 3u);
+  uint32_t blend = (d.flags >> MOPH_DRAW_BLEND_SHIFT) & 3u;
+  f.blend = blend;
   if (flat) {
-    /* rasterize_filled_triangle (rasterizer.c:1000-1085): the blended
-     * branch writes colour only, never depth or id */
+    /* rasterize_filled_triangle: the blended branch writes colour only,
+     * never depth or id */
     if (depth_test && !(zf < L.depth[li]))
       return;
-    float cr, cg, cb;
-    float4 A0[3], A1[3], A2[3];
-    load_tri_attr(frame, st, A0, A1, A2);
-    flat_color(frame, vw, st, d, sm, A0, A1, A2, cr, cg, cb);
     float ca = clamp01f(d.opacity);
     if (sm == MOPH_SM_UNLIT_FLAT)
       ca = (float)cvt_u8(ca * 255.0f) / 255.0f;
     if (blend == MOP_HIP_BLEND_OPAQUE && ca >= 1.0f) {
-      L.hdr[li] = make_float4(cr, cg, cb, 1.0f);
       L.depth[li] = zf;
       L.ids[li] = d.object_id;
-    } else {
-      float4 dc = L.hdr[li];
-      float inv_a = 1.0f - ca;
-      if (blend == MOP_HIP_BLEND_ADDITIVE)
-        L.hdr[li] = make_float4(dc.x + cr * ca, dc.y + cg * ca, dc.z + cb * ca, 1.0f);
-      else if (blend == MOP_HIP_BLEND_MULTIPLY)
-        L.hdr[li] = make_float4(dc.x * cr, dc.y * cg, dc.z * cb, 1.0f);
-      else
-        L.hdr[li] = make_float4(cr * ca + dc.x * inv_a, cg * ca + dc.y * inv_a,
-                                cb * ca + dc.z * inv_a, 1.0f);
     }
+    f.q = true;
+    f.kind = 1;
+    f.fa = ca;
     return;
   }
   float d0 = w0 * r.inv_e[0], d1 = w1 * r.inv_e[1], d2 = w2 * r.inv_e[2];
@@ -2807,27 +2823,76 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
   float zt = b0 * r.sz[0] + b1 * r.sz[1] + b2 * r.sz[2];
   if (depth_test && !(zt < L.depth[li]))
     return;
-  float z, pr, pg, pb, cov;
+  float cov;
   float op = d.opacity < 0.0f ? 0.0f : (d.opacity > 1.0f ? 1.0f : d.opacity);
   if (sm == MOPH_SM_UNLIT_SMOOTH) {
     /* silhouette AA against the depth already in the buffer
      * (rasterizer.c:1421-1426) */
-    z = zt;
+    float pr, pg, pb;
     unlit_smooth_color(frame, st, r, d, w0, w1, w2, pr, pg, pb);
-    bool silhouette = (L.depth[li] - z) > 0.005f;
+    bool silhouette = (L.depth[li] - zt) > 0.005f;
     cov = (min_d < 1.0f && silhouette) ? min_d : 1.0f;
+    f.kind = 2;
+    f.v = make_float4(pr, pg, pb, 0.0f);
   } else {
-    float4 A0[3], A1[3], A2[3];
-    load_tri_attr(frame, st, A0, A1, A2);
-    shade_fragment(frame, vw, st, r, A0, A1, A2, w0, w1, w2, z, pr, pg, pb);
     cov = (min_d < 1.0f && blend != MOP_HIP_BLEND_OPAQUE) ? min_d : 1.0f;
+    f.kind = 0;
+    f.v = make_float4(w0, w1, w2, 0.0f);
   }
   float fa_ = op * cov;
-  if (blend == MOP_HIP_BLEND_OPAQUE && fa_ >= 1.0f) {
-    L.hdr[li] = make_float4(pr, pg, pb, 1.0f);
-    L.depth[li] = z;
+  /* the shaded z is the barycentric zt (shade_fragment's same expression) */
+  if ((blend == MOP_HIP_BLEND_OPAQUE && fa_ >= 1.0f) || fa_ > 0.5f) {
+    L.depth[li] = zt;
     L.ids[li] = d.object_id;
-  } else {
+  }
+  f.q = true;
+  f.fa = fa_;
+}
+
+/* The colours of this wave's pending fragments, one lane per fragment, then
+ * their blends into the tile in the order the fragments were met (each lane
+ * applies those of its own pixel): an opaque one with alpha 1 replaces the
+ * colour, the others blend (rasterizer.c:2101-2131, 1054-1083).  Called by
+ * all lanes of the wave. */
+DEV void ordered_flush(const FrameArgs &frame, const ViewGPU &vw, int li, OrderedLds &L) {
+  uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint32_t n = L.q_n[w];
+  if (n == 0)
+    return;
+  if (lane < n) {
+    uint32_t kind = (L.q_meta[w][lane] >> 8) & 15u;
+    if (kind != 2u) {
+      SubTri st = frame.subtris[L.q_s[w][lane]];
+      const DrawGPU &d = frame.draws[st.draw];
+      float4 A0[3], A1[3], A2[3];
+      load_tri_attr(frame, st, A0, A1, A2);
+      float pr, pg, pb;
+      if (kind == 0u) {
+        RasterTri r;
+        load_raster_tri(vw.w, vw.h, st, r);
+        float4 v = L.q_v[w][lane];
+        float z;
+        shade_fragment(frame, vw, st, r, A0, A1, A2, v.x, v.y, v.z, z, pr, pg, pb);
+      } else {
+        flat_color(frame, vw, st, d, draw_sm(d), A0, A1, A2, pr, pg, pb);
+      }
+      L.q_v[w][lane] = make_float4(pr, pg, pb, 0.0f);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t k = 0; k < n; k++) {
+    uint32_t meta = L.q_meta[w][k];
+    if ((meta & 63u) != lane)
+      continue;
+    uint32_t blend = meta >> 12;
+    float fa_ = L.q_fa[w][k];
+    float4 c = L.q_v[w][k];
+    float pr = c.x, pg = c.y, pb = c.z;
+    if (blend == MOP_HIP_BLEND_OPAQUE && fa_ >= 1.0f) {
+      L.hdr[li] = make_float4(pr, pg, pb, 1.0f);
+      continue;
+    }
     float inv_fa = 1.0f - fa_;
     float4 dc = L.hdr[li];
     if (blend == MOP_HIP_BLEND_ADDITIVE) {
@@ -2844,11 +2909,34 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
       pb = pb * fa_ + dc.z * inv_fa;
     }
     L.hdr[li] = make_float4(pr, pg, pb, 1.0f);
-    if (fa_ > 0.5f) {
-      L.depth[li] = z;
-      L.ids[li] = d.object_id;
-    }
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  L.q_n[w] = 0u;
+}
+
+/* Queue the fragment f of screen triangle s met by this lane (all lanes of
+ * the wave call this; a full queue is flushed first). */
+DEV void ordered_enqueue(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int li,
+                         OrderedLds &L, const OrdFrag &f) {
+  unsigned long long m = __ballot(f.q);
+  if (m == 0ull)
+    return;
+  uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint32_t cnt = (uint32_t)__popcll(m);
+  if (L.q_n[w] + cnt > OQ)
+    ordered_flush(frame, vw, li, L);
+  uint32_t n = L.q_n[w];
+  if (f.q) {
+    uint32_t pos = n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    L.q_s[w][pos] = s;
+    L.q_meta[w][pos] = lane | (f.kind << 8) | (f.blend << 12);
+    L.q_fa[w][pos] = f.fa;
+    L.q_v[w][pos] = f.v;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  L.q_n[w] = n + cnt;
 }
 
 /* aa_plot (rasterizer.c:688-730): coverage blend of the line colour, depth
@@ -2982,6 +3070,7 @@ DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int
   }
   SubTri st = frame.subtris[s];
   if (frame.draws[draw].flags & MOPH_DRAW_WIRE) {
+    ordered_flush(frame, vw, li, L); /* the line blends after the pending colours */
     int bx0, by0, bx1, by1;
     bool ok = wire_bbox(st, vw.w, vw.h, bx0, by0, bx1, by1);
     bool in_bbox = ok && inside && px >= bx0 && px <= bx1 && py >= by0 && py <= by1;
@@ -2998,8 +3087,14 @@ DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int
   /* coverage mask of the wave: skip the triangle when no lane is in its bbox */
   if (__ballot(in_bbox) == 0ull)
     return;
+  OrdFrag f;
+  f.q = false;
+  f.kind = f.blend = 0u;
+  f.fa = 0.0f;
+  f.v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (in_bbox)
-    ordered_fragment(frame, vw, st, r, px, py, li, L);
+    ordered_fragment(frame, vw, st, r, px, py, li, L, f);
+  ordered_enqueue(frame, vw, s, li, L, f);
 }
 
 /* Tags each sorted list entry with the waves of the tile it concerns: wave w
@@ -3082,6 +3177,8 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
     L.depth[li] = vw.depth[idx];
     L.ids[li] = vw.ids[idx];
   }
+  if ((threadIdx.x & 63u) == 0)
+    L.q_n[threadIdx.x >> 6] = 0u;
   OrderedState os = {0xffffffffu, false};
   if (n <= OT_CAP) {
     for (uint32_t i = threadIdx.x; i < n; i += WG)
@@ -3154,6 +3251,7 @@ __global__ __launch_bounds__(WG) void k_tile_ordered(FrameArgs frame, const uint
       lo = hi;
     }
   }
+  ordered_flush(frame, vw, li, L);
   if (os.saved_live && inside)
     L.depth[li] = L.saved[li];
   if (inside) {
