@@ -2628,7 +2628,7 @@ void launch_copy_u8(size_t n, const uint8_t *src, uint8_t *dst, hipStream_t s) {
 /* --------------------------------------------------------------------- */
 
 #define OT 16
-#define OT_CAP 4096
+#define OT_CAP 2048
 
 /* Pixels a wireframe triangle's three lines may plot: Wu's algorithm plots
  * at round(x) and floor(y), floor(y)+1 of points on the edges (the span's
@@ -2747,7 +2747,16 @@ DEV void lds_sort(uint32_t *k, int n) {
 #define OQ 64
 #define OQ_WAVES (OT * OT / 64)
 
+/* A list entry set up once for the wave: the screen triangle's raster
+ * setup, its index and draw.  kind 0: nothing to raster (empty bbox or zero
+ * area), 1: a triangle, 2: a wireframe primitive. */
+struct OrdRec {
+  RasterTri r;
+  uint32_t s, draw, kind;
+};
+
 struct OrderedLds {
+  OrdRec rec[OQ_WAVES][64]; /* the entries of the wave's current list chunk */
   float4 hdr[OT * OT];
   float depth[OT * OT];
   float saved[OT * OT];
@@ -2778,10 +2787,10 @@ struct OrdFrag {
  * it.  None of these depends on a fragment's colour, so the colour and its
  * blend are deferred (f): ordered_flush computes the colours of a wave's
  * pending fragments with one lane each and applies the blends in order. */
-DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTri &st,
+DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, uint32_t draw,
                           const RasterTri &r, int px, int py, int li, OrderedLds &L,
                           OrdFrag &f) {
-  const DrawGPU &d = frame.draws[st.draw];
+  const DrawGPU &d = frame.draws[draw];
   int sm = draw_sm(d);
   bool flat = sm_flat(sm);
   FlatZ fz = {0.0f, 0.0f, 0.0f};
@@ -2829,7 +2838,7 @@ DEV void ordered_fragment(const FrameArgs &frame, const ViewGPU &vw, const SubTr
     /* silhouette AA against the depth already in the buffer
      * (rasterizer.c:1421-1426) */
     float pr, pg, pb;
-    unlit_smooth_color(frame, st, r, d, w0, w1, w2, pr, pg, pb);
+    unlit_smooth_color(frame, frame.subtris[s], r, d, w0, w1, w2, pr, pg, pb);
     bool silhouette = (L.depth[li] - zt) > 0.005f;
     cov = (min_d < 1.0f && silhouette) ? min_d : 1.0f;
     f.kind = 2;
@@ -3053,24 +3062,31 @@ struct OrderedState {
   bool saved_live;
 };
 
-DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int px, int py,
-                      bool inside, int li, OrderedLds &L, OrderedState &os) {
-  uint32_t draw = frame.subtris[s].draw;
-  if (draw != os.cur_draw) {
-    /* end of a draw with depth_write = false: restore its depth */
-    if (os.saved_live && inside)
-      L.depth[li] = L.saved[li];
-    os.saved_live = false;
-    os.cur_draw = draw;
-    if (!(frame.draws[draw].flags & MOPH_DRAW_DEPTH_WRITE)) {
-      if (inside)
-        L.saved[li] = L.depth[li];
-      os.saved_live = true;
-    }
+/* The CPU loop's per-draw state at a change of draw: the end of a draw with
+ * depth_write = false restores the depth it saved at its start
+ * (cpu_backend.c:539-545,622-628). */
+DEV void ordered_draw(const FrameArgs &frame, uint32_t draw, bool inside, int li, OrderedLds &L,
+                      OrderedState &os) {
+  if (draw == os.cur_draw)
+    return;
+  if (os.saved_live && inside)
+    L.depth[li] = L.saved[li];
+  os.saved_live = false;
+  os.cur_draw = draw;
+  if (!(frame.draws[draw].flags & MOPH_DRAW_DEPTH_WRITE)) {
+    if (inside)
+      L.saved[li] = L.depth[li];
+    os.saved_live = true;
   }
-  SubTri st = frame.subtris[s];
-  if (frame.draws[draw].flags & MOPH_DRAW_WIRE) {
+}
+
+/* One list entry (set up in R) against the wave's pixels, in order. */
+DEV void ordered_rec(const FrameArgs &frame, const ViewGPU &vw, const OrdRec &R, int px, int py,
+                     bool inside, int li, OrderedLds &L, OrderedState &os) {
+  ordered_draw(frame, R.draw, inside, li, L, os);
+  if (R.kind == 2u) {
     ordered_flush(frame, vw, li, L); /* the line blends after the pending colours */
+    SubTri st = frame.subtris[R.s];
     int bx0, by0, bx1, by1;
     bool ok = wire_bbox(st, vw.w, vw.h, bx0, by0, bx1, by1);
     bool in_bbox = ok && inside && px >= bx0 && px <= bx1 && py >= by0 && py <= by1;
@@ -3080,10 +3096,8 @@ DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int
       wire_fragment(frame, vw, st, px, py, li, L);
     return;
   }
-  RasterTri r;
-  bool ok = load_raster_tri(vw.w, vw.h, st, r);
-  bool in_bbox = ok && inside && px >= r.es.min_x && px <= r.es.max_x && py >= r.es.min_y &&
-                 py <= r.es.max_y;
+  bool in_bbox = R.kind == 1u && inside && px >= R.r.es.min_x && px <= R.r.es.max_x &&
+                 py >= R.r.es.min_y && py <= R.r.es.max_y;
   /* coverage mask of the wave: skip the triangle when no lane is in its bbox */
   if (__ballot(in_bbox) == 0ull)
     return;
@@ -3093,8 +3107,27 @@ DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int
   f.fa = 0.0f;
   f.v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   if (in_bbox)
-    ordered_fragment(frame, vw, st, r, px, py, li, L, f);
-  ordered_enqueue(frame, vw, s, li, L, f);
+    ordered_fragment(frame, vw, R.s, R.draw, R.r, px, py, li, L, f);
+  ordered_enqueue(frame, vw, R.s, li, L, f);
+}
+
+/* Entry s set up by this lane. */
+DEV void ordered_setup(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, OrdRec &R) {
+  SubTri st = frame.subtris[s];
+  R.s = s;
+  R.draw = st.draw;
+  if (frame.draws[st.draw].flags & MOPH_DRAW_WIRE)
+    R.kind = 2u;
+  else
+    R.kind = load_raster_tri(vw.w, vw.h, st, R.r) ? 1u : 0u;
+}
+
+/* Entry s of an untagged list: set up by every lane of the wave. */
+DEV void ordered_step(const FrameArgs &frame, const ViewGPU &vw, uint32_t s, int px, int py,
+                      bool inside, int li, OrderedLds &L, OrderedState &os) {
+  OrdRec R;
+  ordered_setup(frame, vw, s, R);
+  ordered_rec(frame, vw, R, px, py, inside, li, L, os);
 }
 
 /* Tags each sorted list entry with the waves of the tile it concerns: wave w
@@ -3127,11 +3160,28 @@ DEV void tag_waves(const FrameArgs &frame, const ViewGPU &vw, uint32_t *list, in
 /* The entries of a tagged list this wave walks, in order. */
 DEV void walk_tagged(const FrameArgs &frame, const ViewGPU &vw, const uint32_t *list, int m,
                      int px, int py, bool inside, int li, OrderedLds &L, OrderedState &os) {
-  uint32_t wbit = 1u << (28 + (threadIdx.x >> 6));
-  for (int e = 0; e < m; e++) {
-    uint32_t v = list[e];
-    if (v & wbit)
-      ordered_step(frame, vw, v & 0x0fffffffu, px, py, inside, li, L, os);
+  uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+  uint32_t wbit = 1u << (28 + w);
+  /* 64 list positions at a time: the entries tagged for this wave are set
+   * up once each, one per lane, into the wave's LDS records, then walked in
+   * order from there */
+  for (int base = 0; base < m; base += 64) {
+    int e = base + (int)lane;
+    uint32_t v = e < m ? list[e] : 0u;
+    bool mine = (v & wbit) != 0u;
+    unsigned long long mm = __ballot(mine);
+    if (mm == 0ull)
+      continue;
+    if (mine)
+      ordered_setup(frame, vw, v & 0x0fffffffu,
+                    L.rec[w][__popcll(mm & ((1ull << lane) - 1ull))]);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    int n = __popcll(mm);
+    for (int k = 0; k < n; k++)
+      ordered_rec(frame, vw, L.rec[w][k], px, py, inside, li, L, os);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
