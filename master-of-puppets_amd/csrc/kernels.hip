@@ -2670,6 +2670,39 @@ DEV bool prim_bbox(const FrameArgs &fa, const SubTri &st, int W, int H, int &x0,
   return true;
 }
 
+/* Adds one to counts[tile] for every 16x16 tile of the bbox [tx0..tx0+ntx)
+ * x [ty0..) (nt tiles in all) of each active lane, with one atomic per tile
+ * and wave: the lanes on the same tile take consecutive slots after the
+ * leader's atomic (neighbouring screen triangles share tiles; same-address
+ * atomics from every lane would queue at the memory side).  With `at`, the
+ * slot of each (lane, tile) is passed to at(tile, slot) in the order the
+ * tiles are visited. */
+template <class At>
+DEV void wave_tile_add(uint32_t *counts, uint32_t tile_base, int tiles_x, int tx0, int ty0, int ntx,
+                       int nt, At at) {
+  uint32_t lane = __lane_id();
+  unsigned long long below = (1ull << lane) - 1ull;
+  for (int k = 0;;) {
+    bool pend = k < nt;
+    unsigned long long pm = __ballot(pend);
+    if (!pm)
+      break;
+    int leader = __ffsll((long long)pm) - 1;
+    uint32_t mine = pend ? tile_base + (uint32_t)((ty0 + k / ntx) * tiles_x + tx0 + k % ntx) : ~0u;
+    uint32_t tile = __shfl(mine, leader);
+    bool match = pend && mine == tile;
+    unsigned long long mm = __ballot(match);
+    uint32_t base = 0;
+    if ((int)lane == leader)
+      base = atomicAdd(&counts[tile], (uint32_t)__popcll(mm));
+    base = __shfl(base, leader);
+    if (match) {
+      at(tile, base + (uint32_t)__popcll(mm & below));
+      k++;
+    }
+  }
+}
+
 __global__ __launch_bounds__(WG) void k_bin_count(FrameArgs fa, uint32_t *tile_counts) {
   uint32_t total = fa.counters[CNT_SUBTRI];
   if (total > fa.subtri_cap)
@@ -2680,11 +2713,9 @@ __global__ __launch_bounds__(WG) void k_bin_count(FrameArgs fa, uint32_t *tile_c
     int bx0, by0, bx1, by1;
     if (!prim_bbox(fa, st, vw.w, vw.h, bx0, by0, bx1, by1))
       continue;
-    int tx0 = bx0 / OT, ty0 = by0 / OT, tx1 = bx1 / OT, ty1 = by1 / OT;
-    int tiles_x = (vw.w + OT - 1) / OT;
-    for (int ty = ty0; ty <= ty1; ty++)
-      for (int tx = tx0; tx <= tx1; tx++)
-        atomicAdd(&tile_counts[vw.tile_base + (uint32_t)(ty * tiles_x + tx)], 1u);
+    int tx0 = bx0 / OT, ty0 = by0 / OT, ntx = bx1 / OT - tx0 + 1;
+    wave_tile_add(tile_counts, vw.tile_base, (vw.w + OT - 1) / OT, tx0, ty0, ntx,
+                  ntx * (by1 / OT - ty0 + 1), [](uint32_t, uint32_t) {});
   }
 }
 
@@ -2702,17 +2733,15 @@ __global__ __launch_bounds__(WG) void k_bin_fill(FrameArgs fa, const uint32_t *t
     int bx0, by0, bx1, by1;
     if (!prim_bbox(fa, st, vw.w, vw.h, bx0, by0, bx1, by1))
       continue;
-    int tx0 = bx0 / OT, ty0 = by0 / OT, tx1 = bx1 / OT, ty1 = by1 / OT;
-    int tiles_x = (vw.w + OT - 1) / OT;
-    for (int ty = ty0; ty <= ty1; ty++)
-      for (int tx = tx0; tx <= tx1; tx++) {
-        uint32_t t = vw.tile_base + (uint32_t)(ty * tiles_x + tx);
-        uint32_t pos = tile_start[t] + atomicAdd(&tile_cursor[t], 1u);
-        if (pos < bin_cap)
-          bins[pos] = s;
-        else
-          atomicOr(&fa.counters[CNT_OVERFLOW], 16u);
-      }
+    int tx0 = bx0 / OT, ty0 = by0 / OT, ntx = bx1 / OT - tx0 + 1;
+    wave_tile_add(tile_cursor, vw.tile_base, (vw.w + OT - 1) / OT, tx0, ty0, ntx,
+                  ntx * (by1 / OT - ty0 + 1), [&](uint32_t t, uint32_t slot) {
+                    uint32_t pos = tile_start[t] + slot;
+                    if (pos < bin_cap)
+                      bins[pos] = s;
+                    else
+                      atomicOr(&fa.counters[CNT_OVERFLOW], 16u);
+                  });
   }
 }
 
