@@ -895,6 +895,16 @@ DEV bool fused_setup(const FrameArgs &fa, const ViewGPU &vw, const SubTri &st, u
     fa.big_queue[q] = e;
     fa.big_chk[q] = off;
     fa.big_off[okey] = off;
+    if (off != ~0u) {
+      /* its chunks of 64 bbox rows for k_big_rows */
+      uint32_t nch = (uint32_t)(r.es.max_y - r.es.min_y + 64) >> 6;
+      uint32_t cb = atomicAdd(&fa.counters[CNT_BIGCHUNK], nch);
+      if (cb > fa.chk_cap || nch > fa.chk_cap - cb)
+        atomicOr(&fa.counters[CNT_OVERFLOW], 32u);
+      else
+        for (uint32_t j = 0; j < nch; j++)
+          fa.big_map[cb + j] = make_uint2(q, j);
+    }
     return true;
   }
   if (edges_nondegenerate(r.es)) {
@@ -1058,13 +1068,98 @@ void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
   /* the per-workgroup counts are summed by k_finish */
 }
 
-/* raster_grid sets the number of waves working through the large-triangle
- * queue (0 = 8192) */
-/* Order-free path: one pass (visibility and checkpoints); the shading pass
+/* Order-free path, big screen triangles: one wave per chunk of 64 bbox rows,
+ * so the chunks of a tall triangle are walked side by side rather than one
+ * after another by a single wave.  Lane l takes row l of the chunk: its row
+ * start is the serial chain w_row += e_dy (rasterizer.c:2148-2150) replayed
+ * from the bbox top, then the row is walked as in k_raster_big: checkpoints
+ * every 16 columns, tests only over the columns whose edge values can reach
+ * 0 (row_span), a visibility atomic per covered pixel.  The shading pass
  * finds a big triangle's edge values from the checkpoints. */
+__global__ __launch_bounds__(64) void k_big_rows(FrameArgs fa) {
+  /* an overflowed checkpoint store or chunk map can leave entries of earlier
+   * batches in the map: the frame is run again, and nothing is read here */
+  if (fa.counters[CNT_OVERFLOW] & 32u)
+    return;
+  uint32_t nch = fa.counters[CNT_BIGCHUNK];
+  if (nch > fa.chk_cap)
+    nch = fa.chk_cap;
+  uint32_t nbig = fa.counters[CNT_BIG];
+  for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
+    uint2 m = fa.big_map[c];
+    if (m.x >= nbig) /* uniform */
+      continue;
+    uint32_t s = fa.big_queue[m.x], off = fa.big_chk[m.x];
+    SubTri st;
+    load_subtri(fa, s, st);
+    const DrawGPU &dr = fa.draws[st.draw];
+    const ViewGPU &vw = fa.views[dr.view];
+    RasterTri r;
+    if (off == ~0u || !load_raster_tri_fast(vw.w, vw.h, st, r)) /* uniform */
+      continue;
+    bool flat = sm_flat(draw_sm(dr)); /* uniform */
+    FlatZ fz = {0.0f, 0.0f, 0.0f};
+    if (flat)
+      fz = flat_z_setup(r);
+    int row = (int)m.y * 64 + (int)threadIdx.x;
+    if (row > r.es.max_y - r.es.min_y)
+      continue;
+    /* registers, not the record (vis_update) */
+    unsigned long long *const vis = vw.vis;
+    const size_t pitch = (size_t)vw.w;
+    const unsigned long long seq = (unsigned long long)s + 1ull;
+    float w0 = r.es.w_row[0], w1 = r.es.w_row[1], w2 = r.es.w_row[2], zf = fz.row;
+    for (int j = 0; j < row; j++) {
+      w0 += r.es.e_dy[0];
+      w1 += r.es.e_dy[1];
+      w2 += r.es.e_dy[2];
+      if (flat)
+        zf += fz.dy;
+    }
+    size_t rowbase = (size_t)(r.es.min_y + row) * pitch;
+    float4 *crow = fa.chk + off + (size_t)row * chk_per_row(r.es);
+    int xa, xb;
+    {
+      float wr[3] = {w0, w1, w2};
+      row_span(r, wr, xa, xb);
+    }
+    if (xa > xb)
+      xb = r.es.min_x - 1; /* no pixel of this row passes */
+    int x = r.es.min_x;
+    for (; x < xa && x <= xb; x++) { /* no pixel of these passes the edge test */
+      if (((x - r.es.min_x) & (CHK_SPAN - 1)) == 0)
+        crow[(x - r.es.min_x) >> CHK_SHIFT] = make_float4(w0, w1, w2, zf);
+      w0 += r.es.e_dx[0];
+      w1 += r.es.e_dx[1];
+      w2 += r.es.e_dx[2];
+      zf += fz.dx;
+    }
+    for (; x <= xb; x++) {
+      if (((x - r.es.min_x) & (CHK_SPAN - 1)) == 0)
+        crow[(x - r.es.min_x) >> CHK_SHIFT] = make_float4(w0, w1, w2, zf);
+      float z;
+      bool hit;
+      if (flat) {
+        z = zf;
+        hit = w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f && z == z;
+      } else {
+        hit = frag_depth(r, w0, w1, w2, z);
+      }
+      if (hit)
+        vis_update(vis, rowbase + (size_t)x, ((unsigned long long)depth_key(z) << 32) | seq);
+      w0 += r.es.e_dx[0];
+      w1 += r.es.e_dx[1];
+      w2 += r.es.e_dx[2];
+      zf += fz.dx;
+    }
+  }
+}
+
+/* raster_grid sets the number of waves working through the chunks (0 =
+ * 8192). */
 void launch_raster_big(const FrameArgs &fa, hipStream_t s) {
   uint32_t g = fa.raster_grid ? fa.raster_grid : 8192;
-  launch(k_raster_big<0>, dim3(g), dim3(BIG_WG), 0, s, fa);
+  launch(k_big_rows, dim3(g), dim3(64), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */

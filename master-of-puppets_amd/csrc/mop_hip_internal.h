@@ -162,7 +162,8 @@ enum {
   CNT_CHK = 5,    /* order-free path: row checkpoint entries of big triangles */
   CNT_BINS = 6,   /* ordered path: tile-bin entries the batch needed */
   CNT_TICKET = 7, /* k_finish: workgroups done */
-  CNT_N = 8
+  CNT_BIGCHUNK = 8, /* order-free path: 64-row chunks of big screen triangles */
+  CNT_N = 9
 };
 
 /* Overflow bits that make the runtime grow a scratch store and run the
@@ -239,6 +240,11 @@ struct FrameArgs {
   uint32_t chk_cap;
   uint32_t *big_chk;
   uint32_t *big_off;
+  /* order-free path: big_map[c] = (big-queue entry, chunk) for every chunk
+   * of 64 bbox rows of a big screen triangle with checkpoints.  A chunk
+   * holds at least one bbox row and every row at least one checkpoint, so
+   * chk_cap entries always suffice. */
+  uint2 *big_map;
   float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */
   float4 *cpos;   /* clip_cap * MOPH_MAX_POLY */
   float4 *cattr;  /* clip_cap * MOPH_MAX_POLY * 3 */

@@ -306,6 +306,7 @@ struct MopHipDevice {
   DevArr<uint32_t> blk_counts;
   DevArr<float4> chk;          /* big-triangle row checkpoints */
   DevArr<uint32_t> big_chk, big_off;
+  DevArr<uint2> big_map; /* row chunks of big screen triangles */
   uint32_t chk_cap_hint = 0;
   DevArr<uint8_t> arena; /* per-batch upload: counters, draws, lights, views */
   DevArr<float> skin_bones, morph_targets, morph_weights;
@@ -519,6 +520,7 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->chk.release();
   d->big_chk.release();
   d->big_off.release();
+  d->big_map.release();
   d->arena.release();
   d->skin_bones.release();
   d->morph_targets.release();
@@ -1590,7 +1592,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
         !d->blk_counts.ensure(wg_tri.size() + 1 + CLIP_RASTER_GRID) ||
         !d->chk.ensure(chk_cap) || !d->big_chk.ensure(subtri_cap) ||
-        !d->big_off.ensure((size_t)n_tris + (size_t)clip_cap * 16 + 1))
+        !d->big_off.ensure((size_t)n_tris + (size_t)clip_cap * 16 + 1) ||
+        !d->big_map.ensure(chk_cap))
       return false;
     fa.vscr = d->vscr.p;
     fa.tri_rec = d->tri_rec.p;
@@ -1599,6 +1602,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     fa.chk_cap = chk_cap;
     fa.big_chk = d->big_chk.p;
     fa.big_off = d->big_off.p;
+    fa.big_map = d->big_map.p;
   }
   fa.vpos = d->vpos.p;
   fa.vattr = d->vattr.p;
