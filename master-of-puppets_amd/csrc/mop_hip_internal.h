@@ -361,7 +361,7 @@ void launch_skin(const SkinArgs &a, hipStream_t s);
 void launch_morph(const MorphArgs &a, hipStream_t s);
 
 /* workgroups of the order-free clipped-fan raster (k_clip_raster) */
-#define CLIP_RASTER_GRID 256
+#define CLIP_RASTER_GRID 1024
 void launch_fused_setup(const FrameArgs &fa, hipStream_t s);
 void launch_raster_big(const FrameArgs &fa, hipStream_t s);
 void launch_scan_counts(const FrameArgs &fa, uint32_t *block_sums, hipStream_t s);
