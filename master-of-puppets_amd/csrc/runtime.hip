@@ -292,6 +292,7 @@ struct MopHipDevice {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool join_pending = false;
+  bool shadow_deferred = false; /* direct launches: passes launched at the first join */
   StageTimers tm;
   StageSlot ring[NSTAGE];
   int ring_next = 0;
@@ -1973,8 +1974,19 @@ static void stream_dep(MopHipDevice *d, hipStream_t from, hipStream_t to, hipEve
   dep();
 }
 
-/* Before a launch that reads shadow maps: wait for the side stream's passes. */
+/* Before a launch that reads shadow maps: wait for the side stream's passes
+ * (launching them first when they were deferred). */
 static void join_shadows(MopHipDevice *d) {
+  if (d->shadow_deferred) {
+    HIPCHK(hipStreamWaitEvent(d->side, d->ev_fork, 0));
+    hipStream_t main = d->stream;
+    d->stream = d->side;
+    for (const ShadowJob &j : d->shadow_jobs)
+      run_shadow_job(d, j.map, j.casters);
+    d->stream = main;
+    d->shadow_deferred = false;
+    d->join_pending = true;
+  }
   if (!d->join_pending)
     return;
   stream_dep(d, d->side, d->stream, d->ev_join);
@@ -1989,13 +2001,22 @@ static void frame_body(MopHipDevice *d, const std::vector<MopHipFramebuffer *> &
     /* the shadow passes overlap the vertex, triangle and raster kernels of
      * the views: they run on the side stream (every launch, upload and
      * staging event of run_shadow_job goes to d->stream, swapped here) */
-    stream_dep(d, d->stream, d->side, d->ev_fork);
-    hipStream_t main = d->stream;
-    d->stream = d->side;
-    for (const ShadowJob &j : d->shadow_jobs)
-      run_shadow_job(d, j.map, j.casters);
-    d->stream = main;
-    d->join_pending = true;
+    if (t_plan) {
+      stream_dep(d, d->stream, d->side, d->ev_fork);
+      hipStream_t main = d->stream;
+      d->stream = d->side;
+      for (const ShadowJob &j : d->shadow_jobs)
+        run_shadow_job(d, j.map, j.casters);
+      d->stream = main;
+      d->join_pending = true;
+    } else {
+      /* launched directly, the side stream forks here but its passes are
+       * launched at the first join, after the views' geometry kernels: the
+       * main stream's work then starts without waiting for the host to
+       * launch the passes */
+      HIPCHK(hipEventRecord(d->ev_fork, d->stream));
+      d->shadow_deferred = true;
+    }
   }
   std::vector<std::vector<std::pair<size_t, size_t>>> segs(todo.size());
   size_t rounds = 0;
