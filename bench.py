@@ -414,7 +414,8 @@ def config3_form(frames):
            "mtri_per_s_hbm_resident": round(tris / hbm_s / 1e6, 2),
            "device_ms_one_frame": round(dev_ms, 4),
            "clipped": st["clipped"], "screen_triangles": st["subtris"],
-           "big_screen_triangles": st["big_subtris"], "overflow_flags": 0}
+           "big_screen_triangles": st["big_subtris"],
+           "culled_chunks_of_256_triangles": st["culled_chunks"], "overflow_flags": 0}
     log(f"config 3: {out}")
     for fb in fbs:
         fb.close()
@@ -829,7 +830,7 @@ def main():
     ap.add_argument("--transparent-only", action="store_true",
                     help="only the HBM-resident frames of the transparent 1080p scene "
                          "(for profiling)")
-    ap.add_argument("--config3-frames", type=int, default=10)
+    ap.add_argument("--config3-frames", type=int, default=40)
     ap.add_argument("--batch-only", action="store_true",
                     help="run only the 64-viewport batched form (for profiling)")
     ap.add_argument("--no-tune", action="store_true",

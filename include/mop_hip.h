@@ -427,6 +427,8 @@ typedef struct MopHipFrameStats {
   uint32_t big_subtris;      /* screen triangles walked row-parallel */
   uint32_t shadow_tris;      /* shadow-caster triangles */
   float gpu_ms;              /* device time of the frame (hipEvents) */
+  uint32_t culled_chunks;    /* chunks of 256 triangles of large draws skipped
+                                whole: every triangle trivially rejected */
 } MopHipFrameStats;
 void mop_hip_get_frame_stats(MopHipDevice *dev, MopHipFrameStats *out);
 

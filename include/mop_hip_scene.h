@@ -29,6 +29,12 @@ int mop_hip_scene_add_mesh(MopHipScene *s, const MopHipVertex *v, uint32_t nv,
                            const uint32_t *idx, uint32_t ni, uint32_t object_id,
                            const float *model16, int has_material, float metallic,
                            float roughness);
+/* Rewrite the nv vertices of mesh `mesh` in place (nv must equal its vertex
+ * count): the in-place path of mop_mesh_update_geometry
+ * (viewport.c:1430-1452) through the backend's buffer_update.  Returns 0 or
+ * -1. */
+int mop_hip_scene_update_mesh_vertices(MopHipScene *s, uint32_t mesh, const MopHipVertex *v,
+                                       uint32_t nv);
 int mop_hip_scene_add_light(MopHipScene *s, const MopHipLight *l);
 /* Equirect RGBA32F environment: IBL tables computed on the device and the
  * auto-exposure of environment.c:806-835. NULL removes it. */

@@ -87,7 +87,8 @@ class LoadedMesh(C.Structure):
 
 class FrameStats(C.Structure):
     _fields_ = [("triangles_in", C.c_uint32), ("subtris", C.c_uint32), ("clipped", C.c_uint32),
-                ("big_subtris", C.c_uint32), ("shadow_tris", C.c_uint32), ("gpu_ms", C.c_float)]
+                ("big_subtris", C.c_uint32), ("shadow_tris", C.c_uint32), ("gpu_ms", C.c_float),
+                ("culled_chunks", C.c_uint32)]
 
 
 # entry points of include/mop_hip.h: name -> (restype, argtypes)
@@ -162,6 +163,7 @@ SIGS = {
     "mop_hip_scene_set_clear_color": (None, [VP, Color]),
     "mop_hip_scene_set_post_effects": (None, [VP, C.c_uint32]),
     "mop_hip_scene_set_mesh_blend": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_float]),
+    "mop_hip_scene_update_mesh_vertices": (C.c_int, [VP, C.c_uint32, VP, C.c_uint32]),
     "mop_hip_scene_render": (C.c_uint32, [VP, VP, C.POINTER(Camera), C.c_int, C.c_int]),
     "mop_hip_scene_triangles": (C.c_uint32, [VP]),
     "mop_hip_scene_record": (C.c_uint32, [VP, VP, C.POINTER(Camera), C.c_int, C.c_int]),

@@ -35,7 +35,7 @@ DEV V3 v3norm(V3 v) {
 }
 
 /* mop_mat4_mul_vec4, math.c:196-203: row r = ((m[r]x + m[4+r]y) + m[8+r]z) + m[12+r]w */
-DEV float4 mat4_mul_vec4(const float *m, float x, float y, float z, float w) {
+HD float4 mat4_mul_vec4(const float *m, float x, float y, float z, float w) {
   float4 r;
   r.x = m[0] * x + m[4] * y + m[8] * z + m[12] * w;
   r.y = m[1] * x + m[5] * y + m[9] * z + m[13] * w;

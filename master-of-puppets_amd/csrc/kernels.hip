@@ -9,6 +9,7 @@
  * ordered per-tile loops, and row-per-lane walks for large triangles.
  */
 #include "devmath.h"
+#include "cull.h"
 #include "libmf.h"
 #include "mop_hip_internal.h"
 #include "shade_key.h"
@@ -124,10 +125,11 @@ DEV void read_f2(const uint8_t *base, int off, int fmt, float o[2]) {
   }
 }
 
-__global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
-  /* draw-major grid: the draw is uniform over the workgroup (scalar loads) */
-  const DrawGPU &d = fa.draws[fa.wg_vtx[blockIdx.x]];
-  uint32_t i = (blockIdx.x - d.vtx_wg) * WG + threadIdx.x;
+/* the 256 vertices of per-vertex workgroup b (draw-major: the draw is
+ * uniform over the workgroup, scalar loads) */
+DEV void vertex_work(const FrameArgs &fa, const DrawGPU *draws, uint32_t b, uint32_t di) {
+  const DrawGPU &d = draws[di];
+  uint32_t i = (b - d.vtx_wg) * WG + threadIdx.x;
   if (i >= d.vertex_count)
     return;
   uint32_t v = d.vtx_base + i;
@@ -197,10 +199,208 @@ __global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
   a[2] = make_float4(c[2], c[3], 0.0f, 0.0f);
 }
 
+__global__ __launch_bounds__(WG) void k_vertex(FrameArgs fa) {
+  vertex_work(fa, fa.draws, blockIdx.x, fa.wg_vtx[blockIdx.x]);
+}
+
+/* Chunk-culled batches: the leading vtx_direct per-vertex workgroups (draws
+ * not culled by chunks) one per workgroup as in k_vertex, then the ones
+ * k_cull_vertices kept, walked by the last vtx_lgrid workgroups (a skipped
+ * workgroup costs no launch).  The tables are kernel arguments declared
+ * unaliased, so the vertex stores of one pass of the walk cannot clobber
+ * them and the draw's fields stay scalar loads; the walk's entries are
+ * (workgroup, draw) pairs, the next one loaded while the current one is
+ * worked on. */
+__global__ __launch_bounds__(WG) void k_vertex_list(FrameArgs fa, const DrawGPU *__restrict__ draws,
+                                                    const uint32_t *__restrict__ wg_vtx,
+                                                    const uint2 *__restrict__ list) {
+  if (blockIdx.x < fa.vtx_direct) {
+    vertex_work(fa, draws, blockIdx.x, wg_vtx[blockIdx.x]);
+    return;
+  }
+  uint32_t n = fa.counters[CNT_VLIST], e = blockIdx.x - fa.vtx_direct;
+  uint32_t stride = gridDim.x - fa.vtx_direct;
+  uint2 cur = e < n ? list[e] : make_uint2(0u, 0u);
+  for (; e < n; e += stride) {
+    uint2 nxt = e + stride < n ? list[e + stride] : make_uint2(0u, 0u);
+    vertex_work(fa, draws, cur.x, cur.y);
+    cur = nxt;
+  }
+}
+
 void launch_vertex(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_wg_vtx)
     return;
-  launch(k_vertex, dim3(fa.n_wg_vtx), dim3(WG), 0, s, fa);
+  if (fa.vtx_list)
+    launch(k_vertex_list, dim3(fa.vtx_direct + fa.vtx_lgrid), dim3(WG), 0, s, fa, fa.draws,
+           fa.wg_vtx, (const uint2 *)fa.vtx_list);
+  else
+    launch(k_vertex, dim3(fa.n_wg_vtx), dim3(WG), 0, s, fa);
+}
+
+/* --------------------------------------------------------------------- */
+/* Chunk culling (order-free path, large draws).  A chunk of 256           */
+/* triangles whose object-space box fails one trivial-reject plane with a */
+/* rounding margin (box_rejected, cull.h) holds only triangles the raster */
+/* rejects (rasterizer.c:2469-2476): they add no coverage, no clip entry  */
+/* and no count, so k_tri skips the chunk, and k_vertex transforms only   */
+/* the vertex workgroups the kept chunks use.                             */
+/* --------------------------------------------------------------------- */
+
+/* Per chunk (one workgroup): the box of the positions k_vertex reads for
+ * the vertices of the chunk's valid triangles (indices below vertex_count),
+ * a flag for a non-finite coordinate (such a chunk is never culled), and
+ * the chunk's entry in the reference list of every per-vertex workgroup it
+ * uses: vref[w] counts the chunks using workgroup w, the first VREF_CAP of
+ * them listed at vref[nvwg + w * VREF_CAP ..].  The distinct workgroups of
+ * a chunk are found with an LDS hash set (at most 768 of 1024 slots). */
+__global__ __launch_bounds__(WG) void k_chunk_bounds(ChunkBoundsArgs a) {
+  __shared__ float red[WG / 64][7];
+  __shared__ uint32_t set[1024];
+  for (uint32_t q = threadIdx.x; q < 1024; q += WG)
+    set[q] = ~0u;
+  __syncthreads();
+  uint32_t k = blockIdx.x * WG + threadIdx.x;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  float bad = 0.0f;
+  if (k < a.tri_count) {
+    for (int j = 0; j < 3; j++) {
+      uint32_t i = a.ibuf[3 * k + j];
+      if (i >= a.vertex_count)
+        continue;
+      const uint8_t *vb = a.vbuf + (size_t)i * a.stride;
+      float p[3];
+      if (!a.flex) {
+        const float *f = (const float *)vb;
+        p[0] = f[0]; p[1] = f[1]; p[2] = f[2];
+      } else {
+        read_f3(vb, a.pos_off, a.pos_fmt, p);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (!(fabsf(p[c]) <= 3.402823466e38f))
+          bad = 1.0f;
+        lo[c] = fminf(lo[c], p[c]);
+        hi[c] = fmaxf(hi[c], p[c]);
+      }
+      uint32_t w = i >> 8;
+      for (uint32_t h = (w * 2654435761u) >> 22;; h = (h + 1) & 1023u) {
+        uint32_t old = atomicCAS(&set[h], ~0u, w);
+        if (old == ~0u) {
+          uint32_t slot = atomicAdd(&a.vref[w], 1u);
+          if (slot < VREF_CAP)
+            a.vref[a.nvwg + (size_t)w * VREF_CAP + slot] = blockIdx.x;
+          break;
+        }
+        if (old == w)
+          break;
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      lo[c] = fminf(lo[c], __shfl_xor(lo[c], o));
+      hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], o));
+    }
+    bad = fmaxf(bad, __shfl_xor(bad, o));
+  }
+  if ((threadIdx.x & 63u) == 0) {
+    float *r = red[threadIdx.x >> 6];
+    r[0] = lo[0]; r[1] = lo[1]; r[2] = lo[2];
+    r[3] = hi[0]; r[4] = hi[1]; r[5] = hi[2];
+    r[6] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < WG / 64; w++) {
+      for (int c = 0; c < 3; c++) {
+        lo[c] = fminf(lo[c], red[w][c]);
+        hi[c] = fmaxf(hi[c], red[w][3 + c]);
+      }
+      bad = fmaxf(bad, red[w][6]);
+    }
+    a.box[2 * blockIdx.x] = make_float4(lo[0], lo[1], lo[2], bad);
+    a.box[2 * blockIdx.x + 1] = make_float4(hi[0], hi[1], hi[2], 0.0f);
+  }
+}
+
+void launch_chunk_bounds(const ChunkBoundsArgs &a, hipStream_t s) {
+  uint32_t nb = (a.tri_count + WG - 1) / WG;
+  if (nb)
+    launch(k_chunk_bounds, dim3(nb), dim3(WG), 0, s, a);
+}
+
+/* list[base + rank] = item for the lanes with keep, base from one atomic
+ * per wave (the order of the list is immaterial to its consumers) */
+DEV void wave_append(uint2 *list, uint32_t *count, bool keep, uint2 item) {
+  unsigned long long m = __ballot(keep);
+  uint32_t base = 0;
+  if ((threadIdx.x & 63u) == 0 && m)
+    base = atomicAdd(count, (uint32_t)__popcll(m));
+  base = __shfl(base, 0);
+  if (keep) {
+    uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    list[base + rank] = item;
+  }
+}
+
+/* One lane per per-triangle workgroup b of the culled draws (b >=
+ * tri_direct): kept unless the chunk's box fails a plane; tri_vis[b] and
+ * the list of kept workgroups k_tri_list walks. */
+__global__ __launch_bounds__(WG) void k_cull_chunks(FrameArgs fa) {
+  uint32_t b = fa.tri_direct + blockIdx.x * WG + threadIdx.x;
+  bool keep = false;
+  uint32_t di = 0;
+  if (b < fa.n_wg_tri) {
+    di = fa.wg_tri[b];
+    const DrawGPU &d = fa.draws[di];
+    keep = true;
+    if (d.chunk_box) {
+      uint32_t c = b - d.tri_wg;
+      float4 lo = d.chunk_box[2 * c], hi = d.chunk_box[2 * c + 1];
+      keep = !(lo.w == 0.0f && box_rejected(d.mvp, d.depth_bias, lo, hi));
+    }
+    fa.tri_vis[b] = keep ? 1u : 0u;
+  }
+  unsigned long long culled = __ballot(b < fa.n_wg_tri && !keep);
+  if ((threadIdx.x & 63u) == 0 && culled)
+    atomicAdd(&fa.counters[CNT_CULLED], (uint32_t)__popcll(culled));
+  wave_append(fa.tri_list, &fa.counters[CNT_TLIST], keep, make_uint2(b, di));
+}
+
+/* One lane per per-vertex workgroup b of the culled draws (b >= vtx_direct):
+ * kept when a kept chunk uses it or more than VREF_CAP chunks do. */
+__global__ __launch_bounds__(WG) void k_cull_vertices(FrameArgs fa) {
+  uint32_t b = fa.vtx_direct + blockIdx.x * WG + threadIdx.x;
+  bool need = false;
+  uint32_t di = 0;
+  if (b < fa.n_wg_vtx) {
+    di = fa.wg_vtx[b];
+    const DrawGPU &d = fa.draws[di];
+    need = true;
+    if (d.chunk_box) {
+      uint32_t w = b - d.vtx_wg, nvwg = (d.vertex_count + WG - 1) / WG;
+      uint32_t n = d.vref[w];
+      if (n <= VREF_CAP) {
+        const uint32_t *l = d.vref + nvwg + (size_t)w * VREF_CAP;
+        need = false;
+        for (uint32_t q = 0; q < n; q++)
+          need = need || fa.tri_vis[d.tri_wg + l[q]] != 0;
+      }
+    }
+  }
+  wave_append(fa.vtx_list, &fa.counters[CNT_VLIST], need, make_uint2(b, di));
+}
+
+void launch_cull(const FrameArgs &fa, hipStream_t s) {
+  if (fa.n_wg_tri > fa.tri_direct)
+    launch(k_cull_chunks, dim3((fa.n_wg_tri - fa.tri_direct + WG - 1) / WG), dim3(WG), 0, s, fa);
+  if (fa.n_wg_vtx > fa.vtx_direct)
+    launch(k_cull_vertices, dim3((fa.n_wg_vtx - fa.vtx_direct + WG - 1) / WG), dim3(WG), 0, s,
+           fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -952,13 +1152,13 @@ DEV void count_subtris(uint32_t *blk_counts, bool emitted) {
  * cull (rasterizer.c:2469-2485), trivial accept or a clip-queue entry
  * (2492-2504), then for an accepted triangle the fan checks (2515-2537) and
  * the visibility walk or a big-triangle queue entry. */
-__global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
+DEV bool tri_work(const FrameArgs &fa, const DrawGPU *draws, const ViewGPU *views, uint32_t b,
+                  uint32_t di) {
   /* draw-major grid: the draw (and so the view) is uniform over the
    * workgroup (scalar loads) */
-  uint32_t di = fa.wg_tri[blockIdx.x];
-  const DrawGPU &d = fa.draws[di];
-  const ViewGPU &vw = fa.views[d.view];
-  uint32_t k = (blockIdx.x - d.tri_wg) * WG + threadIdx.x;
+  const DrawGPU &d = draws[di];
+  const ViewGPU &vw = views[d.view];
+  uint32_t k = (b - d.tri_wg) * WG + threadIdx.x;
   uint32_t t = d.tri_base + k;
   bool emitted = false, walk = false;
   RasterTri r;
@@ -1013,7 +1213,42 @@ __global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
   }
   if (walk)
     walk_small_vis(vw, r, sm_flat(draw_sm(d)), ((unsigned long long)t << 4 | FUSED_WHOLE) + 1ull);
-  count_subtris(fa.blk_counts, emitted);
+  return emitted;
+}
+
+__global__ __launch_bounds__(WG) void k_tri(FrameArgs fa) {
+  count_subtris(fa.blk_counts, tri_work(fa, fa.draws, fa.views, blockIdx.x, fa.wg_tri[blockIdx.x]));
+}
+
+/* Chunk-culled batches: the leading tri_direct per-triangle workgroups
+ * (draws not culled by chunks) one per workgroup as in k_tri, then the ones
+ * k_cull_chunks kept, walked by the remaining workgroups as in
+ * k_vertex_list; workgroup g's count goes to blk_counts[g]. */
+__global__ __launch_bounds__(WG) void k_tri_list(FrameArgs fa, const DrawGPU *__restrict__ draws,
+                                                 const ViewGPU *__restrict__ views,
+                                                 const uint32_t *__restrict__ wg_tri,
+                                                 const uint2 *__restrict__ list) {
+  if (blockIdx.x < fa.tri_direct) {
+    count_subtris(fa.blk_counts, tri_work(fa, draws, views, blockIdx.x, wg_tri[blockIdx.x]));
+    return;
+  }
+  __shared__ uint32_t acc;
+  if (threadIdx.x == 0)
+    acc = 0;
+  __syncthreads();
+  uint32_t n = fa.counters[CNT_TLIST], c = 0, e = blockIdx.x - fa.tri_direct;
+  uint32_t stride = gridDim.x - fa.tri_direct;
+  uint2 cur = e < n ? list[e] : make_uint2(0u, 0u);
+  for (; e < n; e += stride) {
+    uint2 nxt = e + stride < n ? list[e + stride] : make_uint2(0u, 0u);
+    c += tri_work(fa, draws, views, cur.x, cur.y) ? 1u : 0u;
+    cur = nxt;
+  }
+  if (c)
+    atomicAdd(&acc, c);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    fa.blk_counts[blockIdx.x] = acc;
 }
 
 /* Fan triangles of the clipped polygons (after k_clip): the same checks and
@@ -1059,13 +1294,31 @@ __global__ __launch_bounds__(WG) void k_clip_raster(FrameArgs fa, uint32_t *blk_
 void launch_fused_setup(const FrameArgs &fa, hipStream_t s) {
   if (!fa.n_tris)
     return;
-  uint32_t nb = fa.n_wg_tri;
-  launch(k_tri, dim3(nb), dim3(WG), 0, s, fa);
+  if (fa.tri_list)
+    launch(k_tri_list, dim3(fa.tri_grid), dim3(WG), 0, s, fa, fa.draws, fa.views, fa.wg_tri,
+           (const uint2 *)fa.tri_list);
+  else
+    launch(k_tri, dim3(fa.n_wg_tri), dim3(WG), 0, s, fa);
   launch_clip(fa, s);
-  /* blk_counts holds nb + CLIP_RASTER_GRID entries */
+  /* blk_counts holds tri_grid + CLIP_RASTER_GRID entries */
   launch(k_clip_raster, dim3(CLIP_RASTER_GRID), dim3(WG), 0, s, fa,
-                     fa.blk_counts + nb);
+                     fa.blk_counts + fa.tri_grid);
   /* the per-workgroup counts are summed by k_finish */
+}
+
+/* workgroups of k_tri_list and k_vertex_list that are resident at once on
+ * the device: the fixed grids of the list walks (a larger grid would start
+ * a second round late, holding its share of the list) */
+void cull_list_grids(uint32_t *tri_grid, uint32_t *vtx_grid) {
+  hipDeviceProp_t prop;
+  int dev = 0, nt = 0, nv = 0;
+  *tri_grid = *vtx_grid = 1024;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess)
+    return;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nt, k_tri_list, WG, 0) == hipSuccess && nt > 0)
+    *tri_grid = (uint32_t)(nt * prop.multiProcessorCount);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nv, k_vertex_list, WG, 0) == hipSuccess && nv > 0)
+    *vtx_grid = (uint32_t)(nv * prop.multiProcessorCount);
 }
 
 /* Order-free path, big screen triangles: one wave per chunk of 64 bbox rows,
@@ -1904,7 +2157,7 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint
 void launch_export_counters(const FrameArgs &fa, hipStream_t s) {
   if (!fa.nviews)
     return;
-  uint32_t nblk = fa.fused ? fa.n_wg_tri + CLIP_RASTER_GRID : 0u;
+  uint32_t nblk = fa.fused ? fa.tri_grid + CLIP_RASTER_GRID : 0u;
   uint32_t grid = (nblk + 4095) / 4096;
   if (grid < 1)
     grid = 1;

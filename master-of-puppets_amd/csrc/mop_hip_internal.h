@@ -140,6 +140,26 @@ struct DrawGPU {
   float light_dir[3]; /* the call's light_dir (unlit models) */
   uint32_t vtx_wg;    /* first workgroup of this draw in the per-vertex grid */
   uint32_t tri_wg;    /* first workgroup of this draw in the per-triangle grid */
+  /* order-free path, large draws: per chunk of 256 triangles the object-space
+   * box (lo.xyz, non-finite flag) (hi.xyz, 0) of the vertices its valid
+   * triangles use, or null (no chunk culling) */
+  const float4 *chunk_box;
+  /* with chunk_box: per-vertex-workgroup chunk references (k_chunk_bounds) */
+  const uint32_t *vref;
+};
+
+/* chunks listed per per-vertex workgroup; one used by more is always kept */
+#define VREF_CAP 16
+
+/* k_chunk_bounds: the boxes of one (index buffer, vertex buffer) pair */
+struct ChunkBoundsArgs {
+  const uint8_t *vbuf;
+  const uint32_t *ibuf;
+  uint32_t tri_count, vertex_count, stride, flex;
+  int32_t pos_off, pos_fmt;
+  float4 *box;   /* 2 per chunk of 256 triangles */
+  uint32_t *vref; /* nvwg counts (zeroed), then nvwg * VREF_CAP chunk indices */
+  uint32_t nvwg;  /* per-vertex workgroups: (vertex_count + 255) / 256 */
 };
 
 /* Screen-space triangle: exactly the MopSwScreenVertex positions the CPU
@@ -163,7 +183,10 @@ enum {
   CNT_BINS = 6,   /* ordered path: tile-bin entries the batch needed */
   CNT_TICKET = 7, /* k_finish: workgroups done */
   CNT_BIGCHUNK = 8, /* order-free path: 64-row chunks of big screen triangles */
-  CNT_N = 9
+  CNT_TLIST = 9,    /* chunk culling: kept per-triangle workgroups */
+  CNT_VLIST = 10,   /* chunk culling: kept per-vertex workgroups */
+  CNT_CULLED = 11,  /* chunk culling: chunks of 256 triangles skipped */
+  CNT_N = 12
 };
 
 /* Overflow bits that make the runtime grow a scratch store and run the
@@ -245,6 +268,21 @@ struct FrameArgs {
    * holds at least one bbox row and every row at least one checkpoint, so
    * chk_cap entries always suffice. */
   uint2 *big_map;
+  /* order-free path, chunk culling (a draw with chunk_box in the batch):
+   * the workgroups of the culled draws follow the first tri_direct /
+   * vtx_direct ones; tri_vis[b] = 0 when the triangles of per-triangle
+   * workgroup b are all trivially rejected; tri_list / vtx_list the kept
+   * workgroups of the culled draws as (workgroup, draw) (counts in
+   * CNT_TLIST / CNT_VLIST), walked by tri_lgrid / vtx_lgrid workgroups;
+   * null without chunk culling */
+  uint32_t *tri_vis;
+  uint2 *tri_list;
+  uint2 *vtx_list;
+  uint32_t tri_direct, vtx_direct;
+  uint32_t tri_lgrid, vtx_lgrid;
+  /* entries of blk_counts the triangle kernel writes (n_wg_tri, or
+   * tri_direct + tri_lgrid with lists); the clipped-fan raster's follow */
+  uint32_t tri_grid;
   float4 *vattr;  /* n_verts * 3: (n.xyz, wp.x) (wp.yz, c.rg) (c.ba, -, -) */
   float4 *cpos;   /* clip_cap * MOPH_MAX_POLY */
   float4 *cattr;  /* clip_cap * MOPH_MAX_POLY * 3 */
@@ -331,6 +369,9 @@ struct ShadowArgsGPU {
 
 /* Kernel launchers (kernels.hip). */
 void launch_vertex(const FrameArgs &fa, hipStream_t s);
+void launch_chunk_bounds(const ChunkBoundsArgs &a, hipStream_t s);
+void launch_cull(const FrameArgs &fa, hipStream_t s);
+void cull_list_grids(uint32_t *tri_grid, uint32_t *vtx_grid);
 void launch_classify(const FrameArgs &fa, hipStream_t s);
 void launch_clip(const FrameArgs &fa, hipStream_t s);
 /* Skinning: bind-pose records in, deformed records out (same layout). */

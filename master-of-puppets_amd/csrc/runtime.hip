@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <map>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -91,7 +92,13 @@ struct MopHipBuffer {
   void *bind_dev = nullptr;
   const void *bind_src = nullptr;
   size_t bind_size = 0;
+  /* contents version, from one process-wide counter: a new or rewritten
+   * buffer never carries a version an earlier one had */
+  uint64_t version = 0;
 };
+
+static std::atomic<uint64_t> g_buf_version{0};
+static void buf_touched(MopHipBuffer *b) { b->version = ++g_buf_version; }
 
 struct MopHipTexture {
   int w = 0, h = 0;
@@ -103,6 +110,7 @@ struct MopHipTexture {
 
 struct PendingDraw {
   DrawGPU g;              /* filled except vtx/tri bases and light_off */
+  MopHipBuffer *vb = nullptr, *ib = nullptr;
   uint32_t light_at;      /* the draw's lights: fb->light_pool[light_at ..] */
   int cls;                /* 0 order-free, 3 ordered, 2 no-op */
   int state;              /* index into the frame's shading-state list */
@@ -216,6 +224,21 @@ template <class T> struct DevArr {
   }
 };
 
+/* Chunk boxes of one (index buffer, vertex buffer) pair for the draw
+ * parameters they were computed with (k_chunk_bounds); stale once either
+ * buffer's version or a parameter changes. */
+struct ChunkBoxes {
+  uint64_t ib_ver = 0, vb_ver = 0;
+  uint32_t tri_count = 0, vertex_count = 0, stride = 0, flex = 0;
+  int32_t pos_off = 0, pos_fmt = 0;
+  uint64_t batch = 0; /* the batch that last computed them */
+  DevArr<float4> box;
+  DevArr<uint32_t> vref; /* per-vertex-workgroup chunk references */
+};
+
+/* draws with at least this many triangles are chunk-culled */
+#define CULL_MIN_TRIS 65536u
+
 struct MopHipShadowMap {
   int size = 0;
   DevArr<uint32_t> keys; /* ordered depth keys: the pass's atomic min target and
@@ -309,6 +332,14 @@ struct MopHipDevice {
   DevArr<uint32_t> big_chk, big_off;
   DevArr<uint2> big_map; /* row chunks of big screen triangles */
   uint32_t chk_cap_hint = 0;
+  /* chunk culling of large order-free draws (MOP_HIP_CHUNK_CULL=0 turns it
+   * off): boxes per (index buffer, vertex buffer), and the per-frame marks */
+  bool chunk_cull = true;
+  uint32_t cull_tri_grid = 1024, cull_vtx_grid = 1024; /* list-walk grids */
+  std::map<std::pair<const MopHipBuffer *, const MopHipBuffer *>, ChunkBoxes> cboxes;
+  uint64_t batch_clock = 0;
+  DevArr<uint32_t> tri_vis;
+  DevArr<uint2> tri_list, vtx_list;
   DevArr<uint8_t> arena; /* per-batch upload: counters, draws, lights, views */
   DevArr<float> skin_bones, morph_targets, morph_weights;
   DevArr<uint32_t> cnt_poly, tri_counts, clip_queue, clip_slot, counters, block_sums,
@@ -468,6 +499,9 @@ extern "C" MopHipDevice *mop_hip_device_create(void) {
   d->force_clip = g_force_clip;
   d->force_chk = g_force_chk;
   d->force_bins = g_force_bins;
+  if (const char *e = getenv("MOP_HIP_CHUNK_CULL"))
+    d->chunk_cull = atoi(e) != 0;
+  cull_list_grids(&d->cull_tri_grid, &d->cull_vtx_grid);
   /* this device's copy of the allocation-free entry point's scratch */
   if (hipGetSymbolAddress((void **)&d->noalloc_scratch, HIP_SYMBOL(scratch)) != hipSuccess ||
       !d->noalloc_scratch) {
@@ -522,6 +556,14 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   d->big_chk.release();
   d->big_off.release();
   d->big_map.release();
+  for (auto &kv : d->cboxes) {
+    kv.second.box.release();
+    kv.second.vref.release();
+  }
+  d->cboxes.clear();
+  d->tri_vis.release();
+  d->tri_list.release();
+  d->vtx_list.release();
   d->arena.release();
   d->skin_bones.release();
   d->morph_targets.release();
@@ -752,6 +794,7 @@ extern "C" MopHipBuffer *mop_hip_buffer_create(MopHipDevice *d, const MopHipBuff
   if (desc->size)
     HIPCHK(hipMemcpy(b->dev, b->host, desc->size, hipMemcpyHostToDevice));
   b->owner = d;
+  buf_touched(b);
   std::lock_guard<std::mutex> lk(d->mu);
   d->host2buf[b->host] = b;
   return b;
@@ -764,6 +807,15 @@ extern "C" void mop_hip_buffer_destroy(MopHipDevice *d, MopHipBuffer *b) {
     (void)hipStreamSynchronize(d->stream);
     std::lock_guard<std::mutex> lk(d->mu);
     d->host2buf.erase(b->host);
+    for (auto it = d->cboxes.begin(); it != d->cboxes.end();) {
+      if (it->first.first == b || it->first.second == b) {
+        it->second.box.release();
+        it->second.vref.release();
+        it = d->cboxes.erase(it);
+      } else {
+        ++it;
+      }
+    }
   }
   (void)hipFree(b->dev);
   if (b->bind_dev)
@@ -781,6 +833,7 @@ extern "C" void mop_hip_buffer_update(MopHipDevice *d, MopHipBuffer *b, const vo
   memcpy((uint8_t *)b->host + offset, data, size);
   HIPCHK(hipMemcpy((uint8_t *)b->dev + offset, (uint8_t *)b->host + offset, size,
                    hipMemcpyHostToDevice));
+  buf_touched(b);
 }
 
 /* A buffer whose contents the device wrote (a mesh decoded in HBM): the
@@ -796,6 +849,7 @@ MopHipBuffer *buffer_adopt_device(MopHipDevice *d, void *dev, size_t size) {
   b->dev = dev;
   b->owner = d;
   b->host_stale = true;
+  buf_touched(b);
   std::lock_guard<std::mutex> lk(d->mu);
   d->host2buf[b->host] = b;
   return b;
@@ -889,6 +943,7 @@ extern "C" int mop_hip_skin_apply(MopHipDevice *d, MopHipBuffer *vb, const void 
   a.aligned = (stride % 4) == 0;
   launch_skin(a, d->stream);
   vb->host_stale = true;
+  buf_touched(vb);
   return 0;
 }
 
@@ -935,6 +990,7 @@ extern "C" int mop_hip_morph_apply(MopHipDevice *d, MopHipBuffer *vb, const void
   a.aligned = (stride % 4) == 0;
   launch_morph(a, d->stream);
   vb->host_stale = true;
+  buf_touched(vb);
   return 0;
 }
 
@@ -1263,6 +1319,8 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
   memcpy(g.model, model.d, sizeof(g.model));
   g.vbuf = (const uint8_t *)c->vertex_buffer->dev;
   g.ibuf = (const uint32_t *)c->index_buffer->dev;
+  pd.vb = c->vertex_buffer;
+  pd.ib = c->index_buffer;
   if (c->texture && !c->texture->hdr && c->texture->w >= 1 && c->texture->h >= 1) {
     g.tex = (const uint8_t *)c->texture->dev;
     g.tex_w = c->texture->w;
@@ -1480,11 +1538,72 @@ struct SegJob {
  * launches (views are independent: own visibility buffer, own state). */
 static void join_shadows(MopHipDevice *d);
 
+/* The chunk boxes a large draw is culled with (*out), computed by
+ * k_chunk_bounds on the main stream when missing or stale.  *out stays null
+ * when the pair's boxes were computed in this batch for other draw
+ * parameters.  Under a plan a computation aborts the plan instead (the
+ * frame is launched again directly). */
+static bool draw_chunk_boxes(MopHipDevice *d, const PendingDraw &pd, const DrawGPU &g,
+                             uint64_t batch, const float4 **out, const uint32_t **vref) {
+  *out = nullptr;
+  *vref = nullptr;
+  ChunkBoxes *cb;
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    cb = &d->cboxes[std::make_pair((const MopHipBuffer *)pd.ib, (const MopHipBuffer *)pd.vb)];
+  }
+  uint32_t flex = (g.flags & MOPH_DRAW_FLEX) ? 1u : 0u;
+  if (cb->box.p && cb->ib_ver == pd.ib->version && cb->vb_ver == pd.vb->version &&
+      cb->tri_count == g.tri_count && cb->vertex_count == g.vertex_count &&
+      cb->stride == g.stride && cb->flex == flex &&
+      (!flex || (cb->pos_off == g.pos_off && cb->pos_fmt == g.pos_fmt))) {
+    *out = cb->box.p;
+    *vref = cb->vref.p;
+    return true;
+  }
+  if (cb->batch == batch)
+    return true;
+  if (t_plan) {
+    t_plan_abort = true;
+    return false;
+  }
+  uint32_t nch = (g.tri_count + 255) / 256, nvwg = (g.vertex_count + 255) / 256;
+  if (!cb->box.ensure((size_t)nch * 2) || !cb->vref.ensure((size_t)nvwg * (1 + VREF_CAP) + 1))
+    return false;
+  cb->ib_ver = pd.ib->version;
+  cb->vb_ver = pd.vb->version;
+  cb->tri_count = g.tri_count;
+  cb->vertex_count = g.vertex_count;
+  cb->stride = g.stride;
+  cb->flex = flex;
+  cb->pos_off = g.pos_off;
+  cb->pos_fmt = g.pos_fmt;
+  cb->batch = batch;
+  ChunkBoundsArgs a;
+  a.vbuf = g.vbuf;
+  a.ibuf = g.ibuf;
+  a.tri_count = g.tri_count;
+  a.vertex_count = g.vertex_count;
+  a.stride = g.stride;
+  a.flex = flex;
+  a.pos_off = g.pos_off;
+  a.pos_fmt = g.pos_fmt;
+  a.box = cb->box.p;
+  a.vref = cb->vref.p;
+  a.nvwg = nvwg;
+  launch_fill_u32(nvwg, cb->vref.p, 0u, d->stream);
+  launch_chunk_bounds(a, d->stream);
+  *out = cb->box.p;
+  *vref = cb->vref.p;
+  return true;
+}
+
 static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ordered) {
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
   std::vector<ViewGPU> views;
   std::vector<uint32_t> wg_vtx, wg_tri;
+  std::vector<std::pair<size_t, const PendingDraw *>> cull_cand;
   uint32_t n_verts = 0, n_tris = 0, tiles = 0, stiles = 0;
   int T = (int)d->shade_tile;
   for (size_t j = 0; j < jobs.size(); j++) {
@@ -1520,13 +1639,34 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         set_err("batch exceeds 2^31 triangles; remaining draws skipped");
         break;
       }
-      g.vtx_wg = (uint32_t)wg_vtx.size();
-      g.tri_wg = (uint32_t)wg_tri.size();
-      wg_vtx.insert(wg_vtx.end(), (g.vertex_count + 255) / 256, (uint32_t)gd.size());
-      wg_tri.insert(wg_tri.end(), (g.tri_count + 255) / 256, (uint32_t)gd.size());
       n_verts += g.vertex_count;
       n_tris += g.tri_count;
+      g.chunk_box = nullptr;
+      if (!ordered && d->chunk_cull && g.tri_count >= CULL_MIN_TRIS && pd.ib && pd.vb &&
+          !pd.vb->bind_dev)
+        cull_cand.push_back(std::make_pair(gd.size(), &pd));
       gd.push_back(g);
+    }
+  }
+  /* draw-major workgroup grids; the workgroups of the chunk-culling
+   * candidates come after all others */
+  std::vector<char> cand(gd.size(), 0);
+  for (const auto &cc : cull_cand)
+    cand[cc.first] = 1;
+  uint32_t direct_vtx = 0, direct_tri = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    for (size_t k = 0; k < gd.size(); k++) {
+      if (cand[k] != pass)
+        continue;
+      DrawGPU &g = gd[k];
+      g.vtx_wg = (uint32_t)wg_vtx.size();
+      g.tri_wg = (uint32_t)wg_tri.size();
+      wg_vtx.insert(wg_vtx.end(), (g.vertex_count + 255) / 256, (uint32_t)k);
+      wg_tri.insert(wg_tri.end(), (g.tri_count + 255) / 256, (uint32_t)k);
+    }
+    if (pass == 0) {
+      direct_vtx = (uint32_t)wg_vtx.size();
+      direct_tri = (uint32_t)wg_tri.size();
     }
   }
   uint32_t clip_cap, subtri_cap;
@@ -1536,6 +1676,25 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (!ensure_frame_scratch(d, n_verts, n_tris, (uint32_t)gd.size(), (uint32_t)lights.size(),
                             (uint32_t)views.size(), fused, clip_cap, subtri_cap))
     return false;
+  /* chunk culling of the large draws of an order-free fused batch; the
+   * per-frame marks start all zero (k_vertex clears what k_cull set) */
+  bool any_cull = false;
+  if (fused && !cull_cand.empty()) {
+    uint64_t batch = ++d->batch_clock;
+    for (const auto &cc : cull_cand) {
+      const float4 *box;
+      const uint32_t *vref;
+      if (!draw_chunk_boxes(d, *cc.second, gd[cc.first], batch, &box, &vref))
+        return false;
+      gd[cc.first].chunk_box = box;
+      gd[cc.first].vref = vref;
+      any_cull = any_cull || box;
+    }
+    if (any_cull && (!d->tri_vis.ensure(wg_tri.size() + 1) ||
+                     !d->tri_list.ensure(wg_tri.size() + 1) ||
+                     !d->vtx_list.ensure(wg_vtx.size() + 1)))
+      return false;
+  }
   /* one upload per batch: [counters (zeroed) | draws | lights | views], each
    * 256-byte aligned */
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -1604,6 +1763,18 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     fa.big_chk = d->big_chk.p;
     fa.big_off = d->big_off.p;
     fa.big_map = d->big_map.p;
+    fa.tri_grid = fa.n_wg_tri;
+    if (any_cull) {
+      /* the kept workgroups are walked by fixed grids */
+      fa.tri_vis = d->tri_vis.p;
+      fa.tri_list = d->tri_list.p;
+      fa.vtx_list = d->vtx_list.p;
+      fa.tri_direct = direct_tri;
+      fa.vtx_direct = direct_vtx;
+      fa.tri_lgrid = std::min<uint32_t>(fa.n_wg_tri - direct_tri, d->cull_tri_grid);
+      fa.vtx_lgrid = std::min<uint32_t>(fa.n_wg_vtx - direct_vtx, d->cull_vtx_grid);
+      fa.tri_grid = fa.tri_direct + fa.tri_lgrid;
+    }
   }
   fa.vpos = d->vpos.p;
   fa.vattr = d->vattr.p;
@@ -1643,6 +1814,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (fa.ndraws) {
     {
       TmScope t(d, TM_VERTEX);
+      if (fa.tri_list)
+        launch_cull(fa, d->stream);
       launch_vertex(fa, d->stream);
     }
     if (fa.fused) {
@@ -2129,6 +2302,7 @@ static void check_frame(MopHipDevice *d, MopHipFramebuffer *fb) {
       st.subtris += c[CNT_SUBTRI];
       st.clipped += c[CNT_CLIP];
       st.big_subtris += c[CNT_BIG];
+      st.culled_chunks += c[CNT_CULLED];
     }
     if (ovf & 8u)
       set_err("a clipped polygon exceeded %d vertices and was dropped", MOPH_MAX_POLY);
@@ -2491,6 +2665,7 @@ static MopHipBuffer *lookup_or_upload(MopHipDevice *d, const void *host, size_t 
     return nullptr;
   }
   HIPCHK(hipMemcpy(b->dev, host, bytes, hipMemcpyHostToDevice));
+  buf_touched(b);
   d->shadow_temp.push_back(b);
   return b;
 }

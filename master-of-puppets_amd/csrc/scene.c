@@ -225,6 +225,26 @@ int mop_hip_scene_add_mesh(MopHipScene *s, const MopHipVertex *v, uint32_t nv,
   return (int)s->mesh_count - 1;
 }
 
+int mop_hip_scene_update_mesh_vertices(MopHipScene *s, uint32_t mesh, const MopHipVertex *v,
+                                       uint32_t nv) {
+  if (!s || !v || mesh >= s->mesh_count || nv != s->meshes[mesh].vertex_count)
+    return -1;
+  SceneMesh *m = &s->meshes[mesh];
+  mop_hip_buffer_update(s->dev, m->vb, v, 0, (size_t)nv * sizeof(MopHipVertex));
+  m->aabb_min = m->aabb_max = v[0].position;
+  for (uint32_t i = 1; i < nv; i++) {
+    MopHipVec3 p = v[i].position;
+    if (p.x < m->aabb_min.x) m->aabb_min.x = p.x;
+    if (p.y < m->aabb_min.y) m->aabb_min.y = p.y;
+    if (p.z < m->aabb_min.z) m->aabb_min.z = p.z;
+    if (p.x > m->aabb_max.x) m->aabb_max.x = p.x;
+    if (p.y > m->aabb_max.y) m->aabb_max.y = p.y;
+    if (p.z > m->aabb_max.z) m->aabb_max.z = p.z;
+  }
+  s->shadow_valid = 0;
+  return 0;
+}
+
 int mop_hip_scene_add_loaded_mesh(MopHipScene *s, MopHipLoadedMesh *lm, uint32_t object_id,
                                   const float *model16, int has_material, float metallic,
                                   float roughness) {

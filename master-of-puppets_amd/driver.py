@@ -44,7 +44,7 @@ class Device:
         self.lib.mop_hip_get_frame_stats(self.dev, C.byref(st))
         return {"triangles_in": st.triangles_in, "subtris": st.subtris, "clipped": st.clipped,
                 "big_subtris": st.big_subtris, "shadow_tris": st.shadow_tris,
-                "gpu_ms": st.gpu_ms}
+                "gpu_ms": st.gpu_ms, "culled_chunks": st.culled_chunks}
 
     def timers(self, on):
         self.lib.mop_hip_timers_enable(self.dev, 1 if on else 0)
@@ -196,6 +196,12 @@ class SceneDriver:
                     raise RuntimeError("add_loaded_mesh failed")
         finally:
             lib.mop_hip_scene_file_free(f)
+
+    def update_vertices(self, k, vertices):
+        """Rewrite mesh k's vertices in place (same count)."""
+        v = np.ascontiguousarray(vertices, np.float32)
+        if self.d.lib.mop_hip_scene_update_mesh_vertices(self.s, k, v.ctypes.data, len(v)) != 0:
+            raise RuntimeError("update_mesh_vertices failed")
 
     def render(self, fb, cam=None):
         return self.d.lib.mop_hip_scene_render(self.s, fb.fb, C.byref(cam or self.cam), fb.w, fb.h)

@@ -123,6 +123,7 @@ def test_config3_full_size_frame_driver():
     print(f"[stats] config3: {st} overflow flags {flags:#x}", flush=True)
     refcore.assert_parity(cmp)
     assert st["clipped"] > 0  # the camera sits inside the terrain
+    assert st["culled_chunks"] > 0  # and the parity holds with chunks of it skipped
 
 
 def _pick_views(views, n=8):
