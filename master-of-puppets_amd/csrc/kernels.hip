@@ -125,6 +125,36 @@ DEV void read_f2(const uint8_t *base, int off, int fmt, float o[2]) {
   }
 }
 
+/* The workgroup -> draw maps of the batch: entry i < n_wg_tri of the
+ * per-triangle grid, then the per-vertex grid; the draw is the last run
+ * starting at or before the workgroup (binary search over the runs, whose
+ * starts do not decrease; a draw with no workgroups shares its start with
+ * the next run). */
+__global__ __launch_bounds__(WG) void k_wg_map(FrameArgs fa) {
+  uint32_t i = blockIdx.x * WG + threadIdx.x;
+  if (i >= fa.n_wg_tri + fa.n_wg_vtx)
+    return;
+  bool tri = i < fa.n_wg_tri;
+  uint32_t b = tri ? i : i - fa.n_wg_tri;
+  uint32_t lo = 0, hi = fa.n_runs; /* runs[lo].start <= b < runs[hi].start */
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    uint4 r = fa.runs[mid];
+    if ((tri ? r.x : r.y) <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  uint32_t *map = (uint32_t *)(tri ? fa.wg_tri : fa.wg_vtx);
+  map[b] = fa.runs[lo].z;
+}
+
+void launch_wg_map(const FrameArgs &fa, hipStream_t s) {
+  uint32_t n = fa.n_wg_tri + fa.n_wg_vtx;
+  if (n && fa.n_runs)
+    launch(k_wg_map, dim3((n + WG - 1) / WG), dim3(WG), 0, s, fa);
+}
+
 /* the 256 vertices of per-vertex workgroup b (draw-major: the draw is
  * uniform over the workgroup, scalar loads) */
 DEV void vertex_work(const FrameArgs &fa, const DrawGPU *draws, uint32_t b, uint32_t di) {
@@ -332,26 +362,43 @@ void launch_chunk_bounds(const ChunkBoundsArgs &a, hipStream_t s) {
     launch(k_chunk_bounds, dim3(nb), dim3(WG), 0, s, a);
 }
 
-/* list[base + rank] = item for the lanes with keep, base from one atomic
- * per wave (the order of the list is immaterial to its consumers) */
-DEV void wave_append(uint2 *list, uint32_t *count, bool keep, uint2 item) {
+#define CULL_WG 1024
+
+/* list[base + rank] = item for the lanes of the workgroup with keep, base
+ * from one atomic per workgroup: atomics on one address are performed one
+ * after another at the memory side, so one per wave (6.4 k for a 12-view
+ * serving batch) took 0.12 ms.  The order of the list is immaterial to its
+ * consumers. */
+DEV void block_append(uint2 *list, uint32_t *count, bool keep, uint2 item) {
+  __shared__ uint32_t wave_base[CULL_WG / 64];
+  __shared__ uint32_t base;
   unsigned long long m = __ballot(keep);
-  uint32_t base = 0;
-  if ((threadIdx.x & 63u) == 0 && m)
-    base = atomicAdd(count, (uint32_t)__popcll(m));
-  base = __shfl(base, 0);
+  uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63u) == 0)
+    wave_base[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t sum = 0;
+    for (int i = 0; i < CULL_WG / 64; i++) {
+      uint32_t c = wave_base[i];
+      wave_base[i] = sum;
+      sum += c;
+    }
+    base = sum ? atomicAdd(count, sum) : 0u;
+  }
+  __syncthreads();
   if (keep) {
     uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-    list[base + rank] = item;
+    list[base + wave_base[w] + rank] = item;
   }
 }
 
 /* One lane per per-triangle workgroup b of the culled draws (b >=
  * tri_direct): kept unless the chunk's box fails a plane; tri_vis[b] and
  * the list of kept workgroups k_tri_list walks. */
-__global__ __launch_bounds__(WG) void k_cull_chunks(FrameArgs fa) {
-  uint32_t b = fa.tri_direct + blockIdx.x * WG + threadIdx.x;
+__global__ __launch_bounds__(CULL_WG) void k_cull_chunks(FrameArgs fa) {
+  uint32_t b = fa.tri_direct + blockIdx.x * CULL_WG + threadIdx.x;
   bool keep = false;
   uint32_t di = 0;
   if (b < fa.n_wg_tri) {
@@ -365,16 +412,13 @@ __global__ __launch_bounds__(WG) void k_cull_chunks(FrameArgs fa) {
     }
     fa.tri_vis[b] = keep ? 1u : 0u;
   }
-  unsigned long long culled = __ballot(b < fa.n_wg_tri && !keep);
-  if ((threadIdx.x & 63u) == 0 && culled)
-    atomicAdd(&fa.counters[CNT_CULLED], (uint32_t)__popcll(culled));
-  wave_append(fa.tri_list, &fa.counters[CNT_TLIST], keep, make_uint2(b, di));
+  block_append(fa.tri_list, &fa.counters[CNT_TLIST], keep, make_uint2(b, di));
 }
 
 /* One lane per per-vertex workgroup b of the culled draws (b >= vtx_direct):
  * kept when a kept chunk uses it or more than VREF_CAP chunks do. */
-__global__ __launch_bounds__(WG) void k_cull_vertices(FrameArgs fa) {
-  uint32_t b = fa.vtx_direct + blockIdx.x * WG + threadIdx.x;
+__global__ __launch_bounds__(CULL_WG) void k_cull_vertices(FrameArgs fa) {
+  uint32_t b = fa.vtx_direct + blockIdx.x * CULL_WG + threadIdx.x;
   bool need = false;
   uint32_t di = 0;
   if (b < fa.n_wg_vtx) {
@@ -392,15 +436,16 @@ __global__ __launch_bounds__(WG) void k_cull_vertices(FrameArgs fa) {
       }
     }
   }
-  wave_append(fa.vtx_list, &fa.counters[CNT_VLIST], need, make_uint2(b, di));
+  block_append(fa.vtx_list, &fa.counters[CNT_VLIST], need, make_uint2(b, di));
 }
 
 void launch_cull(const FrameArgs &fa, hipStream_t s) {
   if (fa.n_wg_tri > fa.tri_direct)
-    launch(k_cull_chunks, dim3((fa.n_wg_tri - fa.tri_direct + WG - 1) / WG), dim3(WG), 0, s, fa);
+    launch(k_cull_chunks, dim3((fa.n_wg_tri - fa.tri_direct + CULL_WG - 1) / CULL_WG),
+           dim3(CULL_WG), 0, s, fa);
   if (fa.n_wg_vtx > fa.vtx_direct)
-    launch(k_cull_vertices, dim3((fa.n_wg_vtx - fa.vtx_direct + WG - 1) / WG), dim3(WG), 0, s,
-           fa);
+    launch(k_cull_vertices, dim3((fa.n_wg_vtx - fa.vtx_direct + CULL_WG - 1) / CULL_WG),
+           dim3(CULL_WG), 0, s, fa);
 }
 
 /* --------------------------------------------------------------------- */
@@ -2131,6 +2176,11 @@ __global__ __launch_bounds__(WG) void k_finish(FrameArgs fa, uint32_t nblk, uint
     uint32_t *o = fa.views[v].cnt_out;
     for (int k = 0; k < CNT_N; k++)
       o[k] = __hip_atomic_load(&fa.counters[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    /* chunks of the culled draws that k_cull_chunks did not keep */
+    o[CNT_CULLED] = fa.tri_list ? fa.n_wg_tri - fa.tri_direct -
+                                      __hip_atomic_load(&fa.counters[CNT_TLIST], __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT)
+                                : 0u;
   }
   if (threadIdx.x == 0 && fa.sticky) {
     /* word 0: overflow bits; words 1-3: the largest clip-queue,

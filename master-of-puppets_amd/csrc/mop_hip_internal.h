@@ -233,9 +233,13 @@ struct FrameArgs {
   uint32_t ndraws;
   uint32_t nviews;
   /* draw-major grids of the per-vertex and per-triangle kernels: workgroup b
-   * works on 256 items of the single draw wg_vtx[b] / wg_tri[b] */
+   * works on 256 items of the single draw wg_vtx[b] / wg_tri[b]; the maps are
+   * written by k_wg_map from the runs (per draw in grid order: its first
+   * per-triangle and per-vertex workgroups, the draw) */
   const uint32_t *wg_vtx;
   const uint32_t *wg_tri;
+  const uint4 *runs;
+  uint32_t n_runs;
   uint32_t n_wg_vtx;
   uint32_t n_wg_tri;
   uint32_t n_verts;
@@ -368,6 +372,7 @@ struct ShadowArgsGPU {
 };
 
 /* Kernel launchers (kernels.hip). */
+void launch_wg_map(const FrameArgs &fa, hipStream_t s);
 void launch_vertex(const FrameArgs &fa, hipStream_t s);
 void launch_chunk_bounds(const ChunkBoundsArgs &a, hipStream_t s);
 void launch_cull(const FrameArgs &fa, hipStream_t s);

@@ -339,6 +339,7 @@ struct MopHipDevice {
   std::map<std::pair<const MopHipBuffer *, const MopHipBuffer *>, ChunkBoxes> cboxes;
   uint64_t batch_clock = 0;
   DevArr<uint32_t> tri_vis;
+  DevArr<uint32_t> wg_map; /* per-triangle then per-vertex workgroup -> draw */
   DevArr<uint2> tri_list, vtx_list;
   DevArr<uint8_t> arena; /* per-batch upload: counters, draws, lights, views */
   DevArr<float> skin_bones, morph_targets, morph_weights;
@@ -562,6 +563,7 @@ extern "C" void mop_hip_device_destroy(MopHipDevice *d) {
   }
   d->cboxes.clear();
   d->tri_vis.release();
+  d->wg_map.release();
   d->tri_list.release();
   d->vtx_list.release();
   d->arena.release();
@@ -1602,7 +1604,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   std::vector<DrawGPU> gd;
   std::vector<MopHipLight> lights;
   std::vector<ViewGPU> views;
-  std::vector<uint32_t> wg_vtx, wg_tri;
+  std::vector<uint4> runs; /* per draw in grid order: first workgroups, draw */
+  uint32_t n_wg_vtx = 0, n_wg_tri = 0;
   std::vector<std::pair<size_t, const PendingDraw *>> cull_cand;
   uint32_t n_verts = 0, n_tris = 0, tiles = 0, stiles = 0;
   int T = (int)d->shade_tile;
@@ -1659,14 +1662,15 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
       if (cand[k] != pass)
         continue;
       DrawGPU &g = gd[k];
-      g.vtx_wg = (uint32_t)wg_vtx.size();
-      g.tri_wg = (uint32_t)wg_tri.size();
-      wg_vtx.insert(wg_vtx.end(), (g.vertex_count + 255) / 256, (uint32_t)k);
-      wg_tri.insert(wg_tri.end(), (g.tri_count + 255) / 256, (uint32_t)k);
+      g.vtx_wg = n_wg_vtx;
+      g.tri_wg = n_wg_tri;
+      runs.push_back(make_uint4(n_wg_tri, n_wg_vtx, (uint32_t)k, 0u));
+      n_wg_vtx += (g.vertex_count + 255) / 256;
+      n_wg_tri += (g.tri_count + 255) / 256;
     }
     if (pass == 0) {
-      direct_vtx = (uint32_t)wg_vtx.size();
-      direct_tri = (uint32_t)wg_tri.size();
+      direct_vtx = n_wg_vtx;
+      direct_tri = n_wg_tri;
     }
   }
   uint32_t clip_cap, subtri_cap;
@@ -1690,9 +1694,9 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
       gd[cc.first].vref = vref;
       any_cull = any_cull || box;
     }
-    if (any_cull && (!d->tri_vis.ensure(wg_tri.size() + 1) ||
-                     !d->tri_list.ensure(wg_tri.size() + 1) ||
-                     !d->vtx_list.ensure(wg_vtx.size() + 1)))
+    if (any_cull && (!d->tri_vis.ensure((size_t)n_wg_tri + 1) ||
+                     !d->tri_list.ensure((size_t)n_wg_tri + 1) ||
+                     !d->vtx_list.ensure((size_t)n_wg_vtx + 1)))
       return false;
   }
   /* one upload per batch: [counters (zeroed) | draws | lights | views], each
@@ -1701,9 +1705,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   size_t oc = 0, od = al(CNT_N * sizeof(uint32_t));
   size_t bd = gd.size() * sizeof(DrawGPU), bl = lights.size() * sizeof(MopHipLight),
          bv = views.size() * sizeof(ViewGPU);
-  size_t bwv = wg_vtx.size() * sizeof(uint32_t), bwt = wg_tri.size() * sizeof(uint32_t);
-  size_t ol = od + al(bd), ov = ol + al(bl), owv = ov + al(bv), owt = owv + al(bwv),
-         total = owt + bwt;
+  size_t br = runs.size() * sizeof(uint4);
+  size_t ol = od + al(bd), ov = ol + al(bl), orun = ov + al(bv), total = orun + br;
   if (!d->arena.ensure(total))
     return false;
   int slot;
@@ -1715,10 +1718,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (bl)
     memcpy(stg + ol, lights.data(), bl);
   memcpy(stg + ov, views.data(), bv);
-  if (bwv)
-    memcpy(stg + owv, wg_vtx.data(), bwv);
-  if (bwt)
-    memcpy(stg + owt, wg_tri.data(), bwt);
+  if (br)
+    memcpy(stg + orun, runs.data(), br);
   upload(d, d->arena.p, stg, total);
   stage_release(d, slot);
 
@@ -1728,10 +1729,15 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.views = (const ViewGPU *)(d->arena.p + ov);
   fa.ndraws = (uint32_t)gd.size();
   fa.nviews = (uint32_t)views.size();
-  fa.wg_vtx = (const uint32_t *)(d->arena.p + owv);
-  fa.wg_tri = (const uint32_t *)(d->arena.p + owt);
-  fa.n_wg_vtx = (uint32_t)wg_vtx.size();
-  fa.n_wg_tri = (uint32_t)wg_tri.size();
+  /* the workgroup -> draw maps are written on the device from the runs */
+  if (!d->wg_map.ensure((size_t)n_wg_vtx + n_wg_tri + 1))
+    return false;
+  fa.runs = (const uint4 *)(d->arena.p + orun);
+  fa.n_runs = (uint32_t)runs.size();
+  fa.wg_tri = d->wg_map.p;
+  fa.wg_vtx = d->wg_map.p + n_wg_tri;
+  fa.n_wg_vtx = n_wg_vtx;
+  fa.n_wg_tri = n_wg_tri;
   fa.n_verts = n_verts;
   fa.n_tris = n_tris;
   fa.clip_cap = clip_cap;
@@ -1750,7 +1756,7 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
     if (d->force_chk)
       chk_cap = d->force_chk;
     if (!d->vscr.ensure(n_verts ? n_verts : 1) || !d->tri_rec.ensure(n_tris ? n_tris : 1) ||
-        !d->blk_counts.ensure(wg_tri.size() + 1 + CLIP_RASTER_GRID) ||
+        !d->blk_counts.ensure((size_t)n_wg_tri + 1 + CLIP_RASTER_GRID) ||
         !d->chk.ensure(chk_cap) || !d->big_chk.ensure(subtri_cap) ||
         !d->big_off.ensure((size_t)n_tris + (size_t)clip_cap * 16 + 1) ||
         !d->big_map.ensure(chk_cap))
@@ -1811,6 +1817,8 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
         !d->bins.ensure(bin_cap) || !d->block_sums.ensure(tiles / (256 * 16) + 2))
       return false;
   }
+  if (fa.ndraws)
+    launch_wg_map(fa, d->stream);
   if (fa.ndraws) {
     {
       TmScope t(d, TM_VERTEX);
