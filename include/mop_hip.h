@@ -336,6 +336,11 @@ int mop_hip_set_tuning(MopHipDevice *dev, uint32_t small_area, uint32_t shade_ti
 int mop_hip_set_shade_waves(MopHipDevice *dev, uint32_t waves);
 /* 1 when fb's frame has finished on the device (non-blocking). */
 int mop_hip_frame_done(MopHipDevice *dev, MopHipFramebuffer *fb);
+/* 1 when the copy mop_hip_framebuffer_readback_async started for fb's
+ * current frame has reached host memory, so the read_* calls of what it
+ * copied return without waiting (non-blocking); mop_hip_frame_done when no
+ * such copy is pending. */
+int mop_hip_readback_done(MopHipDevice *dev, MopHipFramebuffer *fb);
 /* Wait until everything queued on the device has finished. */
 void mop_hip_device_sync(MopHipDevice *dev);
 

@@ -182,6 +182,7 @@ SIGS = {
     "mop_hip_set_default_scratch_limits": (None, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "mop_hip_overflow_reruns": (C.c_uint64, []),
     "mop_hip_framebuffer_readback_async": (None, [VP, VP, C.c_uint32]),
+    "mop_hip_readback_done": (C.c_int, [VP, VP]),
     "mop_hip_shadow_map_read": (C.c_int, [VP, VP, c_float_p]),
     "mop_hip_framebuffer_read_color_downsampled": (c_u8_p, [VP, VP, C.c_int, c_int_p, c_int_p]),
     "mop_hip_draw_skybox": (None, [VP, VP, VP, c_float_p, c_float_p, C.c_float, C.c_float]),

@@ -2407,6 +2407,17 @@ extern "C" int mop_hip_frame_done(MopHipDevice *d, MopHipFramebuffer *fb) {
   return hipEventQuery(fb->done) == hipSuccess ? 1 : 0;
 }
 
+/* 1 when fb's frame has executed and the copy a
+ * mop_hip_framebuffer_readback_async call started for it has reached host
+ * memory; with no such copy pending, mop_hip_frame_done. */
+extern "C" int mop_hip_readback_done(MopHipDevice *d, MopHipFramebuffer *fb) {
+  if (!d || !fb || !fb->executed)
+    return 0;
+  if (fb->rb_pending && fb->rb_seq == fb->exec_seq)
+    return hipEventQuery(fb->rb_done) == hipSuccess ? 1 : 0;
+  return hipEventQuery(fb->done) == hipSuccess ? 1 : 0;
+}
+
 /* Wait for all work queued on the device. */
 extern "C" void mop_hip_device_sync(MopHipDevice *d) {
   if (d)

@@ -68,6 +68,10 @@ class Framebuffer:
     def done(self):
         return self.d.lib.mop_hip_frame_done(self.d.dev, self.fb) == 1
 
+    def readback_done(self):
+        """True when the copy readback_async started has reached the host."""
+        return self.d.lib.mop_hip_readback_done(self.d.dev, self.fb) == 1
+
     def read(self, color=True, depth=True, ids=True, hdr=False):
         lib, dev, fb = self.d.lib, self.d.dev, self.fb
         w, h, n = self.w, self.h, self.w * self.h

@@ -14,7 +14,9 @@ its class is free and the batch's scratch stays within that bound; it waits
 in a bounded queue otherwise, and is rejected when the queue is full.  Continuous batching: while up to `inflight` batches run on the
 device, newly arrived requests are recorded into free slots and launched as
 the next batch; finished batches are read back (RGBA8 to host) and their
-slots freed.  Latency = arrival to host readback.  The plain mode serves the
+slots freed (each result's RGBA8 copy is queued on the copy stream when its
+batch is launched, so copies overlap the next batches' kernels).  Latency =
+arrival to the result on the host.  The plain mode serves the
 same stream one request per launch, synchronously.
 """
 import collections
@@ -136,7 +138,7 @@ class Server:
             max_queue = max(max_queue, len(queue))
             # completions
             progressed = False
-            while inflight and all(fb.done() for _, fb in inflight[0][1]):
+            while inflight and all(fb.readback_done() for _, fb in inflight[0][1]):
                 _, items = inflight.popleft()
                 for req, fb in items:
                     fb.read_color_raw()
@@ -176,6 +178,11 @@ class Server:
                     for req, fb in batch:
                         self.scenes[req[1]][0].record(fb, self._camera(req))
                     self.d.execute([fb for _, fb in batch])
+                    # each RGBA8 result is copied to the host on the copy
+                    # stream as soon as its batch ends, behind the next
+                    # batches' kernels
+                    for _, fb in batch:
+                        fb.readback_async(color=True, depth=False, ids=False)
                     inflight.append((time.perf_counter() - t0, batch))
                     batches += 1
                     batch_sizes.append(len(batch))
