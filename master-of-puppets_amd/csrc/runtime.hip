@@ -236,6 +236,10 @@ struct ChunkBoxes {
   DevArr<uint32_t> vref; /* per-vertex-workgroup chunk references */
 };
 
+/* batches with at most this many per-vertex plus per-triangle workgroups
+ * upload their workgroup -> draw maps; larger ones build them on the device */
+#define WG_MAP_HOST_MAX 32768u
+
 /* draws with at least this many triangles are chunk-culled */
 #define CULL_MIN_TRIS 65536u
 
@@ -1705,7 +1709,12 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   size_t oc = 0, od = al(CNT_N * sizeof(uint32_t));
   size_t bd = gd.size() * sizeof(DrawGPU), bl = lights.size() * sizeof(MopHipLight),
          bv = views.size() * sizeof(ViewGPU);
-  size_t br = runs.size() * sizeof(uint4);
+  /* the workgroup -> draw maps: small ones written here and uploaded with
+   * the tables, large ones (a 12-view serving batch has 0.6 M entries)
+   * written on the device from the runs (k_wg_map, one launch more) */
+  bool host_maps = (size_t)n_wg_vtx + n_wg_tri <= WG_MAP_HOST_MAX;
+  size_t br = host_maps ? ((size_t)n_wg_tri + n_wg_vtx) * sizeof(uint32_t)
+                        : runs.size() * sizeof(uint4);
   size_t ol = od + al(bd), ov = ol + al(bl), orun = ov + al(bv), total = orun + br;
   if (!d->arena.ensure(total))
     return false;
@@ -1718,8 +1727,16 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   if (bl)
     memcpy(stg + ol, lights.data(), bl);
   memcpy(stg + ov, views.data(), bv);
-  if (br)
+  if (br && host_maps) {
+    uint32_t *m = (uint32_t *)(stg + orun);
+    for (const uint4 &r : runs) {
+      const DrawGPU &g = gd[r.z];
+      std::fill(m + r.x, m + r.x + (g.tri_count + 255) / 256, r.z);
+      std::fill(m + n_wg_tri + r.y, m + n_wg_tri + r.y + (g.vertex_count + 255) / 256, r.z);
+    }
+  } else if (br) {
     memcpy(stg + orun, runs.data(), br);
+  }
   upload(d, d->arena.p, stg, total);
   stage_release(d, slot);
 
@@ -1729,13 +1746,17 @@ static bool run_batch(MopHipDevice *d, const std::vector<SegJob> &jobs, bool ord
   fa.views = (const ViewGPU *)(d->arena.p + ov);
   fa.ndraws = (uint32_t)gd.size();
   fa.nviews = (uint32_t)views.size();
-  /* the workgroup -> draw maps are written on the device from the runs */
-  if (!d->wg_map.ensure((size_t)n_wg_vtx + n_wg_tri + 1))
-    return false;
-  fa.runs = (const uint4 *)(d->arena.p + orun);
-  fa.n_runs = (uint32_t)runs.size();
-  fa.wg_tri = d->wg_map.p;
-  fa.wg_vtx = d->wg_map.p + n_wg_tri;
+  if (host_maps) {
+    fa.wg_tri = (const uint32_t *)(d->arena.p + orun);
+    fa.wg_vtx = fa.wg_tri + n_wg_tri;
+  } else {
+    if (!d->wg_map.ensure((size_t)n_wg_vtx + n_wg_tri + 1))
+      return false;
+    fa.runs = (const uint4 *)(d->arena.p + orun);
+    fa.n_runs = (uint32_t)runs.size();
+    fa.wg_tri = d->wg_map.p;
+    fa.wg_vtx = d->wg_map.p + n_wg_tri;
+  }
   fa.n_wg_vtx = n_wg_vtx;
   fa.n_wg_tri = n_wg_tri;
   fa.n_verts = n_verts;
