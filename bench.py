@@ -564,17 +564,18 @@ def cpu_baseline(seconds, pool_cores):
 
 
 def raster_atomic_roofline(lib):
-    """Roofline of the batch's dominant raster kernel, k_tri: its visibility
+    """Roofline of the batch's dominant raster kernel, k_tri_list (the
+    per-triangle kernel of chunk-culled batches, k_tri's walk): its visibility
     atomics (64-bit atomic min, no return value, one per covered fragment)
     over its launch time, both from the committed counter summary of the
     batch, against the rate at which this GPU performs such atomics on
     random addresses, measured now (mop_hip_probe_atomic_rate)."""
-    n = profiled("batch64_counters.md", "atomics_per_launch", "k_tri")
-    us = profiled("batch64_counters.md", "launch_us", "k_tri")
+    n = profiled("batch64_counters.md", "atomics_per_launch", "k_tri_list")
+    us = profiled("batch64_counters.md", "launch_us", "k_tri_list")
     rand = lib.mop_hip_probe_atomic_rate(0)
     coal = lib.mop_hip_probe_atomic_rate(1)
     log(f"atomic probe: {rand:.2f} G/s random, {coal:.2f} G/s with a wave on 512 consecutive bytes")
-    out = {"kernel": "k_tri", "bound": "memory-side atomics", "unit": "G atomics/s",
+    out = {"kernel": "k_tri_list", "bound": "memory-side atomics", "unit": "G atomics/s",
            "peak": round(rand, 2), "peak_coalesced": round(coal, 2),
            "source": "atomics and launch time: profiles/batch64_counters.md; peak: measured now"}
     if n and us and rand > 0:
