@@ -476,6 +476,55 @@ def transparent_form(frames, only=False):
     return out
 
 
+def opaque_tiles_form(frames=100):
+    """The 1080p PBR workload with its opaque draws on the tile path
+    (mop_hip_set_opaque_raster: binned by prefix scan to 16x16 tiles in
+    submission order, colour/depth/id tiles in LDS, ballot coverage) instead
+    of the visibility-buffer minimum: HBM-resident frames/s, the device span
+    of one frame and the stage times, for comparison with the default path.
+    The frames are the same (tests/test_tile_raster.py)."""
+    from mop_amd import driver, scenes
+    dev = driver.Device()
+    dev.opaque_raster(True)
+    sc = scenes.config2()
+    drv = driver.SceneDriver(dev, sc, shadow_size=sc.shadow_size)
+    fbs = [dev.framebuffer(sc.width, sc.height) for _ in range(2)]
+    for k in range(3):
+        drv.render(fbs[k % 2], drv.cam)
+    fbs[0].read()  # an overflowed warm-up frame is grown and run again here
+    dev.sync()
+    dev.overflow_flags(reset=True)
+    t0 = time.perf_counter()
+    for k in range(frames):
+        drv.render(fbs[k % 2], drv.cam)
+    dev.sync()
+    secs = time.perf_counter() - t0
+    check_overflow(dev, "tiled-opaque HBM-resident frames")
+    out = {"path": "opaque draws binned to 16x16 tiles, walked in submission order in LDS",
+           "frames": frames, "frames_per_s_hbm_resident": round(frames / secs, 3)}
+    drv.render(fbs[0], drv.cam)
+    dev.sync()
+    out["device_ms_one_frame"] = round(dev.frame_stats()["gpu_ms"], 4)
+    dev.timers(True)
+    for k in range(20):
+        drv.render(fbs[k % 2], drv.cam)
+    dev.sync()
+    st = {}
+    for name in ("shadow", "vertex", "setup", "raster", "shade", "resolve", "post"):
+        tot, cnt = dev.timer(name)
+        if cnt:
+            st[name] = round(tot / 20, 4)
+    dev.timers(False)
+    check_overflow(dev, "tiled-opaque stage-timed frames")
+    out["stage_ms_per_frame"] = st
+    log(f"tiled opaque 1080p: {out}")
+    for fb in fbs:
+        fb.close()
+    drv.close()
+    dev.close()
+    return out
+
+
 # ---------------------------------------------------------------------------
 # reference CPU baseline
 
@@ -894,6 +943,15 @@ def main():
     sc = c2["sc"]
     c3 = None if args.no_config3 else config3_form(args.config3_frames)
     tr = None if args.no_transparent else transparent_form(args.transparent_frames)
+    tiles = None
+    if not args.no_transparent:
+        try:  # a comparison leg: its failure is reported, not fatal
+            tiles = opaque_tiles_form()
+            tiles["default_path_over_tiled"] = round(
+                c2["extra"]["fps_hbm_resident"] / tiles["frames_per_s_hbm_resident"], 2)
+        except Exception as e:  # noqa: BLE001
+            tiles = {"error": f"{type(e).__name__}: {e}"}
+            log(f"tiled opaque leg failed: {tiles['error']}")
     batch = None if args.no_batch else batched_form(mod, args.batch_rounds, not args.no_tune)
     if batch:
         log(f"batched: {batch}")
@@ -929,6 +987,7 @@ def main():
         "roofline": c2["roofline"],
         "raster_stress_4k": c3,
         "transparent_1080p": tr,
+        "opaque_tile_raster_1080p": tiles,
         "batched_64_viewports": batch,
         "serving": serving,
         "cpu_baseline": cpu,

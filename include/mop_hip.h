@@ -334,6 +334,14 @@ int mop_hip_set_tuning(MopHipDevice *dev, uint32_t small_area, uint32_t shade_ti
  * kernel is compiled for each; fewer waves leave more registers per lane).
  * Results never depend on it.  Returns 0, or -1 for another value. */
 int mop_hip_set_shade_waves(MopHipDevice *dev, uint32_t waves);
+/* Raster path of the opaque, depth-tested and depth-written draws recorded
+ * after the call: 0 (default) the per-pixel (depth, submission index)
+ * minimum of the visibility buffer, 1 the tile path the blended draws take
+ * (prefix-scan binning to 16x16 tiles in submission order, colour/depth/id
+ * tiles in LDS, ballot coverage).  Results are identical; speed differs.
+ * No counterpart in the reference (the CPU backend has one raster).
+ * Returns 0, or -1 for another value. */
+int mop_hip_set_opaque_raster(MopHipDevice *dev, uint32_t tiled);
 /* 1 when fb's frame has finished on the device (non-blocking). */
 int mop_hip_frame_done(MopHipDevice *dev, MopHipFramebuffer *fb);
 /* 1 when the copy mop_hip_framebuffer_readback_async started for fb's

@@ -172,6 +172,7 @@ SIGS = {
     "mop_hip_frame_done": (C.c_int, [VP, VP]),
     "mop_hip_set_tuning": (C.c_int, [VP, C.c_uint32, C.c_uint32, C.c_uint32]),
     "mop_hip_set_shade_waves": (C.c_int, [VP, C.c_uint32]),
+    "mop_hip_set_opaque_raster": (C.c_int, [VP, C.c_uint32]),
     "mop_hip_set_graphs": (None, [VP, C.c_int]),
     "mop_hip_graph_stats": (None, [VP, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "mop_hip_skin_apply": (C.c_int, [VP, VP, VP, C.c_uint32, VP, VP, C.c_uint32]),

@@ -353,6 +353,9 @@ struct MopHipDevice {
   uint32_t clip_cap_hint = 0;
   /* tunables (mop_hip_set_tuning) */
   uint32_t small_area = 64, shade_tile = 16, raster_grid = 0, shade_waves = 5;
+  /* mop_hip_set_opaque_raster: order-free draws recorded from now on take the
+   * tile path (binned, LDS tiles walked in submission order) */
+  uint32_t opaque_tiles = 0;
   DevArr<uint32_t> tile_counts, tile_cursor, bins;
   uint32_t bin_cap_hint = 0;
   /* forced scratch capacities (mop_hip_set_scratch_limits; 0 = none): used
@@ -1419,7 +1422,7 @@ static void record_draw(MopHipDevice *d, MopHipFramebuffer *fb, const MopHipDraw
   /* 0 order-free, 3 ordered (tile path), 2 no-op */
   if (c->wireframe)
     invisible = false; /* lines ignore blend mode and opacity */
-  pd.cls = order_free ? 0 : (invisible ? 2 : 3);
+  pd.cls = order_free && !d->opaque_tiles ? 0 : (invisible ? 2 : 3);
   pd.state = current_state(d, fb);
   fb->draws.push_back(std::move(pd));
   fb->executed = false;
@@ -2418,6 +2421,19 @@ extern "C" int mop_hip_set_tuning(MopHipDevice *d, uint32_t small_area, uint32_t
   d->small_area = small_area;
   d->shade_tile = shade_tile;
   d->raster_grid = raster_grid;
+  return 0;
+}
+
+/* Raster path of the opaque depth-tested draws recorded from now on: 0 (the
+ * default) the visibility buffer's (depth, submission) minimum, 1 the tile
+ * path of the blended draws -- screen triangles binned to 16x16 tiles by
+ * prefix scan in submission order, each tile's colour, depth and id held in
+ * LDS and its list walked in order with ballot coverage skips.  Both give
+ * the CPU backend's frame; the first is faster on the workloads here. */
+extern "C" int mop_hip_set_opaque_raster(MopHipDevice *d, uint32_t tiled) {
+  if (!d || tiled > 1)
+    return -1;
+  d->opaque_tiles = tiled;
   return 0;
 }
 

@@ -25,6 +25,12 @@ class Device:
     def sync(self):
         self.lib.mop_hip_device_sync(self.dev)
 
+    def opaque_raster(self, tiled):
+        """Raster path of the opaque draws recorded from now on: False the
+        visibility-buffer minimum (default), True the binned tile path."""
+        if self.lib.mop_hip_set_opaque_raster(self.dev, 1 if tiled else 0) != 0:
+            raise ValueError("mop_hip_set_opaque_raster failed")
+
     def execute(self, fbs):
         arr = (C.c_void_p * len(fbs))(*[f.fb for f in fbs])
         self.lib.mop_hip_frames_execute(self.dev, arr, len(fbs))
